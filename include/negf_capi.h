@@ -1,0 +1,153 @@
+/*
+ * negf_capi.h -- C ABI of the MI355X (gfx950) per-energy NEGF solve.
+ *
+ * Drop-in boundary for the Energy_Sweep path of the reference
+ * (brian-dellabetta/nonEquilibriumGreensFunction).  The reference's hot-path
+ * subroutines take no arguments and read/write NEGF_Module globals
+ * (NEGF_Module.F90:51-100); these entry points take the same arrays as plain
+ * pointers, so a Fortran caller binds them with iso_c_binding and passes its
+ * module arrays (see the shims in fortran/negf_shims.f90).
+ *
+ * Conventions
+ *  - Complex arrays are COMPLEX(8): interleaved (re, im) doubles, column-major,
+ *    passed as `double*`.  Htotd(i,j,x) is element ((x*n)+j)*n+i (0-based),
+ *    n = NNz = Npy*Npz*Norb, with basis index orb + Norb*(y + Npy*z).
+ *  - Spatial arrays (RhoN, SigXExp, Jx, ...) are REAL(8) (Npx,Npy,Npz), Fortran order.
+ *  - Per-energy arrays (T12, Modec1, ...) are REAL(8)(eStpnum), indexed by eCtr (0-based).
+ *  - Every call returns an int status: 0 = ok; > 0 = LAPACK-style INFO of the first
+ *    singular block (or the NEGF_STATUS_* bits below); < 0 = API/HIP error, with a
+ *    message available from negf_last_error().
+ *  - Single host thread per context.  Device work is asynchronous inside
+ *    negf_submit(); negf_flush() must return before the host reads accumulators
+ *    (the reference reads them at MPI_Gather_Iteration, NEGF_Main.F90:30).
+ *  - There is no host (CPU) fallback: without a usable gfx950 device every entry
+ *    point fails with NEGF_ERR_NO_DEVICE.
+ */
+#ifndef NEGF_CAPI_H
+#define NEGF_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define NEGF_OK 0
+#define NEGF_ERR_ARG (-1)
+#define NEGF_ERR_HIP (-2)
+#define NEGF_ERR_NO_DEVICE (-3)
+#define NEGF_ERR_MEMORY (-4)
+#define NEGF_ERR_STATE (-5)
+/* positive status bits (per energy), combined with a LAPACK INFO in the low 20 bits */
+#define NEGF_STATUS_SR_NOT_CONVERGED (1 << 20) /* Contact_Self_Energy.F90:202-205 */
+#define NEGF_STATUS_SR_NAN (1 << 21)           /* Contact_Self_Energy.F90:182-191 */
+
+/* precision forms */
+#define NEGF_PREC_FP64 0  /* plain complex128 on FP64 MFMA */
+#define NEGF_PREC_MIXED 1 /* complex64 MFMA LU/GEMM with FP64 refinement */
+
+typedef struct negf_ctx negf_ctx;
+
+/* Dimensions and solver options.  Replaces the globals Npx, Npy, Npz, Norb
+ * (NEGF_Module.F90:27), eta_sigma (Constants.F90:7) and the Sancho-Rubio
+ * constants hard-coded at Contact_Self_Energy.F90:141,154. */
+typedef struct {
+  int npx, npy, npz, norb;
+  double eta;          /* eta_sigma */
+  double sr_tol;       /* SR stop: max|change| <= sr_tol */
+  int sr_max_iter;     /* SR iteration cap, reset per call */
+  int sr_extra_eta;    /* 1: SR seed inverts alpha + i*eta*I as Contact_Self_Energy.F90:141 does */
+  int precision;       /* NEGF_PREC_* */
+  int device;          /* HIP device ordinal */
+  double hbm_budget_gb;/* device memory the context may use for in-flight energies; 0 = auto */
+  int max_batch;       /* cap on energies in flight; 0 = auto */
+} negf_config;
+
+/* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
+ * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
+ * reference does. */
+typedef struct {
+  double *T12, *Modec1, *Modec2, *DOSc1, *DOSc2, *EDensity, *HDensity; /* [eStpnum] at eCtr */
+  double *RhoN, *RhoP;                                                  /* (Npx,Npy,Npz) */
+  double *SigXExp, *SigYExp, *SigZExp;                                  /* (Npx,Npy,Npz) */
+  double *ContactCurrent;                                               /* [2] */
+} negf_accum;
+
+const char* negf_last_error(void);
+int negf_device_count(void);
+
+/* Read_In_Allocate (Read_In_Allocate.F90:104-149) analogue: validates the
+ * config and binds a device.  No device memory is reserved yet. */
+int negf_init(negf_ctx** ctx, const negf_config* cfg);
+int negf_finalize(negf_ctx* ctx);
+
+/* Upload Htotd (n,n,Npx), HopX (n,n) and the spin matrices gMx,gMy,gMz (three
+ * column-major Norb x Norb blocks, Read_In_Allocate.F90:186-196).  Call after
+ * Build_Hamiltonian / MPI_Broadcast (NEGF_Main.F90:15-19), i.e. once per
+ * Poisson iteration. */
+int negf_set_hamiltonian(negf_ctx* ctx, const double* Htotd, const double* HopX,
+                         const double* gM);
+
+/* Energy_Sweep (Energy_Sweep.F90:1-46) for one energy, fused on the device:
+ * contacts, RGF, Gn/Gp site blocks and observables.  mu[2], muT give the step
+ * occupations of Energy_Sweep.F90:11-23.  Accumulates into acc at eCtr. */
+int negf_energy_sweep(negf_ctx* ctx, double energy, int eCtr, double deltaE, const double* mu,
+                      double muT, negf_accum* acc);
+
+/* Batched form of the energy loop (NEGF_Main.F90:21-27): queue energies, then
+ * flush.  The device runs queued energies in batches sized to the HBM budget.
+ * status (optional, [count of queued energies]) receives per-energy status in
+ * submission order; sr_iters (optional, 2 per energy) the SR iteration counts. */
+int negf_submit(negf_ctx* ctx, int count, const double* energies, const int* eCtr,
+                const double* deltaE);
+int negf_flush(negf_ctx* ctx, const double* mu, double muT, negf_accum* acc, int* status,
+               int* sr_iters);
+
+/* --- Stage entry points (each mirrors one reference subroutine) ---------- */
+
+/* Contact_Self_Energy, semi-infinite branch (Contact_Self_Energy.F90:68-101):
+ * SigmaL/R and GammaL/R (n,n); dos[2] = this energy's DOSc1, DOSc2 increments;
+ * iters[2] = SR iteration counts (optional). */
+int negf_contact_self_energy(negf_ctx* ctx, double energy, double* SigmaL, double* SigmaR,
+                             double* GammaL, double* GammaR, double* dos, int* iters);
+
+/* Sancho_Rubio (Contact_Self_Energy.F90:110-223) for one lead given alpha, beta. */
+int negf_sancho_rubio(negf_ctx* ctx, const double* alpha, const double* beta, double* Sigma,
+                      int* iters);
+
+/* Get_Retarded_Greens (Recursive_Greens.F90:2-107) on a given InvGRd (n,n,Npx)
+ * and the context's HopX: GRd (n,n,Npx), GRu, GRl (n,n,Npx-1), GRLef, GRRgt
+ * (Npx*n, n).  Any output may be NULL. */
+int negf_retarded(negf_ctx* ctx, const double* InvGRd, double* GRd, double* GRu, double* GRl,
+                  double* GRLef, double* GRRgt);
+
+/* Calculate_Gnp (Calculate_Gnp.F90:1-136): Gnd/Gpd (n,n,Npx), Gnu/Gpu/Gnl/Gpl
+ * (n,n,Npx-1) overwritten (any may be NULL); RhoN/RhoP accumulate. */
+int negf_gnp(negf_ctx* ctx, double f1, double f2, double deltaE, const double* GRLef,
+             const double* GRRgt, const double* GammaL, const double* GammaR, double* Gnd,
+             double* Gnu, double* Gnl, double* Gpd, double* Gpu, double* Gpl, double* RhoN,
+             double* RhoP);
+
+/* Calculate_Observables, non-BdG branches (Calculate_Observables.F90:9-60,85-169)
+ * with spin enabled: per-energy values go to acc->T12 .. HDensity at eCtr, spatial
+ * arrays and ContactCurrent accumulate; Jx/Jy/Jz (optional) accumulate in REAL(8). */
+int negf_observables(negf_ctx* ctx, int eCtr, double f1, double f2, double deltaE,
+                     const double* Gnd, const double* Gpd, const double* Gnu, const double* Gnl,
+                     const double* Gpu, const double* Gpl, const double* GRRgt,
+                     const double* GammaL, const double* GammaR, negf_accum* acc, double* Jx,
+                     double* Jy, double* Jz);
+
+/* --- L1 primitives --------------------------------------------------------- */
+
+/* Invert_Matrix (MKL_Routines_Cluster.F90:2-33): invG = G^-1 for one n x n
+ * host matrix; *info = first zero pivot (1-based) or 0. */
+int negf_invert_matrix(int n, const double* G, double* invG, int* info);
+
+/* zgemm (as called at Recursive_Greens.F90:18-95): C = alpha op(A) op(B) + beta C
+ * for square n x n host matrices, op in 'N','T','C'. */
+int negf_zgemm(char transa, char transb, int n, const double* alpha, const double* A, int lda,
+               const double* B, int ldb, const double* beta, double* C, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEGF_CAPI_H */

@@ -1,0 +1,233 @@
+"""ctypes binding of ``libnegf_hip.so`` (the C ABI declared in include/negf_capi.h).
+
+This is the Python-side twin of the Fortran ``iso_c_binding`` interfaces in
+``fortran/negf_capi.f90``: the same entry points, argument meaning and status
+codes.  Arrays are numpy complex128/float64 in Fortran order (the NEGF_Module
+layouts).  There is no CPU fallback: if the library is missing, or no GPU is
+visible, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libnegf_hip.so")
+
+NEGF_OK = 0
+NEGF_ERR_ARG = -1
+NEGF_ERR_HIP = -2
+NEGF_ERR_NO_DEVICE = -3
+NEGF_ERR_MEMORY = -4
+NEGF_ERR_STATE = -5
+NEGF_STATUS_SR_NOT_CONVERGED = 1 << 20
+NEGF_STATUS_SR_NAN = 1 << 21
+NEGF_PREC_FP64 = 0
+NEGF_PREC_MIXED = 1
+
+# Every entry point include/negf_capi.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "negf_last_error", "negf_device_count", "negf_init", "negf_finalize", "negf_set_hamiltonian",
+    "negf_energy_sweep", "negf_submit", "negf_flush", "negf_contact_self_energy",
+    "negf_sancho_rubio", "negf_retarded", "negf_gnp", "negf_observables", "negf_invert_matrix",
+    "negf_zgemm",
+)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class NegfConfig(ctypes.Structure):
+    _fields_ = [("npx", ctypes.c_int), ("npy", ctypes.c_int), ("npz", ctypes.c_int),
+                ("norb", ctypes.c_int), ("eta", ctypes.c_double), ("sr_tol", ctypes.c_double),
+                ("sr_max_iter", ctypes.c_int), ("sr_extra_eta", ctypes.c_int),
+                ("precision", ctypes.c_int), ("device", ctypes.c_int),
+                ("hbm_budget_gb", ctypes.c_double), ("max_batch", ctypes.c_int)]
+
+
+class NegfAccum(ctypes.Structure):
+    _fields_ = [(k, _dp) for k in ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity",
+                                   "HDensity", "RhoN", "RhoP", "SigXExp", "SigYExp", "SigZExp",
+                                   "ContactCurrent")]
+
+
+class NegfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"negf status {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m nonequilibriumgreensfunction_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.negf_last_error.restype = ctypes.c_char_p
+        L.negf_init.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(NegfConfig)]
+        L.negf_finalize.argtypes = [ctypes.c_void_p]
+        L.negf_set_hamiltonian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.negf_energy_sweep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_double, _dp, ctypes.c_double,
+                                        ctypes.POINTER(NegfAccum)]
+        L.negf_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _dp]
+        L.negf_flush.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.POINTER(NegfAccum),
+                                 _ip, _ip]
+        L.negf_contact_self_energy.argtypes = [ctypes.c_void_p, ctypes.c_double] + [_dp] * 5 + [_ip]
+        L.negf_sancho_rubio.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ip]
+        L.negf_invert_matrix.argtypes = [ctypes.c_int, _dp, _dp, _ip]
+        L.negf_zgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                 _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        if hasattr(L, "negf_retarded"):
+            L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
+            L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
+            L.negf_observables.argtypes = ([ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 3
+                                           + [_dp] * 9 + [ctypes.POINTER(NegfAccum)] + [_dp] * 3)
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    return lib().negf_last_error().decode(errors="replace")
+
+
+def _check(rc):
+    if rc < 0:
+        raise NegfError(rc, last_error())
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _cf(a):
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def device_count() -> int:
+    return lib().negf_device_count()
+
+
+def invert_matrix(G):
+    """Invert_Matrix: returns (inverse, info)."""
+    G = _cf(G)
+    out = np.zeros_like(G)
+    info = ctypes.c_int(0)
+    _check(lib().negf_invert_matrix(G.shape[0], _p(G), _p(out), ctypes.byref(info)))
+    return out, info.value
+
+
+def zgemm(ta, tb, A, B, alpha=1.0, beta=0.0, C=None):
+    A, B = _cf(A), _cf(B)
+    n = A.shape[0]
+    C = np.zeros((n, n), complex, order="F") if C is None else _cf(C).copy(order="F")
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    be = np.array([complex(beta).real, complex(beta).imag])
+    _check(lib().negf_zgemm(ta.encode(), tb.encode(), n, _p(al), _p(A), n, _p(B), n, _p(be), _p(C), n))
+    return C
+
+
+class Accumulators:
+    """The reference's energy-integrated outputs (NEGF_Module.F90:51-97)."""
+
+    PER_ENERGY = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity")
+    SPATIAL = ("RhoN", "RhoP", "SigXExp", "SigYExp", "SigZExp")
+
+    def __init__(self, npx, npy, npz, estpnum):
+        for k in self.PER_ENERGY:
+            setattr(self, k, np.zeros(estpnum))
+        for k in self.SPATIAL:
+            setattr(self, k, np.zeros((npx, npy, npz), order="F"))
+        self.ContactCurrent = np.zeros(2)
+
+    def struct(self):
+        s = NegfAccum()
+        for k in self.PER_ENERGY + self.SPATIAL + ("ContactCurrent",):
+            setattr(s, k, _p(getattr(self, k)))
+        return s
+
+
+class Context:
+    """One device context for a wire of fixed dimensions (negf_init .. negf_finalize)."""
+
+    def __init__(self, npx, npy, npz, norb=4, eta=1e-6, sr_tol=1e-10, sr_max_iter=200,
+                 sr_extra_eta=True, precision=NEGF_PREC_FP64, device=0, hbm_budget_gb=0.0,
+                 max_batch=0):
+        self.dims = (npx, npy, npz, norb)
+        self.n = npy * npz * norb
+        cfg = NegfConfig(npx, npy, npz, norb, eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)),
+                         precision, device, hbm_budget_gb, max_batch)
+        h = ctypes.c_void_p()
+        _check(lib().negf_init(ctypes.byref(h), ctypes.byref(cfg)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().negf_finalize(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_hamiltonian(self, wire):
+        self._wire_keep = (_cf(wire.htotd), _cf(wire.hopx), wire.gm_fortran())
+        H, X, g = self._wire_keep
+        _check(lib().negf_set_hamiltonian(self._h, _p(H), _p(X), _p(g)))
+
+    def energy_sweep(self, energy, eCtr, deltaE, mu, muT, acc: Accumulators):
+        m = np.asarray(mu, float)
+        s = acc.struct()
+        return _check(lib().negf_energy_sweep(self._h, energy, eCtr, deltaE, _p(m), muT,
+                                              ctypes.byref(s)))
+
+    def submit(self, energies, eCtr, deltaE):
+        e = np.ascontiguousarray(energies, float)
+        i = np.ascontiguousarray(eCtr, np.int32)
+        d = np.ascontiguousarray(deltaE, float)
+        _check(lib().negf_submit(self._h, len(e), _p(e), i.ctypes.data_as(_ip), _p(d)))
+        self._queued = getattr(self, "_queued", 0) + len(e)
+
+    def flush(self, mu, muT, acc: Accumulators | None):
+        m = np.asarray(mu, float)
+        nq = getattr(self, "_queued", 0)
+        st = np.zeros(max(nq, 1), np.int32)
+        it = np.zeros(2 * max(nq, 1), np.int32)
+        s = acc.struct() if acc is not None else None
+        rc = lib().negf_flush(self._h, _p(m), muT, ctypes.byref(s) if s is not None else None,
+                              st.ctypes.data_as(_ip), it.ctypes.data_as(_ip))
+        self._queued = 0
+        _check(rc)
+        return st[:nq], it[:2 * nq].reshape(-1, 2)
+
+    def contact_self_energy(self, energy):
+        n = self.n
+        out = [np.zeros((n, n), complex, order="F") for _ in range(4)]
+        dos = np.zeros(2)
+        it = np.zeros(2, np.int32)
+        st = _check(lib().negf_contact_self_energy(self._h, energy, *[_p(o) for o in out], _p(dos),
+                                                   it.ctypes.data_as(_ip)))
+        return dict(SigmaL=out[0], SigmaR=out[1], GammaL=out[2], GammaR=out[3], DOSc1=dos[0],
+                    DOSc2=dos[1], iters=tuple(it), status=st)
+
+    def sancho_rubio(self, alpha, beta):
+        a, b = _cf(alpha), _cf(beta)
+        sig = np.zeros_like(a)
+        it = ctypes.c_int(0)
+        st = _check(lib().negf_sancho_rubio(self._h, _p(a), _p(b), _p(sig), ctypes.byref(it)))
+        return sig, st, it.value

@@ -1,0 +1,253 @@
+// Batched dense complex inverse: blocked Gauss-Jordan elimination with partial
+// (row) pivoting, for gfx950.
+//
+// Replaces Invert_Matrix (MKL_Routines_Cluster.F90:2-33, ZGETRF + ZGETRI).
+// Gauss-Jordan forms the inverse in n^3 complex multiply-adds, the same count
+// as getrf + getri, but every step is a rank-k update of the whole matrix,
+// which maps onto the MFMA GEMM kernel.
+//
+// Pivoting is implicit: rows are never swapped.  Column k is eliminated with
+// pivot row p_k (largest |Re|+|Im| among rows not yet used, as izamax), and
+// pivstep[p_k] = k, pivrow[k] = p_k are recorded.  In place, column k then holds
+// column p_k of the accumulated row transform S, and after all n steps
+//     inv(A)[a][b] = S[pivrow[a]][pivstep[b]],
+// which the final unpermute kernel applies while copying to the output.
+//
+// Blocking.  Columns are processed in outer blocks of kNBO and inner panels of
+// kNBI.  A panel kernel (one 768- or 1024-thread workgroup per matrix, panel rows
+// held in registers) factors kNBI columns and gathers the pivot rows' values in all
+// other columns into Prow.  A rank-kNBI update brings the rest of the outer
+// block up to date; after the whole outer block a rank-kNBO update (the GEMM
+// kernel's Gauss-Jordan epilogue) is applied to every other column:
+//     A[i][j] <- [p(i) not in block] A[i][j] + sum_m S[i][m] Prow[m][j].
+#include "negf_device.h"
+
+namespace negf {
+
+template <typename T>
+hipError_t gemm_gj_update(hipStream_t s, const GemmBatch& g, int batch);
+
+constexpr int kNBI = 8;    // panel width
+constexpr int kNBO = kGjNBO;  // outer block width (= GEMM tile width)
+
+__global__ void k_gj_init(int* pivstep, int* pivrow, int* info, long long sInt, int n,
+                          const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  int* ps = pivstep + (long long)b * sInt;
+  int* pr = pivrow + (long long)b * sInt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    ps[i] = n;
+    pr[i] = 0;
+  }
+}
+
+template <typename T, int R, int NT>
+__global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, int ld,
+                                                             int c0, int nb, int k0, GjWork w,
+                                                             const int* active) {
+  typedef cx<T> V;
+  const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
+  V* A = S + (long long)b * sS;
+  int* pivstep = w.pivstep + (long long)b * w.sInt;
+  int* pivrow = w.pivrow + (long long)b * w.sInt;
+  int* info = w.info + (long long)b * w.sInt;
+  V* prow = (V*)w.prow + (long long)b * w.sProw;
+
+  __shared__ T red_v[NT / 64];
+  __shared__ int red_i[NT / 64];
+  __shared__ V pr[kNBI];
+  __shared__ V psc[kNBI];
+  __shared__ int sp_row;
+  __shared__ int sp_rows[kNBI];
+
+  const int tid = threadIdx.x;
+  V v[R][kNBI];
+  bool used[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * NT;
+    used[r] = true;
+#pragma unroll
+    for (int j = 0; j < kNBI; ++j) v[r][j] = mk<T>(0, 0);
+    if (i < n) {
+      used[r] = pivstep[i] < n;
+#pragma unroll
+      for (int j = 0; j < kNBI; ++j)
+        if (j < nb) v[r][j] = A[(long long)(c0 + j) * ld + i];
+    }
+  }
+
+#pragma unroll
+  for (int jj = 0; jj < kNBI; ++jj) {
+    if (jj >= nb) break;
+    // pivot search: largest |re|+|im| over unused rows, ties to the lower row
+    T best = (T)-1;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * NT;
+      if (i < n && !used[r]) {
+        T m = fabs(v[r][jj].x) + fabs(v[r][jj].y);
+        if (m != m) m = (T)INFINITY;
+        if (m > best || (m == best && i < bi)) { best = m; bi = i; }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      T ob = __shfl_xor(best, off);
+      int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if ((tid & 63) == 0) { red_v[tid >> 6] = best; red_i[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid < 64) {
+      const int nw = NT / 64;
+      best = tid < nw ? red_v[tid] : (T)-1;
+      bi = tid < nw ? red_i[tid] : 0x7fffffff;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        T ob = __shfl_xor(best, off);
+        int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (tid == 0) {
+        sp_row = bi;
+        sp_rows[jj] = bi;
+        if (best == (T)0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
+      }
+    }
+    __syncthreads();
+    const int p = sp_row;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (tid + r * NT == p) {
+#pragma unroll
+        for (int j = 0; j < kNBI; ++j) pr[j] = v[r][j];
+      }
+    __syncthreads();
+    const V inv = cinv(pr[jj]);
+    if (tid < kNBI) psc[tid] = cmul(pr[tid], inv);  // pivot row scaled by 1/pivot
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * NT;
+      if (i >= n) continue;
+      if (i == p) {
+#pragma unroll
+        for (int j = 0; j < kNBI; ++j) v[r][j] = psc[j];
+        v[r][jj] = inv;
+        used[r] = true;
+      } else {
+        const V f = v[r][jj];
+#pragma unroll
+        for (int j = 0; j < kNBI; ++j) v[r][j] = csub(v[r][j], cmul(f, psc[j]));
+        v[r][jj] = cneg(cmul(f, inv));
+      }
+    }
+    if (tid == 0 && p < n) {
+      pivstep[p] = c0 + jj;
+      pivrow[c0 + jj] = p;
+    }
+    __syncthreads();
+  }
+  // write the factored panel back
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * NT;
+    if (i < n) {
+#pragma unroll
+      for (int j = 0; j < kNBI; ++j)
+        if (j < nb) A[(long long)(c0 + j) * ld + i] = v[r][j];
+    }
+  }
+  // gather the new pivot rows in every column outside this panel (untouched here)
+  for (int m = 0; m < nb; ++m) {
+    const int p = sp_rows[m];
+    if (p >= n) continue;  // cannot happen: some unused row always exists
+    V* dst = prow + (long long)(c0 - k0 + m) * n;
+    for (int j = tid; j < n; j += NT) {
+      if (j >= c0 && j < c0 + nb) continue;
+      dst[j] = A[(long long)j * ld + p];
+    }
+  }
+}
+
+// out[a][b] = S[pivrow[a]][pivstep[b]]
+template <typename T>
+__global__ void k_gj_unpermute(const cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
+                               int ldo, GjWork w, int n, const int* active) {
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const cx<T>* A = S + (long long)b * sS;
+  cx<T>* O = out + (long long)b * sOut;
+  const int* pivstep = w.pivstep + (long long)b * w.sInt;
+  const int* pivrow = w.pivrow + (long long)b * w.sInt;
+  const int col = blockIdx.y;
+  const int src = pivstep[col];
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
+    const int pr = pivrow[a];
+    O[(long long)col * ldo + a] = (src < n && pr < n) ? A[(long long)src * ld + pr]
+                                                      : mk<T>((T)NAN, (T)NAN);
+  }
+}
+
+// In-place batched inverse of the n x n matrices S_b (ld, stride sS); the result
+// goes to out_b (ldo, stride sOut), which must not alias S_b.  A zero pivot sets
+// info_b to its step (1-based) if info_b is still 0: callers clear info before a
+// sequence of inversions and read the first failure after it.
+template <typename T>
+hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
+                     int ldo, int n, int batch, const GjWork& w, const int* active) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (n > 3072) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gj_init, dim3((n + 255) / 256, batch), dim3(256), 0, s, w.pivstep, w.pivrow,
+                     w.info, w.sInt, n, active);
+  // panel rows live in registers: R rows per thread, 1024 threads up to n = 2048,
+  // then 768 threads (larger register budget per thread) up to n = 3072.
+  const int R = (n <= 2048) ? (n + 1023) / 1024 : (n + 767) / 768;
+  for (int k0 = 0; k0 < n; k0 += kNBO) {
+    const int kb = (n - k0 < kNBO) ? n - k0 : kNBO;
+    for (int c0 = k0; c0 < k0 + kb; c0 += kNBI) {
+      const int nb = (k0 + kb - c0 < kNBI) ? k0 + kb - c0 : kNBI;
+      switch (R) {
+        case 1: hipLaunchKernelGGL((k_gj_panel<T, 1, 1024>), dim3(batch), dim3(1024), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
+        case 2: hipLaunchKernelGGL((k_gj_panel<T, 2, 1024>), dim3(batch), dim3(1024), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
+        case 3: hipLaunchKernelGGL((k_gj_panel<T, 3, 768>), dim3(batch), dim3(768), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
+        default: hipLaunchKernelGGL((k_gj_panel<T, 4, 768>), dim3(batch), dim3(768), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
+      }
+      if (kb > nb) {  // bring the rest of the outer block up to date (rank nb)
+        GemmBatch g = {};
+        g.A = S + (long long)c0 * ld; g.sA = sS; g.lda = ld;
+        g.B = (const cx<T>*)w.prow + (long long)(c0 - k0) * n + k0; g.sB = w.sProw; g.ldb = n;
+        g.C = S + (long long)k0 * ld; g.sC = sS; g.ldc = ld;
+        g.m = n; g.n = kb; g.k = nb;
+        g.active = active; g.pivstep = w.pivstep; g.sPiv = w.sInt;
+        g.pv_lo = c0; g.pv_hi = c0 + nb;
+        g.skip_lo = c0 - k0; g.skip_hi = c0 - k0 + nb;
+        hipError_t e = gemm_gj_update<T>(s, g, batch);
+        if (e != hipSuccess) return e;
+      }
+    }
+    if (n > kb) {  // rank-kb update of every column outside the outer block
+      GemmBatch g = {};
+      g.A = S + (long long)k0 * ld; g.sA = sS; g.lda = ld;
+      g.B = (const cx<T>*)w.prow; g.sB = w.sProw; g.ldb = n;
+      g.C = S; g.sC = sS; g.ldc = ld;
+      g.m = n; g.n = n; g.k = kb;
+      g.active = active; g.pivstep = w.pivstep; g.sPiv = w.sInt;
+      g.pv_lo = k0; g.pv_hi = k0 + kb;
+      g.skip_lo = k0; g.skip_hi = k0 + kb;
+      hipError_t e = gemm_gj_update<T>(s, g, batch);
+      if (e != hipSuccess) return e;
+    }
+  }
+  hipLaunchKernelGGL((k_gj_unpermute<T>), dim3((n + 255) / 256, n, batch), dim3(256), 0, s, S, sS,
+                     ld, out, sOut, ldo, w, n, active);
+  return hipGetLastError();
+}
+
+template hipError_t gj_invert<double>(hipStream_t, cx<double>*, long long, int, cx<double>*,
+                                      long long, int, int, int, const GjWork&, const int*);
+template hipError_t gj_invert<float>(hipStream_t, cx<float>*, long long, int, cx<float>*,
+                                     long long, int, int, int, const GjWork&, const int*);
+
+}  // namespace negf

@@ -1,0 +1,91 @@
+// Shared device-side definitions for the gfx950 NEGF kernels.
+//
+// Complex numbers are interleaved (re, im) pairs -- the COMPLEX(8) / COMPLEX(4)
+// memory layout of the Fortran host -- and every matrix is column-major.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace negf {
+
+// vector types: double2 / float2 hold one complex number.
+template <typename T> struct cplx;
+template <> struct cplx<double> { typedef double2 type; };
+template <> struct cplx<float> { typedef float2 type; };
+template <typename T> using cx = typename cplx<T>::type;
+
+template <typename T> __host__ __device__ inline cx<T> mk(T r, T i) { cx<T> z; z.x = r; z.y = i; return z; }
+template <typename V> __host__ __device__ inline V cadd(V a, V b) { V z; z.x = a.x + b.x; z.y = a.y + b.y; return z; }
+template <typename V> __host__ __device__ inline V csub(V a, V b) { V z; z.x = a.x - b.x; z.y = a.y - b.y; return z; }
+template <typename V> __host__ __device__ inline V cmul(V a, V b) {
+  V z; z.x = a.x * b.x - a.y * b.y; z.y = a.x * b.y + a.y * b.x; return z;
+}
+// a * conj(b)
+template <typename V> __host__ __device__ inline V cmulc(V a, V b) {
+  V z; z.x = a.x * b.x + a.y * b.y; z.y = a.y * b.x - a.x * b.y; return z;
+}
+template <typename V> __host__ __device__ inline V cconj(V a) { V z; z.x = a.x; z.y = -a.y; return z; }
+template <typename V> __host__ __device__ inline V cneg(V a) { V z; z.x = -a.x; z.y = -a.y; return z; }
+// 1/a = conj(a)/|a|^2, scaled by max(|re|,|im|) so |a|^2 cannot overflow.
+template <typename V> __device__ inline V cinv(V a) {
+  typedef decltype(a.x) R;
+  R s = fmax(fabs(a.x), fabs(a.y));
+  V z;
+  if (s == (R)0) { z.x = (R)1 / s; z.y = (R)0; return z; }
+  R ar = a.x / s, ai = a.y / s;
+  R d = (ar * ar + ai * ai) * s;
+  z.x = ar / d; z.y = -ai / d;
+  return z;
+}
+
+// Matrix operation selectors (BLAS op flags).
+enum Op { OP_N = 0, OP_T = 1, OP_C = 2 };
+// GEMM epilogues: standard alpha*AB + beta*C, or the Gauss-Jordan update.
+enum Epi { EPI_STD = 0, EPI_GJ = 1 };
+
+// A strided batch of complex GEMMs, C_b = alpha*op(A_b)*op(B_b) + beta*C_b, with
+// X_b = X + b*sX (elements).  blockIdx.z selects b, through `active` if given.
+// In the Gauss-Jordan form (EPI_GJ) the product is C_b = keep(C_b) + op(A_b)op(B_b)
+// where keep() zeroes the rows r with pv_lo <= pivstep_b[r] < pv_hi and the
+// columns in [skip_lo, skip_hi) are left untouched.
+struct GemmBatch {
+  const void* A;
+  const void* B;
+  void* C;
+  long long sA, sB, sC;
+  int m, n, k;
+  int lda, ldb, ldc;
+  const int* active;
+  const int* pivstep;
+  long long sPiv;
+  int pv_lo, pv_hi;
+  int skip_lo, skip_hi;
+};
+
+// Spin matrices gMx, gMy, gMz (norb <= 4) as kernel arguments.
+struct SpinMats {
+  double re[3][4][4];  // [matrix][row][col]
+  double im[3][4][4];
+};
+
+// Per-matrix workspace of the batched Gauss-Jordan inverse (gj_inverse.hip).
+struct GjWork {
+  void* prow;       // kNBO x n complex, row-major (row m at prow + m*n)
+  int* pivstep;     // n: step at which each row became a pivot (n = not yet)
+  int* pivrow;      // n: pivot row of each step
+  int* info;        // 1: first zero pivot (1-based) or 0
+  long long sProw;  // batch stride of prow (complex elements)
+  long long sInt;   // batch stride of pivstep, pivrow and info (ints)
+};
+
+template <typename T>
+hipError_t gemm_batched(hipStream_t s, int opa, int opb, const GemmBatch& g, int batch,
+                        cx<T> alpha, cx<T> beta);
+template <typename T>
+hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
+                     int ldo, int n, int batch, const GjWork& w, const int* active);
+constexpr int kGjNBO = 64;
+
+}  // namespace negf
+
+extern "C" void negf_set_error(const char* fmt, ...);

@@ -1,0 +1,885 @@
+// Host orchestration of the per-energy NEGF solve on one gfx950 GPU, behind the
+// C ABI of include/negf_capi.h.
+//
+// Pipeline for a batch of B energies of one wire (all kernels batched over B):
+//  1. Sancho-Rubio for both leads of every energy, 2B independent chains
+//     (Contact_Self_Energy.F90:68-101, :110-223); converged chains drop out of
+//     the active list, so the loop costs what the slowest chain needs.
+//  2. RGF pass 1 (forward): gL_x = inv(A_x - X' gL_{x-1} X) into slot x
+//     (Recursive_Greens.F90:16-34).
+//  3. RGF pass 2 (backward): the last block column G_xL = gL_x X G_{x+1,L}
+//     (:53-63) is formed from slot x, which is then overwritten with
+//     gR_x = inv(A_x - X gR_{x+1} X') (:67-86).  Each G_xL feeds the right-lead
+//     Gn/Gp site blocks of slice x and, at x = 1, T12.
+//  4. RGF pass 3 (forward): the first block column G_x1 = gR_x X' G_{x-1,1}
+//     (:88-96) and the left-lead Gn/Gp site blocks (Calculate_Gnp.F90).
+// So only L + 10 n x n blocks per energy are resident (the reference keeps ~15 L),
+// block columns are streamed, and Gn/Gp are reduced to the site-diagonal traces
+// the outputs need (Calculate_Gnp.F90:118-124, Calculate_Observables.F90:10-53)
+// without forming the n x n x L Gnd/Gpd arrays.
+#include "negf_device.h"
+#include "negf_capi.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+namespace negf {
+template <typename T>
+hipError_t launch_assemble(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, int, double,
+                           const double*, double, const cx<T>*, long long, const cx<T>*, long long,
+                           const cx<T>*, long long, const cx<T>*, int, int, const int*);
+template <typename T>
+hipError_t launch_bd_mul(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, long long,
+                         const cx<T>*, int, int, const int*);
+template <typename T>
+hipError_t launch_siteblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
+                             int, double*, long long, long long, const SpinMats&, int, const int*);
+template <typename T>
+hipError_t launch_inner(hipStream_t, const cx<T>*, long long, const cx<T>*, long long, int,
+                        double*, long long, int, const int*);
+template <typename T>
+hipError_t launch_eye(hipStream_t, cx<T>*, long long, int, int, const int*);
+template <typename T>
+hipError_t launch_add(hipStream_t, cx<T>*, long long, const cx<T>*, long long, const cx<T>*,
+                      long long, long long, int, const int*);
+template <typename T>
+hipError_t launch_copy3(hipStream_t, cx<T>*, const cx<T>*, cx<T>*, const cx<T>*, cx<T>*,
+                        const cx<T>*, long long, long long, int, const int*);
+template <typename T>
+hipError_t launch_sr_alpha(hipStream_t, cx<T>*, cx<T>*, long long, const cx<T>*, const cx<T>*,
+                           const double*, int, double, int, int);
+template <typename T>
+hipError_t launch_maxabs(hipStream_t, const cx<T>*, long long, long long, double*, int,
+                         const int*);
+template <typename T>
+hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long long, int, int,
+                            double*, long long);
+}  // namespace negf
+
+using namespace negf;
+typedef cx<double> zd;
+
+// ---------------------------------------------------------------------------
+static thread_local char g_err[2048];
+extern "C" void negf_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+extern "C" const char* negf_last_error(void) { return g_err; }
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      negf_set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(_e));    \
+      return NEGF_ERR_HIP;                                                                   \
+    }                                                                                        \
+  } while (0)
+
+static const double kPi = 3.14159265358979323846;
+static const double kQ = 1.602e-19;   // Constants.F90:4
+static const double kH = 6.626e-34;   // Constants.F90:6
+
+// ---------------------------------------------------------------------------
+struct Layout {
+  long long nn;
+  // per-energy region (complex elements)
+  long long Es, o_slots, o_M, o_colA, o_colB, o_Y, o_Bm, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
+      o_prow;
+  // per-chain Sancho-Rubio region (complex elements)
+  long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
+  long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
+  long long site_per_e;  // doubles of site output per energy
+  void init(int n, int L, int nsites) {
+    nn = (long long)n * n;
+    long long o = 0;
+    o_slots = o; o += (long long)L * nn;
+    o_M = o; o += nn;
+    o_colA = o; o += nn;
+    o_colB = o; o += nn;
+    o_Y = o; o += nn;
+    o_Bm = o; o += nn;
+    o_P = o; o += nn;
+    o_SigL = o; o += nn;
+    o_SigR = o; o += nn;
+    o_GamL = o; o += nn;
+    o_GamR = o; o += nn;
+    o_prow = o; o += (long long)kGjNBO * n;
+    Es = (o + 31) & ~31LL;
+    o = 0;
+    c_alpha = o; o += nn;
+    c_work = o; o += nn;
+    c_tmp = o; o += nn;
+    c_t = o; o += nn;
+    c_tt = o; o += nn;
+    c_Toldt = o; o += nn;
+    c_CapT = o; o += nn;
+    c_W1 = o; o += nn;
+    c_W2 = o; o += nn;
+    c_W3 = o; o += nn;
+    c_W4 = o; o += nn;
+    c_prow = o; o += (long long)kGjNBO * n;
+    Cs = (o + 31) & ~31LL;
+    Is = ((2LL * n + 1) + 63) & ~63LL;
+    site_per_e = 2LL * 4 * L * nsites;
+  }
+};
+
+struct negf_ctx {
+  negf_config cfg;
+  int n = 0, L = 0, nsites = 0;
+  hipStream_t st = nullptr;
+  bool have_h = false;
+  bool blockdiag = true;
+  zd* H = nullptr;       // Htotd (n,n,L)
+  zd* X = nullptr;       // HopX
+  zd* betaL = nullptr;   // -HopX'
+  zd* betaR = nullptr;   // -HopX
+  zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
+  SpinMats sp;
+  Layout lay;
+  int cap = 0;           // energies per batch
+  zd* E = nullptr;       // energy arena (cap * Es)
+  zd* C = nullptr;       // chain arena (2cap * Cs)
+  int* Ie = nullptr;     // energy ints (cap * Is)
+  int* Ic = nullptr;     // chain ints (2cap * Is)
+  double* d_energy = nullptr;  // cap
+  double* d_scal = nullptr;    // cap * 8
+  double* d_site = nullptr;    // cap * site_per_e
+  double* d_mc = nullptr;      // 2cap
+  int* d_active = nullptr;     // 2cap
+  // queue
+  std::vector<double> qE, qdE;
+  std::vector<int> qIdx;
+};
+
+static int check_ctx(negf_ctx* c) {
+  if (!c) {
+    negf_set_error("null context");
+    return NEGF_ERR_ARG;
+  }
+  if (!c->have_h) {
+    negf_set_error("negf_set_hamiltonian has not been called");
+    return NEGF_ERR_STATE;
+  }
+  return 0;
+}
+
+extern "C" int negf_device_count(void) {
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess) return 0;
+  return nd;
+}
+
+static int require_device(int dev) {
+  int nd = negf_device_count();
+  if (nd <= 0) {
+    negf_set_error("no HIP device available: the NEGF library has no CPU fallback");
+    return NEGF_ERR_NO_DEVICE;
+  }
+  if (dev < 0 || dev >= nd) {
+    negf_set_error("device %d out of range (%d devices)", dev, nd);
+    return NEGF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(dev));
+  return 0;
+}
+
+static void free_arena(negf_ctx* c) {
+  hipFree(c->E); hipFree(c->C); hipFree(c->Ie); hipFree(c->Ic);
+  hipFree(c->d_energy); hipFree(c->d_scal); hipFree(c->d_site); hipFree(c->d_mc);
+  hipFree(c->d_active);
+  c->E = c->C = nullptr;
+  c->Ie = c->Ic = nullptr;
+  c->d_energy = c->d_scal = c->d_site = c->d_mc = nullptr;
+  c->d_active = nullptr;
+  c->cap = 0;
+}
+
+extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
+  if (!out || !cfg) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  *out = nullptr;
+  if (cfg->npx < 1 || cfg->npy < 1 || cfg->npz < 1 ||
+      !(cfg->norb == 1 || cfg->norb == 2 || cfg->norb == 4)) {
+    negf_set_error("bad dimensions npx=%d npy=%d npz=%d norb=%d (norb must be 1, 2 or 4)",
+                   cfg->npx, cfg->npy, cfg->npz, cfg->norb);
+    return NEGF_ERR_ARG;
+  }
+  const int n = cfg->npy * cfg->npz * cfg->norb;
+  if (n > 3072) {
+    negf_set_error("block size n=%d exceeds 3072", n);
+    return NEGF_ERR_ARG;
+  }
+  if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED) {
+    negf_set_error("unknown precision form %d", cfg->precision);
+    return NEGF_ERR_ARG;
+  }
+  int rc = require_device(cfg->device);
+  if (rc) return rc;
+  negf_ctx* c = new negf_ctx();
+  c->cfg = *cfg;
+  if (c->cfg.sr_max_iter < 0) c->cfg.sr_max_iter = 0;
+  c->n = n;
+  c->L = cfg->npx;
+  c->nsites = cfg->npy * cfg->npz;
+  c->lay.init(n, c->L, c->nsites);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    negf_set_error("hipStreamCreate failed");
+    delete c;
+    return NEGF_ERR_HIP;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int negf_finalize(negf_ctx* c) {
+  if (!c) return 0;
+  hipSetDevice(c->cfg.device);
+  hipStreamSynchronize(c->st);
+  free_arena(c);
+  hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
+  hipStreamDestroy(c->st);
+  delete c;
+  return 0;
+}
+
+extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const double* HopX,
+                                    const double* gM) {
+  if (!c || !Htotd || !HopX) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const int n = c->n, L = c->L, no = c->cfg.norb;
+  const long long nn = (long long)n * n;
+  const zd* hx = (const zd*)HopX;
+  // HopX structure: site-block-diagonal (Build_Hamiltonian.F90:62-64) or dense
+  bool bd = true;
+  for (int j = 0; j < n && bd; ++j)
+    for (int i = 0; i < n; ++i)
+      if (i / no != j / no && (hx[(long long)j * n + i].x != 0.0 || hx[(long long)j * n + i].y != 0.0)) {
+        bd = false;
+        break;
+      }
+  c->blockdiag = bd;
+  if (!c->H) {
+    HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
+    HIPCHK(hipMalloc(&c->X, sizeof(zd) * nn));
+    HIPCHK(hipMalloc(&c->betaL, sizeof(zd) * nn));
+    HIPCHK(hipMalloc(&c->betaR, sizeof(zd) * nn));
+  }
+  HIPCHK(hipMemcpy(c->H, Htotd, sizeof(zd) * nn * L, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->X, HopX, sizeof(zd) * nn, hipMemcpyHostToDevice));
+  std::vector<zd> b(nn);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      zd v = hx[(long long)i * n + j];  // HopX(j,i)
+      b[(long long)j * n + i] = mk<double>(-v.x, v.y);  // -conj(HopX(j,i)) = (-HopX')(i,j)
+    }
+  HIPCHK(hipMemcpy(c->betaL, b.data(), sizeof(zd) * nn, hipMemcpyHostToDevice));
+  for (long long k = 0; k < nn; ++k) b[k] = mk<double>(-hx[k].x, -hx[k].y);
+  HIPCHK(hipMemcpy(c->betaR, b.data(), sizeof(zd) * nn, hipMemcpyHostToDevice));
+  std::memset(&c->sp, 0, sizeof(c->sp));
+  if (gM) {
+    const zd* g = (const zd*)gM;
+    for (int m = 0; m < 3; ++m)
+      for (int col = 0; col < no; ++col)
+        for (int row = 0; row < no; ++row) {
+          zd v = g[m * no * no + col * no + row];
+          c->sp.re[m][row][col] = v.x;
+          c->sp.im[m][row][col] = v.y;
+        }
+  }
+  if (!bd && !c->tmpP) HIPCHK(hipMalloc(&c->tmpP, sizeof(zd) * nn * 2));
+  c->have_h = true;
+  return 0;
+}
+
+// Allocate the batch arena for `want` energies (capped by the HBM budget).
+static int ensure_arena(negf_ctx* c, int want) {
+  if (c->cap >= want && c->E) return 0;
+  const Layout& y = c->lay;
+  size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(3 * y.Is) +
+                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2) + 2 * sizeof(int);
+  size_t freeb = 0, totalb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  double budget = c->cfg.hbm_budget_gb > 0 ? c->cfg.hbm_budget_gb * 1e9 : 0.6 * (double)freeb;
+  if (budget > 0.9 * (double)freeb + (c->E ? 0.0 : 0.0)) budget = 0.9 * (double)freeb;
+  int cap = (int)std::min<double>(want, budget / (double)per_e);
+  if (c->cfg.max_batch > 0) cap = std::min(cap, c->cfg.max_batch);
+  if (cap < 1) {
+    negf_set_error("one energy needs %.2f GB of device memory; budget %.2f GB", per_e / 1e9,
+                   budget / 1e9);
+    return NEGF_ERR_MEMORY;
+  }
+  if (c->cap >= cap && c->E) return 0;
+  free_arena(c);
+  HIPCHK(hipMalloc(&c->E, sizeof(zd) * (size_t)y.Es * cap));
+  HIPCHK(hipMalloc(&c->C, sizeof(zd) * (size_t)y.Cs * 2 * cap));
+  HIPCHK(hipMalloc(&c->Ie, sizeof(int) * (size_t)y.Is * cap));
+  HIPCHK(hipMalloc(&c->Ic, sizeof(int) * (size_t)y.Is * 2 * cap));
+  HIPCHK(hipMalloc(&c->d_energy, sizeof(double) * cap));
+  HIPCHK(hipMalloc(&c->d_scal, sizeof(double) * 8 * cap));
+  HIPCHK(hipMalloc(&c->d_site, sizeof(double) * (size_t)y.site_per_e * cap));
+  HIPCHK(hipMalloc(&c->d_mc, sizeof(double) * 2 * cap));
+  HIPCHK(hipMalloc(&c->d_active, sizeof(int) * 2 * cap));
+  c->cap = cap;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// small wrappers
+static GemmBatch gb(const zd* A, long long sA, int lda, const zd* B, long long sB, int ldb, zd* C,
+                    long long sC, int ldc, int n, const int* active = nullptr) {
+  GemmBatch g = {};
+  g.A = A; g.sA = sA; g.lda = lda;
+  g.B = B; g.sB = sB; g.ldb = ldb;
+  g.C = C; g.sC = sC; g.ldc = ldc;
+  g.m = n; g.n = n; g.k = n;
+  g.active = active;
+  return g;
+}
+static const zd kOne = {1.0, 0.0}, kZero = {0.0, 0.0}, kMone = {-1.0, 0.0};
+
+#define TRY(expr)                                                                     \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      negf_set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(_e)); \
+      return NEGF_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+static GjWork gjw(zd* prow_base, long long sProw, int* ints, long long sInt, int n) {
+  GjWork w;
+  w.prow = prow_base;
+  w.sProw = sProw;
+  w.pivstep = ints;
+  w.pivrow = ints + n;
+  w.info = ints + 2 * n;
+  w.sInt = sInt;
+  return w;
+}
+
+// Sancho-Rubio for nchains chains laid out in the chain arena (lead-major:
+// chains [0, nl) use betaA/betadA, [nl, nchains) use betaB/betadB).  Expects
+// chain alpha and work (= seed matrix) filled.  Writes Sigma for chain c to
+// sig0 + c*sSig (c < nl) or sig1 + (c-nl)*sSig.  iters/status per chain.
+static int run_sr(negf_ctx* c, int nl, int nr, const zd* betaA, const zd* betadA, const zd* betaB,
+                  const zd* betadB, zd* sig0, zd* sig1, long long sSig, std::vector<int>& iters,
+                  std::vector<int>& status) {
+  const int n = c->n;
+  const Layout& y = c->lay;
+  const int nc = nl + nr;
+  const long long Cs = y.Cs, nn = y.nn;
+  hipStream_t s = c->st;
+  zd* C0 = c->C;
+  auto P = [&](long long off) { return C0 + off; };
+  GjWork w = gjw(P(y.c_prow), Cs, c->Ic, y.Is, n);
+  iters.assign(nc, 0);
+  status.assign(nc, 0);
+  // clear INFO of every chain slot
+  TRY(hipMemsetAsync(c->Ic, 0, sizeof(int) * (size_t)y.Is * nc, s));
+  // tmp = inv(seed)   (Contact_Self_Energy.F90:141)
+  TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nc, w, nullptr));
+  // t = -tmp*betad, tt = -tmp*beta   (:147, :149)
+  for (int grp = 0; grp < 2; ++grp) {
+    const int cnt = grp ? nr : nl;
+    if (!cnt) continue;
+    const long long off = grp ? (long long)nl * Cs : 0;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(off + y.c_tmp), Cs, n, grp ? betadB : betadA, 0, n,
+                                               P(off + y.c_t), Cs, n, n), cnt, kMone, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(off + y.c_tmp), Cs, n, grp ? betaB : betaA, 0, n,
+                                               P(off + y.c_tt), Cs, n, n), cnt, kMone, kZero));
+  }
+  TRY(launch_copy3<double>(s, P(y.c_CapT), P(y.c_t), nullptr, nullptr, nullptr, nullptr, Cs, nn, nc,
+                           nullptr));
+  TRY(launch_eye<double>(s, P(y.c_Toldt), Cs, n, nc, nullptr));
+  std::vector<int> act(nc);
+  for (int i = 0; i < nc; ++i) act[i] = i;
+  std::vector<double> mc(nc);
+  int it = 0;
+  const double tol = c->cfg.sr_tol;
+  while (!act.empty() && it < c->cfg.sr_max_iter) {  // (:154)
+    ++it;
+    const int na = (int)act.size();
+    int* A = c->d_active;
+    TRY(hipMemcpyAsync(A, act.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+    // W1 = Toldt*tt   (:157)
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs,
+                                               n, n, A), na, kOne, kZero));
+    // work = I - t*tt - tt*t   (:160-162)
+    TRY(launch_eye<double>(s, P(y.c_work), Cs, n, na, A));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n,
+                                               n, A), na, kMone, kOne));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n,
+                                               n, A), na, kMone, kOne));
+    // tmp = inv(work)   (:163)
+    TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, na, w, A));
+    // t' = tmp*t*t, tt' = tmp*tt*tt   (:170-176)
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n,
+                                               n, A), na, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n,
+                                               n, A), na, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n,
+                                               n, A), na, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n,
+                                               n, A), na, kOne, kZero));
+    // change = Toldt*t'   (:178);  CapT += change   (:180)
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
+                                               n, A), na, kOne, kZero));
+    TRY(launch_add<double>(s, P(y.c_CapT), Cs, P(y.c_CapT), Cs, P(y.c_W2), Cs, nn, na, A));
+    TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, c->d_mc, na, A));
+    TRY(launch_copy3<double>(s, P(y.c_Toldt), P(y.c_W1), P(y.c_t), P(y.c_W3), P(y.c_tt), P(y.c_W4),
+                             Cs, nn, na, A));
+    TRY(hipMemcpyAsync(mc.data(), c->d_mc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    std::vector<int> keep;
+    for (int ch : act) {
+      iters[ch] = it;
+      const double m = mc[ch];
+      if (m != m) {
+        status[ch] |= NEGF_STATUS_SR_NAN;  // (:182-191): stop this chain
+      } else if (m > tol) {
+        keep.push_back(ch);
+      }
+    }
+    act.swap(keep);
+  }
+  for (int ch : act) status[ch] |= NEGF_STATUS_SR_NOT_CONVERGED;
+  // g = inv(alpha + beta*CapT)  (:195-197);  Sigma = beta*g*beta'  (:208-209)
+  for (int grp = 0; grp < 2; ++grp) {
+    const int cnt = grp ? nr : nl;
+    if (!cnt) continue;
+    const long long off = grp ? (long long)nl * Cs : 0;
+    const zd* beta = grp ? betaB : betaA;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(off + y.c_CapT), Cs, n, P(off + y.c_W1),
+                                               Cs, n, n), cnt, kOne, kZero));
+  }
+  TRY(launch_add<double>(s, P(y.c_work), Cs, P(y.c_W1), Cs, P(y.c_alpha), Cs, nn, nc, nullptr));
+  TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_W2), Cs, n, n, nc, w, nullptr));
+  for (int grp = 0; grp < 2; ++grp) {
+    const int cnt = grp ? nr : nl;
+    if (!cnt) continue;
+    const long long off = grp ? (long long)nl * Cs : 0;
+    const zd* beta = grp ? betaB : betaA;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(off + y.c_W2), Cs, n, P(off + y.c_W3),
+                                               Cs, n, n), cnt, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_C, gb(P(off + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
+                                               sSig, n, n), cnt, kOne, kZero));
+  }
+  // LAPACK INFO of the chain inverses
+  std::vector<int> info(nc);
+  for (int ch = 0; ch < nc; ++ch)
+    TRY(hipMemcpyAsync(&info[ch], c->Ic + (size_t)ch * y.Is + 2 * n, sizeof(int),
+                       hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  for (int ch = 0; ch < nc; ++ch)
+    if (info[ch] && !(status[ch] & 0xFFFFF)) status[ch] |= (info[ch] & 0xFFFFF);
+  return 0;
+}
+
+// Result of one energy (host side, before accumulation).
+struct EnergyResult {
+  double scal[8];
+  std::vector<double> site;
+  int status = 0;
+  int iters[2] = {0, 0};
+};
+
+// Fused solve of nb energies (nb <= cap) resident in the arena.
+static int run_batch(negf_ctx* c, int nb, const double* energies, std::vector<EnergyResult>& res) {
+  const int n = c->n, L = c->L, no = c->cfg.norb, ns = c->nsites;
+  const Layout& y = c->lay;
+  const long long Es = y.Es, nn = y.nn;
+  hipStream_t s = c->st;
+  zd* E0 = c->E;
+  auto Q = [&](long long off) { return E0 + off; };
+  TRY(hipMemcpyAsync(c->d_energy, energies, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  TRY(hipMemsetAsync(c->d_scal, 0, sizeof(double) * 8 * nb, s));
+  TRY(hipMemsetAsync(c->Ie, 0, sizeof(int) * (size_t)y.Is * nb, s));
+  // ---- contacts: chains [0,nb) left lead, [nb,2nb) right lead
+  const zd* Hend0 = c->H;
+  const zd* Hend1 = c->H + (size_t)(L - 1) * nn;
+  TRY(launch_sr_alpha<double>(s, c->C + y.c_alpha, c->C + y.c_work, y.Cs, Hend0, Hend1,
+                              c->d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
+  std::vector<int> iters, sst;
+  // left: beta = -HopX', betad = -HopX;  right: beta = -HopX, betad = -HopX'
+  int rc = run_sr(c, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL), Q(y.o_SigR), Es,
+                  iters, sst);
+  if (rc) return rc;
+  TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, c->d_scal + 4, 8));
+  TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, c->d_scal + 5, 8));
+
+  GjWork w = gjw(Q(y.o_prow), Es, c->Ie, y.Is, n);
+  const bool bd = c->blockdiag;
+  auto slot = [&](int x) { return Q(y.o_slots + (long long)x * nn); };
+  auto hx = [&](int x) { return (const zd*)(c->H + (size_t)x * nn); };
+  // Assemble M = zI - H_x - [SigL if x==0] - [SigR if x==L-1] - sandwich(g) into o_M.
+  auto assemble = [&](int x, const zd* g, int mode) -> int {
+    const zd* S1 = (x == 0) ? Q(y.o_SigL) : nullptr;
+    const zd* S2 = (x == L - 1) ? Q(y.o_SigR) : nullptr;
+    if (g && !bd) {
+      // dense HopX: P = X' g X (mode 1) or X g X' (mode 2) with two GEMMs, then subtract
+      zd* T1 = Q(y.o_P);
+      zd* T2 = Q(y.o_Y);
+      if (mode == 1) {
+        TRY(gemm_batched<double>(s, OP_C, OP_N, gb(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+      } else {
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
+        TRY(gemm_batched<double>(s, OP_N, OP_C, gb(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+      }
+      TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, c->d_energy, c->cfg.eta, S1,
+                                  Es, S2, Es, T2, Es, c->X, n, nb, nullptr));
+      return 0;
+    }
+    TRY(launch_assemble<double>(s, no, g ? mode : 0, Q(y.o_M), Es, hx(x), 1, -1.0, c->d_energy,
+                                c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr));
+    return 0;
+  };
+  // Y = X G (conj=0) or X' G (conj=1)
+  auto xmul = [&](int conj, zd* Y, const zd* G) -> int {
+    if (bd) {
+      TRY(launch_bd_mul<double>(s, no, conj, Y, Es, G, Es, c->X, n, nb, nullptr));
+    } else {
+      TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gb(c->X, 0, n, G, Es, n, Y, Es, n, n), nb,
+                               kOne, kZero));
+    }
+    return 0;
+  };
+  // Gn/Gp site blocks of column block G (lead 0 = left/Gamma_L, 1 = right/Gamma_R) at slice x
+  auto siteblocks = [&](int lead, int x, const zd* G) -> int {
+    const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(G, Es, n, Gam, Es, n, Q(y.o_Bm), Es, n, n), nb, kOne,
+                             kZero));
+    double* out = c->d_site + ((long long)lead * 4 * L + x) * ns;
+    TRY(launch_siteblocks<double>(s, no, Q(y.o_Bm), Es, G, Es, n, out, y.site_per_e,
+                                  (long long)L * ns, c->sp, nb, nullptr));
+    return 0;
+  };
+  // scal[idx] = Re <Gam_lead * Bm, G>
+  auto trace_term = [&](int lead, const zd* G, int idx) -> int {
+    const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(Gam, Es, n, Q(y.o_Bm), Es, n, Q(y.o_P), Es, n, n), nb,
+                             kOne, kZero));
+    TRY(launch_inner<double>(s, Q(y.o_P), Es, G, Es, n, c->d_scal + idx, 8, nb, nullptr));
+    return 0;
+  };
+  int r;
+  // ---- pass 1: gL
+  for (int x = 0; x < L; ++x) {
+    if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
+    TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
+  }
+  // ---- pass 2: last column and gR
+  zd* colA = Q(y.o_colA);
+  zd* colB = Q(y.o_colB);
+  TRY(launch_copy3<double>(s, colA, slot(L - 1), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
+  if ((r = siteblocks(1, L - 1, colA))) return r;
+  if ((r = trace_term(1, colA, 3))) return r;          // Modec2, right-lead term
+  if (L == 1 && (r = trace_term(0, colA, 0))) return r; // T12
+  if ((r = assemble(L - 1, nullptr, 2))) return r;
+  TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(L - 1), Es, n, n, nb, w, nullptr));
+  for (int x = L - 2; x >= 0; --x) {
+    if ((r = xmul(0, Q(y.o_Y), colA))) return r;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                             kOne, kZero));
+    if ((r = siteblocks(1, x, colB))) return r;
+    if (x == 0 && (r = trace_term(0, colB, 0))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
+    if ((r = assemble(x, slot(x + 1), 2))) return r;
+    TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
+    std::swap(colA, colB);
+  }
+  // ---- pass 3: first column
+  colA = Q(y.o_colA);
+  colB = Q(y.o_colB);
+  TRY(launch_copy3<double>(s, colA, slot(0), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
+  if ((r = siteblocks(0, 0, colA))) return r;
+  if ((r = trace_term(0, colA, 1))) return r;            // Modec1, left-lead term
+  if (L == 1 && (r = trace_term(1, colA, 2))) return r;  // Modec2, left-lead term
+  for (int x = 1; x < L; ++x) {
+    if ((r = xmul(1, Q(y.o_Y), colA))) return r;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                             kOne, kZero));
+    if ((r = siteblocks(0, x, colB))) return r;
+    if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;
+    std::swap(colA, colB);
+  }
+  // ---- results to host
+  res.assign(nb, EnergyResult());
+  std::vector<double> scal(8 * nb);
+  std::vector<double> site((size_t)y.site_per_e * nb);
+  std::vector<int> ints((size_t)y.Is * nb);
+  TRY(hipMemcpyAsync(scal.data(), c->d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(site.data(), c->d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(ints.data(), c->Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  for (int b = 0; b < nb; ++b) {
+    EnergyResult& e = res[b];
+    std::memcpy(e.scal, &scal[8 * b], sizeof(e.scal));
+    e.site.assign(site.begin() + (size_t)b * y.site_per_e, site.begin() + (size_t)(b + 1) * y.site_per_e);
+    int st = sst[b] | sst[nb + b];
+    const int info = ints[(size_t)b * y.Is + 2 * n];
+    if (info && !(st & 0xFFFFF)) st |= (info & 0xFFFFF);
+    e.status = st;
+    e.iters[0] = iters[b];
+    e.iters[1] = iters[nb + b];
+  }
+  return 0;
+}
+
+// Accumulate one energy's results in the reference's output layout.
+static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eCtr, double deltaE,
+                       const double* mu, double muT, negf_accum* acc) {
+  if (!acc) return;
+  const int L = c->L, ns = c->nsites;
+  const double f1 = (energy <= muT - mu[0]) ? 1.0 : 0.0;  // Energy_Sweep.F90:11-23
+  const double f2 = (energy <= muT - mu[1]) ? 1.0 : 0.0;
+  const double w = deltaE / (2.0 * kPi);
+  const double T = e.scal[0];
+  const size_t q = (size_t)L * ns;
+  const double* sl = e.site.data();          // left lead: tr, spin x, y, z
+  const double* sr = e.site.data() + 4 * q;  // right lead
+  double ed = 0.0, hd = 0.0;
+  for (int x = 0; x < L; ++x)
+    for (int sidx = 0; sidx < ns; ++sidx) {
+      const size_t k = (size_t)x * ns + sidx;
+      const double trN = f1 * sl[k] + f2 * sr[k];
+      const double trP = (1.0 - f1) * sl[k] + (1.0 - f2) * sr[k];
+      ed += trN / (2.0 * kPi);
+      hd += trP / (2.0 * kPi);
+      const size_t o = (size_t)x + (size_t)L * sidx;  // (x, y, z) with sidx = y + Npy*z
+      if (acc->RhoN) acc->RhoN[o] += trN * w;
+      if (acc->RhoP) acc->RhoP[o] += trP * w;
+      double* sg[3] = {acc->SigXExp, acc->SigYExp, acc->SigZExp};
+      for (int a = 0; a < 3; ++a)
+        if (sg[a]) sg[a][o] += (f1 * sl[(1 + a) * q + k] + f2 * sr[(1 + a) * q + k]) * w;
+    }
+  if (eCtr >= 0) {
+    if (acc->T12) acc->T12[eCtr] += T;
+    if (acc->Modec1) acc->Modec1[eCtr] += e.scal[1] + T;
+    if (acc->Modec2) acc->Modec2[eCtr] += e.scal[2] + e.scal[3];
+    if (acc->DOSc1) acc->DOSc1[eCtr] += e.scal[4];
+    if (acc->DOSc2) acc->DOSc2[eCtr] += e.scal[5];
+    if (acc->EDensity) acc->EDensity[eCtr] += ed;
+    if (acc->HDensity) acc->HDensity[eCtr] += hd;
+  }
+  if (acc->ContactCurrent) {  // Calculate_Observables.F90:102-103
+    acc->ContactCurrent[0] += kQ * deltaE / kH * (T * (f2 - f1));
+    acc->ContactCurrent[1] += kQ * deltaE / kH * (T * (f1 - f2));
+  }
+}
+
+extern "C" int negf_submit(negf_ctx* c, int count, const double* energies, const int* eCtr,
+                           const double* deltaE) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count < 0 || (count > 0 && (!energies || !deltaE))) {
+    negf_set_error("bad submit arguments");
+    return NEGF_ERR_ARG;
+  }
+  for (int i = 0; i < count; ++i) {
+    c->qE.push_back(energies[i]);
+    c->qdE.push_back(deltaE[i]);
+    c->qIdx.push_back(eCtr ? eCtr[i] : -1);
+  }
+  return 0;
+}
+
+extern "C" int negf_flush(negf_ctx* c, const double* mu, double muT, negf_accum* acc, int* status,
+                          int* sr_iters) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!mu) {
+    negf_set_error("mu is null");
+    return NEGF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const int total = (int)c->qE.size();
+  if (total == 0) return 0;
+  if ((rc = ensure_arena(c, total))) return rc;
+  int first_bad = 0;
+  std::vector<EnergyResult> res;
+  for (int i0 = 0; i0 < total; i0 += c->cap) {
+    const int nb = std::min(c->cap, total - i0);
+    if ((rc = run_batch(c, nb, &c->qE[i0], res))) {
+      c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
+      return rc;
+    }
+    for (int b = 0; b < nb; ++b) {
+      const int i = i0 + b;
+      accumulate(c, res[b], c->qE[i], c->qIdx[i], c->qdE[i], mu, muT, acc);
+      if (status) status[i] = res[b].status;
+      if (sr_iters) {
+        sr_iters[2 * i] = res[b].iters[0];
+        sr_iters[2 * i + 1] = res[b].iters[1];
+      }
+      if (res[b].status && !first_bad) first_bad = res[b].status;
+    }
+  }
+  c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
+  return first_bad;
+}
+
+extern "C" int negf_energy_sweep(negf_ctx* c, double energy, int eCtr, double deltaE,
+                                 const double* mu, double muT, negf_accum* acc) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->qE.empty()) {
+    negf_set_error("negf_energy_sweep called with energies still queued; flush first");
+    return NEGF_ERR_STATE;
+  }
+  if ((rc = negf_submit(c, 1, &energy, &eCtr, &deltaE))) return rc;
+  return negf_flush(c, mu, muT, acc, nullptr, nullptr);
+}
+
+extern "C" int negf_contact_self_energy(negf_ctx* c, double energy, double* SigmaL, double* SigmaR,
+                                        double* GammaL, double* GammaR, double* dos, int* iters) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if ((rc = ensure_arena(c, 1))) return rc;
+  const int n = c->n, L = c->L;
+  const Layout& y = c->lay;
+  const long long nn = y.nn;
+  hipStream_t s = c->st;
+  HIPCHK(hipMemcpyAsync(c->d_energy, &energy, sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(launch_sr_alpha<double>(s, c->C + y.c_alpha, c->C + y.c_work, y.Cs, c->H,
+                                 c->H + (size_t)(L - 1) * nn, c->d_energy, 1, c->cfg.eta,
+                                 c->cfg.sr_extra_eta, n));
+  std::vector<int> it, st;
+  zd* SL = c->E + y.o_SigL;
+  zd* SR = c->E + y.o_SigR;
+  if ((rc = run_sr(c, 1, 1, c->betaL, c->betaR, c->betaR, c->betaL, SL, SR, y.Es, it, st))) return rc;
+  HIPCHK(launch_gamma_dos<double>(s, SL, y.Es, c->E + y.o_GamL, y.Es, n, 1, c->d_mc, 1));
+  HIPCHK(launch_gamma_dos<double>(s, SR, y.Es, c->E + y.o_GamR, y.Es, n, 1, c->d_mc + 1, 1));
+  double d[2];
+  HIPCHK(hipMemcpyAsync(d, c->d_mc, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+  if (SigmaL) HIPCHK(hipMemcpyAsync(SigmaL, SL, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  if (SigmaR) HIPCHK(hipMemcpyAsync(SigmaR, SR, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  if (GammaL) HIPCHK(hipMemcpyAsync(GammaL, c->E + y.o_GamL, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  if (GammaR) HIPCHK(hipMemcpyAsync(GammaR, c->E + y.o_GamR, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (dos) { dos[0] = d[0]; dos[1] = d[1]; }
+  if (iters) { iters[0] = it[0]; iters[1] = it[1]; }
+  return st[0] ? st[0] : st[1];
+}
+
+extern "C" int negf_sancho_rubio(negf_ctx* c, const double* alpha, const double* beta,
+                                 double* Sigma, int* iters) {
+  if (!c || !alpha || !beta || !Sigma) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  int rc = ensure_arena(c, 1);
+  if (rc) return rc;
+  const int n = c->n;
+  const Layout& y = c->lay;
+  const long long nn = y.nn;
+  hipStream_t s = c->st;
+  // beta and beta' go into the (unused) energy-slot region
+  zd* dB = c->E + y.o_slots;
+  zd* dBd = c->E + y.o_M;
+  zd* sig = c->E + y.o_SigL;
+  std::vector<zd> bd(nn), w(nn);
+  const zd* bh = (const zd*)beta;
+  const zd* ah = (const zd*)alpha;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      zd v = bh[(long long)i * n + j];
+      bd[(long long)j * n + i] = mk<double>(v.x, -v.y);
+    }
+  for (long long k = 0; k < nn; ++k) w[k] = ah[k];
+  if (c->cfg.sr_extra_eta)
+    for (int i = 0; i < n; ++i) w[(long long)i * n + i].y += c->cfg.eta;
+  HIPCHK(hipMemcpyAsync(dB, beta, sizeof(zd) * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dBd, bd.data(), sizeof(zd) * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->C + y.c_alpha, alpha, sizeof(zd) * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->C + y.c_work, w.data(), sizeof(zd) * nn, hipMemcpyHostToDevice, s));
+  std::vector<int> it, st;
+  if ((rc = run_sr(c, 1, 0, dB, dBd, dB, dBd, sig, sig, y.Es, it, st))) return rc;
+  HIPCHK(hipMemcpyAsync(Sigma, sig, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (iters) *iters = it[0];
+  return st[0];
+}
+
+// ---------------------------------------------------------------------------
+// L1 primitives on host matrices (one-shot device buffers).
+extern "C" int negf_invert_matrix(int n, const double* G, double* invG, int* info) {
+  if (n <= 0 || !G || !invG) {
+    negf_set_error("bad arguments");
+    return NEGF_ERR_ARG;
+  }
+  if (n > 3072) {
+    negf_set_error("n=%d exceeds 3072", n);
+    return NEGF_ERR_ARG;
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  int rc = require_device(dev);
+  if (rc) return rc;
+  const long long nn = (long long)n * n;
+  zd *dA = nullptr, *dO = nullptr, *prow = nullptr;
+  int* ints = nullptr;
+  HIPCHK(hipMalloc(&dA, sizeof(zd) * nn));
+  HIPCHK(hipMalloc(&dO, sizeof(zd) * nn));
+  HIPCHK(hipMalloc(&prow, sizeof(zd) * kGjNBO * n));
+  HIPCHK(hipMalloc(&ints, sizeof(int) * (2 * n + 1)));
+  HIPCHK(hipMemcpy(dA, G, sizeof(zd) * nn, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(ints, 0, sizeof(int) * (2 * n + 1)));
+  GjWork w = gjw(prow, 0, ints, 0, n);
+  HIPCHK(gj_invert<double>(nullptr, dA, 0, n, dO, 0, n, n, 1, w, nullptr));
+  int inf = 0;
+  HIPCHK(hipMemcpy(invG, dO, sizeof(zd) * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&inf, ints + 2 * n, sizeof(int), hipMemcpyDeviceToHost));
+  hipFree(dA); hipFree(dO); hipFree(prow); hipFree(ints);
+  if (info) *info = inf;
+  return 0;
+}
+
+static int op_of(char t) {
+  if (t == 'N' || t == 'n') return OP_N;
+  if (t == 'T' || t == 't') return OP_T;
+  if (t == 'C' || t == 'c') return OP_C;
+  return -1;
+}
+
+extern "C" int negf_zgemm(char transa, char transb, int n, const double* alpha, const double* A,
+                          int lda, const double* B, int ldb, const double* beta, double* C,
+                          int ldc) {
+  const int oa = op_of(transa), ob = op_of(transb);
+  if (n <= 0 || oa < 0 || ob < 0 || !alpha || !beta || !A || !B || !C || lda < n || ldb < n ||
+      ldc < n) {
+    negf_set_error("bad zgemm arguments");
+    return NEGF_ERR_ARG;
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  int rc = require_device(dev);
+  if (rc) return rc;
+  zd *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  HIPCHK(hipMalloc(&dA, sizeof(zd) * (size_t)lda * n));
+  HIPCHK(hipMalloc(&dB, sizeof(zd) * (size_t)ldb * n));
+  HIPCHK(hipMalloc(&dC, sizeof(zd) * (size_t)ldc * n));
+  HIPCHK(hipMemcpy(dA, A, sizeof(zd) * (size_t)lda * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, sizeof(zd) * (size_t)ldb * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dC, C, sizeof(zd) * (size_t)ldc * n, hipMemcpyHostToDevice));
+  HIPCHK(gemm_batched<double>(nullptr, oa, ob, gb(dA, 0, lda, dB, 0, ldb, dC, 0, ldc, n), 1,
+                              mk<double>(alpha[0], alpha[1]), mk<double>(beta[0], beta[1])));
+  HIPCHK(hipMemcpy(C, dC, sizeof(zd) * (size_t)ldc * n, hipMemcpyDeviceToHost));
+  hipFree(dA); hipFree(dB); hipFree(dC);
+  return 0;
+}

@@ -1,0 +1,504 @@
+// Bandwidth-bound kernels of the per-energy NEGF solve (gfx950).
+//
+// The dense n^3 work runs in the MFMA GEMM and the Gauss-Jordan inverse; what
+// remains is O(n^2) per slice and is HBM/L2-bound:
+//  * k_assemble   -- the RGF slice matrix  z I - H_x - Sigma - X' g X,  fused:
+//                    the reference forms it from two dense zgemm with -HopX
+//                    (Recursive_Greens.F90:18-20, :70-72) plus the InvGRd
+//                    assembly (Energy_Sweep.F90:31-38); HopX is site-block-diagonal
+//                    (Build_Hamiltonian.F90:62-64), so the sandwich costs
+//                    norb^2 multiply-adds per element instead of 2n;
+//  * k_bd_mul     -- HopX * G and HopX' * G for the block-column recursions
+//                    (Recursive_Greens.F90:61-62, :94-95);
+//  * k_siteblocks -- the norb x norb site-diagonal blocks of B G' (B = G Gamma),
+//                    reduced to the traces the density (Calculate_Gnp.F90:118-124)
+//                    and spin (Calculate_Observables.F90:10-30) need;
+//  * k_inner      -- Re sum_ij P_ij conj(Q_ij), the trace form of T12 and the
+//                    injected modes (Calculate_Observables.F90:48-53, :86-100);
+//  * Sancho-Rubio helpers (Contact_Self_Energy.F90:110-223).
+// Every reduction runs in a fixed order inside one workgroup, so an energy's
+// result does not depend on what else is in its batch.
+#include "negf_device.h"
+
+namespace negf {
+
+// ---------------------------------------------------------------------------
+// Slice assembly.  For batch item b (an energy):
+//   M_b = hsign*H + [diag] z_b I - Sig1_b - Sig2_b - D_b
+// with z_b = energy[b] + i*eta and, by MODE,
+//   MODE 0: D = 0,  MODE 1: D = X' g_b X,  MODE 2: D = X g_b X',  MODE 3: D = P_b.
+// X is site-block-diagonal with norb x norb blocks (MODE 1/2 read only those).
+template <typename T, int NORB, int MODE>
+__global__ void k_assemble(cx<T>* M, long long sM, const cx<T>* H, int add_diag, T hsign,
+                           const double* energy, double eta, const cx<T>* S1, long long sS1,
+                           const cx<T>* S2, long long sS2, const cx<T>* g, long long sg,
+                           const cx<T>* X, int n, const int* active) {
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int c = blockIdx.y;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const long long e = (long long)c * n + a;
+  cx<T> v = H[e];
+  v.x *= hsign;
+  v.y *= hsign;
+  if (add_diag && a == c) {
+    v.x += (T)energy[b];
+    v.y += (T)eta;
+  }
+  if (S1) v = csub(v, S1[(long long)b * sS1 + e]);
+  if (S2) v = csub(v, S2[(long long)b * sS2 + e]);
+  if (MODE == 1 || MODE == 2) {
+    const cx<T>* G = g + (long long)b * sg;
+    const int a0 = (a / NORB) * NORB, r = a - a0;
+    const int c0 = (c / NORB) * NORB, s = c - c0;
+    cx<T> d = mk<T>(0, 0);
+#pragma unroll
+    for (int p = 0; p < NORB; ++p) {
+      // t = (g X)[a0+p][c] (MODE 1)  or  (g X')[a0+p][c] (MODE 2)
+      cx<T> t = mk<T>(0, 0);
+#pragma unroll
+      for (int q = 0; q < NORB; ++q) {
+        const cx<T> gv = G[(long long)(c0 + q) * n + a0 + p];
+        if (MODE == 1)
+          t = cadd(t, cmul(gv, X[(long long)(c0 + s) * n + c0 + q]));
+        else
+          t = cadd(t, cmulc(gv, X[(long long)(c0 + q) * n + c0 + s]));
+      }
+      if (MODE == 1)
+        d = cadd(d, cmulc(t, X[(long long)(a0 + r) * n + a0 + p]));  // conj(X[p][r]) t
+      else
+        d = cadd(d, cmul(X[(long long)(a0 + p) * n + a0 + r], t));   // X[r][p] t
+    }
+    v = csub(v, d);
+  } else if (MODE == 3) {
+    v = csub(v, g[(long long)b * sg + e]);
+  }
+  M[(long long)b * sM + e] = v;
+}
+
+// Y_b = X G_b (CONJ = 0) or X' G_b (CONJ = 1), X site-block-diagonal.
+template <typename T, int NORB, int CONJ>
+__global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, const cx<T>* X,
+                         int n, const int* active) {
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int col = blockIdx.y;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int a0 = (a / NORB) * NORB, r = a - a0;
+  const cx<T>* Gb = G + (long long)b * sG + (long long)col * n;
+  cx<T> v = mk<T>(0, 0);
+#pragma unroll
+  for (int q = 0; q < NORB; ++q) {
+    if (CONJ)
+      v = cadd(v, cmul(cconj(X[(long long)(a0 + r) * n + a0 + q]), Gb[a0 + q]));
+    else
+      v = cadd(v, cmul(X[(long long)(a0 + q) * n + a0 + r], Gb[a0 + q]));
+  }
+  Y[(long long)b * sY + (long long)col * n + a] = v;
+}
+
+// Site-diagonal blocks D_s = (B G')[site s] for 64 rows per workgroup (4 waves
+// split the k range; partial sums are combined in LDS in a fixed order).
+// Writes, per site s of slice x:  out[0] = Re tr D,  out[1+a] = Re tr(D sigma_a).
+// out layout: out_b + q*qstride + s  (q = 0..3).
+template <typename T, int NORB>
+__global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long sB,
+                                                    const cx<T>* G, long long sG, int n,
+                                                    double* out, long long sOut, long long qstride,
+                                                    SpinMats sp, const int* active) {
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const cx<T>* Bb = Bm + (long long)b * sB;
+  const cx<T>* Gb = G + (long long)b * sG;
+  const int row0 = blockIdx.x * 64;
+  const int tr = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  const int a = row0 + tr;
+  const int a0 = (a / NORB) * NORB;
+  double accr[NORB], acci[NORB];
+#pragma unroll
+  for (int q = 0; q < NORB; ++q) accr[q] = acci[q] = 0.0;
+  if (a < n) {
+    const int kbeg = (n * kq) / 4, kend = (n * (kq + 1)) / 4;
+    for (int k = kbeg; k < kend; ++k) {
+      const cx<T> bv = Bb[(long long)k * n + a];
+#pragma unroll
+      for (int q = 0; q < NORB; ++q) {
+        const cx<T> gv = Gb[(long long)k * n + a0 + q];
+        accr[q] += (double)bv.x * gv.x + (double)bv.y * gv.y;
+        acci[q] += (double)bv.y * gv.x - (double)bv.x * gv.y;
+      }
+    }
+  }
+  __shared__ double sr[4][64][NORB], si[4][64][NORB];
+#pragma unroll
+  for (int q = 0; q < NORB; ++q) {
+    sr[kq][tr][q] = accr[q];
+    si[kq][tr][q] = acci[q];
+  }
+  __syncthreads();
+  // one thread per site of this row group
+  const int nsite_grp = 64 / NORB;
+  if (threadIdx.x < nsite_grp) {
+    const int sl = threadIdx.x;
+    const int s = (row0 / NORB) + sl;
+    if (s * NORB < n) {
+      double dr[NORB][NORB], di[NORB][NORB];
+#pragma unroll
+      for (int r = 0; r < NORB; ++r)
+#pragma unroll
+        for (int q = 0; q < NORB; ++q) {
+          const int t = sl * NORB + r;
+          dr[r][q] = ((sr[0][t][q] + sr[1][t][q]) + sr[2][t][q]) + sr[3][t][q];
+          di[r][q] = ((si[0][t][q] + si[1][t][q]) + si[2][t][q]) + si[3][t][q];
+        }
+      double* o = out + (long long)b * sOut + s;
+      double trace = 0.0;
+#pragma unroll
+      for (int r = 0; r < NORB; ++r) trace += dr[r][r];
+      o[0] = trace;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < NORB; ++r)
+#pragma unroll
+          for (int q = 0; q < NORB; ++q)
+            acc += dr[r][q] * sp.re[m][q][r] - di[r][q] * sp.im[m][q][r];
+        o[(m + 1) * qstride] = acc;
+      }
+    }
+  }
+}
+
+// out[b*sOut] = Re sum_{ij} P_ij conj(Q_ij) over an n x n block (one workgroup).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_inner(const cx<T>* P, long long sP, const cx<T>* Q,
+                                                long long sQ, long long nn, double* out,
+                                                long long sOut, const int* active) {
+  const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
+  const cx<T>* Pb = P + (long long)b * sP;
+  const cx<T>* Qb = Q + (long long)b * sQ;
+  double acc = 0.0;
+  for (long long i = threadIdx.x; i < nn; i += 1024) {
+    const cx<T> p = Pb[i], q = Qb[i];
+    acc += (double)p.x * q.x + (double)p.y * q.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  __shared__ double red[16];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    out[(long long)b * sOut] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched elementwise helpers over n x n blocks (blockIdx.y = batch slot).
+
+// Y = I
+template <typename T>
+__global__ void k_eye(cx<T>* Y, long long sY, int n, const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  const long long nn = (long long)n * n;
+  cx<T>* Yb = Y + (long long)b * sY;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(i % n), c = (int)(i / n);
+    Yb[i] = mk<T>(r == c ? (T)1 : (T)0, 0);
+  }
+}
+
+// Y = X + Z  (Z may alias Y)
+template <typename T>
+__global__ void k_add(cx<T>* Y, long long sY, const cx<T>* X, long long sX, const cx<T>* Z,
+                      long long sZ, long long nn, const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x)
+    Y[(long long)b * sY + i] = cadd(X[(long long)b * sX + i], Z[(long long)b * sZ + i]);
+}
+
+// up to three batched copies Y_k = X_k in one pass
+template <typename T>
+__global__ void k_copy3(cx<T>* Y0, const cx<T>* X0, cx<T>* Y1, const cx<T>* X1, cx<T>* Y2,
+                        const cx<T>* X2, long long stride, long long nn, const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  const long long o = (long long)b * stride;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x) {
+    Y0[o + i] = X0[o + i];
+    if (Y1) Y1[o + i] = X1[o + i];
+    if (Y2) Y2[o + i] = X2[o + i];
+  }
+}
+
+// Sancho-Rubio seed (Contact_Self_Energy.F90:71-76, :83-88, :141): for chain c
+// (lead = c / nb, energy = c % nb)  alpha = (E + i eta) I - Hend,
+// work = alpha + extra * i eta I.
+template <typename T>
+__global__ void k_sr_alpha(cx<T>* alpha, cx<T>* work, long long stride, const cx<T>* H0,
+                           const cx<T>* H1, const double* energy, int nb, double eta, int extra,
+                           int n) {
+  const int c = blockIdx.y;
+  const int lead = c / nb, e = c % nb;
+  const cx<T>* H = lead ? H1 : H0;
+  const long long nn = (long long)n * n;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(i % n), col = (int)(i / n);
+    cx<T> v = cneg(H[i]);
+    if (r == col) {
+      v.x += (T)energy[e];
+      v.y += (T)eta;
+    }
+    alpha[(long long)c * stride + i] = v;
+    if (r == col && extra) v.y += (T)eta;
+    work[(long long)c * stride + i] = v;
+  }
+}
+
+// out[c] = max_ij |X_ij| (NaN if any entry is NaN), one workgroup per chain.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_maxabs(const cx<T>* X, long long sX, long long nn,
+                                                 double* out, const int* active) {
+  const int c = active ? active[blockIdx.x] : (int)blockIdx.x;
+  const cx<T>* Xc = X + (long long)c * sX;
+  double m = 0.0;
+  bool nan = false;
+  for (long long i = threadIdx.x; i < nn; i += 1024) {
+    const double v = hypot((double)Xc[i].x, (double)Xc[i].y);
+    if (v != v) nan = true;
+    m = fmax(m, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmax(m, __shfl_xor(m, off));
+    nan = nan || __shfl_xor((int)nan, off);
+  }
+  __shared__ double red[16];
+  __shared__ int rn[16];
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = m;
+    rn[threadIdx.x >> 6] = nan;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    int an = 0;
+    for (int w = 0; w < 16; ++w) {
+      t = fmax(t, red[w]);
+      an |= rn[w];
+    }
+    out[c] = an ? (double)NAN : t;
+  }
+}
+
+// Gamma = i (Sigma - Sigma')  (Contact_Self_Energy.F90:100-101) and
+// dos[c] = -sum_i Im Sigma_ii / pi  (:94-97); one workgroup per (column block, chain).
+template <typename T>
+__global__ void k_gamma(const cx<T>* Sig, long long sS, cx<T>* Gam, long long sG, int n) {
+  const int c = blockIdx.z;
+  const int col = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const cx<T>* S = Sig + (long long)c * sS;
+  const cx<T> a = S[(long long)col * n + r], bt = S[(long long)r * n + col];
+  // i (a - conj(bt))
+  const T dr = a.x - bt.x, di = a.y + bt.y;
+  Gam[(long long)c * sG + (long long)col * n + r] = mk<T>(-di, dr);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dos(const cx<T>* Sig, long long sS, int n, double* out,
+                                             long long sOut) {
+  const int c = blockIdx.x;
+  const cx<T>* S = Sig + (long long)c * sS;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += (double)S[(long long)i * n + i].y;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    out[(long long)c * sOut] = -(((red[0] + red[1]) + red[2]) + red[3]) / 3.14159265358979323846;
+}
+
+// precision conversion (mixed-precision form)
+template <typename Td, typename Ts>
+__global__ void k_convert(cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX, long long nn,
+                          const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cx<Ts> v = X[(long long)b * sX + i];
+    Y[(long long)b * sY + i] = mk<Td>((Td)v.x, (Td)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+static inline dim3 grid_nn(long long nn, int batch) {
+  long long blocks = (nn + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  return dim3((unsigned)blocks, batch);
+}
+
+template <typename T>
+hipError_t launch_assemble(hipStream_t s, int norb, int mode, cx<T>* M, long long sM,
+                           const cx<T>* H, int add_diag, double hsign, const double* energy,
+                           double eta, const cx<T>* S1, long long sS1, const cx<T>* S2,
+                           long long sS2, const cx<T>* g, long long sg, const cx<T>* X, int n,
+                           int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  dim3 grid((n + 255) / 256, n, batch);
+#define NEGF_ASM(NO, MO)                                                                    \
+  if (norb == NO && mode == MO) {                                                           \
+    hipLaunchKernelGGL((k_assemble<T, NO, MO>), grid, dim3(256), 0, s, M, sM, H, add_diag,   \
+                       (T)hsign, energy, eta, S1, sS1, S2, sS2, g, sg, X, n, active);        \
+    return hipGetLastError();                                                               \
+  }
+#define NEGF_ASM_N(NO) NEGF_ASM(NO, 0) NEGF_ASM(NO, 1) NEGF_ASM(NO, 2) NEGF_ASM(NO, 3)
+  NEGF_ASM_N(1) NEGF_ASM_N(2) NEGF_ASM_N(4)
+#undef NEGF_ASM_N
+#undef NEGF_ASM
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_bd_mul(hipStream_t s, int norb, int conj, cx<T>* Y, long long sY,
+                         const cx<T>* G, long long sG, const cx<T>* X, int n, int batch,
+                         const int* active) {
+  if (batch <= 0) return hipSuccess;
+  dim3 grid((n + 255) / 256, n, batch);
+#define NEGF_BD(NO, CJ)                                                                       \
+  if (norb == NO && conj == CJ) {                                                             \
+    hipLaunchKernelGGL((k_bd_mul<T, NO, CJ>), grid, dim3(256), 0, s, Y, sY, G, sG, X, n, active); \
+    return hipGetLastError();                                                                 \
+  }
+  NEGF_BD(1, 0) NEGF_BD(1, 1) NEGF_BD(2, 0) NEGF_BD(2, 1) NEGF_BD(4, 0) NEGF_BD(4, 1)
+#undef NEGF_BD
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long sB,
+                             const cx<T>* G, long long sG, int n, double* out, long long sOut,
+                             long long qstride, const SpinMats& sp, int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  dim3 grid((n + 63) / 64, 1, batch);
+#define NEGF_SB(NO)                                                                            \
+  if (norb == NO) {                                                                            \
+    hipLaunchKernelGGL((k_siteblocks<T, NO>), grid, dim3(256), 0, s, Bm, sB, G, sG, n, out, sOut, \
+                       qstride, sp, active);                                                   \
+    return hipGetLastError();                                                                  \
+  }
+  NEGF_SB(1) NEGF_SB(2) NEGF_SB(4)
+#undef NEGF_SB
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_inner(hipStream_t s, const cx<T>* P, long long sP, const cx<T>* Q, long long sQ,
+                        int n, double* out, long long sOut, int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_inner<T>), dim3(batch), dim3(1024), 0, s, P, sP, Q, sQ, (long long)n * n,
+                     out, sOut, active);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_eye(hipStream_t s, cx<T>* Y, long long sY, int n, int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_eye<T>), grid_nn((long long)n * n, batch), dim3(256), 0, s, Y, sY, n, active);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_add(hipStream_t s, cx<T>* Y, long long sY, const cx<T>* X, long long sX,
+                      const cx<T>* Z, long long sZ, long long nn, int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_add<T>), grid_nn(nn, batch), dim3(256), 0, s, Y, sY, X, sX, Z, sZ, nn, active);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_copy3(hipStream_t s, cx<T>* Y0, const cx<T>* X0, cx<T>* Y1, const cx<T>* X1,
+                        cx<T>* Y2, const cx<T>* X2, long long stride, long long nn, int batch,
+                        const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_copy3<T>), grid_nn(nn, batch), dim3(256), 0, s, Y0, X0, Y1, X1, Y2, X2,
+                     stride, nn, active);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sr_alpha(hipStream_t s, cx<T>* alpha, cx<T>* work, long long stride,
+                           const cx<T>* H0, const cx<T>* H1, const double* energy, int nb,
+                           double eta, int extra, int n) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_sr_alpha<T>), grid_nn((long long)n * n, 2 * nb), dim3(256), 0, s, alpha,
+                     work, stride, H0, H1, energy, nb, eta, extra, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_maxabs(hipStream_t s, const cx<T>* X, long long sX, long long nn, double* out,
+                         int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_maxabs<T>), dim3(batch), dim3(1024), 0, s, X, sX, nn, out, active);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_gamma_dos(hipStream_t s, const cx<T>* Sig, long long sS, cx<T>* Gam,
+                            long long sG, int n, int batch, double* dos, long long sDos) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_gamma<T>), dim3((n + 255) / 256, n, batch), dim3(256), 0, s, Sig, sS, Gam,
+                     sG, n);
+  hipLaunchKernelGGL((k_dos<T>), dim3(batch), dim3(256), 0, s, Sig, sS, n, dos, sDos);
+  return hipGetLastError();
+}
+
+template <typename Td, typename Ts>
+hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX,
+                          long long nn, int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_convert<Td, Ts>), grid_nn(nn, batch), dim3(256), 0, s, Y, sY, X, sX, nn,
+                     active);
+  return hipGetLastError();
+}
+
+#define NEGF_INST(T)                                                                               \
+  template hipError_t launch_assemble<T>(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,   \
+                                         int, double, const double*, double, const cx<T>*,         \
+                                         long long, const cx<T>*, long long, const cx<T>*,         \
+                                         long long, const cx<T>*, int, int, const int*);           \
+  template hipError_t launch_bd_mul<T>(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,     \
+                                       long long, const cx<T>*, int, int, const int*);             \
+  template hipError_t launch_siteblocks<T>(hipStream_t, int, const cx<T>*, long long,              \
+                                           const cx<T>*, long long, int, double*, long long,       \
+                                           long long, const SpinMats&, int, const int*);           \
+  template hipError_t launch_inner<T>(hipStream_t, const cx<T>*, long long, const cx<T>*,          \
+                                      long long, int, double*, long long, int, const int*);        \
+  template hipError_t launch_eye<T>(hipStream_t, cx<T>*, long long, int, int, const int*);         \
+  template hipError_t launch_add<T>(hipStream_t, cx<T>*, long long, const cx<T>*, long long,       \
+                                    const cx<T>*, long long, long long, int, const int*);          \
+  template hipError_t launch_copy3<T>(hipStream_t, cx<T>*, const cx<T>*, cx<T>*, const cx<T>*,     \
+                                      cx<T>*, const cx<T>*, long long, long long, int, const int*); \
+  template hipError_t launch_sr_alpha<T>(hipStream_t, cx<T>*, cx<T>*, long long, const cx<T>*,     \
+                                         const cx<T>*, const double*, int, double, int, int);      \
+  template hipError_t launch_maxabs<T>(hipStream_t, const cx<T>*, long long, long long, double*,   \
+                                       int, const int*);                                           \
+  template hipError_t launch_gamma_dos<T>(hipStream_t, const cx<T>*, long long, cx<T>*, long long, \
+                                          int, int, double*, long long);
+NEGF_INST(double)
+NEGF_INST(float)
+#undef NEGF_INST
+template hipError_t launch_convert<float, double>(hipStream_t, cx<float>*, long long,
+                                                  const cx<double>*, long long, long long, int,
+                                                  const int*);
+template hipError_t launch_convert<double, float>(hipStream_t, cx<double>*, long long,
+                                                  const cx<float>*, long long, long long, int,
+                                                  const int*);
+
+}  // namespace negf
