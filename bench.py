@@ -1,0 +1,178 @@
+"""Throughput benchmark of the per-energy NEGF solve on one MI355X.
+
+A step is one batch of energy points of one wire solved end to end on the GPU
+(Sancho-Rubio contacts, forward/backward RGF, Gn/Gp site blocks, T12, modes,
+density and spin).  Default workload: the plain-FP64 mid-size wire (4-band TI
+model, Npx=100, Npy=Npz=16 -> block n=1024, 512 energies in [-0.4, 0.4], Anderson
+disorder W=0.5, seed 1); each step takes a strided slice of that energy grid so
+every step samples the whole range.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the FP64 MFMA
+complex GEMM, trailing Gauss-Jordan updates included), timed with HIP events on
+the stream it runs on; `cpu_baseline` times the complex128 CPU oracle (MKL) on
+the host on a bounded sample of the same wire, extrapolated to Npx=100.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from nonequilibriumgreensfunction_amd import capi  # noqa: E402
+from nonequilibriumgreensfunction_amd.hamiltonian import build_wire, energy_grid  # noqa: E402
+
+PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+PEAK_FP32_MFMA = 157.3
+
+CONFIGS = {
+    "c1": dict(npx=20, npy=6, npz=6, ne=64, emin=-0.3, emax=0.3, W=0.0, seed=1, per_step=64),
+    "c2": dict(npx=100, npy=16, npz=16, ne=512, emin=-0.4, emax=0.4, W=0.5, seed=1, per_step=64),
+    "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=16),
+}
+ETA, SR_TOL, SR_MAX = 1e-6, 1e-10, 200
+MU, MUT = (-0.1, 0.0), 0.0
+
+
+def f_useful(n, L, k_left, k_right):
+    """Irreducible dense flops of one energy point: 2L inverses, 2(L-1) column
+    GEMMs, 2L G*Gamma GEMMs, 1 for T12 and (2 + 9K) per lead for Sancho-Rubio,
+    each n^3 complex multiply-adds = 8 n^3 flops."""
+    return 8.0 * n**3 * (6 * L + 3 + 9 * (k_left + k_right))
+
+
+def cpu_baseline(cfg, energy, threads):
+    """Oracle (C++ complex128 + MKL, the reference's operation order) on the host:
+    one energy at Npx=2 and Npx=4 of the same cross-section, extrapolated linearly
+    in Npx to the workload's Npx."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # noqa: E402  (checker only)
+    O.set_threads(threads)
+    t = {}
+    for L in (2, 4):
+        w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
+        t0 = time.perf_counter()
+        O.energy_sweep(w, energy, 0.0, MU, MUT, ETA, SR_TOL, SR_MAX, True)
+        t[L] = time.perf_counter() - t0
+    per_slice = (t[4] - t[2]) / 2.0
+    t_full = t[2] + per_slice * (cfg["npx"] - 2)
+    return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port",
+                sample=(f"oracle, 1 energy (E={energy:.4f}) at Npx=2 ({t[2]:.2f}s) and Npx=4 "
+                        f"({t[4]:.2f}s), Npy=Npz={cfg['npy']}, MKL {threads} threads; "
+                        f"extrapolated linearly to Npx={cfg['npx']}: {t_full:.1f}s per energy"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "single-GPU build: this benchmark runs on exactly one GPU "
+                                   "(energies shard across GPUs with no communication until the "
+                                   "final sums; only --gpus 1 is measured)"}))
+        sys.exit(2)
+
+    cfg = CONFIGS[args.config]
+    per = args.per_step or cfg["per_step"]
+    n = cfg["npy"] * cfg["npz"] * 4
+    L = cfg["npx"]
+    grid = energy_grid(cfg["emin"], cfg["emax"], cfg["ne"])
+    dE = grid[1] - grid[0]
+    nsteps = args.warmup + args.steps
+    stride = max(1, min(nsteps, cfg["ne"] // per))
+
+    def step_indices(k):
+        idx = (np.arange(per) * stride + k) % cfg["ne"]
+        return idx
+
+    print(f"[bench] building wire {args.config}: Npx={L} Npy=Npz={cfg['npy']} n={n}", file=sys.stderr,
+          flush=True)
+    w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
+    ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
+                       max_batch=per)
+    ctx.set_hamiltonian(w)
+    acc = capi.Accumulators(L, cfg["npy"], cfg["npz"], cfg["ne"])
+
+    def run_step(k):
+        idx = step_indices(k)
+        ctx.submit(grid[idx], idx, np.full(len(idx), dE))
+        st, it = ctx.flush(MU, MUT, acc)
+        return st, it
+
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        run_step(k)
+        print(f"[bench] warmup step {k} {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+
+    capi.profile_begin()
+    times, iters, status = [], [], []
+    t_start = time.perf_counter()
+    for k in range(args.warmup, nsteps):
+        t0 = time.perf_counter()
+        st, it = run_step(k)  # flush() returns after the device stream has drained
+        times.append(time.perf_counter() - t0)
+        iters.append(it)
+        status.append(st)
+        print(f"[bench] step {k} {times[-1]:.2f}s", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t_start
+    g_flops, g_ms, g_launches = capi.profile_end()
+    ctx.close()
+
+    iters = np.concatenate(iters)
+    npts = per * args.steps
+    value = npts / elapsed
+    useful = sum(f_useful(n, L, a, b) for a, b in iters)
+    gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    result = {
+        "metric": "energy-points/s (plain FP64 form, one GPU)",
+        "value": value,
+        "unit": "energy-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 4-band TI wire, host-built Hamiltonian)",
+        "config": {"workload": (f"{args.config}: Npx={L}, Npy=Npz={cfg['npy']} (block {n}), "
+                                f"{cfg['ne']} energies in [{cfg['emin']},{cfg['emax']}], "
+                                f"Anderson W={cfg['W']} seed {cfg['seed']}, eta={ETA}, "
+                                f"SR tol {SR_TOL} max {SR_MAX}"),
+                   "energies_per_step": per, "precision": "fp64"},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(times, 50)),
+                       "p99": 1e3 * float(np.percentile(times, 99)),
+                       "note": "energies of a step complete together: per-energy solve latency = step time"},
+        "sr_iters_mean": float(iters.mean()),
+        "status_nonzero": int(sum(int(np.count_nonzero(s)) for s in status)),
+        "path_tflops_useful": useful / elapsed / 1e12,
+        "path_frac_fp64_peak": useful / elapsed / 1e12 / PEAK_FP64_MFMA,
+        "roofline": {"bound": "mfma", "kernel": "k_gemm (FP64 MFMA complex GEMM incl. Gauss-Jordan updates)",
+                     "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                     "frac": gemm_tflops / PEAK_FP64_MFMA, "traffic": None,
+                     "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
+                     "gemm_time_share": (g_ms * 1e-3) / elapsed},
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(cfg, float(grid[cfg["ne"] // 3]), threads)
+        print(f"[bench] cpu baseline {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

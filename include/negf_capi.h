@@ -147,6 +147,14 @@ int negf_invert_matrix(int n, const double* G, double* invG, int* info);
 int negf_zgemm(char transa, char transb, int n, const double* alpha, const double* A, int lda,
                const double* B, int ldb, const double* beta, double* C, int ldc);
 
+/* --- Instrumentation (no counterpart in the reference) ---------------------
+ * Time every MFMA GEMM launch (the dominant kernel) with HIP events on the
+ * stream it is launched on, between begin and end.  end() synchronises the
+ * device and returns the launches' algorithmic flops (8 m n k per complex
+ * product), their summed duration in ms and the launch count. */
+int negf_profile_begin(void);
+int negf_profile_end(double* flops, double* ms, long long* launches);
+
 #ifdef __cplusplus
 }
 #endif

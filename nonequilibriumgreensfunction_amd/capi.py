@@ -32,7 +32,7 @@ EXPORTS = (
     "negf_last_error", "negf_device_count", "negf_init", "negf_finalize", "negf_set_hamiltonian",
     "negf_energy_sweep", "negf_submit", "negf_flush", "negf_contact_self_energy",
     "negf_sancho_rubio", "negf_retarded", "negf_gnp", "negf_observables", "negf_invert_matrix",
-    "negf_zgemm",
+    "negf_zgemm", "negf_profile_begin", "negf_profile_end",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -84,6 +84,7 @@ def lib():
         L.negf_invert_matrix.argtypes = [ctypes.c_int, _dp, _dp, _ip]
         L.negf_zgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -109,6 +110,17 @@ def _p(a):
 
 def _cf(a):
     return np.asfortranarray(a, dtype=np.complex128)
+
+
+def profile_begin():
+    _check(lib().negf_profile_begin())
+
+
+def profile_end():
+    """(flops, ms, launches) of the MFMA GEMM launches since profile_begin()."""
+    f, m, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+    _check(lib().negf_profile_end(ctypes.byref(f), ctypes.byref(m), ctypes.byref(n)))
+    return f.value, m.value, n.value
 
 
 def device_count() -> int:
@@ -231,3 +243,50 @@ class Context:
         it = ctypes.c_int(0)
         st = _check(lib().negf_sancho_rubio(self._h, _p(a), _p(b), _p(sig), ctypes.byref(it)))
         return sig, st, it.value
+
+    # --- stage entry points (one reference subroutine each) -----------------
+    def retarded(self, invgrd):
+        """Get_Retarded_Greens on a given InvGRd: dict of GRd, GRu, GRl, GRLef, GRRgt."""
+        npx, npy, npz, norb = self.dims
+        n, L = self.n, npx
+        A = _cf(invgrd)
+        Lm = max(L - 1, 1)
+        o = dict(GRd=np.zeros((n, n, L), complex, order="F"),
+                 GRu=np.zeros((n, n, Lm), complex, order="F"),
+                 GRl=np.zeros((n, n, Lm), complex, order="F"),
+                 GRLef=np.zeros((n * L, n), complex, order="F"),
+                 GRRgt=np.zeros((n * L, n), complex, order="F"))
+        o["status"] = _check(lib().negf_retarded(self._h, _p(A), _p(o["GRd"]), _p(o["GRu"]),
+                                                 _p(o["GRl"]), _p(o["GRLef"]), _p(o["GRRgt"])))
+        return o
+
+    def gnp(self, f1, f2, deltaE, GRLef, GRRgt, GammaL, GammaR):
+        """Calculate_Gnp: dict of Gnd..Gpl and this call's RhoN/RhoP increments."""
+        npx, npy, npz, norb = self.dims
+        n, L = self.n, npx
+        Lm = max(L - 1, 1)
+        o = {k: np.zeros((n, n, L if k.endswith("d") else Lm), complex, order="F")
+             for k in ("Gnd", "Gnu", "Gnl", "Gpd", "Gpu", "Gpl")}
+        o["RhoN"] = np.zeros((L, npy, npz), order="F")
+        o["RhoP"] = np.zeros((L, npy, npz), order="F")
+        ins = [_cf(GRLef), _cf(GRRgt), _cf(GammaL), _cf(GammaR)]
+        _check(lib().negf_gnp(self._h, f1, f2, deltaE, *[_p(a) for a in ins],
+                              _p(o["Gnd"]), _p(o["Gnu"]), _p(o["Gnl"]), _p(o["Gpd"]), _p(o["Gpu"]),
+                              _p(o["Gpl"]), _p(o["RhoN"]), _p(o["RhoP"])))
+        return o
+
+    def observables(self, f1, f2, deltaE, g, GRRgt, GammaL, GammaR):
+        """Calculate_Observables for one energy (eCtr 0): dict of T12, Modec1, Modec2,
+        EDensity, HDensity, SigXExp.., Jx/Jy/Jz and ContactCurrent increments."""
+        npx, npy, npz, norb = self.dims
+        acc = Accumulators(npx, npy, npz, 1)
+        J = [np.zeros((npx, npy, npz), order="F") for _ in range(3)]
+        ins = [_cf(g[k]) for k in ("Gnd", "Gpd", "Gnu", "Gnl", "Gpu", "Gpl")]
+        s = acc.struct()
+        _check(lib().negf_observables(self._h, 0, f1, f2, deltaE, *[_p(a) for a in ins],
+                                      _p(_cf(GRRgt)), _p(_cf(GammaL)), _p(_cf(GammaR)),
+                                      ctypes.byref(s), *[_p(j) for j in J]))
+        out = {k: getattr(acc, k)[0] for k in ("T12", "Modec1", "Modec2", "EDensity", "HDensity")}
+        out.update(SigXExp=acc.SigXExp, SigYExp=acc.SigYExp, SigZExp=acc.SigZExp,
+                   ContactCurrent=acc.ContactCurrent, Jx=J[0], Jy=J[1], Jz=J[2])
+        return out

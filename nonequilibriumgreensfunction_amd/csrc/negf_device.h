@@ -86,6 +86,18 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
                      int ldo, int n, int batch, const GjWork& w, const int* active);
 constexpr int kGjNBO = 64;
 
+// Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
+// every launch is bracketed by HIP events on its own stream and its algorithmic
+// flop count (8 m n k per complex product) is recorded.
+struct GemmTimer {
+  bool on = false;
+  size_t used = 0;
+  hipEvent_t* ev = nullptr;  // 2 per launch
+  double* flops = nullptr;
+  size_t cap = 0;
+};
+GemmTimer& gemm_timer();
+
 }  // namespace negf
 
 extern "C" void negf_set_error(const char* fmt, ...);

@@ -336,6 +336,180 @@ __global__ void k_convert(cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX
 }
 
 // ---------------------------------------------------------------------------
+// Stage-wise observables on materialised Gn/Gp blocks (Calculate_Observables).
+
+// out[b*n + i] = Re M_b[i][i]
+template <typename T>
+__global__ void k_diag_re(const cx<T>* M, long long sM, int n, double* out) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[(long long)b * n + i] = (double)M[(long long)b * sM + (long long)i * n + i].x;
+}
+
+// out = Re sum_ij P_ij Q_ji  (= Re tr(P Q)), one workgroup
+template <typename T>
+__global__ __launch_bounds__(1024) void k_trace_prod(const cx<T>* P, const cx<T>* Q, int n,
+                                                     double* out) {
+  double acc = 0.0;
+  const long long nn = (long long)n * n;
+  for (long long e = threadIdx.x; e < nn; e += 1024) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    const cx<T> p = P[e], q = Q[(long long)i * n + j];
+    acc += (double)p.x * q.x - (double)p.y * q.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  __shared__ double red[16];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    *out = t;
+  }
+}
+
+// Per (slice x, site s = y + npy*z): spin Re tr(Gn_site sigma_a) (Calculate_Observables.F90:10-30)
+// and the bond-current terms of :111-168 before the e^2/hbar and dE/2pi factors:
+//   y: Re i [H(a,b)Gn(b,a) - Gn(a,b)H(b,a) - H(a,b)Gp(b,a) + Gp(a,b)H(b,a)], b on site y+1
+//   z: the same with b on site z+1;  x: HopX with Gnl/Gnu/Gpl/Gpu of slice x.
+// out[(q * L + x) * nsites + s], q = 0..2 spin, 3 Jx, 4 Jy, 5 Jz.
+template <typename T, int NORB>
+__global__ void k_site_obs(const cx<T>* Gnd, const cx<T>* Gpd, const cx<T>* Gnu, const cx<T>* Gnl,
+                           const cx<T>* Gpu, const cx<T>* Gpl, const cx<T>* H, const cx<T>* X,
+                           int n, int L, int npy, int npz, SpinMats sp, double* out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.y;
+  const int ns = npy * npz;
+  if (s >= ns) return;
+  const int y = s % npy, z = s / npy;
+  const long long nn = (long long)n * n;
+  const long long bx = (long long)x * nn;
+  const int idx = NORB * s;  // (y-1)*Norb + (z-1)*NN, 0-based
+  auto at = [&](const cx<T>* M, long long base, int i, int j) { return M[base + (long long)j * n + i]; };
+  double spin[3] = {0, 0, 0};
+  for (int ii = 0; ii < NORB; ++ii)
+    for (int jj = 0; jj < NORB; ++jj) {
+      const cx<T> g = at(Gnd, bx, idx + ii, idx + jj);
+      for (int m = 0; m < 3; ++m)
+        spin[m] += (double)g.x * sp.re[m][jj][ii] - (double)g.y * sp.im[m][jj][ii];
+    }
+  // Re(i*c) = -Im(c)
+  auto jterm = [&](int b0) {
+    double acc = 0.0;
+    for (int ii = 0; ii < NORB; ++ii)
+      for (int jj = 0; jj < NORB; ++jj) {
+        const int a = idx + ii, b = b0 + jj;
+        const cx<T> hab = at(H, bx, a, b), hba = at(H, bx, b, a);
+        cx<T> c = cmul(hab, at(Gnd, bx, b, a));
+        c = csub(c, cmul(at(Gnd, bx, a, b), hba));
+        c = csub(c, cmul(hab, at(Gpd, bx, b, a)));
+        c = cadd(c, cmul(at(Gpd, bx, a, b), hba));
+        acc += -(double)c.y;
+      }
+    return acc;
+  };
+  double jx = 0.0, jy = 0.0, jz = 0.0;
+  if (y < npy - 1) jy = jterm(idx + NORB);
+  if (z < npz - 1) jz = jterm(idx + NORB * npy);
+  if (x < L - 1) {
+    for (int ii = 0; ii < NORB; ++ii)
+      for (int jj = 0; jj < NORB; ++jj) {
+        const int a = idx + ii, b = idx + jj;
+        const cx<T> hx = X[(long long)b * n + a];
+        cx<T> c = cmul(hx, at(Gnl, bx, b, a));
+        c = csub(c, cmul(at(Gnu, bx, a, b), cconj(hx)));
+        c = csub(c, cmul(hx, at(Gpl, bx, b, a)));
+        c = cadd(c, cmul(at(Gpu, bx, a, b), cconj(hx)));
+        jx += -(double)c.y;
+      }
+  }
+  const long long q = (long long)L * ns, o = (long long)x * ns + s;
+  out[o] = spin[0];
+  out[q + o] = spin[1];
+  out[2 * q + o] = spin[2];
+  out[3 * q + o] = jx;
+  out[4 * q + o] = jy;
+  out[5 * q + o] = jz;
+}
+
+// Y = a P + b Q (real a, b) over nn elements
+template <typename T>
+__global__ void k_lincomb(cx<T>* Y, const cx<T>* P, const cx<T>* Q, double a, double b,
+                          long long nn) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cx<T> p = P[i], q = Q[i];
+    Y[i] = mk<T>((T)(a * p.x + b * q.x), (T)(a * p.y + b * q.y));
+  }
+}
+
+template <typename T>
+hipError_t launch_lincomb(hipStream_t s, cx<T>* Y, const cx<T>* P, const cx<T>* Q, double a,
+                          double b, long long nn) {
+  if (nn <= 0) return hipSuccess;
+  const long long blocks = (nn + 255) / 256 < 2048 ? (nn + 255) / 256 : 2048;
+  hipLaunchKernelGGL((k_lincomb<T>), dim3((unsigned)blocks), dim3(256), 0, s, Y, P, Q, a, b, nn);
+  return hipGetLastError();
+}
+template hipError_t launch_lincomb<double>(hipStream_t, cx<double>*, const cx<double>*,
+                                           const cx<double>*, double, double, long long);
+
+// Y_b += I
+template <typename T>
+__global__ void k_add_eye(cx<T>* Y, long long sY, int n) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Y[(long long)b * sY + (long long)i * n + i].x += (T)1;
+}
+
+template <typename T>
+hipError_t launch_diag_re(hipStream_t s, const cx<T>* M, long long sM, int n, int batch,
+                          double* out) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_diag_re<T>), dim3((n + 255) / 256, batch), dim3(256), 0, s, M, sM, n, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_trace_prod(hipStream_t s, const cx<T>* P, const cx<T>* Q, int n, double* out) {
+  hipLaunchKernelGGL((k_trace_prod<T>), dim3(1), dim3(1024), 0, s, P, Q, n, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_site_obs(hipStream_t s, int norb, const cx<T>* Gnd, const cx<T>* Gpd,
+                           const cx<T>* Gnu, const cx<T>* Gnl, const cx<T>* Gpu, const cx<T>* Gpl,
+                           const cx<T>* H, const cx<T>* X, int n, int L, int npy, int npz,
+                           const SpinMats& sp, double* out) {
+  dim3 grid((npy * npz + 63) / 64, L);
+#define NEGF_SO(NO)                                                                              \
+  if (norb == NO) {                                                                              \
+    hipLaunchKernelGGL((k_site_obs<T, NO>), grid, dim3(64), 0, s, Gnd, Gpd, Gnu, Gnl, Gpu, Gpl, H, \
+                       X, n, L, npy, npz, sp, out);                                              \
+    return hipGetLastError();                                                                    \
+  }
+  NEGF_SO(1) NEGF_SO(2) NEGF_SO(4)
+#undef NEGF_SO
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_add_eye(hipStream_t s, cx<T>* Y, long long sY, int n, int batch) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_add_eye<T>), dim3((n + 255) / 256, batch), dim3(256), 0, s, Y, sY, n);
+  return hipGetLastError();
+}
+
+template hipError_t launch_diag_re<double>(hipStream_t, const cx<double>*, long long, int, int, double*);
+template hipError_t launch_trace_prod<double>(hipStream_t, const cx<double>*, const cx<double>*, int,
+                                              double*);
+template hipError_t launch_site_obs<double>(hipStream_t, int, const cx<double>*, const cx<double>*,
+                                            const cx<double>*, const cx<double>*, const cx<double>*,
+                                            const cx<double>*, const cx<double>*, const cx<double>*,
+                                            int, int, int, int, const SpinMats&, double*);
+template hipError_t launch_add_eye<double>(hipStream_t, cx<double>*, long long, int, int);
+
+// ---------------------------------------------------------------------------
 // Host launchers
 static inline dim3 grid_nn(long long nn, int batch) {
   long long blocks = (nn + 255) / 256;

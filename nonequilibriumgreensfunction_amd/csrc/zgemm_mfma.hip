@@ -17,6 +17,8 @@
 // inverse (gj_inverse.hip).
 #include "negf_device.h"
 
+#include <cstdlib>
+
 namespace negf {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -193,13 +195,39 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
     }
 }
 
+GemmTimer& gemm_timer() {
+  static GemmTimer t;
+  return t;
+}
+
+static hipEvent_t* timer_slot(double flops) {
+  GemmTimer& t = gemm_timer();
+  if (!t.on) return nullptr;
+  if (t.used == t.cap) {
+    size_t nc = t.cap ? 2 * t.cap : 4096;
+    hipEvent_t* ev = (hipEvent_t*)realloc(t.ev, sizeof(hipEvent_t) * 2 * nc);
+    double* fl = (double*)realloc(t.flops, sizeof(double) * nc);
+    if (!ev || !fl) return nullptr;
+    for (size_t i = 2 * t.cap; i < 2 * nc; ++i) hipEventCreate(&ev[i]);
+    t.ev = ev;
+    t.flops = fl;
+    t.cap = nc;
+  }
+  t.flops[t.used] = flops;
+  return &t.ev[2 * t.used++];
+}
+
 template <typename T, int OPA, int OPB, int EPI>
 static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alpha, cx<T> beta) {
   if (batch <= 0 || g.m <= 0 || g.n <= 0) return hipSuccess;
   int tm = (g.m + kBM - 1) / kBM, tn = (g.n + kBN - 1) / kBN;
   int nt = tm * tn;
   dim3 grid(((nt + 7) / 8) * 8, 1, batch);
+  const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
+  hipEvent_t* ev = timer_slot(8.0 * g.m * (double)ncols * g.k * batch);
+  if (ev) hipEventRecord(ev[0], s);
   hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI>), grid, dim3(kThreads), 0, s, g, tm, nt, alpha, beta);
+  if (ev) hipEventRecord(ev[1], s);
   return hipGetLastError();
 }
 

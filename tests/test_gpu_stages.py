@@ -1,0 +1,100 @@
+"""GPU parity of the stage entry points, each against the oracle's restatement of
+the same reference subroutine on identical inputs: Get_Retarded_Greens
+(Recursive_Greens.F90), Calculate_Gnp, Calculate_Observables (including the bond
+currents, computed in REAL(8)) and Sancho_Rubio.  Relative tolerance 1e-10 of
+each array's max-norm (GRd..GRRgt, Gn/Gp); the Gamma-derived quantities use the
+scales explained in test_gpu_sweep.py."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("nonequilibriumgreensfunction_amd.capi")
+from nonequilibriumgreensfunction_amd.hamiltonian import build_wire  # noqa: E402
+
+import oracle as O  # noqa: E402
+
+ETA = 1e-6
+
+
+def _close(name, got, ref, rtol=1e-10, scale=0.0):
+    got, ref = np.asarray(got), np.asarray(ref)
+    err = np.abs(got - ref).max()
+    lim = rtol * max(np.abs(ref).max(), scale)
+    assert err <= lim, f"{name}: {err:.3e} > {lim:.3e}"
+
+
+CASES = [((5, 2, 2), dict(), 1.6), ((4, 3, 2), dict(disorder=0.5, seed=3, flux=0.3, zeeman=0.02), 0.2),
+         ((1, 2, 2), dict(), 1.2), ((3, 2, 3), dict(disorder=0.9, seed=1), -1.7)]
+
+
+def _inputs(dims, kw, E):
+    w = build_wire(*dims, **kw)
+    c = O.contact_self_energy(w, E, ETA)
+    A = O.assemble_invg(w, c["SigmaL"], c["SigmaR"], E, ETA)
+    return w, c, A
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_retarded_matches_oracle(case):
+    w, c, A = _inputs(*CASES[case])
+    ref = O.retarded(A, w.hopx)
+    with capi.Context(w.npx, w.npy, w.npz, w.norb, eta=ETA) as ctx:
+        ctx.set_hamiltonian(w)
+        got = ctx.retarded(A)
+    assert got["status"] == 0
+    for k in ("GRd", "GRu", "GRl", "GRLef", "GRRgt"):
+        if w.npx == 1 and k in ("GRu", "GRl"):
+            continue
+        _close(k, got[k], ref[k])
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_gnp_and_observables_match_oracle(case):
+    w, c, A = _inputs(*CASES[case])
+    r = O.retarded(A, w.hopx)
+    f1, f2, dE = 1.0, 0.0, 0.0123
+    ref = O.gnp(w, f1, f2, dE, r["GRLef"], r["GRRgt"], c["GammaL"], c["GammaR"])
+    with capi.Context(w.npx, w.npy, w.npz, w.norb, eta=ETA) as ctx:
+        ctx.set_hamiltonian(w)
+        got = ctx.gnp(f1, f2, dE, r["GRLef"], r["GRRgt"], c["GammaL"], c["GammaR"])
+        for k in ("Gnd", "Gpd", "Gnu", "Gnl", "Gpu", "Gpl", "RhoN", "RhoP"):
+            if w.npx == 1 and k[-1] in "ul":
+                continue
+            _close(k, got[k], ref[k])
+        oref = O.observables(w, f1, f2, dE, ref, r["GRRgt"], c["GammaL"], c["GammaR"])
+        ogot = ctx.observables(f1, f2, dE, ref, r["GRRgt"], c["GammaL"], c["GammaR"])
+    nm = max(1.0, abs(oref["Modec1"]))
+    for k in ("T12", "Modec1", "Modec2"):
+        _close(k, ogot[k], oref[k], scale=nm)
+    for k in ("EDensity", "HDensity", "ContactCurrent"):
+        _close(k, ogot[k], oref[k])
+    rho = abs(oref["EDensity"]) * dE
+    for k, rk in (("SigXExp", "SigX"), ("SigYExp", "SigY"), ("SigZExp", "SigZ")):
+        _close(k, ogot[k], oref[rk], scale=rho)
+    jscale = max(np.abs(oref[k]).max() for k in ("Jx", "Jy", "Jz"))
+    for k in ("Jx", "Jy", "Jz"):
+        _close(k, ogot[k], oref[k], rtol=1e-9, scale=jscale)
+
+
+def test_sancho_rubio_entry_matches_oracle():
+    w = build_wire(3, 3, 2, zeeman=0.02)
+    n = w.n
+    for E in (-0.3, 1.1):
+        alpha = (E + 1j * ETA) * np.eye(n) - w.htotd[:, :, 0]
+        for beta in (-w.hopx.conj().T, -w.hopx):
+            ref, st, it = O.sancho_rubio(alpha, beta, ETA, 1e-10, 200)
+            with capi.Context(3, 3, 2, 4, eta=ETA, sr_tol=1e-10, sr_max_iter=200) as ctx:
+                ctx.set_hamiltonian(w)
+                got, st2, it2 = ctx.sancho_rubio(alpha, beta)
+            assert (st, it) == (st2, it2)
+            _close("Sigma", got, ref)
+
+
+def test_sancho_rubio_reports_nonconvergence():
+    w = build_wire(3, 2, 2)
+    with capi.Context(3, 2, 2, 4, eta=ETA, sr_tol=1e-10, sr_max_iter=1) as ctx:
+        ctx.set_hamiltonian(w)
+        res = ctx.contact_self_energy(0.9)
+    assert res["status"] & capi.NEGF_STATUS_SR_NOT_CONVERGED
+    assert res["iters"] == (1, 1)
