@@ -72,9 +72,13 @@ def test_gnp_and_observables_match_oracle(case):
     rho = abs(oref["EDensity"]) * dE
     for k, rk in (("SigXExp", "SigX"), ("SigYExp", "SigY"), ("SigZExp", "SigZ")):
         _close(k, ogot[k], oref[rk], scale=rho)
-    jscale = max(np.abs(oref[k]).max() for k in ("Jx", "Jy", "Jz"))
+    # a bond current is a difference of H*Gn products (Calculate_Observables.F90:119-154)
+    # that cancels to ~0 in a gap: judge it on the scale of those products
+    q, hbar = 1.602e-19, 6.626e-34 / (2 * np.pi)
+    terms = (q * q / hbar) * dE / (2 * np.pi) * w.norb**2 * max(
+        np.abs(w.htotd).max(), np.abs(w.hopx).max()) * max(np.abs(ref["Gnd"]).max(), np.abs(ref["Gpd"]).max())
     for k in ("Jx", "Jy", "Jz"):
-        _close(k, ogot[k], oref[k], rtol=1e-9, scale=jscale)
+        _close(k, ogot[k], oref[k], rtol=1e-10, scale=terms)
 
 
 def test_sancho_rubio_entry_matches_oracle():
