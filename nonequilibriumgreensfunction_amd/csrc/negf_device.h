@@ -70,7 +70,7 @@ struct SpinMats {
 
 // Per-matrix workspace of the batched Gauss-Jordan inverse (gj_inverse.hip).
 struct GjWork {
-  void* prow;       // kNBO x n complex, row-major (row m at prow + m*n)
+  void* prow;       // kGjNBO x n complex, row-major (row m at prow + m*n)
   int* pivstep;     // n: step at which each row became a pivot (n = not yet)
   int* pivrow;      // n: pivot row of each step
   int* info;        // 1: first zero pivot (1-based) or 0
@@ -84,7 +84,7 @@ hipError_t gemm_batched(hipStream_t s, int opa, int opb, const GemmBatch& g, int
 template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
-constexpr int kGjNBO = 64;
+constexpr int kGjNBO = 64;  // Gauss-Jordan outer block width (columns per trailing update)
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
 // every launch is bracketed by HIP events on its own stream and its algorithmic

@@ -27,6 +27,9 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
 
 namespace negf {
 template <typename T>
@@ -132,19 +135,11 @@ struct Layout {
   }
 };
 
-struct negf_ctx {
-  negf_config cfg;
-  int n = 0, L = 0, nsites = 0;
+// One independent stream of energy batches: its own HIP stream and arena.  Two
+// lanes run concurrently (one host thread each), so the latency-bound panel
+// factorizations of one lane overlap the MFMA GEMMs of the other.
+struct Lane {
   hipStream_t st = nullptr;
-  bool have_h = false;
-  bool blockdiag = true;
-  zd* H = nullptr;       // Htotd (n,n,L)
-  zd* X = nullptr;       // HopX
-  zd* betaL = nullptr;   // -HopX'
-  zd* betaR = nullptr;   // -HopX
-  zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
-  SpinMats sp;
-  Layout lay;
   int cap = 0;           // energies per batch
   zd* E = nullptr;       // energy arena (cap * Es)
   zd* C = nullptr;       // chain arena (2cap * Cs)
@@ -155,6 +150,23 @@ struct negf_ctx {
   double* d_site = nullptr;    // cap * site_per_e
   double* d_mc = nullptr;      // 2cap
   int* d_active = nullptr;     // 2cap
+};
+
+struct negf_ctx {
+  negf_config cfg;
+  int n = 0, L = 0, nsites = 0;
+  hipStream_t st = nullptr;  // stream of the stage entry points (= lane 0's)
+  bool have_h = false;
+  bool blockdiag = true;
+  zd* H = nullptr;       // Htotd (n,n,L)
+  zd* X = nullptr;       // HopX
+  zd* betaL = nullptr;   // -HopX'
+  zd* betaR = nullptr;   // -HopX
+  zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
+  SpinMats sp;
+  Layout lay;
+  Lane lanes[2];
+  int nlanes = 2;
   // queue
   std::vector<double> qE, qdE;
   std::vector<int> qIdx;
@@ -193,14 +205,16 @@ static int require_device(int dev) {
 }
 
 static void free_arena(negf_ctx* c) {
-  hipFree(c->E); hipFree(c->C); hipFree(c->Ie); hipFree(c->Ic);
-  hipFree(c->d_energy); hipFree(c->d_scal); hipFree(c->d_site); hipFree(c->d_mc);
-  hipFree(c->d_active);
-  c->E = c->C = nullptr;
-  c->Ie = c->Ic = nullptr;
-  c->d_energy = c->d_scal = c->d_site = c->d_mc = nullptr;
-  c->d_active = nullptr;
-  c->cap = 0;
+  for (Lane& l : c->lanes) {
+    hipFree(l.E); hipFree(l.C); hipFree(l.Ie); hipFree(l.Ic);
+    hipFree(l.d_energy); hipFree(l.d_scal); hipFree(l.d_site); hipFree(l.d_mc);
+    hipFree(l.d_active);
+    l.E = l.C = nullptr;
+    l.Ie = l.Ic = nullptr;
+    l.d_energy = l.d_scal = l.d_site = l.d_mc = nullptr;
+    l.d_active = nullptr;
+    l.cap = 0;
+  }
 }
 
 extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
@@ -233,11 +247,13 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
   c->L = cfg->npx;
   c->nsites = cfg->npy * cfg->npz;
   c->lay.init(n, c->L, c->nsites);
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-    negf_set_error("hipStreamCreate failed");
-    delete c;
-    return NEGF_ERR_HIP;
-  }
+  for (Lane& l : c->lanes)
+    if (hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) != hipSuccess) {
+      negf_set_error("hipStreamCreate failed");
+      delete c;
+      return NEGF_ERR_HIP;
+    }
+  c->st = c->lanes[0].st;
   *out = c;
   return 0;
 }
@@ -245,10 +261,10 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
 extern "C" int negf_finalize(negf_ctx* c) {
   if (!c) return 0;
   hipSetDevice(c->cfg.device);
-  hipStreamSynchronize(c->st);
+  for (Lane& l : c->lanes) hipStreamSynchronize(l.st);
   free_arena(c);
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
-  hipStreamDestroy(c->st);
+  for (Lane& l : c->lanes) hipStreamDestroy(l.st);
   delete c;
   return 0;
 }
@@ -305,35 +321,40 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
   return 0;
 }
 
-// Allocate the batch arena for `want` energies (capped by the HBM budget).
+// Allocate the lanes' arenas for `want` energies in flight in total (capped by the
+// HBM budget); each lane gets half.
 static int ensure_arena(negf_ctx* c, int want) {
-  if (c->cap >= want && c->E) return 0;
+  const int nl = want >= 2 ? c->nlanes : 1;
+  const int per_lane = (want + nl - 1) / nl;
+  if (c->lanes[0].E && c->lanes[0].cap >= per_lane && (nl == 1 || c->lanes[1].E)) return 0;
   const Layout& y = c->lay;
   size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(3 * y.Is) +
                  sizeof(double) * (size_t)(y.site_per_e + 8 + 2) + 2 * sizeof(int);
+  free_arena(c);
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
   double budget = c->cfg.hbm_budget_gb > 0 ? c->cfg.hbm_budget_gb * 1e9 : 0.6 * (double)freeb;
-  if (budget > 0.9 * (double)freeb + (c->E ? 0.0 : 0.0)) budget = 0.9 * (double)freeb;
-  int cap = (int)std::min<double>(want, budget / (double)per_e);
-  if (c->cfg.max_batch > 0) cap = std::min(cap, c->cfg.max_batch);
+  if (budget > 0.9 * (double)freeb) budget = 0.9 * (double)freeb;
+  int cap = (int)std::min<double>(per_lane, budget / nl / (double)per_e);
+  if (c->cfg.max_batch > 0) cap = std::min(cap, std::max(1, c->cfg.max_batch / nl));
   if (cap < 1) {
     negf_set_error("one energy needs %.2f GB of device memory; budget %.2f GB", per_e / 1e9,
                    budget / 1e9);
     return NEGF_ERR_MEMORY;
   }
-  if (c->cap >= cap && c->E) return 0;
-  free_arena(c);
-  HIPCHK(hipMalloc(&c->E, sizeof(zd) * (size_t)y.Es * cap));
-  HIPCHK(hipMalloc(&c->C, sizeof(zd) * (size_t)y.Cs * 2 * cap));
-  HIPCHK(hipMalloc(&c->Ie, sizeof(int) * (size_t)y.Is * cap));
-  HIPCHK(hipMalloc(&c->Ic, sizeof(int) * (size_t)y.Is * 2 * cap));
-  HIPCHK(hipMalloc(&c->d_energy, sizeof(double) * cap));
-  HIPCHK(hipMalloc(&c->d_scal, sizeof(double) * 8 * cap));
-  HIPCHK(hipMalloc(&c->d_site, sizeof(double) * (size_t)y.site_per_e * cap));
-  HIPCHK(hipMalloc(&c->d_mc, sizeof(double) * 2 * cap));
-  HIPCHK(hipMalloc(&c->d_active, sizeof(int) * 2 * cap));
-  c->cap = cap;
+  for (int li = 0; li < nl; ++li) {
+    Lane& l = c->lanes[li];
+    HIPCHK(hipMalloc(&l.E, sizeof(zd) * (size_t)y.Es * cap));
+    HIPCHK(hipMalloc(&l.C, sizeof(zd) * (size_t)y.Cs * 2 * cap));
+    HIPCHK(hipMalloc(&l.Ie, sizeof(int) * (size_t)y.Is * cap));
+    HIPCHK(hipMalloc(&l.Ic, sizeof(int) * (size_t)y.Is * 2 * cap));
+    HIPCHK(hipMalloc(&l.d_energy, sizeof(double) * cap));
+    HIPCHK(hipMalloc(&l.d_scal, sizeof(double) * 8 * cap));
+    HIPCHK(hipMalloc(&l.d_site, sizeof(double) * (size_t)y.site_per_e * cap));
+    HIPCHK(hipMalloc(&l.d_mc, sizeof(double) * 2 * cap));
+    HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 2 * cap));
+    l.cap = cap;
+  }
   return 0;
 }
 
@@ -375,21 +396,21 @@ static GjWork gjw(zd* prow_base, long long sProw, int* ints, long long sInt, int
 // chains [0, nl) use betaA/betadA, [nl, nchains) use betaB/betadB).  Expects
 // chain alpha and work (= seed matrix) filled.  Writes Sigma for chain c to
 // sig0 + c*sSig (c < nl) or sig1 + (c-nl)*sSig.  iters/status per chain.
-static int run_sr(negf_ctx* c, int nl, int nr, const zd* betaA, const zd* betadA, const zd* betaB,
+static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const zd* betadA, const zd* betaB,
                   const zd* betadB, zd* sig0, zd* sig1, long long sSig, std::vector<int>& iters,
                   std::vector<int>& status) {
   const int n = c->n;
   const Layout& y = c->lay;
   const int nc = nl + nr;
   const long long Cs = y.Cs, nn = y.nn;
-  hipStream_t s = c->st;
-  zd* C0 = c->C;
+  hipStream_t s = ln.st;
+  zd* C0 = ln.C;
   auto P = [&](long long off) { return C0 + off; };
-  GjWork w = gjw(P(y.c_prow), Cs, c->Ic, y.Is, n);
+  GjWork w = gjw(P(y.c_prow), Cs, ln.Ic, y.Is, n);
   iters.assign(nc, 0);
   status.assign(nc, 0);
   // clear INFO of every chain slot
-  TRY(hipMemsetAsync(c->Ic, 0, sizeof(int) * (size_t)y.Is * nc, s));
+  TRY(hipMemsetAsync(ln.Ic, 0, sizeof(int) * (size_t)y.Is * nc, s));
   // tmp = inv(seed)   (Contact_Self_Energy.F90:141)
   TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nc, w, nullptr));
   // t = -tmp*betad, tt = -tmp*beta   (:147, :149)
@@ -413,7 +434,7 @@ static int run_sr(negf_ctx* c, int nl, int nr, const zd* betaA, const zd* betadA
   while (!act.empty() && it < c->cfg.sr_max_iter) {  // (:154)
     ++it;
     const int na = (int)act.size();
-    int* A = c->d_active;
+    int* A = ln.d_active;
     TRY(hipMemcpyAsync(A, act.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
     // W1 = Toldt*tt   (:157)
     TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs,
@@ -439,10 +460,10 @@ static int run_sr(negf_ctx* c, int nl, int nr, const zd* betaA, const zd* betadA
     TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
                                                n, A), na, kOne, kZero));
     TRY(launch_add<double>(s, P(y.c_CapT), Cs, P(y.c_CapT), Cs, P(y.c_W2), Cs, nn, na, A));
-    TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, c->d_mc, na, A));
+    TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, ln.d_mc, na, A));
     TRY(launch_copy3<double>(s, P(y.c_Toldt), P(y.c_W1), P(y.c_t), P(y.c_W3), P(y.c_tt), P(y.c_W4),
                              Cs, nn, na, A));
-    TRY(hipMemcpyAsync(mc.data(), c->d_mc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    TRY(hipMemcpyAsync(mc.data(), ln.d_mc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     TRY(hipStreamSynchronize(s));
     std::vector<int> keep;
     for (int ch : act) {
@@ -481,7 +502,7 @@ static int run_sr(negf_ctx* c, int nl, int nr, const zd* betaA, const zd* betadA
   // LAPACK INFO of the chain inverses
   std::vector<int> info(nc);
   for (int ch = 0; ch < nc; ++ch)
-    TRY(hipMemcpyAsync(&info[ch], c->Ic + (size_t)ch * y.Is + 2 * n, sizeof(int),
+    TRY(hipMemcpyAsync(&info[ch], ln.Ic + (size_t)ch * y.Is + 2 * n, sizeof(int),
                        hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
   for (int ch = 0; ch < nc; ++ch)
@@ -498,30 +519,30 @@ struct EnergyResult {
 };
 
 // Fused solve of nb energies (nb <= cap) resident in the arena.
-static int run_batch(negf_ctx* c, int nb, const double* energies, std::vector<EnergyResult>& res) {
+static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std::vector<EnergyResult>& res) {
   const int n = c->n, L = c->L, no = c->cfg.norb, ns = c->nsites;
   const Layout& y = c->lay;
   const long long Es = y.Es, nn = y.nn;
-  hipStream_t s = c->st;
-  zd* E0 = c->E;
+  hipStream_t s = ln.st;
+  zd* E0 = ln.E;
   auto Q = [&](long long off) { return E0 + off; };
-  TRY(hipMemcpyAsync(c->d_energy, energies, sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  TRY(hipMemsetAsync(c->d_scal, 0, sizeof(double) * 8 * nb, s));
-  TRY(hipMemsetAsync(c->Ie, 0, sizeof(int) * (size_t)y.Is * nb, s));
+  TRY(hipMemcpyAsync(ln.d_energy, energies, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  TRY(hipMemsetAsync(ln.d_scal, 0, sizeof(double) * 8 * nb, s));
+  TRY(hipMemsetAsync(ln.Ie, 0, sizeof(int) * (size_t)y.Is * nb, s));
   // ---- contacts: chains [0,nb) left lead, [nb,2nb) right lead
   const zd* Hend0 = c->H;
   const zd* Hend1 = c->H + (size_t)(L - 1) * nn;
-  TRY(launch_sr_alpha<double>(s, c->C + y.c_alpha, c->C + y.c_work, y.Cs, Hend0, Hend1,
-                              c->d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
+  TRY(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, Hend0, Hend1,
+                              ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
   std::vector<int> iters, sst;
   // left: beta = -HopX', betad = -HopX;  right: beta = -HopX, betad = -HopX'
-  int rc = run_sr(c, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL), Q(y.o_SigR), Es,
+  int rc = run_sr(c, ln, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL), Q(y.o_SigR), Es,
                   iters, sst);
   if (rc) return rc;
-  TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, c->d_scal + 4, 8));
-  TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, c->d_scal + 5, 8));
+  TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
+  TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
 
-  GjWork w = gjw(Q(y.o_prow), Es, c->Ie, y.Is, n);
+  GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
   const bool bd = c->blockdiag;
   auto slot = [&](int x) { return Q(y.o_slots + (long long)x * nn); };
   auto hx = [&](int x) { return (const zd*)(c->H + (size_t)x * nn); };
@@ -540,11 +561,11 @@ static int run_batch(negf_ctx* c, int nb, const double* energies, std::vector<En
         TRY(gemm_batched<double>(s, OP_N, OP_N, gb(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
         TRY(gemm_batched<double>(s, OP_N, OP_C, gb(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
       }
-      TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, c->d_energy, c->cfg.eta, S1,
+      TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy, c->cfg.eta, S1,
                                   Es, S2, Es, T2, Es, c->X, n, nb, nullptr));
       return 0;
     }
-    TRY(launch_assemble<double>(s, no, g ? mode : 0, Q(y.o_M), Es, hx(x), 1, -1.0, c->d_energy,
+    TRY(launch_assemble<double>(s, no, g ? mode : 0, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy,
                                 c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr));
     return 0;
   };
@@ -563,7 +584,7 @@ static int run_batch(negf_ctx* c, int nb, const double* energies, std::vector<En
     const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
     TRY(gemm_batched<double>(s, OP_N, OP_N, gb(G, Es, n, Gam, Es, n, Q(y.o_Bm), Es, n, n), nb, kOne,
                              kZero));
-    double* out = c->d_site + ((long long)lead * 4 * L + x) * ns;
+    double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
     TRY(launch_siteblocks<double>(s, no, Q(y.o_Bm), Es, G, Es, n, out, y.site_per_e,
                                   (long long)L * ns, c->sp, nb, nullptr));
     return 0;
@@ -573,7 +594,7 @@ static int run_batch(negf_ctx* c, int nb, const double* energies, std::vector<En
     const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
     TRY(gemm_batched<double>(s, OP_N, OP_N, gb(Gam, Es, n, Q(y.o_Bm), Es, n, Q(y.o_P), Es, n, n), nb,
                              kOne, kZero));
-    TRY(launch_inner<double>(s, Q(y.o_P), Es, G, Es, n, c->d_scal + idx, 8, nb, nullptr));
+    TRY(launch_inner<double>(s, Q(y.o_P), Es, G, Es, n, ln.d_scal + idx, 8, nb, nullptr));
     return 0;
   };
   int r;
@@ -621,9 +642,9 @@ static int run_batch(negf_ctx* c, int nb, const double* energies, std::vector<En
   std::vector<double> scal(8 * nb);
   std::vector<double> site((size_t)y.site_per_e * nb);
   std::vector<int> ints((size_t)y.Is * nb);
-  TRY(hipMemcpyAsync(scal.data(), c->d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
-  TRY(hipMemcpyAsync(site.data(), c->d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
-  TRY(hipMemcpyAsync(ints.data(), c->Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(scal.data(), ln.d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(site.data(), ln.d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(ints.data(), ln.Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
   for (int b = 0; b < nb; ++b) {
     EnergyResult& e = res[b];
@@ -708,25 +729,57 @@ extern "C" int negf_flush(negf_ctx* c, const double* mu, double muT, negf_accum*
   HIPCHK(hipSetDevice(c->cfg.device));
   const int total = (int)c->qE.size();
   if (total == 0) return 0;
-  if ((rc = ensure_arena(c, total))) return rc;
-  int first_bad = 0;
-  std::vector<EnergyResult> res;
-  for (int i0 = 0; i0 < total; i0 += c->cap) {
-    const int nb = std::min(c->cap, total - i0);
-    if ((rc = run_batch(c, nb, &c->qE[i0], res))) {
-      c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
-      return rc;
-    }
-    for (int b = 0; b < nb; ++b) {
-      const int i = i0 + b;
-      accumulate(c, res[b], c->qE[i], c->qIdx[i], c->qdE[i], mu, muT, acc);
-      if (status) status[i] = res[b].status;
-      if (sr_iters) {
-        sr_iters[2 * i] = res[b].iters[0];
-        sr_iters[2 * i + 1] = res[b].iters[1];
+  if ((rc = ensure_arena(c, total))) {
+    c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
+    return rc;
+  }
+  // chunks of one lane's capacity, taken in order by whichever lane is free
+  const int cap = c->lanes[0].cap;
+  const int nchunks = (total + cap - 1) / cap;
+  const int nl = (c->lanes[1].E && nchunks > 1) ? 2 : 1;
+  std::vector<std::vector<EnergyResult>> res(nchunks);
+  std::vector<int> chunk_rc(nchunks, 0);
+  std::atomic<int> next(0);
+  std::vector<std::string> errs(nl);
+  auto worker = [&](int li) {
+    hipSetDevice(c->cfg.device);
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= nchunks) break;
+      const int i0 = k * cap, nb = std::min(cap, total - i0);
+      chunk_rc[k] = run_batch(c, c->lanes[li], nb, &c->qE[i0], res[k]);
+      if (chunk_rc[k]) {
+        errs[li] = negf_last_error();
+        next.store(nchunks);
+        break;
       }
-      if (res[b].status && !first_bad) first_bad = res[b].status;
     }
+  };
+  if (nl == 2) {
+    std::thread t1(worker, 1);
+    worker(0);
+    t1.join();
+  } else {
+    worker(0);
+  }
+  for (int k = 0; k < nchunks; ++k)
+    if (chunk_rc[k]) {
+      for (const std::string& e : errs)
+        if (!e.empty()) negf_set_error("%s", e.c_str());
+      c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
+      return chunk_rc[k];
+    }
+  // accumulate in submission order: the sums do not depend on lane scheduling
+  int first_bad = 0;
+  for (int i = 0; i < total; ++i) {
+    const EnergyResult& r = res[i / cap][i % cap];
+    accumulate(c, r, c->qE[i], c->qIdx[i], c->qdE[i], mu, muT, acc);
+    if (status) status[i] = r.status;
+    if (sr_iters) {
+      sr_iters[2 * i] = r.iters[0];
+      sr_iters[2 * i + 1] = r.iters[1];
+    }
+    if (r.status && !first_bad) first_bad = r.status;
   }
   c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
   return first_bad;
@@ -750,26 +803,27 @@ extern "C" int negf_contact_self_energy(negf_ctx* c, double energy, double* Sigm
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->cfg.device));
   if ((rc = ensure_arena(c, 1))) return rc;
+  Lane& ln = c->lanes[0];
   const int n = c->n, L = c->L;
   const Layout& y = c->lay;
   const long long nn = y.nn;
   hipStream_t s = c->st;
-  HIPCHK(hipMemcpyAsync(c->d_energy, &energy, sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(launch_sr_alpha<double>(s, c->C + y.c_alpha, c->C + y.c_work, y.Cs, c->H,
-                                 c->H + (size_t)(L - 1) * nn, c->d_energy, 1, c->cfg.eta,
+  HIPCHK(hipMemcpyAsync(ln.d_energy, &energy, sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, c->H,
+                                 c->H + (size_t)(L - 1) * nn, ln.d_energy, 1, c->cfg.eta,
                                  c->cfg.sr_extra_eta, n));
   std::vector<int> it, st;
-  zd* SL = c->E + y.o_SigL;
-  zd* SR = c->E + y.o_SigR;
-  if ((rc = run_sr(c, 1, 1, c->betaL, c->betaR, c->betaR, c->betaL, SL, SR, y.Es, it, st))) return rc;
-  HIPCHK(launch_gamma_dos<double>(s, SL, y.Es, c->E + y.o_GamL, y.Es, n, 1, c->d_mc, 1));
-  HIPCHK(launch_gamma_dos<double>(s, SR, y.Es, c->E + y.o_GamR, y.Es, n, 1, c->d_mc + 1, 1));
+  zd* SL = ln.E + y.o_SigL;
+  zd* SR = ln.E + y.o_SigR;
+  if ((rc = run_sr(c, ln, 1, 1, c->betaL, c->betaR, c->betaR, c->betaL, SL, SR, y.Es, it, st))) return rc;
+  HIPCHK(launch_gamma_dos<double>(s, SL, y.Es, ln.E + y.o_GamL, y.Es, n, 1, ln.d_mc, 1));
+  HIPCHK(launch_gamma_dos<double>(s, SR, y.Es, ln.E + y.o_GamR, y.Es, n, 1, ln.d_mc + 1, 1));
   double d[2];
-  HIPCHK(hipMemcpyAsync(d, c->d_mc, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(d, ln.d_mc, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
   if (SigmaL) HIPCHK(hipMemcpyAsync(SigmaL, SL, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
   if (SigmaR) HIPCHK(hipMemcpyAsync(SigmaR, SR, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
-  if (GammaL) HIPCHK(hipMemcpyAsync(GammaL, c->E + y.o_GamL, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
-  if (GammaR) HIPCHK(hipMemcpyAsync(GammaR, c->E + y.o_GamR, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  if (GammaL) HIPCHK(hipMemcpyAsync(GammaL, ln.E + y.o_GamL, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
+  if (GammaR) HIPCHK(hipMemcpyAsync(GammaR, ln.E + y.o_GamR, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (dos) { dos[0] = d[0]; dos[1] = d[1]; }
   if (iters) { iters[0] = it[0]; iters[1] = it[1]; }
@@ -785,14 +839,15 @@ extern "C" int negf_sancho_rubio(negf_ctx* c, const double* alpha, const double*
   HIPCHK(hipSetDevice(c->cfg.device));
   int rc = ensure_arena(c, 1);
   if (rc) return rc;
+  Lane& ln = c->lanes[0];
   const int n = c->n;
   const Layout& y = c->lay;
   const long long nn = y.nn;
   hipStream_t s = c->st;
   // beta and beta' go into the (unused) energy-slot region
-  zd* dB = c->E + y.o_slots;
-  zd* dBd = c->E + y.o_M;
-  zd* sig = c->E + y.o_SigL;
+  zd* dB = ln.E + y.o_slots;
+  zd* dBd = ln.E + y.o_M;
+  zd* sig = ln.E + y.o_SigL;
   std::vector<zd> bd(nn), w(nn);
   const zd* bh = (const zd*)beta;
   const zd* ah = (const zd*)alpha;
@@ -806,10 +861,10 @@ extern "C" int negf_sancho_rubio(negf_ctx* c, const double* alpha, const double*
     for (int i = 0; i < n; ++i) w[(long long)i * n + i].y += c->cfg.eta;
   HIPCHK(hipMemcpyAsync(dB, beta, sizeof(zd) * nn, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(dBd, bd.data(), sizeof(zd) * nn, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->C + y.c_alpha, alpha, sizeof(zd) * nn, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->C + y.c_work, w.data(), sizeof(zd) * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ln.C + y.c_alpha, alpha, sizeof(zd) * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ln.C + y.c_work, w.data(), sizeof(zd) * nn, hipMemcpyHostToDevice, s));
   std::vector<int> it, st;
-  if ((rc = run_sr(c, 1, 0, dB, dBd, dB, dBd, sig, sig, y.Es, it, st))) return rc;
+  if ((rc = run_sr(c, ln, 1, 0, dB, dBd, dB, dBd, sig, sig, y.Es, it, st))) return rc;
   HIPCHK(hipMemcpyAsync(Sigma, sig, sizeof(zd) * nn, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (iters) *iters = it[0];

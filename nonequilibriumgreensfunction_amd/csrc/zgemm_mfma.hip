@@ -18,6 +18,7 @@
 #include "negf_device.h"
 
 #include <cstdlib>
+#include <mutex>
 
 namespace negf {
 
@@ -42,7 +43,7 @@ template <> struct Mfma<float> {
   }
 };
 
-constexpr int kBM = 64, kBN = 64, kBK = 16, kThreads = 256;
+constexpr int kBK = 16, kThreads = 256;
 constexpr int kPad = 4;
 
 template <typename T, int OP>
@@ -63,117 +64,161 @@ __device__ inline int xcd_tile(int bid, int ntiles) {
   return (bid & 7) * per + (bid >> 3);
 }
 
-template <typename T, int OPA, int OPB, int EPI>
+// Workgroup tile (32 SM) x (32 SN): 2 x 2 waves, each owning SM x SN MFMA blocks of
+// 16 x 16.  K advances in slabs of 16 through a two-stage LDS ring: the next slab
+// is fetched into registers and written to the idle stage while the current
+// stage feeds the MFMAs, so each slab costs one barrier.
+// PRE = 1 (Gauss-Jordan epilogue with short K): the old C values and pivot flags
+// are loaded before the K loop, so their latency overlaps the A/B traffic.
+template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int PRE>
 __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int ntiles,
                                                    cx<T> alpha, cx<T> beta) {
   typedef typename Mfma<T>::acc acc_t;
+  constexpr int BM = 32 * SM, BN = 32 * SN;
+  constexpr int LA = BM * kBK / kThreads, LB = BN * kBK / kThreads;
   const int b = g.active ? g.active[blockIdx.z] : (int)blockIdx.z;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   if (tile >= ntiles) return;
-  const int m0 = (tile % tiles_m) * kBM;
-  const int n0 = (tile / tiles_m) * kBN;
+  const int m0 = (tile % tiles_m) * BM;
+  const int n0 = (tile / tiles_m) * BN;
   if (m0 >= g.m || n0 >= g.n) return;
-  if (EPI == EPI_GJ && n0 >= g.skip_lo && n0 + kBN <= g.skip_hi) return;
+  if (EPI == EPI_GJ && n0 >= g.skip_lo && n0 + BN <= g.skip_hi) return;
 
   const cx<T>* A = (const cx<T>*)g.A + (long long)b * g.sA;
   const cx<T>* B = (const cx<T>*)g.B + (long long)b * g.sB;
   cx<T>* C = (cx<T>*)g.C + (long long)b * g.sC;
   const int K = g.k, M = g.m, N = g.n;
 
-  __shared__ T sA[2][kBK][kBM + kPad];
-  __shared__ T sB[2][kBK][kBN + kPad];
+  __shared__ T sA[2][2][kBK][BM + kPad];  // [stage][re/im][k][m]
+  __shared__ T sB[2][2][kBK][BN + kPad];  // [stage][re/im][k][n]
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * 16 * SM, wn = (wave >> 1) * 16 * SN;
   const int lr = lane & 15, lk = lane >> 4;
 
-  acc_t cr[2][2], ci[2][2];
+  acc_t cr[SM][SN], ci[SM][SN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < SM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < SN; ++j) {
       cr[i][j] = acc_t{0, 0, 0, 0};
       ci[i][j] = acc_t{0, 0, 0, 0};
     }
 
-  cx<T> ra[4], rb[4];
+  const int* piv = (EPI == EPI_GJ) ? g.pivstep + (long long)b * g.sPiv : nullptr;
+  cx<T> cpre[PRE ? SM : 1][PRE ? SN : 1][PRE ? 4 : 1];
+  bool kpre[PRE ? SM : 1][PRE ? 4 : 1];
+  if (PRE) {
+#pragma unroll
+    for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + 16 * sm + Mfma<T>::row(lk, r);
+        const int ps = row < M ? piv[row] : 0;
+        kpre[sm][r] = !(ps >= g.pv_lo && ps < g.pv_hi);
+#pragma unroll
+        for (int sn = 0; sn < SN; ++sn) {
+          const int col = n0 + wn + 16 * sn + lr;
+          const bool ok = row < M && col < N && !(col >= g.skip_lo && col < g.skip_hi);
+          cpre[sm][sn][r] = ok ? C[(long long)col * g.ldc + row] : mk<T>(0, 0);
+        }
+      }
+  }
+
+  cx<T> ra[LA], rb[LB];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int c = tid + kThreads * i;
+    for (int i = 0; i < LA; ++i) {
+      const int c = tid + kThreads * i;
       if (OPA == OP_N) {
-        int mm = c & 63, kk = c >> 6;
-        int r = m0 + mm, q = k0 + kk;
+        const int mm = c % BM, kk = c / BM, r = m0 + mm, q = k0 + kk;
         ra[i] = load_op<T, OPA>(A + (long long)q * g.lda + r, r < M && q < K);
       } else {
-        int kk = c & 15, mm = c >> 4;
-        int r = m0 + mm, q = k0 + kk;
+        const int kk = c % kBK, mm = c / kBK, r = m0 + mm, q = k0 + kk;
         ra[i] = load_op<T, OPA>(A + (long long)r * g.lda + q, r < M && q < K);
       }
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int c = tid + kThreads * i;
       if (OPB == OP_N) {
-        int kk = c & 15, nn = c >> 4;
-        int q = k0 + kk, col = n0 + nn;
+        const int kk = c % kBK, nn = c / kBK, q = k0 + kk, col = n0 + nn;
         rb[i] = load_op<T, OPB>(B + (long long)col * g.ldb + q, q < K && col < N);
       } else {
-        int nn = c & 63, kk = c >> 6;
-        int q = k0 + kk, col = n0 + nn;
+        const int nn = c % BN, kk = c / BN, q = k0 + kk, col = n0 + nn;
         rb[i] = load_op<T, OPB>(B + (long long)q * g.ldb + col, q < K && col < N);
       }
     }
   };
-  auto stash = [&]() {
+  auto stash = [&](int st) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int c = tid + kThreads * i;
+    for (int i = 0; i < LA; ++i) {
+      const int c = tid + kThreads * i;
       int mm, kk;
-      if (OPA == OP_N) { mm = c & 63; kk = c >> 6; } else { kk = c & 15; mm = c >> 4; }
-      sA[0][kk][mm] = ra[i].x;
-      sA[1][kk][mm] = ra[i].y;
-      int nn;
-      if (OPB == OP_N) { kk = c & 15; nn = c >> 4; } else { nn = c & 63; kk = c >> 6; }
-      sB[0][kk][nn] = rb[i].x;
-      sB[1][kk][nn] = rb[i].y;
+      if (OPA == OP_N) { mm = c % BM; kk = c / BM; } else { kk = c % kBK; mm = c / kBK; }
+      sA[st][0][kk][mm] = ra[i].x;
+      sA[st][1][kk][mm] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int c = tid + kThreads * i;
+      int nn, kk;
+      if (OPB == OP_N) { kk = c % kBK; nn = c / kBK; } else { nn = c % BN; kk = c / BN; }
+      sB[st][0][kk][nn] = rb[i].x;
+      sB[st][1][kk][nn] = rb[i].y;
     }
   };
 
-  fetch(0);
-  for (int k0 = 0; k0 < K; k0 += kBK) {
-    stash();
-    __syncthreads();
-    if (k0 + kBK < K) fetch(k0 + kBK);
+  const int nk = (K + kBK - 1) / kBK;
+  if (nk > 0) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int st = kt & 1;
+    if (kt + 1 < nk) fetch((kt + 1) * kBK);
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
       const int kk = ks * 4 + lk;
-      T ar[2], ai[2], nai[2], br[2], bi[2];
+      T ar[SM], ai[SM], nai[SM], br[SN], bi[SN];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        ar[s] = sA[0][kk][wm + 16 * s + lr];
-        ai[s] = sA[1][kk][wm + 16 * s + lr];
+      for (int s = 0; s < SM; ++s) {
+        ar[s] = sA[st][0][kk][wm + 16 * s + lr];
+        ai[s] = sA[st][1][kk][wm + 16 * s + lr];
         nai[s] = -ai[s];
-        br[s] = sB[0][kk][wn + 16 * s + lr];
-        bi[s] = sB[1][kk][wn + 16 * s + lr];
       }
 #pragma unroll
-      for (int sm = 0; sm < 2; ++sm)
+      for (int s = 0; s < SN; ++s) {
+        br[s] = sB[st][0][kk][wn + 16 * s + lr];
+        bi[s] = sB[st][1][kk][wn + 16 * s + lr];
+      }
 #pragma unroll
-        for (int sn = 0; sn < 2; ++sn) {
+      for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+        for (int sn = 0; sn < SN; ++sn) {
           cr[sm][sn] = Mfma<T>::mma(ar[sm], br[sn], cr[sm][sn]);
           ci[sm][sn] = Mfma<T>::mma(ar[sm], bi[sn], ci[sm][sn]);
+        }
+#pragma unroll
+      for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+        for (int sn = 0; sn < SN; ++sn) {
           cr[sm][sn] = Mfma<T>::mma(nai[sm], bi[sn], cr[sm][sn]);
           ci[sm][sn] = Mfma<T>::mma(ai[sm], br[sn], ci[sm][sn]);
         }
     }
+    if (kt + 1 < nk) stash(st ^ 1);
     __syncthreads();
   }
 
   // Epilogue: accumulator element r of lane (lk, lr) is C[Mfma<T>::row(lk, r)][lr]
   // of its 16x16 block.
-  const int* piv = (EPI == EPI_GJ) ? g.pivstep + (long long)b * g.sPiv : nullptr;
 #pragma unroll
-  for (int sm = 0; sm < 2; ++sm)
+  for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
-    for (int sn = 0; sn < 2; ++sn) {
+    for (int sn = 0; sn < SN; ++sn) {
       const int col = n0 + wn + 16 * sn + lr;
       if (col >= N) continue;
       if (EPI == EPI_GJ && col >= g.skip_lo && col < g.skip_hi) continue;
@@ -183,7 +228,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
         if (row >= M) continue;
         cx<T>* pc = C + (long long)col * g.ldc + row;
         cx<T> v = mk<T>(cr[sm][sn][r], ci[sm][sn][r]);
-        if (EPI == EPI_GJ) {
+        if (EPI == EPI_GJ && PRE) {
+          if (kpre[sm][r]) v = cadd(v, cpre[sm][sn][r]);
+        } else if (EPI == EPI_GJ) {
           const int ps = piv[row];
           if (!(ps >= g.pv_lo && ps < g.pv_hi)) v = cadd(v, *pc);
         } else {
@@ -195,40 +242,78 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
     }
 }
 
+// Tile configurations (SM, SN): 0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128.
+static int g_tile_std = -1;  // -1: choose by shape
+static int g_tile_gj = -1;
+void gemm_set_tiles(int std_cfg, int gj_cfg) {
+  g_tile_std = std_cfg;
+  g_tile_gj = gj_cfg;
+}
+
 GemmTimer& gemm_timer() {
   static GemmTimer t;
   return t;
 }
 
-static hipEvent_t* timer_slot(double flops) {
+static std::mutex g_timer_mu;
+
+// Reserves an event pair for one launch; the handles are copied out under the lock
+// (the event array may grow while another host thread launches).
+static bool timer_slot(double flops, hipEvent_t* e0, hipEvent_t* e1) {
   GemmTimer& t = gemm_timer();
-  if (!t.on) return nullptr;
+  if (!t.on) return false;
+  std::lock_guard<std::mutex> lock(g_timer_mu);
   if (t.used == t.cap) {
     size_t nc = t.cap ? 2 * t.cap : 4096;
     hipEvent_t* ev = (hipEvent_t*)realloc(t.ev, sizeof(hipEvent_t) * 2 * nc);
-    double* fl = (double*)realloc(t.flops, sizeof(double) * nc);
-    if (!ev || !fl) return nullptr;
-    for (size_t i = 2 * t.cap; i < 2 * nc; ++i) hipEventCreate(&ev[i]);
+    if (!ev) return false;
     t.ev = ev;
+    double* fl = (double*)realloc(t.flops, sizeof(double) * nc);
+    if (!fl) return false;
     t.flops = fl;
+    for (size_t i = 2 * t.cap; i < 2 * nc; ++i) hipEventCreate(&ev[i]);
     t.cap = nc;
   }
   t.flops[t.used] = flops;
-  return &t.ev[2 * t.used++];
+  *e0 = t.ev[2 * t.used];
+  *e1 = t.ev[2 * t.used + 1];
+  ++t.used;
+  return true;
+}
+
+template <typename T, int OPA, int OPB, int EPI, int SM, int SN>
+static hipError_t launch_cfg(hipStream_t s, const GemmBatch& g, int batch, cx<T> alpha, cx<T> beta) {
+  constexpr int BM = 32 * SM, BN = 32 * SN;
+  int tm = (g.m + BM - 1) / BM, tn = (g.n + BN - 1) / BN;
+  int nt = tm * tn;
+  dim3 grid(((nt + 7) / 8) * 8, 1, batch);
+  if (EPI == EPI_GJ && g.k <= 32)
+    hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 1>), grid, dim3(kThreads), 0, s, g, tm, nt,
+                       alpha, beta);
+  else
+    hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 0>), grid, dim3(kThreads), 0, s, g, tm, nt,
+                       alpha, beta);
+  return hipGetLastError();
 }
 
 template <typename T, int OPA, int OPB, int EPI>
 static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alpha, cx<T> beta) {
   if (batch <= 0 || g.m <= 0 || g.n <= 0) return hipSuccess;
-  int tm = (g.m + kBM - 1) / kBM, tn = (g.n + kBN - 1) / kBN;
-  int nt = tm * tn;
-  dim3 grid(((nt + 7) / 8) * 8, 1, batch);
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
-  hipEvent_t* ev = timer_slot(8.0 * g.m * (double)ncols * g.k * batch);
-  if (ev) hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI>), grid, dim3(kThreads), 0, s, g, tm, nt, alpha, beta);
-  if (ev) hipEventRecord(ev[1], s);
-  return hipGetLastError();
+  hipEvent_t ev0, ev1;
+  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, &ev0, &ev1);
+  if (timed) hipEventRecord(ev0, s);
+  int cfg = (EPI == EPI_GJ) ? g_tile_gj : g_tile_std;
+  if (cfg < 0) cfg = 0;
+  hipError_t e;
+  switch (cfg) {
+    case 1: e = launch_cfg<T, OPA, OPB, EPI, 4, 2>(s, g, batch, alpha, beta); break;
+    case 2: e = launch_cfg<T, OPA, OPB, EPI, 2, 4>(s, g, batch, alpha, beta); break;
+    case 3: e = launch_cfg<T, OPA, OPB, EPI, 4, 4>(s, g, batch, alpha, beta); break;
+    default: e = launch_cfg<T, OPA, OPB, EPI, 2, 2>(s, g, batch, alpha, beta); break;
+  }
+  if (timed) hipEventRecord(ev1, s);
+  return e;
 }
 
 template <typename T>
