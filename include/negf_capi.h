@@ -136,6 +136,29 @@ int negf_observables(negf_ctx* ctx, int eCtr, double f1, double f2, double delta
                      const double* GammaL, const double* GammaR, negf_accum* acc, double* Jx,
                      double* Jy, double* Jz);
 
+/* --- Fortran-friendly forms ------------------------------------------------
+ * The same operations with the accumulators passed as separate arrays, so a
+ * Fortran caller can pass its (non-TARGET) module arrays by reference.  Bit k of
+ * `mask` marks the k-th array argument present, in negf_accum's field order
+ * (0 T12 ... 12 ContactCurrent); for negf_observables_f bits 13-15 mark Jx, Jy, Jz. */
+int negf_energy_sweep_f(negf_ctx* ctx, double energy, int eCtr, double deltaE, const double* mu,
+                        double muT, int mask, double* T12, double* Modec1, double* Modec2,
+                        double* DOSc1, double* DOSc2, double* EDensity, double* HDensity,
+                        double* RhoN, double* RhoP, double* SigXExp, double* SigYExp,
+                        double* SigZExp, double* ContactCurrent);
+int negf_flush_f(negf_ctx* ctx, const double* mu, double muT, int mask, double* T12,
+                 double* Modec1, double* Modec2, double* DOSc1, double* DOSc2, double* EDensity,
+                 double* HDensity, double* RhoN, double* RhoP, double* SigXExp, double* SigYExp,
+                 double* SigZExp, double* ContactCurrent);
+int negf_observables_f(negf_ctx* ctx, int eCtr, double f1, double f2, double deltaE,
+                       const double* Gnd, const double* Gpd, const double* Gnu, const double* Gnl,
+                       const double* Gpu, const double* Gpl, const double* GRRgt,
+                       const double* GammaL, const double* GammaR, int mask, double* T12,
+                       double* Modec1, double* Modec2, double* DOSc1, double* DOSc2,
+                       double* EDensity, double* HDensity, double* RhoN, double* RhoP,
+                       double* SigXExp, double* SigYExp, double* SigZExp, double* ContactCurrent,
+                       double* Jx, double* Jy, double* Jz);
+
 /* --- L1 primitives --------------------------------------------------------- */
 
 /* Invert_Matrix (MKL_Routines_Cluster.F90:2-33): invG = G^-1 for one n x n

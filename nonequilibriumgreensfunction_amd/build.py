@@ -67,18 +67,20 @@ def build_fortran() -> str | None:
     moddir = os.path.join(LIBDIR, "mod")
     os.makedirs(moddir, exist_ok=True)
     objs = []
-    for src in ("negf_capi.f90", "negf_state.f90", "negf_shims.f90"):
+    for src in ("negf_capi.f90", "negf_state.f90", "negf_shims.F90"):
         path = os.path.join(FSRC, src)
-        if not os.path.exists(path):
-            continue
-        obj = os.path.join(OBJDIR, src.replace(".f90", ".o"))
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
         _run([fc, "-O2", "-fPIC", "-module-dir", moddir, "-I" + moddir, "-c", path, "-o", obj])
         objs.append(obj)
-    if not objs:
-        return None
     out = os.path.join(LIBDIR, "libnegf_fortran.so")
     _run([fc, "-shared", "-fPIC", "-o", out] + objs +
          ["-L" + LIBDIR, "-Wl,-rpath,$ORIGIN", "-lnegf_hip"])
+    # host driver used by the tests: a Fortran program calling the shims
+    drv = os.path.join(ROOT, "tests", "fortran", "negf_fdriver.f90")
+    if os.path.exists(drv):
+        _run([fc, "-O2", "-module-dir", moddir, "-I" + moddir, drv, "-o",
+              os.path.join(LIBDIR, "negf_fdriver"), "-L" + LIBDIR, "-Wl,-rpath,$ORIGIN",
+              "-lnegf_fortran", "-lnegf_hip"])
     return out
 
 

@@ -32,7 +32,8 @@ EXPORTS = (
     "negf_last_error", "negf_device_count", "negf_init", "negf_finalize", "negf_set_hamiltonian",
     "negf_energy_sweep", "negf_submit", "negf_flush", "negf_contact_self_energy",
     "negf_sancho_rubio", "negf_retarded", "negf_gnp", "negf_observables", "negf_invert_matrix",
-    "negf_zgemm", "negf_profile_begin", "negf_profile_end",
+    "negf_zgemm", "negf_profile_begin", "negf_profile_end", "negf_energy_sweep_f", "negf_flush_f",
+    "negf_observables_f",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)

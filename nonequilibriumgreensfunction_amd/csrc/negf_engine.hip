@@ -1320,3 +1320,56 @@ extern "C" int negf_observables(negf_ctx* c, int eCtr, double f1, double f2, dou
   }
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Fortran-friendly forms: the accumulators as separate array arguments (a Fortran
+// caller passes its module arrays by reference, no TARGET attribute needed); bit k
+// of `mask` marks the k-th array as present, in the field order of negf_accum.
+static negf_accum accum_from(int mask, double* const* a) {
+  negf_accum acc;
+  double** f[13] = {&acc.T12, &acc.Modec1, &acc.Modec2, &acc.DOSc1, &acc.DOSc2, &acc.EDensity,
+                    &acc.HDensity, &acc.RhoN, &acc.RhoP, &acc.SigXExp, &acc.SigYExp,
+                    &acc.SigZExp, &acc.ContactCurrent};
+  for (int k = 0; k < 13; ++k) *f[k] = (mask >> k & 1) ? a[k] : nullptr;
+  return acc;
+}
+
+extern "C" int negf_energy_sweep_f(negf_ctx* c, double energy, int eCtr, double deltaE,
+                                   const double* mu, double muT, int mask, double* T12,
+                                   double* Modec1, double* Modec2, double* DOSc1, double* DOSc2,
+                                   double* EDensity, double* HDensity, double* RhoN, double* RhoP,
+                                   double* SigXExp, double* SigYExp, double* SigZExp,
+                                   double* ContactCurrent) {
+  double* a[13] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
+                   SigYExp, SigZExp, ContactCurrent};
+  negf_accum acc = accum_from(mask, a);
+  return negf_energy_sweep(c, energy, eCtr, deltaE, mu, muT, &acc);
+}
+
+extern "C" int negf_flush_f(negf_ctx* c, const double* mu, double muT, int mask, double* T12,
+                            double* Modec1, double* Modec2, double* DOSc1, double* DOSc2,
+                            double* EDensity, double* HDensity, double* RhoN, double* RhoP,
+                            double* SigXExp, double* SigYExp, double* SigZExp,
+                            double* ContactCurrent) {
+  double* a[13] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
+                   SigYExp, SigZExp, ContactCurrent};
+  negf_accum acc = accum_from(mask, a);
+  return negf_flush(c, mu, muT, &acc, nullptr, nullptr);
+}
+
+extern "C" int negf_observables_f(negf_ctx* c, int eCtr, double f1, double f2, double deltaE,
+                                  const double* Gnd, const double* Gpd, const double* Gnu,
+                                  const double* Gnl, const double* Gpu, const double* Gpl,
+                                  const double* GRRgt, const double* GammaL, const double* GammaR,
+                                  int mask, double* T12, double* Modec1, double* Modec2,
+                                  double* DOSc1, double* DOSc2, double* EDensity, double* HDensity,
+                                  double* RhoN, double* RhoP, double* SigXExp, double* SigYExp,
+                                  double* SigZExp, double* ContactCurrent, double* Jx, double* Jy,
+                                  double* Jz) {
+  double* a[13] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
+                   SigYExp, SigZExp, ContactCurrent};
+  negf_accum acc = accum_from(mask, a);
+  return negf_observables(c, eCtr, f1, f2, deltaE, Gnd, Gpd, Gnu, Gnl, Gpu, Gpl, GRRgt, GammaL,
+                          GammaR, &acc, (mask >> 13 & 1) ? Jx : nullptr,
+                          (mask >> 14 & 1) ? Jy : nullptr, (mask >> 15 & 1) ? Jz : nullptr);
+}

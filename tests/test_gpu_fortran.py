@@ -1,0 +1,69 @@
+"""GPU parity through the Fortran boundary: a Fortran host program
+(tests/fortran/negf_fdriver.f90) fills the NEGF_Module-named state and calls the
+reference-named shims -- Energy_Sweep (fused), the per-stage chain
+Contact_Self_Energy -> Get_Retarded_Greens -> Calculate_Gnp ->
+Calculate_Observables, and Energy_Sweep_Submit + NEGF_Flush -- exactly as
+NEGF_Main's energy loop does.  Results are compared with the CPU oracle under
+the plain-FP64 criterion of tests/parity.py."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from nonequilibriumgreensfunction_amd.hamiltonian import build_wire, energy_grid  # noqa: E402
+
+import parity as P  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "nonequilibriumgreensfunction_amd", "lib", "negf_fdriver")
+ETA, TOL, MAXIT = 1e-6, 1e-10, 200
+MU, MUT = (-0.1, 0.0), 0.0
+
+
+def run_driver(tmp_path, w, energies, staged, submit):
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    with open(fin, "wb") as f:
+        np.array([w.npx, w.npy, w.npz, w.norb, len(energies), int(staged), int(submit), MAXIT],
+                 np.int32).tofile(f)
+        np.array([ETA, TOL, MU[0], MU[1], MUT], np.float64).tofile(f)
+        np.asarray(energies, np.float64).tofile(f)
+        np.asfortranarray(w.htotd).ravel(order="F").tofile(f)
+        np.asfortranarray(w.hopx).ravel(order="F").tofile(f)
+        for m in w.gm:
+            np.asfortranarray(m).ravel(order="F").tofile(f)
+    print(f"fortran driver: staged={staged} submit={submit} dims={w.npx, w.npy, w.npz}", flush=True)
+    r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    raw = np.fromfile(fout, np.uint8)
+    ne, sp = len(energies), w.npx * w.npy * w.npz
+    d = np.frombuffer(raw[: 8 * (7 * ne + 8 * sp + 2)].tobytes(), np.float64)
+    out = {}
+    for i, k in enumerate(("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity")):
+        out[k] = d[i * ne:(i + 1) * ne]
+    o = 7 * ne
+    for i, k in enumerate(("RhoN", "RhoP", "SigX", "SigY", "SigZ", "Jx", "Jy", "Jz")):
+        out[k] = d[o + i * sp:o + (i + 1) * sp]
+    out["ContactCurrent"] = d[o + 8 * sp:o + 8 * sp + 2]
+    out["errorFlag"] = int(np.frombuffer(raw[8 * (7 * ne + 8 * sp + 2):].tobytes(), np.int32)[0])
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+@pytest.mark.parametrize("staged,submit", [(False, False), (True, False), (False, True)])
+def test_fortran_host_matches_oracle(tmp_path, staged, submit):
+    w = build_wire(5, 2, 3, disorder=0.4, seed=6, flux=0.2, zeeman=0.02)
+    energies = list(energy_grid(-1.7, 1.7, 5))
+    got = run_driver(tmp_path, w, energies, staged, submit)
+    ref, noise = P.reference(w, energies, energies[1] - energies[0], mu=MU, muT=MUT)
+    assert got["errorFlag"] == 0
+    P.check_all(got, ref, noise)
+    if staged:  # the per-stage path also produces the bond currents
+        q, hbar = 1.602e-19, 6.626e-34 / (2 * np.pi)
+        # current terms are H*Gn products: e^2/hbar * norb^2 * max|H| * max(Gn dE/2pi)
+        rho = float(np.abs(ref["RhoN"]).max())
+        jscale = q * q / hbar * w.norb**2 * np.abs(w.htotd).max() * rho
+        for k in ("Jx", "Jy", "Jz"):
+            P.check(k, got[k], ref[k], noise[k], jscale)

@@ -1,11 +1,10 @@
 """GPU parity of the fused per-energy solve (Energy_Sweep.F90:1-46 and everything
 below it) against the complex128 CPU oracle on identical Hamiltonians.
 
-Tolerance (plain FP64 form): every output array agrees with the oracle to a
-relative 1e-10 of its max-norm; per-energy scalars to 1e-10 relative with an
-absolute floor of 1e-10 times the quantity's natural scale (the number of
-propagating channels for T12 and the modes, the DOS magnitude for DOSc, the
-site density for the spin expectation values)."""
+Tolerance (plain FP64 form, tests/parity.py): every output agrees with the oracle
+to a relative 1e-10 of its natural scale (the open-channel count for T12 and the
+modes, the density for spin), or -- where the problem is less stable than that --
+to within 10x the oracle's own spread under a different MKL summation order."""
 import numpy as np
 import pytest
 
@@ -15,29 +14,9 @@ capi = pytest.importorskip("nonequilibriumgreensfunction_amd.capi")
 from nonequilibriumgreensfunction_amd.hamiltonian import build_wire, energy_grid  # noqa: E402
 
 import oracle as O  # noqa: E402
+import parity as P  # noqa: E402
 
-RTOL = 1e-10
 MU, MUT = (-0.1, 0.0), 0.0
-
-
-def _oracle_sweep(w, energies, dE, eta, tol, maxit):
-    L = w.npx
-    acc = dict(T12=[], Modec1=[], Modec2=[], DOSc1=[], DOSc2=[], EDensity=[], HDensity=[])
-    sp = {k: np.zeros((L, w.npy, w.npz)) for k in ("RhoN", "RhoP", "SigX", "SigY", "SigZ")}
-    cc = np.zeros(2)
-    iters = []
-    for E in energies:
-        r = O.energy_sweep(w, E, dE, MU, MUT, eta, tol, maxit, True)
-        for k in acc:
-            acc[k].append(r[k])
-        for k in sp:
-            sp[k] += r[k]
-        cc += r["ContactCurrent"]
-        iters.append(r["iters"])
-    out = {k: np.array(v) for k, v in acc.items()}
-    out.update(RhoN=sp["RhoN"], RhoP=sp["RhoP"], SigXExp=sp["SigX"], SigYExp=sp["SigY"],
-               SigZExp=sp["SigZ"], ContactCurrent=cc, iters=np.array(iters))
-    return out
 
 
 def _gpu_sweep(w, energies, dE, eta, tol, maxit, batched=True, max_batch=0):
@@ -56,27 +35,16 @@ def _gpu_sweep(w, energies, dE, eta, tol, maxit, batched=True, max_batch=0):
     return acc, st, it
 
 
-def _close(name, got, ref, scale=0.0):
-    got, ref = np.asarray(got), np.asarray(ref)
-    err = np.abs(got - ref).max()
-    lim = RTOL * max(np.abs(ref).max(), scale)
-    assert err <= lim, f"{name}: max err {err:.3e} > {lim:.3e} (max |ref| {np.abs(ref).max():.3e})"
+def _as_dict(acc):
+    d = {k: getattr(acc, k) for k in P.PER_E + ("ContactCurrent",)}
+    for k, a in (("RhoN", "RhoN"), ("RhoP", "RhoP"), ("SigX", "SigXExp"), ("SigY", "SigYExp"),
+                 ("SigZ", "SigZExp")):
+        d[k] = np.asarray(getattr(acc, a)).ravel(order="F")
+    return d
 
 
-def _compare(acc, ref, it=None):
-    nmodes = max(1.0, float(np.abs(ref["Modec1"]).max()))
-    _close("T12", acc.T12, ref["T12"], nmodes)
-    _close("Modec1", acc.Modec1, ref["Modec1"], nmodes)
-    _close("Modec2", acc.Modec2, ref["Modec2"], nmodes)
-    _close("DOSc1", acc.DOSc1, ref["DOSc1"])
-    _close("DOSc2", acc.DOSc2, ref["DOSc2"])
-    for k in ("EDensity", "HDensity", "RhoN", "RhoP", "ContactCurrent"):
-        _close(k, getattr(acc, k), ref[k])
-    # |Tr(Gn_site sigma)| <= Tr(Gn_site): spin is judged on the density's scale (it
-    # vanishes by symmetry without Zeeman or flux, leaving only rounding noise)
-    rho = float(np.abs(ref["RhoN"]).max())
-    for k in ("SigXExp", "SigYExp", "SigZExp"):
-        _close(k, getattr(acc, k), ref[k], rho)
+def _compare(acc, ref, noise, it=None):
+    P.check_all(_as_dict(acc), ref, noise)
     if it is not None:
         assert np.array_equal(np.asarray(it), ref["iters"]), (it, ref["iters"])
 
@@ -94,10 +62,10 @@ CASES = [
 def test_sweep_matches_oracle(case):
     npx, npy, npz, kw, energies = CASES[case]
     w = build_wire(npx, npy, npz, **kw)
-    ref = _oracle_sweep(w, energies, 0.01, 1e-6, 1e-10, 200)
+    ref, noise = P.reference(w, energies, 0.01, mu=MU, muT=MUT)
     acc, st, it = _gpu_sweep(w, energies, 0.01, 1e-6, 1e-10, 200)
     assert np.all(st == 0), st
-    _compare(acc, ref, it)
+    _compare(acc, ref, noise, it)
 
 
 def test_sweep_single_calls_equal_batch():
@@ -118,10 +86,10 @@ def test_sweep_c1_subset():
     grid = energy_grid(-0.3, 0.3, 64)
     energies = list(grid[[0, 21, 40, 63]])
     dE = grid[1] - grid[0]
-    ref = _oracle_sweep(w, energies, dE, 1e-6, 1e-10, 200)
+    ref, noise = P.reference(w, energies, dE, mu=MU, muT=MUT)
     acc, st, it = _gpu_sweep(w, energies, dE, 1e-6, 1e-10, 200)
     assert np.all(st == 0), st
-    _compare(acc, ref, it)
+    _compare(acc, ref, noise, it)
 
 
 def test_contact_self_energy_matches_oracle():
@@ -133,9 +101,9 @@ def test_contact_self_energy_matches_oracle():
             got = ctx.contact_self_energy(E)
         assert got["iters"] == ref["iters"]
         for k in ("SigmaL", "SigmaR"):
-            _close(k, got[k], ref[k])
+            P.check(k, got[k], ref[k], 0.0)
         # Gamma = i(Sigma - Sigma') cancels to O(eta) inside the gap: its error is
         # judged on the scale of the Sigma it is formed from
         for k, sk in (("GammaL", "SigmaL"), ("GammaR", "SigmaR")):
-            _close(k, got[k], ref[k], float(np.abs(ref[sk]).max()))
-        _close("DOSc1", got["DOSc1"], ref["DOSc1"])
+            P.check(k, got[k], ref[k], 0.0, float(np.abs(ref[sk]).max()))
+        P.check("DOSc1", got["DOSc1"], ref["DOSc1"], 0.0)
