@@ -10,7 +10,11 @@ wrong sign, a transposed or un-conjugated operand, an off-by-one slice) fails:
   against its own fixed-point equation for multi-orbital leads;
 * clean-wire transmission against the number of propagating Bloch modes;
 * the Ward identity G(GL+GR)G' = i(G-G') - 2 eta GG', two-terminal reciprocity,
-  Hermiticity of Gn, and vanishing spin density under time reversal.
+  Hermiticity of Gn, and vanishing spin density under time reversal;
+* clean-wire injected modes (Modec = 2 x channels), the contact DOS of a chain
+  against its closed form, spin as a plain matrix trace of the Gn site blocks for
+  a field along each axis, and bond currents against Kirchhoff's law and the
+  transmission (sum over a cross-section = 2 T q^2/h dE).
 """
 import numpy as np
 import pytest
@@ -199,6 +203,93 @@ def test_spin_vanishes_with_time_reversal():
     wz = build_wire(4, 2, 3, disorder=0.6, seed=2, zeeman=0.05)
     r = O.energy_sweep(wz, -1.6, 0.01, mu=(0.0, 0.0), eta=1e-6)
     assert np.abs(r["SigX"]).max() > 1e-3 * np.abs(r["RhoN"]).max()
+
+
+@pytest.mark.parametrize("dims,E,modes", [((4, 2, 2), 1.6, 2), ((3, 1, 1), -5.0, 2), ((3, 2, 1), -6.0, 4),
+                                          ((3, 3, 2), -6.5, 10)])
+def test_clean_wire_injected_modes_are_twice_the_channels(dims, E, modes):
+    # reflectionless wire: Tr(GL G11 GL G11') = M and Tr(GL G1L GR G1L') = T = M, so
+    # Modec1 = Re Tr(GL (Gn+Gp)_11) = 2M, and likewise Modec2 at the right end
+    w = build_wire(*dims)
+    assert bloch_mode_count(w, E) == modes
+    r = O.energy_sweep(w, E, 0.0, eta=ETA)
+    assert abs(r["Modec1"] - 2 * modes) < 2e-3
+    assert abs(r["Modec2"] - 2 * modes) < 2e-3
+
+
+@pytest.mark.parametrize("E", [-1.5, 0.4, 1.9])
+def test_contact_dos_chain_closed_form(E):
+    # single-orbital chain (eps0 = 0, hop -t): Sigma = t^2 g_s, DOSc = -Im Sigma / pi
+    t = 1.0
+    w = build_wire(3, 1, 1, model="metal", params=dict(t=t, eps=0.0))
+    z = E + 1j * ETA
+    root = np.sqrt(z * z - 4 * t * t)
+    g = (z - root) / (2 * t * t)
+    if g.imag > 0:
+        g = (z + root) / (2 * t * t)
+    r = O.energy_sweep(w, E, 0.0, eta=ETA)
+    want = -(t * t * g).imag / np.pi
+    # the 2*eta decimation seed moves Sigma by O(eta)
+    assert abs(r["DOSc1"] - want) < 10 * ETA
+    assert abs(r["DOSc2"] - want) < 10 * ETA
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_spin_is_site_trace_of_gn_with_each_matrix(axis):
+    # Zeeman field along one axis, Peierls flux, disorder: the spin output equals
+    # Re Tr(Gn_site sigma) formed from the Gn blocks (pinned against the reference's
+    # Calculate_Gnp) with a plain matrix trace -- a transposed sigma flips the sign of
+    # the imaginary (y) component, a wrong site offset moves it
+    w = build_wire(3, 2, 2, disorder=0.4, seed=5, flux=0.3)
+    for s in range(w.n // 4):
+        w.htotd[4 * s:4 * s + 4, 4 * s:4 * s + 4, :] += 0.05 * w.gm[axis][:, :, None]
+    n, L = w.n, w.npx
+    E, dE = -2.5, 0.01
+    c = O.contact_self_energy(w, E, ETA)
+    A = O.assemble_invg(w, c["SigmaL"], c["SigmaR"], E, ETA)
+    rr = O.retarded(A, w.hopx)
+    g = O.gnp(w, 1.0, 0.0, dE, rr["GRLef"], rr["GRRgt"], c["GammaL"], c["GammaR"])
+    ob = O.observables(w, 1.0, 0.0, dE, g, rr["GRRgt"], c["GammaL"], c["GammaR"])
+    for a, k in enumerate(("SigX", "SigY", "SigZ")):
+        want = np.zeros((L, w.npy, w.npz))
+        for x in range(L):
+            for y in range(w.npy):
+                for zz in range(w.npz):
+                    i0 = (y + zz * w.npy) * 4
+                    blk = g["Gnd"][i0:i0 + 4, i0:i0 + 4, x]
+                    want[x, y, zz] = np.trace(blk @ w.gm[a]).real * dE / (2 * np.pi)
+        got = np.asarray(ob[k]).reshape(L, w.npy, w.npz, order="F")
+        assert np.abs(got - want).max() <= 1e-13 * max(np.abs(want).max(), 1e-3 * dE)
+    assert np.abs(np.asarray(ob[("SigX", "SigY", "SigZ")[axis]])).max() > 1e-4 * dE
+
+
+def test_bond_currents_conserve_and_carry_twice_the_transmission():
+    # f1 = 1, f2 = 0 (Gp injects from the right): the electron minus hole particle
+    # currents through every cross-section sum to 2 T q^2/h dE, and every interior
+    # site obeys Kirchhoff's law (Jx, Jy, Jz all flow from lower to higher index);
+    # both hold up to the O(eta) leakage of the broadened device
+    q, hbar = 1.602e-19, 6.626e-34 / (2 * np.pi)
+    w = build_wire(6, 3, 2, disorder=0.8, seed=3, flux=0.3)
+    dE = 0.02
+    r = O.energy_sweep(w, 1.6, dE, mu=(-2.0, 0.0), muT=0.0, eta=ETA)
+    L, ny, nz = w.npx, w.npy, w.npz
+    J = {k: np.asarray(r[k]).reshape(L, ny, nz, order="F") for k in ("Jx", "Jy", "Jz")}
+    assert r["T12"] > 1.0
+    unit = 2 * r["T12"] * q * q / (2 * np.pi * hbar) * dE
+    through = J["Jx"][:-1].sum(axis=(1, 2))
+    assert np.abs(through / unit - 1).max() < 1e-4
+    div = J["Jx"] + J["Jy"] + J["Jz"]
+    div[1:] -= J["Jx"][:-1]
+    div[:, 1:] -= J["Jy"][:, :-1]
+    div[:, :, 1:] -= J["Jz"][:, :, :-1]
+    jmax = max(np.abs(J[k]).max() for k in J)
+    assert np.abs(J["Jy"]).max() > 1e-3 * jmax  # transverse currents are exercised
+    assert np.abs(div[1:-1]).max() < 1e-5 * jmax
+    # the contact currents are the same T q dE/h, with opposite signs
+    assert abs(r["ContactCurrent"][1] - q * dE / (2 * np.pi * hbar) * r["T12"]) < 1e-12 * abs(r["ContactCurrent"][1])
+    assert r["ContactCurrent"][0] == -r["ContactCurrent"][1]
+    # the energy-resolved density is the sum of the site densities
+    assert abs(r["EDensity"] * dE - np.sum(r["RhoN"])) < 1e-12 * abs(r["EDensity"] * dE)
 
 
 def test_error_paths():
