@@ -47,25 +47,67 @@ def f_useful(n, L, k_left, k_right):
     return 8.0 * n**3 * (6 * L + 3 + 9 * (k_left + k_right))
 
 
-def cpu_baseline(cfg, energy, threads):
-    """Oracle (C++ complex128 + MKL, the reference's operation order) on the host:
-    one energy at Npx=2 and Npx=4 of the same cross-section, extrapolated linearly
-    in Npx to the workload's Npx."""
+def cpu_baseline(cfg, energy, threads, budget_s=20.0):
+    """Oracle (C++ complex128 + MKL, the reference's operation order) on the host,
+    on a bounded sample: one energy of the same cross-section at two short lengths
+    Npx = a < b (the time of one energy is affine in Npx: Sancho-Rubio plus L
+    slices), extrapolated to the workload's Npx.  b grows until the sample has
+    used about budget_s seconds of CPU time, or reaches the workload's Npx."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # noqa: E402  (checker only)
     O.set_threads(threads)
-    t = {}
-    for L in (2, 4):
+
+    def timed(L):
         w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
         t0 = time.perf_counter()
         O.energy_sweep(w, energy, 0.0, MU, MUT, ETA, SR_TOL, SR_MAX, True)
-        t[L] = time.perf_counter() - t0
-    per_slice = (t[4] - t[2]) / 2.0
-    t_full = t[2] + per_slice * (cfg["npx"] - 2)
+        return time.perf_counter() - t0
+
+    a = 2
+    t = {a: timed(a)}
+    b = a
+    used = t[a]
+    while b < cfg["npx"] and used < 0.5 * budget_s:
+        b = min(cfg["npx"], max(b + 2, 2 * b))
+        t[b] = timed(b)
+        used += t[b]
+    if b == cfg["npx"]:
+        t_full = t[b]
+    else:
+        per_slice = max(t[b] - t[a], 0.0) / (b - a)
+        t_full = t[a] + per_slice * (cfg["npx"] - a)
+    lens = sorted(t)
     return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port",
-                sample=(f"oracle, 1 energy (E={energy:.4f}) at Npx=2 ({t[2]:.2f}s) and Npx=4 "
-                        f"({t[4]:.2f}s), Npy=Npz={cfg['npy']}, MKL {threads} threads; "
-                        f"extrapolated linearly to Npx={cfg['npx']}: {t_full:.1f}s per energy"))
+                sample=(f"oracle, 1 energy (E={energy:.4f}), Npy=Npz={cfg['npy']}, MKL {threads} "
+                        f"threads, at Npx=" + ", ".join(f"{L} ({t[L]:.1f}s)" for L in lens) +
+                        f"; {used:.1f}s of CPU work, extrapolated affinely to Npx={cfg['npx']}: "
+                        f"{t_full:.1f}s per energy"))
+
+
+def traffic_from_profiles(config):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy-kernel
+    calibration), if one exists for this workload; else None."""
+    path = os.path.join(ROOT, "profiles", f"k_gemm_fp64_traffic_{config}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: reads and writes mib MiB per launch."""
+    import torch
+    x = torch.ones(mib * 2**20 // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(8):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read_per_launch": mib * 2**20,
+                      "copy_calibration_bytes_written_per_launch": mib * 2**20, "launches": 8}))
 
 
 def main():
@@ -77,7 +119,16 @@ def main():
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
+    ap.add_argument("--profile-every", type=int, default=32,
+                    help="time one MFMA GEMM launch in this many with HIP events")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB MiB 8 times (the known-size "
+                         "reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: this benchmark runs on exactly one GPU "
                                    "(energies shard across GPUs with no communication until the "
@@ -101,7 +152,7 @@ def main():
           flush=True)
     w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
     ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
-                       max_batch=per)
+                       max_batch=per, flags=args.flags)
     ctx.set_hamiltonian(w)
     acc = capi.Accumulators(L, cfg["npy"], cfg["npz"], cfg["ne"])
 
@@ -116,7 +167,7 @@ def main():
         run_step(k)
         print(f"[bench] warmup step {k} {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
 
-    capi.profile_begin()
+    capi.profile_begin(args.profile_every)
     times, iters, status = [], [], []
     t_start = time.perf_counter()
     for k in range(args.warmup, nsteps):
@@ -127,14 +178,18 @@ def main():
         status.append(st)
         print(f"[bench] step {k} {times[-1]:.2f}s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
-    g_flops, g_ms, g_launches = capi.profile_end()
+    prof = capi.profile_end()
     ctx.close()
 
     iters = np.concatenate(iters)
     npts = per * args.steps
     value = npts / elapsed
     useful = sum(f_useful(n, L, a, b) for a, b in iters)
+    g_flops, g_ms, g_launches = prof["fp64_gemm"]
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    kernels = {k: {"tflops": (f / (m * 1e-3) / 1e12 if m > 0 else 0.0), "launches": c,
+                   "avg_launch_ms": m / max(1, c), "busy_ms": m}
+               for k, (f, m, c) in prof.items() if c}
     result = {
         "metric": "energy-points/s (plain FP64 form, one GPU)",
         "value": value,
@@ -160,11 +215,16 @@ def main():
         "status_nonzero": int(sum(int(np.count_nonzero(s)) for s in status)),
         "path_tflops_useful": useful / elapsed / 1e12,
         "path_frac_fp64_peak": useful / elapsed / 1e12 / PEAK_FP64_MFMA,
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (FP64 MFMA complex GEMM incl. Gauss-Jordan updates)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_gemm<double,N,N> (FP64 MFMA complex product, n x n x n per matrix)",
                      "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / PEAK_FP64_MFMA, "traffic": None,
+                     "frac": gemm_tflops / PEAK_FP64_MFMA, "traffic": traffic_from_profiles(args.config),
                      "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
-                     "gemm_time_share": (g_ms * 1e-3) / elapsed},
+                     "note": ("achieved = algorithmic 8 n^3 flops per matrix x batch / HIP-event duration "
+                              "on the launching stream; two lanes run concurrently, so durations include "
+                              "the other lane's kernels sharing the CUs")},
+        "kernels": kernels,
+        "flags": args.flags,
     }
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()

@@ -60,7 +60,19 @@ typedef struct {
   int device;          /* HIP device ordinal */
   double hbm_budget_gb;/* device memory the context may use for in-flight energies; 0 = auto */
   int max_batch;       /* cap on energies in flight; 0 = auto */
+  int flags;           /* NEGF_FLAG_* (0 = default algorithms) */
 } negf_config;
+
+/* Algorithm selection (results agree to rounding either way).
+ * Default fused RGF: one forward pass of left-connected inverses gL_x, then both
+ * block columns by block back-substitution (L inverses per energy, 2L resident
+ * blocks).  THREE_PASS: the reference's structure with right-connected inverses
+ * gR_x for the first column (2L inverses, L resident blocks; for tight memory).
+ * Default Sancho-Rubio: when the two end slices are bitwise equal (clean leads of
+ * the same material) one decimation serves both leads, whose (t, tt) sequences
+ * are each other's swapped; SR_SEPARATE_LEADS runs one decimation per lead. */
+#define NEGF_FLAG_RGF_THREE_PASS 1
+#define NEGF_FLAG_SR_SEPARATE_LEADS 2
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
@@ -171,11 +183,16 @@ int negf_zgemm(char transa, char transb, int n, const double* alpha, const doubl
                const double* B, int ldb, const double* beta, double* C, int ldc);
 
 /* --- Instrumentation (no counterpart in the reference) ---------------------
- * Time every MFMA GEMM launch (the dominant kernel) with HIP events on the
- * stream it is launched on, between begin and end.  end() synchronises the
- * device and returns the launches' algorithmic flops (8 m n k per complex
- * product), their summed duration in ms and the launch count. */
-int negf_profile_begin(void);
+ * Time MFMA GEMM launches (the dominant kernel) with HIP events on the stream
+ * each is launched on, between begin and end: one launch in sample_every, in
+ * launch order (an event pair per launch costs host time and stream markers, so
+ * a uniform sample keeps the timed run close to an untimed one; 1 = every
+ * launch).  end() synchronises the
+ * device and returns, per kind, the launches' algorithmic flops (8 m n k per
+ * complex product), their summed duration in ms and the launch count; each
+ * argument is an array of 4: [0] FP64 product, [1] FP64 Gauss-Jordan trailing
+ * update, [2] FP32 product, [3] FP32 trailing update. */
+int negf_profile_begin(int sample_every);
 int negf_profile_end(double* flops, double* ms, long long* launches);
 
 #ifdef __cplusplus

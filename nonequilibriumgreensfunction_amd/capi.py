@@ -45,7 +45,11 @@ class NegfConfig(ctypes.Structure):
                 ("norb", ctypes.c_int), ("eta", ctypes.c_double), ("sr_tol", ctypes.c_double),
                 ("sr_max_iter", ctypes.c_int), ("sr_extra_eta", ctypes.c_int),
                 ("precision", ctypes.c_int), ("device", ctypes.c_int),
-                ("hbm_budget_gb", ctypes.c_double), ("max_batch", ctypes.c_int)]
+                ("hbm_budget_gb", ctypes.c_double), ("max_batch", ctypes.c_int),
+                ("flags", ctypes.c_int)]
+
+FLAG_RGF_THREE_PASS = 1
+FLAG_SR_SEPARATE_LEADS = 2
 
 
 class NegfAccum(ctypes.Structure):
@@ -85,7 +89,8 @@ def lib():
         L.negf_invert_matrix.argtypes = [ctypes.c_int, _dp, _dp, _ip]
         L.negf_zgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
-        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.negf_profile_begin.argtypes = [ctypes.c_int]
+        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 4
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -113,15 +118,21 @@ def _cf(a):
     return np.asfortranarray(a, dtype=np.complex128)
 
 
-def profile_begin():
-    _check(lib().negf_profile_begin())
+def profile_begin(sample_every=1):
+    _check(lib().negf_profile_begin(int(sample_every)))
+
+
+PROFILE_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update")
 
 
 def profile_end():
-    """(flops, ms, launches) of the MFMA GEMM launches since profile_begin()."""
-    f, m, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
-    _check(lib().negf_profile_end(ctypes.byref(f), ctypes.byref(m), ctypes.byref(n)))
-    return f.value, m.value, n.value
+    """{kind: (flops, ms, launches)} of the MFMA GEMM launches since profile_begin(),
+    kinds as in PROFILE_KINDS."""
+    f = (ctypes.c_double * 4)()
+    m = (ctypes.c_double * 4)()
+    n = (ctypes.c_longlong * 4)()
+    _check(lib().negf_profile_end(f, m, n))
+    return {k: (f[i], m[i], n[i]) for i, k in enumerate(PROFILE_KINDS)}
 
 
 def device_count() -> int:
@@ -172,11 +183,11 @@ class Context:
 
     def __init__(self, npx, npy, npz, norb=4, eta=1e-6, sr_tol=1e-10, sr_max_iter=200,
                  sr_extra_eta=True, precision=NEGF_PREC_FP64, device=0, hbm_budget_gb=0.0,
-                 max_batch=0):
+                 max_batch=0, flags=0):
         self.dims = (npx, npy, npz, norb)
         self.n = npy * npz * norb
         cfg = NegfConfig(npx, npy, npz, norb, eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)),
-                         precision, device, hbm_budget_gb, max_batch)
+                         precision, device, hbm_budget_gb, max_batch, flags)
         h = ctypes.c_void_p()
         _check(lib().negf_init(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h

@@ -88,12 +88,17 @@ constexpr int kGjNBO = 64;  // Gauss-Jordan outer block width (columns per trail
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
 // every launch is bracketed by HIP events on its own stream and its algorithmic
-// flop count (8 m n k per complex product) is recorded.
+// flop count (8 m n k per complex product) is recorded with its kind:
+// 0 FP64 product, 1 FP64 Gauss-Jordan trailing update, 2 FP32 product, 3 FP32 update.
+constexpr int kTimerKinds = 4;
 struct GemmTimer {
   bool on = false;
+  int every = 1;         // time one launch in `every` (uniform sample)
+  unsigned long long seen = 0;
   size_t used = 0;
   hipEvent_t* ev = nullptr;  // 2 per launch
   double* flops = nullptr;
+  int* kind = nullptr;
   size_t cap = 0;
 };
 GemmTimer& gemm_timer();

@@ -2,21 +2,24 @@
 // C ABI of include/negf_capi.h.
 //
 // Pipeline for a batch of B energies of one wire (all kernels batched over B):
-//  1. Sancho-Rubio for both leads of every energy, 2B independent chains
-//     (Contact_Self_Energy.F90:68-101, :110-223); converged chains drop out of
-//     the active list, so the loop costs what the slowest chain needs.
-//  2. RGF pass 1 (forward): gL_x = inv(A_x - X' gL_{x-1} X) into slot x
-//     (Recursive_Greens.F90:16-34).
-//  3. RGF pass 2 (backward): the last block column G_xL = gL_x X G_{x+1,L}
-//     (:53-63) is formed from slot x, which is then overwritten with
-//     gR_x = inv(A_x - X gR_{x+1} X') (:67-86).  Each G_xL feeds the right-lead
-//     Gn/Gp site blocks of slice x and, at x = 1, T12.
-//  4. RGF pass 3 (forward): the first block column G_x1 = gR_x X' G_{x-1,1}
-//     (:88-96) and the left-lead Gn/Gp site blocks (Calculate_Gnp.F90).
-// So only L + 10 n x n blocks per energy are resident (the reference keeps ~15 L),
-// block columns are streamed, and Gn/Gp are reduced to the site-diagonal traces
+//  1. Sancho-Rubio for both leads of every energy (Contact_Self_Energy.F90:68-101,
+//     :110-223).  When the end slices are equal one decimation serves both leads
+//     (run_sr_paired); otherwise 2B independent chains.  Converged chains drop out
+//     of the active list, so the loop costs what the slowest chain needs.
+//  2. Default two-pass RGF.  Forward: gL_x = inv(A_x - X' gL_{x-1} X) into slot x
+//     (Recursive_Greens.F90:16-34) and the forward substitution z_x = gL_x X' z_{x-1}
+//     of the block-tridiagonal system for block column 1.  Backward: the last block
+//     column G_xL = gL_x X G_{x+1,L} (:53-63) and, by back substitution, the first
+//     block column G_x1 = z_x + gL_x X G_{x+1,1} -- the same blocks the reference
+//     gets from its right-connected gR (:67-96), with L inverses instead of 2L.
+//     Each slice's two column blocks feed the Gn/Gp site blocks of both leads;
+//     T12 and the Modec terms come from the end slices.
+//  3. Three-pass RGF (NEGF_FLAG_RGF_THREE_PASS): pass 2 overwrites slot x with
+//     gR_x = inv(A_x - X gR_{x+1} X') and pass 3 forms G_x1 = gR_x X' G_{x-1,1}; it
+//     keeps L + 10 blocks per energy resident instead of 2L + 10.
+// Block columns are streamed, and Gn/Gp are reduced to the site-diagonal traces
 // the outputs need (Calculate_Gnp.F90:118-124, Calculate_Observables.F90:10-53)
-// without forming the n x n x L Gnd/Gpd arrays.
+// without forming the n x n x L Gnd/Gpd arrays (the reference keeps ~15 L blocks).
 #include "negf_device.h"
 #include "negf_capi.h"
 
@@ -27,6 +30,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <iterator>
 #include <atomic>
 #include <mutex>
 #include <thread>
@@ -94,16 +98,19 @@ static const double kH = 6.626e-34;   // Constants.F90:6
 struct Layout {
   long long nn;
   // per-energy region (complex elements)
-  long long Es, o_slots, o_M, o_colA, o_colB, o_Y, o_Bm, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
+  long long Es, o_slots, o_z, o_M, o_colA, o_colB, o_Y, o_Bm, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
       o_prow;
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
   long long site_per_e;  // doubles of site output per energy
-  void init(int n, int L, int nsites) {
+  // zslots: L more blocks per energy for the forward substitution chain of the
+  // two-pass RGF (first block column from the left-connected inverses)
+  void init(int n, int L, int nsites, bool zslots) {
     nn = (long long)n * n;
     long long o = 0;
     o_slots = o; o += (long long)L * nn;
+    o_z = o; if (zslots) o += (long long)L * nn;
     o_M = o; o += nn;
     o_colA = o; o += nn;
     o_colB = o; o += nn;
@@ -149,7 +156,7 @@ struct Lane {
   double* d_scal = nullptr;    // cap * 8
   double* d_site = nullptr;    // cap * site_per_e
   double* d_mc = nullptr;      // 2cap
-  int* d_active = nullptr;     // 2cap
+  int* d_active = nullptr;     // 4cap: Sancho-Rubio active lists
 };
 
 struct negf_ctx {
@@ -158,6 +165,7 @@ struct negf_ctx {
   hipStream_t st = nullptr;  // stream of the stage entry points (= lane 0's)
   bool have_h = false;
   bool blockdiag = true;
+  bool ends_equal = false;  // H slice 1 == H slice L bitwise: one decimation serves both leads
   zd* H = nullptr;       // Htotd (n,n,L)
   zd* X = nullptr;       // HopX
   zd* betaL = nullptr;   // -HopX'
@@ -246,7 +254,7 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
   c->n = n;
   c->L = cfg->npx;
   c->nsites = cfg->npy * cfg->npz;
-  c->lay.init(n, c->L, c->nsites);
+  c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS));
   for (Lane& l : c->lanes)
     if (hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) != hipSuccess) {
       negf_set_error("hipStreamCreate failed");
@@ -288,6 +296,7 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
         break;
       }
   c->blockdiag = bd;
+  c->ends_equal = std::memcmp(Htotd, Htotd + 2 * (size_t)(L - 1) * nn, sizeof(zd) * nn) == 0;
   if (!c->H) {
     HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
     HIPCHK(hipMalloc(&c->X, sizeof(zd) * nn));
@@ -352,7 +361,7 @@ static int ensure_arena(negf_ctx* c, int want) {
     HIPCHK(hipMalloc(&l.d_scal, sizeof(double) * 8 * cap));
     HIPCHK(hipMalloc(&l.d_site, sizeof(double) * (size_t)y.site_per_e * cap));
     HIPCHK(hipMalloc(&l.d_mc, sizeof(double) * 2 * cap));
-    HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 2 * cap));
+    HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 4 * cap));
     l.cap = cap;
   }
   return 0;
@@ -510,6 +519,150 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
   return 0;
 }
 
+// Sancho-Rubio for both leads of nb energies whose end slices are equal
+// (alpha_L = alpha_R).  With beta_L = -X', beta_R = -X the reference's two
+// decimations (Contact_Self_Energy.F90:141-180) start from t_R = tt_L, tt_R = t_L,
+// and the update tmp = inv(I - t tt - tt t), t' = tmp t t, tt' = tmp tt tt maps the
+// swapped pair to the swapped pair, so one sequence (t, tt) serves both: the left
+// lead sums change_L = Toldt_L t' with Toldt_L = prod tt, the right lead sums
+// change_R = Toldt_R tt' with Toldt_R = prod t.  Each lead stops accumulating at
+// its own convergence (its own iteration count, as separate calls would); the
+// shared steps run while either lead is active.  Per iteration: 1 inverse and 10
+// GEMMs instead of 2 and 16.  Chain b (< nb) holds the shared state and the left
+// lead, chain nb + b the right lead's Toldt, CapT and work matrices.
+static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long long sSig,
+                         std::vector<int>& iters, std::vector<int>& status) {
+  const int n = c->n;
+  const Layout& y = c->lay;
+  const long long Cs = y.Cs, nn = y.nn;
+  const long long R = (long long)nb * Cs;  // right-lead chain offset
+  hipStream_t s = ln.st;
+  zd* C0 = ln.C;
+  auto P = [&](long long off) { return C0 + off; };
+  GjWork w = gjw(P(y.c_prow), Cs, ln.Ic, y.Is, n);
+  iters.assign(2 * nb, 0);
+  status.assign(2 * nb, 0);
+  TRY(hipMemsetAsync(ln.Ic, 0, sizeof(int) * (size_t)y.Is * 2 * nb, s));
+  // tmp = inv(seed); t = -tmp*betad_L = tmp X, tt = -tmp*beta_L = tmp X'  (:141-149)
+  TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nb, w, nullptr));
+  TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, c->betaR, 0, n, P(y.c_t), Cs, n, n), nb,
+                           kMone, kZero));
+  TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, c->betaL, 0, n, P(y.c_tt), Cs, n, n),
+                           nb, kMone, kZero));
+  // CapT_L = t_L = t, CapT_R = t_R = tt; Toldt = I for both
+  TRY(launch_copy3<double>(s, P(y.c_CapT), P(y.c_t), P(R + y.c_CapT), P(y.c_tt), nullptr, nullptr, Cs,
+                           nn, nb, nullptr));
+  TRY(launch_eye<double>(s, P(y.c_Toldt), Cs, n, 2 * nb, nullptr));
+  std::vector<int> actL(nb), actR(nb);
+  for (int i = 0; i < nb; ++i) actL[i] = actR[i] = i;
+  std::vector<double> mc(2 * nb);
+  int it = 0;
+  const double tol = c->cfg.sr_tol;
+  while ((!actL.empty() || !actR.empty()) && it < c->cfg.sr_max_iter) {  // (:154)
+    ++it;
+    std::vector<int> U;
+    std::set_union(actL.begin(), actL.end(), actR.begin(), actR.end(), std::back_inserter(U));
+    const int nL = (int)actL.size(), nR = (int)actR.size(), nU = (int)U.size();
+    int* AL = ln.d_active;
+    int* AR = ln.d_active + nb;
+    int* AU = ln.d_active + 2 * nb;
+    std::vector<int> lists(3 * (size_t)nb, 0);
+    std::copy(actL.begin(), actL.end(), lists.begin());
+    std::copy(actR.begin(), actR.end(), lists.begin() + nb);
+    std::copy(U.begin(), U.end(), lists.begin() + 2 * nb);
+    TRY(hipMemcpyAsync(ln.d_active, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice, s));
+    // W1_L = Toldt_L*tt, W1_R = Toldt_R*t   (:157)
+    if (nL)
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs, n,
+                                                 n, AL), nL, kOne, kZero));
+    if (nR)
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(R + y.c_Toldt), Cs, n, P(y.c_t), Cs, n,
+                                                 P(R + y.c_W1), Cs, n, n, AR), nR, kOne, kZero));
+    // work = I - t*tt - tt*t; tmp = inv(work)   (:160-163)
+    TRY(launch_eye<double>(s, P(y.c_work), Cs, n, nU, AU));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n, n,
+                                               AU), nU, kMone, kOne));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n, n,
+                                               AU), nU, kMone, kOne));
+    TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nU, w, AU));
+    // t' = tmp*t*t -> W3, tt' = tmp*tt*tt -> W4   (:170-176)
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n, n,
+                                               AU), nU, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n, n,
+                                               AU), nU, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n, n,
+                                               AU), nU, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n, n,
+                                               AU), nU, kOne, kZero));
+    // change_L = Toldt_L*t', change_R = Toldt_R*tt'; CapT += change   (:178-180)
+    if (nL) {
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
+                                                 n, AL), nL, kOne, kZero));
+      TRY(launch_add<double>(s, P(y.c_CapT), Cs, P(y.c_CapT), Cs, P(y.c_W2), Cs, nn, nL, AL));
+      TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, ln.d_mc, nL, AL));
+      TRY(launch_copy3<double>(s, P(y.c_Toldt), P(y.c_W1), nullptr, nullptr, nullptr, nullptr, Cs, nn,
+                               nL, AL));
+    }
+    if (nR) {
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(R + y.c_W1), Cs, n, P(y.c_W4), Cs, n, P(R + y.c_W2),
+                                                 Cs, n, n, AR), nR, kOne, kZero));
+      TRY(launch_add<double>(s, P(R + y.c_CapT), Cs, P(R + y.c_CapT), Cs, P(R + y.c_W2), Cs, nn, nR, AR));
+      TRY(launch_maxabs<double>(s, P(R + y.c_W2), Cs, nn, ln.d_mc + nb, nR, AR));
+      TRY(launch_copy3<double>(s, P(R + y.c_Toldt), P(R + y.c_W1), nullptr, nullptr, nullptr, nullptr,
+                               Cs, nn, nR, AR));
+    }
+    TRY(launch_copy3<double>(s, P(y.c_t), P(y.c_W3), P(y.c_tt), P(y.c_W4), nullptr, nullptr, Cs, nn, nU,
+                             AU));
+    TRY(hipMemcpyAsync(mc.data(), ln.d_mc, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    for (int lead = 0; lead < 2; ++lead) {
+      std::vector<int>& act = lead ? actR : actL;
+      std::vector<int> keep;
+      for (int b : act) {
+        const int ch = lead * nb + b;
+        iters[ch] = it;
+        const double m = mc[ch];
+        if (m != m) {
+          status[ch] |= NEGF_STATUS_SR_NAN;  // (:182-191): stop this lead
+        } else if (m > tol) {
+          keep.push_back(b);
+        }
+      }
+      act.swap(keep);
+    }
+  }
+  for (int b : actL) status[b] |= NEGF_STATUS_SR_NOT_CONVERGED;
+  for (int b : actR) status[nb + b] |= NEGF_STATUS_SR_NOT_CONVERGED;
+  // g = inv(alpha + beta*CapT), Sigma = beta*g*beta' for each lead   (:195-209)
+  for (int grp = 0; grp < 2; ++grp) {
+    const zd* beta = grp ? c->betaR : c->betaL;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(grp * R + y.c_CapT), Cs, n,
+                                               P(grp * R + y.c_W1), Cs, n, n), nb, kOne, kZero));
+  }
+  TRY(launch_add<double>(s, P(y.c_work), Cs, P(y.c_W1), Cs, P(y.c_alpha), Cs, nn, 2 * nb, nullptr));
+  TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_W2), Cs, n, n, 2 * nb, w, nullptr));
+  for (int grp = 0; grp < 2; ++grp) {
+    const zd* beta = grp ? c->betaR : c->betaL;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(grp * R + y.c_W2), Cs, n,
+                                               P(grp * R + y.c_W3), Cs, n, n), nb, kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_C, gb(P(grp * R + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
+                                               sSig, n, n), nb, kOne, kZero));
+  }
+  std::vector<int> info(2 * nb);
+  for (int ch = 0; ch < 2 * nb; ++ch)
+    TRY(hipMemcpyAsync(&info[ch], ln.Ic + (size_t)ch * y.Is + 2 * n, sizeof(int),
+                       hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  for (int b = 0; b < nb; ++b) {
+    // the shared inverses belong to both leads' decimations
+    const int shared = info[b];
+    if (shared && !(status[b] & 0xFFFFF)) status[b] |= (shared & 0xFFFFF);
+    const int right = info[nb + b] ? info[nb + b] : shared;
+    if (right && !(status[nb + b] & 0xFFFFF)) status[nb + b] |= (right & 0xFFFFF);
+  }
+  return 0;
+}
+
 // Result of one energy (host side, before accumulation).
 struct EnergyResult {
   double scal[8];
@@ -517,6 +670,34 @@ struct EnergyResult {
   int status = 0;
   int iters[2] = {0, 0};
 };
+
+// Per-energy scalars, site sums, INFO and SR iteration counts back to the host.
+static int collect(negf_ctx* c, Lane& ln, int nb, const std::vector<int>& iters,
+                   const std::vector<int>& sst, std::vector<EnergyResult>& res) {
+  const int n = c->n;
+  const Layout& y = c->lay;
+  hipStream_t s = ln.st;
+  res.assign(nb, EnergyResult());
+  std::vector<double> scal(8 * nb);
+  std::vector<double> site((size_t)y.site_per_e * nb);
+  std::vector<int> ints((size_t)y.Is * nb);
+  TRY(hipMemcpyAsync(scal.data(), ln.d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(site.data(), ln.d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(ints.data(), ln.Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  for (int b = 0; b < nb; ++b) {
+    EnergyResult& e = res[b];
+    std::memcpy(e.scal, &scal[8 * b], sizeof(e.scal));
+    e.site.assign(site.begin() + (size_t)b * y.site_per_e, site.begin() + (size_t)(b + 1) * y.site_per_e);
+    int st = sst[b] | sst[nb + b];
+    const int info = ints[(size_t)b * y.Is + 2 * n];
+    if (info && !(st & 0xFFFFF)) st |= (info & 0xFFFFF);
+    e.status = st;
+    e.iters[0] = iters[b];
+    e.iters[1] = iters[nb + b];
+  }
+  return 0;
+}
 
 // Fused solve of nb energies (nb <= cap) resident in the arena.
 static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std::vector<EnergyResult>& res) {
@@ -536,8 +717,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
                               ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
   std::vector<int> iters, sst;
   // left: beta = -HopX', betad = -HopX;  right: beta = -HopX, betad = -HopX'
-  int rc = run_sr(c, ln, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL), Q(y.o_SigR), Es,
-                  iters, sst);
+  const bool paired = c->ends_equal && !(c->cfg.flags & NEGF_FLAG_SR_SEPARATE_LEADS);
+  int rc = paired ? run_sr_paired(c, ln, nb, Q(y.o_SigL), Q(y.o_SigR), Es, iters, sst)
+                  : run_sr(c, ln, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL),
+                           Q(y.o_SigR), Es, iters, sst);
   if (rc) return rc;
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
@@ -598,7 +781,54 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     return 0;
   };
   int r;
-  // ---- pass 1: gL
+  if (!(c->cfg.flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    // ---- two-pass RGF.  Forward: gL_x (Recursive_Greens.F90:16-34) and the forward
+    // substitution z_x = gL_x X' z_{x-1}, z_1 = gL_1, of A Y = E_1 (block column 1).
+    auto zs = [&](int x) { return Q(y.o_z + (long long)x * nn); };
+    for (int x = 0; x < L; ++x) {
+      if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
+      TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
+      if (x == 0) {
+        TRY(launch_copy3<double>(s, zs(0), slot(0), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
+      } else {
+        if ((r = xmul(1, Q(y.o_Y), zs(x - 1)))) return r;
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
+                                 kOne, kZero));
+      }
+    }
+    // Backward: the last column G_xL = gL_x X G_{x+1,L}, G_LL = gL_L (:53-63), and the
+    // back substitution G_x1 = z_x + gL_x X G_{x+1,1}, G_L1 = z_L, in place of z_x.
+    // Slice x then has both columns: the Gn/Gp site blocks of both leads, T12 at x = 1
+    // (Calculate_Observables.F90:86-100) and the Modec terms at the ends (:48-53).
+    zd* colA = slot(L - 1);  // G_{x+1,L}
+    zd* colB = nullptr;      // G_{x,L}
+    zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
+    int pb = 0;
+    for (int x = L - 1; x >= 0; --x) {
+      if (x < L - 1) {
+        colB = bufs[pb];
+        if ((r = xmul(0, Q(y.o_Y), colA))) return r;
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                                 kOne, kZero));
+        if ((r = xmul(0, Q(y.o_P), zs(x + 1)))) return r;
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_P), Es, n, zs(x), Es, n, n), nb,
+                                 kOne, kOne));
+      }
+      const zd* GxL = (x < L - 1) ? colB : colA;
+      if ((r = siteblocks(1, x, GxL))) return r;
+      if (x == L - 1 && (r = trace_term(1, GxL, 3))) return r;  // Modec2, right-lead term
+      if (x == 0 && (r = trace_term(0, GxL, 0))) return r;      // T12
+      if ((r = siteblocks(0, x, zs(x)))) return r;
+      if (x == L - 1 && (r = trace_term(1, zs(x), 2))) return r;  // Modec2, left-lead term
+      if (x == 0 && (r = trace_term(0, zs(x), 1))) return r;      // Modec1, left-lead term
+      if (x < L - 1) {
+        colA = colB;
+        pb ^= 1;
+      }
+    }
+    return collect(c, ln, nb, iters, sst, res);
+  }
+  // ---- three-pass RGF, pass 1: gL
   for (int x = 0; x < L; ++x) {
     if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
     TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
@@ -637,27 +867,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;
     std::swap(colA, colB);
   }
-  // ---- results to host
-  res.assign(nb, EnergyResult());
-  std::vector<double> scal(8 * nb);
-  std::vector<double> site((size_t)y.site_per_e * nb);
-  std::vector<int> ints((size_t)y.Is * nb);
-  TRY(hipMemcpyAsync(scal.data(), ln.d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
-  TRY(hipMemcpyAsync(site.data(), ln.d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
-  TRY(hipMemcpyAsync(ints.data(), ln.Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
-  TRY(hipStreamSynchronize(s));
-  for (int b = 0; b < nb; ++b) {
-    EnergyResult& e = res[b];
-    std::memcpy(e.scal, &scal[8 * b], sizeof(e.scal));
-    e.site.assign(site.begin() + (size_t)b * y.site_per_e, site.begin() + (size_t)(b + 1) * y.site_per_e);
-    int st = sst[b] | sst[nb + b];
-    const int info = ints[(size_t)b * y.Is + 2 * n];
-    if (info && !(st & 0xFFFFF)) st |= (info & 0xFFFFF);
-    e.status = st;
-    e.iters[0] = iters[b];
-    e.iters[1] = iters[nb + b];
-  }
-  return 0;
+  return collect(c, ln, nb, iters, sst, res);
 }
 
 // Accumulate one energy's results in the reference's output layout.
@@ -941,9 +1151,11 @@ extern "C" int negf_zgemm(char transa, char transb, int n, const double* alpha, 
 
 // ---------------------------------------------------------------------------
 // Instrumentation: MFMA GEMM launch timing (no counterpart in the reference).
-extern "C" int negf_profile_begin(void) {
+extern "C" int negf_profile_begin(int sample_every) {
   GemmTimer& t = gemm_timer();
   t.used = 0;
+  t.seen = 0;
+  t.every = sample_every > 1 ? sample_every : 1;
   t.on = true;
   return 0;
 }
@@ -952,16 +1164,21 @@ extern "C" int negf_profile_end(double* flops, double* ms, long long* launches) 
   GemmTimer& t = gemm_timer();
   t.on = false;
   HIPCHK(hipDeviceSynchronize());
-  double f = 0.0, m = 0.0;
+  double f[kTimerKinds] = {}, m[kTimerKinds] = {};
+  long long cnt[kTimerKinds] = {};
   for (size_t i = 0; i < t.used; ++i) {
     float e = 0.f;
     HIPCHK(hipEventElapsedTime(&e, t.ev[2 * i], t.ev[2 * i + 1]));
-    f += t.flops[i];
-    m += e;
+    const int k = t.kind[i];
+    f[k] += t.flops[i];
+    m[k] += e;
+    ++cnt[k];
   }
-  if (flops) *flops = f;
-  if (ms) *ms = m;
-  if (launches) *launches = (long long)t.used;
+  for (int k = 0; k < kTimerKinds; ++k) {
+    if (flops) flops[k] = f[k];
+    if (ms) ms[k] = m[k];
+    if (launches) launches[k] = cnt[k];
+  }
   t.used = 0;
   return 0;
 }

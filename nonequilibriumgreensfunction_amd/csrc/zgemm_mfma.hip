@@ -259,10 +259,11 @@ static std::mutex g_timer_mu;
 
 // Reserves an event pair for one launch; the handles are copied out under the lock
 // (the event array may grow while another host thread launches).
-static bool timer_slot(double flops, hipEvent_t* e0, hipEvent_t* e1) {
+static bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
   GemmTimer& t = gemm_timer();
   if (!t.on) return false;
   std::lock_guard<std::mutex> lock(g_timer_mu);
+  if (t.seen++ % (unsigned long long)t.every != 0) return false;
   if (t.used == t.cap) {
     size_t nc = t.cap ? 2 * t.cap : 4096;
     hipEvent_t* ev = (hipEvent_t*)realloc(t.ev, sizeof(hipEvent_t) * 2 * nc);
@@ -271,10 +272,14 @@ static bool timer_slot(double flops, hipEvent_t* e0, hipEvent_t* e1) {
     double* fl = (double*)realloc(t.flops, sizeof(double) * nc);
     if (!fl) return false;
     t.flops = fl;
+    int* kd = (int*)realloc(t.kind, sizeof(int) * nc);
+    if (!kd) return false;
+    t.kind = kd;
     for (size_t i = 2 * t.cap; i < 2 * nc; ++i) hipEventCreate(&ev[i]);
     t.cap = nc;
   }
   t.flops[t.used] = flops;
+  t.kind[t.used] = kind;
   *e0 = t.ev[2 * t.used];
   *e1 = t.ev[2 * t.used + 1];
   ++t.used;
@@ -301,7 +306,8 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   if (batch <= 0 || g.m <= 0 || g.n <= 0) return hipSuccess;
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
-  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, &ev0, &ev1);
+  const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0);
+  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, kind, &ev0, &ev1);
   if (timed) hipEventRecord(ev0, s);
   int cfg = (EPI == EPI_GJ) ? g_tile_gj : g_tile_std;
   if (cfg < 0) cfg = 0;

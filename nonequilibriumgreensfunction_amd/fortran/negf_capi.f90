@@ -22,6 +22,7 @@ module negf_capi
     integer(c_int) :: device
     real(c_double) :: hbm_budget_gb
     integer(c_int) :: max_batch
+    integer(c_int) :: flags
   end type negf_config
 
   interface

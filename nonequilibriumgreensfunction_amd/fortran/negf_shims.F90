@@ -37,6 +37,7 @@ module negf_shim_ctl
   integer(c_int), save :: negf_shim_sr_extra_eta = 1
   integer(c_int), save :: negf_shim_precision = NEGF_PREC_FP64
   integer(c_int), save :: negf_shim_device = 0
+  integer(c_int), save :: negf_shim_flags = 0      ! NEGF_FLAG_* algorithm selection
   logical, save :: negf_shim_staged = .false.
 contains
   subroutine negf_shim_fail(where, rc)
@@ -66,6 +67,7 @@ subroutine NEGF_Prepare_Device
     cfg%device = negf_shim_device
     cfg%hbm_budget_gb = 0d0
     cfg%max_batch = 0
+    cfg%flags = negf_shim_flags
     rc = negf_init(negf_shim_ctx, cfg)
     if (rc /= 0) call negf_shim_fail("negf_init", rc)
     negf_shim_h_ite = -huge(1)

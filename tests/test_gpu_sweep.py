@@ -19,10 +19,10 @@ import parity as P  # noqa: E402
 MU, MUT = (-0.1, 0.0), 0.0
 
 
-def _gpu_sweep(w, energies, dE, eta, tol, maxit, batched=True, max_batch=0):
+def _gpu_sweep(w, energies, dE, eta, tol, maxit, batched=True, max_batch=0, flags=0):
     acc = capi.Accumulators(w.npx, w.npy, w.npz, len(energies))
     with capi.Context(w.npx, w.npy, w.npz, w.norb, eta=eta, sr_tol=tol, sr_max_iter=maxit,
-                      max_batch=max_batch) as ctx:
+                      max_batch=max_batch, flags=flags) as ctx:
         ctx.set_hamiltonian(w)
         if batched:
             ctx.submit(energies, np.arange(len(energies)), np.full(len(energies), dE))
@@ -64,6 +64,28 @@ def test_sweep_matches_oracle(case):
     w = build_wire(npx, npy, npz, **kw)
     ref, noise = P.reference(w, energies, 0.01, mu=MU, muT=MUT)
     acc, st, it = _gpu_sweep(w, energies, 0.01, 1e-6, 1e-10, 200)
+    assert np.all(st == 0), st
+    _compare(acc, ref, noise, it)
+
+
+def _wire_variant(which):
+    w = build_wire(6, 3, 2, disorder=0.6, seed=9, flux=0.2, zeeman=0.02)
+    if which == "unequal_ends":
+        w.htotd[:, :, -1] += 0.15 * np.eye(w.n)  # a potential step on the right lead
+    return w
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+@pytest.mark.parametrize("which", ["equal_ends", "unequal_ends"])
+def test_sweep_algorithm_variants_match_oracle(which, flags):
+    """Two-pass vs three-pass RGF (flag 1) and the shared vs per-lead Sancho-Rubio
+    decimation (flag 2; shared only applies when the end slices are equal)."""
+    assert capi.FLAG_RGF_THREE_PASS == 1 and capi.FLAG_SR_SEPARATE_LEADS == 2
+    w = _wire_variant(which)
+    energies = [-1.7, -0.2, 0.15, 1.2]
+    print(f"variant {which} flags={flags}", flush=True)
+    ref, noise = P.reference(w, energies, 0.01, mu=MU, muT=MUT)
+    acc, st, it = _gpu_sweep(w, energies, 0.01, 1e-6, 1e-10, 200, flags=flags)
     assert np.all(st == 0), st
     _compare(acc, ref, noise, it)
 
