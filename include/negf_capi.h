@@ -73,6 +73,10 @@ typedef struct {
  * are each other's swapped; SR_SEPARATE_LEADS runs one decimation per lead. */
 #define NEGF_FLAG_RGF_THREE_PASS 1
 #define NEGF_FLAG_SR_SEPARATE_LEADS 2
+/* negf_flush runs on the calling thread and one HIP stream instead of two
+ * concurrent lanes (for tools that cannot follow kernels dispatched from two
+ * host threads, e.g. per-dispatch hardware counter collection). */
+#define NEGF_FLAG_SINGLE_LANE 4
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the

@@ -50,6 +50,7 @@ class NegfConfig(ctypes.Structure):
 
 FLAG_RGF_THREE_PASS = 1
 FLAG_SR_SEPARATE_LEADS = 2
+FLAG_SINGLE_LANE = 4
 
 
 class NegfAccum(ctypes.Structure):

@@ -333,7 +333,7 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
 // Allocate the lanes' arenas for `want` energies in flight in total (capped by the
 // HBM budget); each lane gets half.
 static int ensure_arena(negf_ctx* c, int want) {
-  const int nl = want >= 2 ? c->nlanes : 1;
+  const int nl = (want >= 2 && !(c->cfg.flags & NEGF_FLAG_SINGLE_LANE)) ? c->nlanes : 1;
   const int per_lane = (want + nl - 1) / nl;
   if (c->lanes[0].E && c->lanes[0].cap >= per_lane && (nl == 1 || c->lanes[1].E)) return 0;
   const Layout& y = c->lay;
