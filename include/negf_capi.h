@@ -194,8 +194,9 @@ int negf_zgemm(char transa, char transb, int n, const double* alpha, const doubl
  * launch).  end() synchronises the
  * device and returns, per kind, the launches' algorithmic flops (8 m n k per
  * complex product), their summed duration in ms and the launch count; each
- * argument is an array of 4: [0] FP64 product, [1] FP64 Gauss-Jordan trailing
- * update, [2] FP32 product, [3] FP32 trailing update. */
+ * argument is an array of 8 indexed by 4 * (3M complex form) + 2 * (FP32) +
+ * (Gauss-Jordan trailing update): [0] FP64 product, [1] FP64 update, [2] FP32
+ * product, [3] FP32 update, [4..7] the same in the 3M form. */
 int negf_profile_begin(int sample_every);
 int negf_profile_end(double* flops, double* ms, long long* launches);
 

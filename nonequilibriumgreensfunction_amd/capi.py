@@ -91,7 +91,7 @@ def lib():
         L.negf_zgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.negf_profile_begin.argtypes = [ctypes.c_int]
-        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 4
+        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 8
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -123,15 +123,16 @@ def profile_begin(sample_every=1):
     _check(lib().negf_profile_begin(int(sample_every)))
 
 
-PROFILE_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update")
+PROFILE_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update",
+                 "fp64_gemm_3m", "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
 
 
 def profile_end():
     """{kind: (flops, ms, launches)} of the MFMA GEMM launches since profile_begin(),
     kinds as in PROFILE_KINDS."""
-    f = (ctypes.c_double * 4)()
-    m = (ctypes.c_double * 4)()
-    n = (ctypes.c_longlong * 4)()
+    f = (ctypes.c_double * 8)()
+    m = (ctypes.c_double * 8)()
+    n = (ctypes.c_longlong * 8)()
     _check(lib().negf_profile_end(f, m, n))
     return {k: (f[i], m[i], n[i]) for i, k in enumerate(PROFILE_KINDS)}
 

@@ -223,6 +223,7 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
         g.active = active; g.pivstep = w.pivstep; g.sPiv = w.sInt;
         g.pv_lo = c0; g.pv_hi = c0 + nb;
         g.skip_lo = c0 - k0; g.skip_hi = c0 - k0 + nb;
+        g.m3 = w.m3;
         hipError_t e = gemm_gj_update<T>(s, g, batch);
         if (e != hipSuccess) return e;
       }
@@ -236,6 +237,7 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
       g.active = active; g.pivstep = w.pivstep; g.sPiv = w.sInt;
       g.pv_lo = k0; g.pv_hi = k0 + kb;
       g.skip_lo = k0; g.skip_hi = k0 + kb;
+      g.m3 = w.m3;
       hipError_t e = gemm_gj_update<T>(s, g, batch);
       if (e != hipSuccess) return e;
     }

@@ -60,6 +60,8 @@ struct GemmBatch {
   long long sPiv;
   int pv_lo, pv_hi;
   int skip_lo, skip_hi;
+  int m3;  // 1: 3-product complex MFMA form (normwise-accurate, 3/4 of the work);
+           // 0: 4-product form (each of Re, Im accurate relative to its own terms)
 };
 
 // Spin matrices gMx, gMy, gMz (norb <= 4) as kernel arguments.
@@ -76,6 +78,7 @@ struct GjWork {
   int* info;        // 1: first zero pivot (1-based) or 0
   long long sProw;  // batch stride of prow (complex elements)
   long long sInt;   // batch stride of pivstep, pivrow and info (ints)
+  int m3 = 0;       // complex product form of the trailing updates (GemmBatch::m3)
 };
 
 template <typename T>
@@ -88,9 +91,9 @@ constexpr int kGjNBO = 64;  // Gauss-Jordan outer block width (columns per trail
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
 // every launch is bracketed by HIP events on its own stream and its algorithmic
-// flop count (8 m n k per complex product) is recorded with its kind:
-// 0 FP64 product, 1 FP64 Gauss-Jordan trailing update, 2 FP32 product, 3 FP32 update.
-constexpr int kTimerKinds = 4;
+// flop count (8 m n k per complex product) is recorded with its kind
+// = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing update).
+constexpr int kTimerKinds = 8;
 struct GemmTimer {
   bool on = false;
   int every = 1;         // time one launch in `every` (uniform sample)

@@ -379,6 +379,15 @@ static GemmBatch gb(const zd* A, long long sA, int lda, const zd* B, long long s
   g.active = active;
   return g;
 }
+// RGF products and inverses: 3M complex form (normwise accuracy is what G, Gn
+// and T need); Sancho-Rubio, Sigma and Gamma keep the 4-product form, whose
+// componentwise accuracy holds the O(eta) anti-Hermitian part of Sigma in a gap.
+static GemmBatch gbr(const zd* A, long long sA, int lda, const zd* B, long long sB, int ldb, zd* C,
+                     long long sC, int ldc, int n, const int* active = nullptr) {
+  GemmBatch g = gb(A, sA, lda, B, sB, ldb, C, sC, ldc, n, active);
+  g.m3 = 1;
+  return g;
+}
 static const zd kOne = {1.0, 0.0}, kZero = {0.0, 0.0}, kMone = {-1.0, 0.0};
 
 #define TRY(expr)                                                                     \
@@ -726,6 +735,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
 
   GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
+  w.m3 = 1;
   const bool bd = c->blockdiag;
   auto slot = [&](int x) { return Q(y.o_slots + (long long)x * nn); };
   auto hx = [&](int x) { return (const zd*)(c->H + (size_t)x * nn); };
@@ -738,11 +748,11 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       zd* T1 = Q(y.o_P);
       zd* T2 = Q(y.o_Y);
       if (mode == 1) {
-        TRY(gemm_batched<double>(s, OP_C, OP_N, gb(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+        TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
       } else {
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
-        TRY(gemm_batched<double>(s, OP_N, OP_C, gb(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
+        TRY(gemm_batched<double>(s, OP_N, OP_C, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
       }
       TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy, c->cfg.eta, S1,
                                   Es, S2, Es, T2, Es, c->X, n, nb, nullptr));
@@ -757,7 +767,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     if (bd) {
       TRY(launch_bd_mul<double>(s, no, conj, Y, Es, G, Es, c->X, n, nb, nullptr));
     } else {
-      TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gb(c->X, 0, n, G, Es, n, Y, Es, n, n), nb,
+      TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gbr(c->X, 0, n, G, Es, n, Y, Es, n, n), nb,
                                kOne, kZero));
     }
     return 0;
@@ -765,7 +775,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   // Gn/Gp site blocks of column block G (lead 0 = left/Gamma_L, 1 = right/Gamma_R) at slice x
   auto siteblocks = [&](int lead, int x, const zd* G) -> int {
     const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(G, Es, n, Gam, Es, n, Q(y.o_Bm), Es, n, n), nb, kOne,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(G, Es, n, Gam, Es, n, Q(y.o_Bm), Es, n, n), nb, kOne,
                              kZero));
     double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
     TRY(launch_siteblocks<double>(s, no, Q(y.o_Bm), Es, G, Es, n, out, y.site_per_e,
@@ -775,7 +785,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   // scal[idx] = Re <Gam_lead * Bm, G>
   auto trace_term = [&](int lead, const zd* G, int idx) -> int {
     const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(Gam, Es, n, Q(y.o_Bm), Es, n, Q(y.o_P), Es, n, n), nb,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(Gam, Es, n, Q(y.o_Bm), Es, n, Q(y.o_P), Es, n, n), nb,
                              kOne, kZero));
     TRY(launch_inner<double>(s, Q(y.o_P), Es, G, Es, n, ln.d_scal + idx, 8, nb, nullptr));
     return 0;
@@ -792,7 +802,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         TRY(launch_copy3<double>(s, zs(0), slot(0), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
       } else {
         if ((r = xmul(1, Q(y.o_Y), zs(x - 1)))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
                                  kOne, kZero));
       }
     }
@@ -808,10 +818,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       if (x < L - 1) {
         colB = bufs[pb];
         if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
                                  kOne, kZero));
         if ((r = xmul(0, Q(y.o_P), zs(x + 1)))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_P), Es, n, zs(x), Es, n, n), nb,
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_P), Es, n, zs(x), Es, n, n), nb,
                                  kOne, kOne));
       }
       const zd* GxL = (x < L - 1) ? colB : colA;
@@ -844,7 +854,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(L - 1), Es, n, n, nb, w, nullptr));
   for (int x = L - 2; x >= 0; --x) {
     if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
                              kOne, kZero));
     if ((r = siteblocks(1, x, colB))) return r;
     if (x == 0 && (r = trace_term(0, colB, 0))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
@@ -861,7 +871,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   if (L == 1 && (r = trace_term(1, colA, 2))) return r;  // Modec2, left-lead term
   for (int x = 1; x < L; ++x) {
     if ((r = xmul(1, Q(y.o_Y), colA))) return r;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
                              kOne, kZero));
     if ((r = siteblocks(0, x, colB))) return r;
     if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;

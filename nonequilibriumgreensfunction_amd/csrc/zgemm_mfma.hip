@@ -1,10 +1,18 @@
 // Batched complex GEMM on CDNA4 matrix cores (gfx950).
 //
-// The complex product is built from four real MFMA products per 16x16x4 step:
-//   Re C += Re A Re B - Im A Im B,   Im C += Re A Im B + Im A Re B
-// using v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32).
+// Two forms of the complex product per 16x16x4 step, chosen per launch
+// (GemmBatch::m3):
+//  * 4-product form: Re C += Re A Re B - Im A Im B, Im C += Re A Im B + Im A Re B.
+//    Each component is accurate relative to its own terms, so a small imaginary
+//    part (the O(eta) anti-Hermitian part of a lead self-energy inside a gap)
+//    keeps its relative accuracy.
+//  * 3M form: P1 += Re A Re B, P2 += Im A Im B, P3 += (Re A + Im A)(Re B + Im B),
+//    and at the end Re C = P1 - P2, Im C = P3 - P1 - P2: 3/4 of the matrix-core
+//    work, sums formed in registers.  Its error is normwise,
+//    O(eps (|Re A|+|Im A|)(|Re B|+|Im B|)) per term for both components.
+// Uses v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32).
 // A 256-thread workgroup owns a 64x64 tile of C; its four wave64s each own a
-// 32x32 quadrant (2x2 MFMA blocks, separate real and imaginary accumulators).
+// 32x32 quadrant (2x2 MFMA blocks; three accumulators in the 3M form, two otherwise).
 // Tiles of op(A) (64x16) and op(B) (16x64) are staged through LDS as split
 // real/imaginary planes, k-major, so that each MFMA operand read is 16
 // consecutive LDS words; the next K slab is fetched into registers while the
@@ -70,8 +78,12 @@ __device__ inline int xcd_tile(int bid, int ntiles) {
 // stage feeds the MFMAs, so each slab costs one barrier.
 // PRE = 1 (Gauss-Jordan epilogue with short K): the old C values and pivot flags
 // are loaded before the K loop, so their latency overlaps the A/B traffic.
-template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int PRE>
-__global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int ntiles,
+// Two workgroups per CU (launch bound): the FP64 64x64 variant then fits 199 VGPRs
+// with no AGPR accumulators and no spills, and the second workgroup hides the
+// LDS/global latency that one wave per SIMD leaves exposed (measured 53 -> 70 TF/s
+// algorithmic at n = 1024, 55 -> 75 TF/s at n = 2304).
+template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int PRE, int M3>
+__global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, int ntiles,
                                                    cx<T> alpha, cx<T> beta) {
   typedef typename Mfma<T>::acc acc_t;
   constexpr int BM = 32 * SM, BN = 32 * SN;
@@ -97,13 +109,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
   const int wm = (wave & 1) * 16 * SM, wn = (wave >> 1) * 16 * SN;
   const int lr = lane & 15, lk = lane >> 4;
 
-  acc_t cr[SM][SN], ci[SM][SN];
+  acc_t p1[SM][SN], p2[SM][SN], p3[SM][SN];
 #pragma unroll
   for (int i = 0; i < SM; ++i)
 #pragma unroll
     for (int j = 0; j < SN; ++j) {
-      cr[i][j] = acc_t{0, 0, 0, 0};
-      ci[i][j] = acc_t{0, 0, 0, 0};
+      p1[i][j] = acc_t{0, 0, 0, 0};
+      p2[i][j] = acc_t{0, 0, 0, 0};
+      p3[i][j] = acc_t{0, 0, 0, 0};
     }
 
   const int* piv = (EPI == EPI_GJ) ? g.pivstep + (long long)b * g.sPiv : nullptr;
@@ -182,32 +195,45 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
       const int kk = ks * 4 + lk;
-      T ar[SM], ai[SM], nai[SM], br[SN], bi[SN];
+      T ar[SM], ai[SM], as[SM], br[SN], bi[SN], bs[SN];
 #pragma unroll
       for (int s = 0; s < SM; ++s) {
         ar[s] = sA[st][0][kk][wm + 16 * s + lr];
         ai[s] = sA[st][1][kk][wm + 16 * s + lr];
-        nai[s] = -ai[s];
+        as[s] = M3 ? ar[s] + ai[s] : -ai[s];
       }
 #pragma unroll
       for (int s = 0; s < SN; ++s) {
         br[s] = sB[st][0][kk][wn + 16 * s + lr];
         bi[s] = sB[st][1][kk][wn + 16 * s + lr];
+        bs[s] = br[s] + bi[s];
       }
+      if (M3) {
 #pragma unroll
-      for (int sm = 0; sm < SM; ++sm)
+        for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
-        for (int sn = 0; sn < SN; ++sn) {
-          cr[sm][sn] = Mfma<T>::mma(ar[sm], br[sn], cr[sm][sn]);
-          ci[sm][sn] = Mfma<T>::mma(ar[sm], bi[sn], ci[sm][sn]);
-        }
+          for (int sn = 0; sn < SN; ++sn) {
+            p1[sm][sn] = Mfma<T>::mma(ar[sm], br[sn], p1[sm][sn]);
+            p2[sm][sn] = Mfma<T>::mma(ai[sm], bi[sn], p2[sm][sn]);
+            p3[sm][sn] = Mfma<T>::mma(as[sm], bs[sn], p3[sm][sn]);
+          }
+      } else {
+        // 4-product form: p1 = Re C, p3 = Im C (p2 unused)
 #pragma unroll
-      for (int sm = 0; sm < SM; ++sm)
+        for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
-        for (int sn = 0; sn < SN; ++sn) {
-          cr[sm][sn] = Mfma<T>::mma(nai[sm], bi[sn], cr[sm][sn]);
-          ci[sm][sn] = Mfma<T>::mma(ai[sm], br[sn], ci[sm][sn]);
-        }
+          for (int sn = 0; sn < SN; ++sn) {
+            p1[sm][sn] = Mfma<T>::mma(ar[sm], br[sn], p1[sm][sn]);
+            p3[sm][sn] = Mfma<T>::mma(ar[sm], bi[sn], p3[sm][sn]);
+          }
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+          for (int sn = 0; sn < SN; ++sn) {
+            p1[sm][sn] = Mfma<T>::mma(as[sm], bi[sn], p1[sm][sn]);
+            p3[sm][sn] = Mfma<T>::mma(ai[sm], br[sn], p3[sm][sn]);
+          }
+      }
     }
     if (kt + 1 < nk) stash(st ^ 1);
     __syncthreads();
@@ -227,7 +253,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
         const int row = m0 + wm + 16 * sm + Mfma<T>::row(lk, r);
         if (row >= M) continue;
         cx<T>* pc = C + (long long)col * g.ldc + row;
-        cx<T> v = mk<T>(cr[sm][sn][r], ci[sm][sn][r]);
+        const T q1 = p1[sm][sn][r], q2 = p2[sm][sn][r];
+        cx<T> v = M3 ? mk<T>(q1 - q2, p3[sm][sn][r] - q1 - q2) : mk<T>(q1, p3[sm][sn][r]);
         if (EPI == EPI_GJ && PRE) {
           if (kpre[sm][r]) v = cadd(v, cpre[sm][sn][r]);
         } else if (EPI == EPI_GJ) {
@@ -242,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int tiles_m, int
     }
 }
 
-// Tile configurations (SM, SN): 0 = 64x64, 1 = 128x64, 2 = 64x128, 3 = 128x128.
+// Tile configurations (SM, SN): 0 = 64x64, 1 = 128x64, 2 = 64x128.
 static int g_tile_std = -1;  // -1: choose by shape
 static int g_tile_gj = -1;
 void gemm_set_tiles(int std_cfg, int gj_cfg) {
@@ -286,17 +313,17 @@ static bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
   return true;
 }
 
-template <typename T, int OPA, int OPB, int EPI, int SM, int SN>
+template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int M3>
 static hipError_t launch_cfg(hipStream_t s, const GemmBatch& g, int batch, cx<T> alpha, cx<T> beta) {
   constexpr int BM = 32 * SM, BN = 32 * SN;
   int tm = (g.m + BM - 1) / BM, tn = (g.n + BN - 1) / BN;
   int nt = tm * tn;
   dim3 grid(((nt + 7) / 8) * 8, 1, batch);
   if (EPI == EPI_GJ && g.k <= 32)
-    hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 1>), grid, dim3(kThreads), 0, s, g, tm, nt,
+    hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 1, M3>), grid, dim3(kThreads), 0, s, g, tm, nt,
                        alpha, beta);
   else
-    hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 0>), grid, dim3(kThreads), 0, s, g, tm, nt,
+    hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 0, M3>), grid, dim3(kThreads), 0, s, g, tm, nt,
                        alpha, beta);
   return hipGetLastError();
 }
@@ -306,17 +333,24 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   if (batch <= 0 || g.m <= 0 || g.n <= 0) return hipSuccess;
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
-  const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0);
+  const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
   const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, kind, &ev0, &ev1);
   if (timed) hipEventRecord(ev0, s);
   int cfg = (EPI == EPI_GJ) ? g_tile_gj : g_tile_std;
   if (cfg < 0) cfg = 0;
   hipError_t e;
-  switch (cfg) {
-    case 1: e = launch_cfg<T, OPA, OPB, EPI, 4, 2>(s, g, batch, alpha, beta); break;
-    case 2: e = launch_cfg<T, OPA, OPB, EPI, 2, 4>(s, g, batch, alpha, beta); break;
-    case 3: e = launch_cfg<T, OPA, OPB, EPI, 4, 4>(s, g, batch, alpha, beta); break;
-    default: e = launch_cfg<T, OPA, OPB, EPI, 2, 2>(s, g, batch, alpha, beta); break;
+  if (g.m3) {
+    switch (cfg) {
+      case 1: e = launch_cfg<T, OPA, OPB, EPI, 4, 2, 1>(s, g, batch, alpha, beta); break;
+      case 2: e = launch_cfg<T, OPA, OPB, EPI, 2, 4, 1>(s, g, batch, alpha, beta); break;
+      default: e = launch_cfg<T, OPA, OPB, EPI, 2, 2, 1>(s, g, batch, alpha, beta); break;
+    }
+  } else {
+    switch (cfg) {
+      case 1: e = launch_cfg<T, OPA, OPB, EPI, 4, 2, 0>(s, g, batch, alpha, beta); break;
+      case 2: e = launch_cfg<T, OPA, OPB, EPI, 2, 4, 0>(s, g, batch, alpha, beta); break;
+      default: e = launch_cfg<T, OPA, OPB, EPI, 2, 2, 0>(s, g, batch, alpha, beta); break;
+    }
   }
   if (timed) hipEventRecord(ev1, s);
   return e;
