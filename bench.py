@@ -1,16 +1,23 @@
 """Throughput benchmark of the per-energy NEGF solve on one MI355X.
 
 A step is one batch of energy points of one wire solved end to end on the GPU
-(Sancho-Rubio contacts, forward/backward RGF, Gn/Gp site blocks, T12, modes,
-density and spin).  Default workload: the plain-FP64 mid-size wire (4-band TI
-model, Npx=100, Npy=Npz=16 -> block n=1024, 512 energies in [-0.4, 0.4], Anderson
-disorder W=0.5, seed 1); each step takes a strided slice of that energy grid so
-every step samples the whole range.
+(Sancho-Rubio contacts, RGF, Gn/Gp site blocks, T12, modes, density and spin).
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the FP64 MFMA
-complex GEMM, trailing Gauss-Jordan updates included), timed with HIP events on
-the stream it runs on; `cpu_baseline` times the complex128 CPU oracle (MKL) on
-the host on a bounded sample of the same wire, extrapolated to Npx=100.
+Default workload (the configuration the energy-points/s target is stated on):
+the 4-band TI wire with Npx=200, Npy=Npz=24 (block n=2304), 1024 energies in
+[-0.3, 0.3], Anderson disorder W=0.3 seed 1, eta=1e-6, Sancho-Rubio tol 1e-10
+/ max 200; each step takes a strided slice of that grid so every step samples
+the whole range.  The timed form is the mixed-precision form (FP64 contacts,
+inverses and the T12 chain; complex64 first-column substitution and site-block
+products), the fastest form meeting the <= 1e-8 relative transmission error;
+the plain FP64 form is timed on the same energies right after, and the mixed
+form's error against it is reported for every output (the plain form itself is
+pinned to the complex128 CPU oracle by the test suite).
+
+Prints one JSON line.  `roofline` is for the dominant kernel, timed with HIP
+events on the stream it runs on (a uniform sample of its launches inside the
+timed region); `cpu_baseline` times the complex128 CPU oracle (MKL) on the host
+on a bounded sample of the same wire.
 """
 from __future__ import annotations
 
@@ -30,17 +37,22 @@ from nonequilibriumgreensfunction_amd.hamiltonian import build_wire, energy_grid
 
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32_MFMA = 157.3
-# dominant kernel: the FP64 complex product in the 3M form (RGF column, substitution
-# and G*Gamma products), k_gemm<double, N, N, plain epilogue, 64x64, 3M>
-DOMINANT = "fp64_gemm_3m"
 
 CONFIGS = {
-    "c1": dict(npx=20, npy=6, npz=6, ne=64, emin=-0.3, emax=0.3, W=0.0, seed=1, per_step=64),
-    "c2": dict(npx=100, npy=16, npz=16, ne=512, emin=-0.4, emax=0.4, W=0.5, seed=1, per_step=64),
-    "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=16),
+    # small, CPU-runnable: 64 energies of a clean 6x6 wire
+    "c1": dict(npx=20, npy=6, npz=6, ne=64, emin=-0.3, emax=0.3, W=0.0, seed=1, per_step=64,
+               budget_gb=0.0),
+    # plain FP64 mid-size: block 1024, Npx=100, W=0.5
+    "c2": dict(npx=100, npy=16, npz=16, ne=512, emin=-0.4, emax=0.4, W=0.5, seed=1, per_step=64,
+               budget_gb=0.0),
+    # precision target size: block 2304, Npx=200, W=0.3
+    "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=8,
+               budget_gb=220.0),
 }
 ETA, SR_TOL, SR_MAX = 1e-6, 1e-10, 200
 MU, MUT = (-0.1, 0.0), 0.0
+OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", "RhoN", "RhoP",
+           "SigXExp", "SigYExp", "SigZExp")
 
 
 def f_useful(n, L, k_left, k_right):
@@ -50,48 +62,68 @@ def f_useful(n, L, k_left, k_right):
     return 8.0 * n**3 * (6 * L + 3 + 9 * (k_left + k_right))
 
 
-def cpu_baseline(cfg, energy, threads, budget_s=20.0):
-    """Oracle (C++ complex128 + MKL, the reference's operation order) on the host,
-    on a bounded sample: one energy of the same cross-section at two short lengths
-    Npx = a < b (the time of one energy is affine in Npx: Sancho-Rubio plus L
-    slices), extrapolated to the workload's Npx.  b grows until the sample has
-    used about budget_s seconds of CPU time, or reaches the workload's Npx."""
+def cpu_baseline(cfg, energy, threads, k_mean):
+    """The complex128 CPU oracle (MKL, the reference's operation order) timed on the
+    host on a bounded sample of one energy of the same wire, extrapolated to the
+    workload's Npx.  The time of one energy is 2 x (Sancho-Rubio of one lead) +
+    Npx x (per-slice RGF, Gn/Gp and observables).  For blocks up to 1024 a whole
+    energy at Npx = 2 and 4 is timed and the line through them extrapolated; for
+    larger blocks the pieces are timed: one lead's decimation at 1 and 3
+    iterations (per-iteration and fixed cost, scaled to the GPU run's mean
+    iteration count k_mean) and the RGF/Gnp/observables stages at Npx = 2."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # noqa: E402  (checker only)
     O.set_threads(threads)
+    n = cfg["npy"] * cfg["npz"] * 4
 
-    def timed(L):
-        w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
+    def wire(L):
+        return build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
+
+    if n <= 1024:
+        t = {}
+        for L in (2, 4):
+            w = wire(L)
+            t0 = time.perf_counter()
+            O.energy_sweep(w, energy, 0.0, MU, MUT, ETA, SR_TOL, SR_MAX, True)
+            t[L] = time.perf_counter() - t0
+        per_slice = max(t[4] - t[2], 0.0) / 2.0
+        t_full = t[2] + per_slice * (cfg["npx"] - 2)
+        sample = (f"oracle, 1 energy (E={energy:.4f}), Npy=Npz={cfg['npy']}, MKL {threads} threads, "
+                  f"at Npx=2 ({t[2]:.1f}s) and 4 ({t[4]:.1f}s); extrapolated affinely to "
+                  f"Npx={cfg['npx']}: {t_full:.1f}s per energy")
+        return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port",
+                    sample=sample)
+    w = wire(2)
+    alpha = (energy + 1j * ETA) * np.eye(n) - w.htotd[:, :, 0]
+    beta = -w.hopx.conj().T
+    ts = {}
+    for it in (1, 3):
         t0 = time.perf_counter()
-        O.energy_sweep(w, energy, 0.0, MU, MUT, ETA, SR_TOL, SR_MAX, True)
-        return time.perf_counter() - t0
-
-    a = 2
-    t = {a: timed(a)}
-    b = a
-    used = t[a]
-    while b < cfg["npx"] and used < 0.5 * budget_s:
-        b = min(cfg["npx"], max(b + 2, 2 * b))
-        t[b] = timed(b)
-        used += t[b]
-    if b == cfg["npx"]:
-        t_full = t[b]
-    else:
-        per_slice = max(t[b] - t[a], 0.0) / (b - a)
-        t_full = t[a] + per_slice * (cfg["npx"] - a)
-    lens = sorted(t)
-    return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port",
-                sample=(f"oracle, 1 energy (E={energy:.4f}), Npy=Npz={cfg['npy']}, MKL {threads} "
-                        f"threads, at Npx=" + ", ".join(f"{L} ({t[L]:.1f}s)" for L in lens) +
-                        f"; {used:.1f}s of CPU work, extrapolated affinely to Npx={cfg['npx']}: "
-                        f"{t_full:.1f}s per energy"))
+        O.sancho_rubio(alpha, beta, ETA, SR_TOL, it, True)
+        ts[it] = time.perf_counter() - t0
+    per_it = max(ts[3] - ts[1], 0.0) / 2.0
+    fixed = max(ts[1] - per_it, 0.0)
+    c = O.contact_self_energy(w, energy, ETA, SR_TOL, 1, True)  # Sigma values do not change the cost
+    t0 = time.perf_counter()
+    A = O.assemble_invg(w, c["SigmaL"], c["SigmaR"], energy, ETA)
+    r = O.retarded(A, w.hopx)
+    g = O.gnp(w, 1.0, 0.0, 0.01, r["GRLef"], r["GRRgt"], c["GammaL"], c["GammaR"])
+    O.observables(w, 1.0, 0.0, 0.01, g, r["GRRgt"], c["GammaL"], c["GammaR"])
+    t_st = time.perf_counter() - t0
+    t_full = 2.0 * (fixed + per_it * k_mean) + t_st / 2.0 * cfg["npx"]
+    sample = (f"oracle pieces at Npy=Npz={cfg['npy']} (n={n}), E={energy:.4f}, MKL {threads} threads: "
+              f"one lead's Sancho-Rubio at 1 and 3 iterations ({ts[1]:.1f}s, {ts[3]:.1f}s -> "
+              f"{per_it:.2f}s/iteration), RGF+Gnp+observables at Npx=2 ({t_st:.1f}s); "
+              f"{ts[1] + ts[3] + t_st:.1f}s of CPU work; one energy = 2 leads x ({fixed:.2f}s + "
+              f"{k_mean:.1f} GPU-measured iterations) + {cfg['npx']} slices = {t_full:.0f}s")
+    return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port", sample=sample)
 
 
-def traffic_from_profiles(config):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+def traffic_from_profiles(config, kind):
+    """HBM bytes per launch of the dominant kernel, from a committed rocprofv3
     --pmc summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy-kernel
-    calibration), if one exists for this workload; else None."""
-    path = os.path.join(ROOT, "profiles", f"k_gemm_fp64_traffic_{config}.md")
+    calibration) for this workload and form, if one exists; else None."""
+    path = os.path.join(ROOT, "profiles", f"k_gemm_{kind}_traffic_{config}.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -113,14 +145,73 @@ def copy_calibration(mib):
                       "copy_calibration_bytes_written_per_launch": mib * 2**20, "launches": 8}))
 
 
+def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmup, profile_every,
+             label):
+    """Warm up, then time `steps` steps of one form; returns timings, the
+    accumulators of the last timed step and the GEMM profile."""
+    L = cfg["npx"]
+    prec = capi.NEGF_PREC_MIXED if precision == "mixed" else capi.NEGF_PREC_FP64
+    ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
+                       max_batch=per, flags=flags, precision=prec, hbm_budget_gb=cfg["budget_gb"])
+    ctx.set_hamiltonian(w)
+
+    def run_step(k, acc):
+        idx = step_indices(k)
+        ctx.submit(grid[idx], idx, np.full(len(idx), dE))
+        return ctx.flush(MU, MUT, acc)
+
+    for k in range(warmup):
+        t0 = time.perf_counter()
+        run_step(k, capi.Accumulators(L, cfg["npy"], cfg["npz"], cfg["ne"]))
+        print(f"[bench] {label} warmup step {k} {time.perf_counter() - t0:.2f}s", file=sys.stderr,
+              flush=True)
+    capi.profile_begin(profile_every)
+    times, iters, status = [], [], []
+    acc = None
+    t_start = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        acc = capi.Accumulators(L, cfg["npy"], cfg["npz"], cfg["ne"])
+        t0 = time.perf_counter()
+        st, it = run_step(k, acc)  # flush() returns after the device streams have drained
+        times.append(time.perf_counter() - t0)
+        iters.append(it)
+        status.append(st)
+        print(f"[bench] {label} step {k} {times[-1]:.2f}s", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t_start
+    prof = capi.profile_end()
+    ctx.close()
+    return dict(times=times, elapsed=elapsed, iters=np.concatenate(iters), status=status, acc=acc,
+                prof=prof, last_step=warmup + steps - 1)
+
+
+def errors_vs(acc, ref):
+    """Max error of every output relative to its natural scale (T12 and the modes:
+    the open-channel count, at least 1; spin: the density; others: their max)."""
+    out = {}
+    nm = max(1.0, float(np.abs(ref.Modec1).max()))
+    rho = float(np.abs(ref.RhoN).max())
+    for k in OUTPUTS:
+        a, b = np.asarray(getattr(acc, k)), np.asarray(getattr(ref, k))
+        scale = max(float(np.abs(b).max()), 1e-300)
+        if k in ("T12", "Modec1", "Modec2"):
+            scale = nm
+        if k.startswith("Sig"):
+            scale = max(scale, rho)
+        out[k] = float(np.abs(a - b).max()) / scale
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"))
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain-FP64 comparison step of the mixed form")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
     ap.add_argument("--profile-every", type=int, default=32,
@@ -145,56 +236,51 @@ def main():
     grid = energy_grid(cfg["emin"], cfg["emax"], cfg["ne"])
     dE = grid[1] - grid[0]
     nsteps = args.warmup + args.steps
-    stride = max(1, min(nsteps, cfg["ne"] // per))
+    stride = max(1, cfg["ne"] // per)
+    shift = max(1, stride // max(1, nsteps))
 
     def step_indices(k):
-        idx = (np.arange(per) * stride + k) % cfg["ne"]
-        return idx
+        return (np.arange(per) * stride + k * shift) % cfg["ne"]
 
     print(f"[bench] building wire {args.config}: Npx={L} Npy=Npz={cfg['npy']} n={n}", file=sys.stderr,
           flush=True)
     w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
-    ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
-                       max_batch=per, flags=args.flags)
-    ctx.set_hamiltonian(w)
-    acc = capi.Accumulators(L, cfg["npy"], cfg["npz"], cfg["ne"])
-
-    def run_step(k):
-        idx = step_indices(k)
-        ctx.submit(grid[idx], idx, np.full(len(idx), dE))
-        st, it = ctx.flush(MU, MUT, acc)
-        return st, it
-
-    for k in range(args.warmup):
-        t0 = time.perf_counter()
-        run_step(k)
-        print(f"[bench] warmup step {k} {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
-
-    capi.profile_begin(args.profile_every)
-    times, iters, status = [], [], []
-    t_start = time.perf_counter()
-    for k in range(args.warmup, nsteps):
-        t0 = time.perf_counter()
-        st, it = run_step(k)  # flush() returns after the device stream has drained
-        times.append(time.perf_counter() - t0)
-        iters.append(it)
-        status.append(st)
-        print(f"[bench] step {k} {times[-1]:.2f}s", file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t_start
-    prof = capi.profile_end()
-    ctx.close()
-
-    iters = np.concatenate(iters)
+    run = run_form(w, cfg, args.precision, args.flags, per, grid, dE, step_indices, args.steps,
+                   args.warmup, args.profile_every, args.precision)
     npts = per * args.steps
+    elapsed = run["elapsed"]
     value = npts / elapsed
+    compare = None
+    if args.precision == "mixed" and not args.no_compare:
+        # the plain FP64 form on the same energies as the last timed step
+        last = run["last_step"]
+        base = run_form(w, cfg, "fp64", args.flags, per, grid, dE, lambda k: step_indices(last), 1, 0,
+                        args.profile_every, "fp64")
+        plain_rate = per / base["elapsed"]
+        compare = {
+            "plain_fp64_energy_points_per_s": plain_rate,
+            "speedup_vs_plain_fp64": value / plain_rate,
+            "error_vs_plain_fp64": errors_vs(run["acc"], base["acc"]),
+            "note": ("errors: max |mixed - fp64| over the last timed step's energies / the output's "
+                     "natural scale; the plain FP64 form agrees with the complex128 CPU oracle to "
+                     "1e-10 in the test suite"),
+        }
+
+    iters = run["iters"]
     useful = sum(f_useful(n, L, a, b) for a, b in iters)
-    g_flops, g_ms, g_launches = prof[DOMINANT]
+    prof = run["prof"]
+    dominant = "fp32_gemm_3m" if args.precision == "mixed" else "fp64_gemm_3m"
+    peak = PEAK_FP32_MFMA if args.precision == "mixed" else PEAK_FP64_MFMA
+    g_flops, g_ms, g_launches = prof[dominant]
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     kernels = {k: {"tflops": (f / (m * 1e-3) / 1e12 if m > 0 else 0.0), "launches": c,
-                   "avg_launch_ms": m / max(1, c), "busy_ms": m}
+                   "avg_launch_ms": m / max(1, c)}
                for k, (f, m, c) in prof.items() if c}
+    kname = ("k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form, n x n x n per matrix)"
+             if args.precision == "mixed" else
+             "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
     result = {
-        "metric": "energy-points/s (plain FP64 form, one GPU)",
+        "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
         "value": value,
         "unit": "energy-points/s",
         "n_gpus": 1,
@@ -204,37 +290,38 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+c64 (mixed)" if args.precision == "mixed" else "f64",
         "data": "synthetic (seeded 4-band TI wire, host-built Hamiltonian)",
         "config": {"workload": (f"{args.config}: Npx={L}, Npy=Npz={cfg['npy']} (block {n}), "
                                 f"{cfg['ne']} energies in [{cfg['emin']},{cfg['emax']}], "
                                 f"Anderson W={cfg['W']} seed {cfg['seed']}, eta={ETA}, "
-                                f"SR tol {SR_TOL} max {SR_MAX}"),
-                   "energies_per_step": per, "precision": "fp64"},
-        "latency_ms": {"p50": 1e3 * float(np.percentile(times, 50)),
-                       "p99": 1e3 * float(np.percentile(times, 99)),
-                       "note": "energies of a step complete together: per-energy solve latency = step time"},
+                                f"SR tol {SR_TOL} max {SR_MAX}; {args.precision} form"),
+                   "energies_per_step": per, "precision": args.precision, "flags": args.flags},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(run["times"], 50)),
+                       "p99": 1e3 * float(np.percentile(run["times"], 99)),
+                       "note": ("a step's energies complete together: per-energy solve latency = "
+                                "step time")},
         "sr_iters_mean": float(iters.mean()),
-        "status_nonzero": int(sum(int(np.count_nonzero(s)) for s in status)),
+        "status_nonzero": int(sum(int(np.count_nonzero(s)) for s in run["status"])),
         "path_tflops_useful": useful / elapsed / 1e12,
         "path_frac_fp64_peak": useful / elapsed / 1e12 / PEAK_FP64_MFMA,
-        "roofline": {"bound": "mfma",
-                     "kernel": ("k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form, "
-                                "n x n x n per matrix)"),
-                     "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / PEAK_FP64_MFMA, "traffic": traffic_from_profiles(args.config),
+        "roofline": {"bound": "mfma", "kernel": kname,
+                     "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
+                     "frac": gemm_tflops / peak, "traffic": traffic_from_profiles(args.config, dominant),
                      "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
                      "note": ("achieved = algorithmic 8 n^3 flops per complex product x batch / HIP-event "
                               "duration on the launching stream (one launch in --profile-every sampled); the "
                               "3M form executes 6 n^3 on the matrix cores; two lanes run concurrently, so "
                               "durations include the other lane's kernels sharing the CUs")},
         "kernels": kernels,
-        "flags": args.flags,
     }
+    if compare is not None:
+        result["compare"] = compare
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         threads = min(args.cpu_threads, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(cfg, float(grid[cfg["ne"] // 3]), threads)
+        result["cpu_baseline"] = cpu_baseline(cfg, float(grid[cfg["ne"] // 3]), threads,
+                                              float(iters.mean()))
         print(f"[bench] cpu baseline {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     print(json.dumps(result), flush=True)
 

@@ -15,8 +15,8 @@
 //
 // Blocking.  Columns are processed in outer blocks of kNBO and inner panels of
 // kNBI.  A panel kernel (one 768- or 1024-thread workgroup per matrix, panel rows
-// held in registers) factors kNBI columns and gathers the pivot rows' values in all
-// other columns into Prow.  A rank-kNBI update brings the rest of the outer
+// held in registers) factors kNBI columns; a gather kernel then copies the pivot
+// rows' values in all other columns into Prow.  A rank-kNBI update brings the rest of the outer
 // block up to date; after the whole outer block a rank-kNBO update (the GEMM
 // kernel's Gauss-Jordan epilogue) is applied to every other column:
 //     A[i][j] <- [p(i) not in block] A[i][j] + sum_m S[i][m] Prow[m][j].
@@ -41,11 +41,18 @@ __global__ void k_gj_init(int* pivstep, int* pivrow, int* info, long long sInt, 
   }
 }
 
+// Two barriers per pivot column: (1) after each wave has published its partial
+// (max, row), every wave reduces the NT/64 partials itself -- no broadcast barrier
+// -- and the thread holding the pivot row publishes the row scaled by 1/pivot;
+// (2) after that, every thread eliminates.  The LDS scratch alternates between two
+// buffers by column parity, so column jj+1 never overwrites what a slower wave
+// still reads for column jj.
 template <typename T, int R, int NT>
 __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, int ld,
                                                              int c0, int nb, int k0, GjWork w,
                                                              const int* active) {
   typedef cx<T> V;
+  constexpr int NW = NT / 64;
   const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
   V* A = S + (long long)b * sS;
   int* pivstep = w.pivstep + (long long)b * w.sInt;
@@ -53,14 +60,11 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
   int* info = w.info + (long long)b * w.sInt;
   V* prow = (V*)w.prow + (long long)b * w.sProw;
 
-  __shared__ T red_v[NT / 64];
-  __shared__ int red_i[NT / 64];
-  __shared__ V pr[kNBI];
-  __shared__ V psc[kNBI];
-  __shared__ int sp_row;
-  __shared__ int sp_rows[kNBI];
+  __shared__ T red_v[2][NW];
+  __shared__ int red_i[2][NW];
+  __shared__ V psc[2][kNBI + 1];  // pivot row scaled by 1/pivot; [kNBI] = 1/pivot
 
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   V v[R][kNBI];
   bool used[R];
 #pragma unroll
@@ -79,76 +83,71 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
 
 #pragma unroll
   for (int jj = 0; jj < kNBI; ++jj) {
-    if (jj >= nb) break;
-    // pivot search: largest |re|+|im| over unused rows, ties to the lower row
-    T best = (T)-1;
-    int bi = 0x7fffffff;
+    if (jj < nb) {
+      const int par = jj & 1;
+      // pivot search: largest |re|+|im| over unused rows, ties to the lower row
+      T best = (T)-1;
+      int bi = 0x7fffffff;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int i = tid + r * NT;
-      if (i < n && !used[r]) {
-        T m = fabs(v[r][jj].x) + fabs(v[r][jj].y);
-        if (m != m) m = (T)INFINITY;
-        if (m > best || (m == best && i < bi)) { best = m; bi = i; }
+      for (int r = 0; r < R; ++r) {
+        const int i = tid + r * NT;
+        if (i < n && !used[r]) {
+          T m = fabs(v[r][jj].x) + fabs(v[r][jj].y);
+          if (m != m) m = (T)INFINITY;
+          if (m > best || (m == best && i < bi)) { best = m; bi = i; }
+        }
       }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      T ob = __shfl_xor(best, off);
-      int oi = __shfl_xor(bi, off);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-    }
-    if ((tid & 63) == 0) { red_v[tid >> 6] = best; red_i[tid >> 6] = bi; }
-    __syncthreads();
-    if (tid < 64) {
-      const int nw = NT / 64;
-      best = tid < nw ? red_v[tid] : (T)-1;
-      bi = tid < nw ? red_i[tid] : 0x7fffffff;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         T ob = __shfl_xor(best, off);
         int oi = __shfl_xor(bi, off);
         if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
       }
+      if (lane == 0) { red_v[par][tid >> 6] = best; red_i[par][tid >> 6] = bi; }
+      __syncthreads();  // (1)
+      best = lane < NW ? red_v[par][lane] : (T)-1;
+      bi = lane < NW ? red_i[par][lane] : 0x7fffffff;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        T ob = __shfl_xor(best, off);
+        int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      const int p = bi;  // the same in every wave
       if (tid == 0) {
-        sp_row = bi;
-        sp_rows[jj] = bi;
         if (best == (T)0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
+        if (p < n) {
+          pivstep[p] = c0 + jj;
+          pivrow[c0 + jj] = p;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (tid + r * NT == p) {
+          const V inv = cinv(v[r][jj]);
+#pragma unroll
+          for (int j = 0; j < kNBI; ++j) psc[par][j] = cmul(v[r][j], inv);
+          psc[par][kNBI] = inv;
+        }
+      __syncthreads();  // (2)
+      const V inv = psc[par][kNBI];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = tid + r * NT;
+        if (i >= n) continue;
+        if (i == p) {
+#pragma unroll
+          for (int j = 0; j < kNBI; ++j) v[r][j] = psc[par][j];
+          v[r][jj] = inv;
+          used[r] = true;
+        } else {
+          const V f = v[r][jj];
+#pragma unroll
+          for (int j = 0; j < kNBI; ++j) v[r][j] = csub(v[r][j], cmul(f, psc[par][j]));
+          v[r][jj] = cneg(cmul(f, inv));
+        }
       }
     }
-    __syncthreads();
-    const int p = sp_row;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (tid + r * NT == p) {
-#pragma unroll
-        for (int j = 0; j < kNBI; ++j) pr[j] = v[r][j];
-      }
-    __syncthreads();
-    const V inv = cinv(pr[jj]);
-    if (tid < kNBI) psc[tid] = cmul(pr[tid], inv);  // pivot row scaled by 1/pivot
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int i = tid + r * NT;
-      if (i >= n) continue;
-      if (i == p) {
-#pragma unroll
-        for (int j = 0; j < kNBI; ++j) v[r][j] = psc[j];
-        v[r][jj] = inv;
-        used[r] = true;
-      } else {
-        const V f = v[r][jj];
-#pragma unroll
-        for (int j = 0; j < kNBI; ++j) v[r][j] = csub(v[r][j], cmul(f, psc[j]));
-        v[r][jj] = cneg(cmul(f, inv));
-      }
-    }
-    if (tid == 0 && p < n) {
-      pivstep[p] = c0 + jj;
-      pivrow[c0 + jj] = p;
-    }
-    __syncthreads();
   }
   // write the factored panel back
 #pragma unroll
@@ -160,16 +159,33 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
         if (j < nb) A[(long long)(c0 + j) * ld + i] = v[r][j];
     }
   }
-  // gather the new pivot rows in every column outside this panel (untouched here)
-  for (int m = 0; m < nb; ++m) {
-    const int p = sp_rows[m];
-    if (p >= n) continue;  // cannot happen: some unused row always exists
-    V* dst = prow + (long long)(c0 - k0 + m) * n;
-    for (int j = tid; j < n; j += NT) {
-      if (j >= c0 && j < c0 + nb) continue;
-      dst[j] = A[(long long)j * ld + p];
+}
+
+// Gathers the pivot rows of panel [c0, c0 + nb) in every column outside the panel
+// into Prow (row c0 - k0 + m of the outer block's 64 x n panel).  In a
+// column-major matrix each pivot-row element is its own cache line, so the gather
+// is spread over many workgroups (one per 64 columns of each matrix) instead of
+// being done by the single panel workgroup of that matrix.
+template <typename T>
+__global__ __launch_bounds__(64) void k_gj_gather(const cx<T>* S, long long sS, int n, int ld, int c0,
+                                                  int nb, int k0, GjWork w, const int* active) {
+  typedef cx<T> V;
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  const V* A = S + (long long)b * sS;
+  const int* pivrow = w.pivrow + (long long)b * w.sInt;
+  V* prow = (V*)w.prow + (long long)b * w.sProw;
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= n || (j >= c0 && j < c0 + nb)) return;
+  V t[kNBI];
+#pragma unroll
+  for (int m = 0; m < kNBI; ++m)
+    if (m < nb) {
+      const int p = pivrow[c0 + m];
+      t[m] = A[(long long)j * ld + (p < n ? p : 0)];
     }
-  }
+#pragma unroll
+  for (int m = 0; m < kNBI; ++m)
+    if (m < nb) prow[(long long)(c0 - k0 + m) * n + j] = t[m];
 }
 
 // out[a][b] = S[pivrow[a]][pivstep[b]]
@@ -214,6 +230,8 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
         case 3: hipLaunchKernelGGL((k_gj_panel<T, 3, 768>), dim3(batch), dim3(768), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
         default: hipLaunchKernelGGL((k_gj_panel<T, 4, 768>), dim3(batch), dim3(768), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
       }
+      hipLaunchKernelGGL((k_gj_gather<T>), dim3((n + 63) / 64, batch), dim3(64), 0, s, S, sS, n, ld, c0,
+                         nb, k0, w, active);
       if (kb > nb) {  // bring the rest of the outer block up to date (rank nb)
         GemmBatch g = {};
         g.A = S + (long long)c0 * ld; g.sA = sS; g.lda = ld;

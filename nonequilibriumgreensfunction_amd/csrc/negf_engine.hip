@@ -66,6 +66,14 @@ hipError_t launch_maxabs(hipStream_t, const cx<T>*, long long, long long, double
 template <typename T>
 hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long long, int, int,
                             double*, long long);
+template <typename Td, typename Ts>
+hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
+                          const int*);
+// complex128 -> complex64 copies of nn elements for `batch` strided matrices
+static inline hipError_t to_c64(hipStream_t s, cx<float>* Y, long long sY, const cx<double>* X,
+                                long long sX, long long nn, int batch) {
+  return launch_convert<float, double>(s, Y, sY, X, sX, nn, batch, nullptr);
+}
 }  // namespace negf
 
 using namespace negf;
@@ -100,17 +108,24 @@ struct Layout {
   // per-energy region (complex elements)
   long long Es, o_slots, o_z, o_M, o_colA, o_colB, o_Y, o_Bm, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
       o_prow;
+  // mixed-precision form: complex64 region (offsets in complex64 elements from o_f32)
+  long long o_f32, f_z, f_g, f_G, f_Y, f_Bm, f_P, f_GamL, f_GamR;
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
   long long site_per_e;  // doubles of site output per energy
   // zslots: L more blocks per energy for the forward substitution chain of the
   // two-pass RGF (first block column from the left-connected inverses)
-  void init(int n, int L, int nsites, bool zslots) {
+  // mixed: the first-column chain and the site-block products run in complex64
+  void init(int n, int L, int nsites, bool zslots, bool mixed) {
     nn = (long long)n * n;
     long long o = 0;
     o_slots = o; o += (long long)L * nn;
-    o_z = o; if (zslots) o += (long long)L * nn;
+    o_z = o; if (zslots && !mixed) o += (long long)L * nn;
+    o_f32 = o;
+    f_z = 0; f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_Bm = f_Y + nn;
+    f_P = f_Bm + nn; f_GamL = f_P + nn; f_GamR = f_GamL + nn;
+    if (mixed) o += (f_GamR + nn + 1) / 2;  // two complex64 per complex128 slot
     o_M = o; o += nn;
     o_colA = o; o += nn;
     o_colB = o; o += nn;
@@ -171,6 +186,7 @@ struct negf_ctx {
   zd* betaL = nullptr;   // -HopX'
   zd* betaR = nullptr;   // -HopX
   zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
+  cx<float>* X32 = nullptr;  // HopX in complex64 (mixed-precision form)
   SpinMats sp;
   Layout lay;
   Lane lanes[2];
@@ -254,7 +270,12 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
   c->n = n;
   c->L = cfg->npx;
   c->nsites = cfg->npy * cfg->npz;
-  c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS));
+  if (cfg->precision == NEGF_PREC_MIXED && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    negf_set_error("the mixed-precision form runs on the two-pass RGF only");
+    return NEGF_ERR_ARG;
+  }
+  c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS),
+              cfg->precision == NEGF_PREC_MIXED);
   for (Lane& l : c->lanes)
     if (hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) != hipSuccess) {
       negf_set_error("hipStreamCreate failed");
@@ -272,6 +293,7 @@ extern "C" int negf_finalize(negf_ctx* c) {
   for (Lane& l : c->lanes) hipStreamSynchronize(l.st);
   free_arena(c);
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
+  hipFree(c->X32);
   for (Lane& l : c->lanes) hipStreamDestroy(l.st);
   delete c;
   return 0;
@@ -326,6 +348,11 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
         }
   }
   if (!bd && !c->tmpP) HIPCHK(hipMalloc(&c->tmpP, sizeof(zd) * nn * 2));
+  if (c->cfg.precision == NEGF_PREC_MIXED) {
+    if (!c->X32) HIPCHK(hipMalloc(&c->X32, sizeof(cx<float>) * nn));
+    HIPCHK(to_c64(c->st, c->X32, 0, c->X, 0, nn, 1));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
   c->have_h = true;
   return 0;
 }
@@ -791,6 +818,91 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     return 0;
   };
   int r;
+  if (c->cfg.precision == NEGF_PREC_MIXED) {
+    // ---- mixed-precision two-pass RGF.  FP64: the inverses gL_x and the last block
+    // column G_xL (T12 = Re tr(GamL G1L GamR G1L') stays an FP64 product chain).
+    // complex64 (3M MFMA): the forward/back substitution for the first block column
+    // and every G*Gamma site-block product -- they feed densities, spin and the
+    // injected modes only, at complex64 accuracy.
+    typedef cx<float> zf;
+    zf* F0 = (zf*)Q(y.o_f32);
+    auto F = [&](long long off) { return F0 + off; };
+    const long long Es32 = 2 * Es;  // complex64 stride between energies
+    auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
+    const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
+    auto gbf = [&](const zf* A, const zf* B, zf* C) {
+      GemmBatch g = {};
+      g.A = A; g.sA = Es32; g.lda = n;
+      g.B = B; g.sB = Es32; g.ldb = n;
+      g.C = C; g.sC = Es32; g.ldc = n;
+      g.m = n; g.n = n; g.k = n;
+      g.m3 = 1;
+      return g;
+    };
+    TRY(to_c64(s, F(y.f_GamL), Es32, Q(y.o_GamL), Es, nn, nb));
+    TRY(to_c64(s, F(y.f_GamR), Es32, Q(y.o_GamR), Es, nn, nb));
+    // Bm32 = G32 Gam_lead and the site blocks of Bm32 G32' for one lead at slice x
+    auto site32 = [&](int lead, int x, const zf* G32) -> int {
+      TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(G32, F(lead ? y.f_GamR : y.f_GamL), F(y.f_Bm)), nb,
+                              kOnef, kZerof));
+      double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
+      TRY(launch_siteblocks<float>(s, no, F(y.f_Bm), Es32, G32, Es32, n, out, y.site_per_e,
+                                   (long long)L * ns, c->sp, nb, nullptr));
+      return 0;
+    };
+    // scal[idx] = Re <Gam_lead Bm32, G32> (Bm32 from the last site32 call)
+    auto trace32 = [&](int lead, const zf* G32, int idx) -> int {
+      TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(lead ? y.f_GamR : y.f_GamL), F(y.f_Bm), F(y.f_P)), nb,
+                              kOnef, kZerof));
+      TRY(launch_inner<float>(s, F(y.f_P), Es32, G32, Es32, n, ln.d_scal + idx, 8, nb, nullptr));
+      return 0;
+    };
+    // forward: gL_x in FP64, z_x = gL_x X' z_{x-1} in complex64
+    for (int x = 0; x < L; ++x) {
+      if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
+      TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
+      if (x == 0) {
+        TRY(to_c64(s, zs(0), Es32, slot(0), Es, nn, nb));
+      } else {
+        TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
+        TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, nb, nullptr));
+        TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kZerof));
+      }
+    }
+    // backward: G_xL in FP64; G_x1 = z_x + gL_x X G_{x+1,1} in complex64 (in place of z_x)
+    zd* colA = slot(L - 1);
+    zd* colB = nullptr;
+    zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
+    int pb = 0;
+    for (int x = L - 1; x >= 0; --x) {
+      if (x < L - 1) {
+        colB = bufs[pb];
+        if ((r = xmul(0, Q(y.o_Y), colA))) return r;
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                                 kOne, kZero));
+        TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
+        TRY(launch_bd_mul<float>(s, no, 0, F(y.f_Y), Es32, zs(x + 1), Es32, c->X32, n, nb, nullptr));
+        TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kOnef));
+      }
+      const zd* GxL = (x < L - 1) ? colB : colA;
+      if (x == 0) {  // T12 = Re tr(GamL G1L GamR G1L') from FP64 products
+        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(GxL, Es, n, Q(y.o_GamR), Es, n, Q(y.o_Bm), Es, n, n),
+                                 nb, kOne, kZero));
+        if ((r = trace_term(0, GxL, 0))) return r;
+      }
+      TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, nn, nb));
+      if ((r = site32(1, x, F(y.f_G)))) return r;
+      if (x == L - 1 && (r = trace32(1, F(y.f_G), 3))) return r;  // Modec2, right-lead term
+      if ((r = site32(0, x, zs(x)))) return r;
+      if (x == L - 1 && (r = trace32(1, zs(x), 2))) return r;  // Modec2, left-lead term
+      if (x == 0 && (r = trace32(0, zs(x), 1))) return r;      // Modec1, left-lead term
+      if (x < L - 1) {
+        colA = colB;
+        pb ^= 1;
+      }
+    }
+    return collect(c, ln, nb, iters, sst, res);
+  }
   if (!(c->cfg.flags & NEGF_FLAG_RGF_THREE_PASS)) {
     // ---- two-pass RGF.  Forward: gL_x (Recursive_Greens.F90:16-34) and the forward
     // substitution z_x = gL_x X' z_{x-1}, z_1 = gL_1, of A Y = E_1 (block column 1).
