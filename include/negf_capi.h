@@ -186,6 +186,56 @@ int negf_invert_matrix(int n, const double* G, double* invG, int* info);
 int negf_zgemm(char transa, char transb, int n, const double* alpha, const double* A, int lda,
                const double* B, int ldb, const double* beta, double* C, int ldc);
 
+/* --- Serving and batching of whole transport solves (no single counterpart in
+ * the reference: it runs one wire per process, NEGF_Main.F90:21-38) ----------
+ * A server owns one GPU.  Each request is one wire -- its own dimensions,
+ * Hamiltonian (Htotd, HopX, gM) and energy list -- solved exactly as
+ * negf_submit/negf_flush would solve it; several requests are in flight at once:
+ *  - admission: a request is admitted when its resident footprint (Htotd and
+ *    the coupling matrices on the device) fits the server's HBM budget next to
+ *    the admitted ones; otherwise it waits (first come, first served);
+ *  - continuous batching: the energies of admitted requests are cut into chunks
+ *    that two device lanes take in arrival order, so requests that arrive while
+ *    others run share the GPU as soon as memory allows;
+ *  - each lane owns a fixed working pool, re-carved per chunk for that request's
+ *    block size and length; a request whose single energy does not fit a pool is
+ *    rejected at submission (NEGF_ERR_MEMORY);
+ *  - results are accumulated per request in energy order, so a request's
+ *    outputs are identical alone and in any mix.
+ * The request's host arrays (Htotd, HopX, gM, energies) must stay valid, and its
+ * outputs untouched, until negf_server_wait() returns for it. */
+typedef struct negf_server negf_server;
+
+typedef struct {
+  double eta, sr_tol;
+  int sr_max_iter, sr_extra_eta, precision, device, flags;
+  double hbm_budget_gb; /* device memory for resident Hamiltonians + lane pools; 0 = 85% of free */
+  double lane_pool_gb;  /* working pool per lane; 0 = a quarter of the budget */
+} negf_server_config;
+
+typedef struct {
+  int npx, npy, npz, norb;
+  const double* Htotd; /* (n, n, npx) complex128 */
+  const double* HopX;  /* (n, n) complex128 */
+  const double* gM;    /* gMx, gMy, gMz (norb x norb each) complex128, or NULL */
+  int n_energies;
+  const double* energies;
+  double deltaE;
+  double mu[2], muT;
+  negf_accum acc; /* outputs; per-energy arrays indexed 0 .. n_energies-1 */
+} negf_request;
+
+int negf_server_create(negf_server** srv, const negf_server_config* cfg);
+/* Queue a request; *id identifies it.  Returns immediately. */
+int negf_server_submit(negf_server* srv, const negf_request* req, long long* id);
+/* 1 in *done when the request's outputs are complete. */
+int negf_server_poll(negf_server* srv, long long id, int* done);
+/* Block until the request completes; *status = its first nonzero per-energy status
+ * (or a negative error code), latency_ms = submission to completion. */
+int negf_server_wait(negf_server* srv, long long id, int* status, double* latency_ms);
+/* Waits for every request, then releases the GPU resources. */
+int negf_server_destroy(negf_server* srv);
+
 /* --- Instrumentation (no counterpart in the reference) ---------------------
  * Time MFMA GEMM launches (the dominant kernel) with HIP events on the stream
  * each is launched on, between begin and end: one launch in sample_every, in

@@ -33,7 +33,8 @@ EXPORTS = (
     "negf_energy_sweep", "negf_submit", "negf_flush", "negf_contact_self_energy",
     "negf_sancho_rubio", "negf_retarded", "negf_gnp", "negf_observables", "negf_invert_matrix",
     "negf_zgemm", "negf_profile_begin", "negf_profile_end", "negf_energy_sweep_f", "negf_flush_f",
-    "negf_observables_f",
+    "negf_observables_f", "negf_server_create", "negf_server_submit", "negf_server_poll",
+    "negf_server_wait", "negf_server_destroy",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -57,6 +58,20 @@ class NegfAccum(ctypes.Structure):
     _fields_ = [(k, _dp) for k in ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity",
                                    "HDensity", "RhoN", "RhoP", "SigXExp", "SigYExp", "SigZExp",
                                    "ContactCurrent")]
+
+
+class NegfServerConfig(ctypes.Structure):
+    _fields_ = [("eta", ctypes.c_double), ("sr_tol", ctypes.c_double), ("sr_max_iter", ctypes.c_int),
+                ("sr_extra_eta", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
+                ("flags", ctypes.c_int), ("hbm_budget_gb", ctypes.c_double),
+                ("lane_pool_gb", ctypes.c_double)]
+
+
+class NegfRequest(ctypes.Structure):
+    _fields_ = [("npx", ctypes.c_int), ("npy", ctypes.c_int), ("npz", ctypes.c_int),
+                ("norb", ctypes.c_int), ("Htotd", _dp), ("HopX", _dp), ("gM", _dp),
+                ("n_energies", ctypes.c_int), ("energies", _dp), ("deltaE", ctypes.c_double),
+                ("mu", ctypes.c_double * 2), ("muT", ctypes.c_double), ("acc", NegfAccum)]
 
 
 class NegfError(RuntimeError):
@@ -91,6 +106,12 @@ def lib():
         L.negf_zgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.negf_profile_begin.argtypes = [ctypes.c_int]
+        L.negf_server_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(NegfServerConfig)]
+        L.negf_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(NegfRequest),
+                                         ctypes.POINTER(ctypes.c_longlong)]
+        L.negf_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _ip]
+        L.negf_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _ip, _dp]
+        L.negf_server_destroy.argtypes = [ctypes.c_void_p]
         L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 8
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
@@ -304,3 +325,64 @@ class Context:
         out.update(SigXExp=acc.SigXExp, SigYExp=acc.SigYExp, SigZExp=acc.SigZExp,
                    ContactCurrent=acc.ContactCurrent, Jx=J[0], Jy=J[1], Jz=J[2])
         return out
+
+
+class Server:
+    """A serving/batching instance on one GPU (negf_server_* in include/negf_capi.h):
+    whole transport solves of different wires admitted by HBM footprint and run
+    concurrently, each accumulated in energy order."""
+
+    def __init__(self, eta=1e-6, sr_tol=1e-10, sr_max_iter=200, sr_extra_eta=True,
+                 precision=NEGF_PREC_FP64, device=0, flags=0, hbm_budget_gb=0.0, lane_pool_gb=0.0):
+        cfg = NegfServerConfig(eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)), precision, device,
+                               flags, hbm_budget_gb, lane_pool_gb)
+        h = ctypes.c_void_p()
+        _check(lib().negf_server_create(ctypes.byref(h), ctypes.byref(cfg)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, wire, energies, deltaE, mu=(-0.1, 0.0), muT=0.0):
+        """Queue one wire's energy list; returns (request id, Accumulators).  The
+        accumulators are complete once wait(id) has returned."""
+        H, X, g = _cf(wire.htotd), _cf(wire.hopx), wire.gm_fortran()
+        e = np.ascontiguousarray(energies, float)
+        acc = Accumulators(wire.npx, wire.npy, wire.npz, len(e))
+        req = NegfRequest()
+        req.npx, req.npy, req.npz, req.norb = wire.npx, wire.npy, wire.npz, wire.norb
+        req.Htotd, req.HopX, req.gM = _p(H), _p(X), _p(g)
+        req.n_energies = len(e)
+        req.energies = _p(e)
+        req.deltaE = float(deltaE)
+        req.mu[0], req.mu[1] = float(mu[0]), float(mu[1])
+        req.muT = float(muT)
+        req.acc = acc.struct()
+        rid = ctypes.c_longlong()
+        _check(lib().negf_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
+        self._keep[rid.value] = (H, X, g, e, acc, req)
+        return rid.value, acc
+
+    def poll(self, rid):
+        d = ctypes.c_int()
+        _check(lib().negf_server_poll(self._h, rid, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid):
+        """Block until the request completes; returns (status, latency_ms)."""
+        st = ctypes.c_int()
+        lat = ctypes.c_double()
+        _check(lib().negf_server_wait(self._h, rid, ctypes.byref(st), ctypes.byref(lat)))
+        self._keep.pop(rid, None)
+        return st.value, lat.value
+
+    def close(self):
+        if self._h:
+            lib().negf_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+

@@ -1712,3 +1712,7 @@ extern "C" int negf_observables_f(negf_ctx* c, int eCtr, double f1, double f2, d
                           GammaR, &acc, (mask >> 13 & 1) ? Jx : nullptr,
                           (mask >> 14 & 1) ? Jy : nullptr, (mask >> 15 & 1) ? Jz : nullptr);
 }
+
+// ---------------------------------------------------------------------------
+// Serving and batching of whole transport solves.
+#include "negf_server.inl"
