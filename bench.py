@@ -49,6 +49,21 @@ CONFIGS = {
     "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=8,
                budget_gb=220.0),
 }
+# Many-wire workloads run through the serving / batching layer (negf_server):
+#  c3 -- a seeded sample of the batching configuration (256 disorder realisations x
+#        32 energies, Npy=Npz in {8,12,16,20}, Npx in [40,120], W in [0,1], g*B in
+#        [0,0.05], flux in [0,2]): `instances` realisations x `energies_per` of their 32
+#        energies, all submitted at once;
+#  c5 -- a serving stream with Poisson arrivals at `rate` requests/s, per request
+#        Npy=Npz in `ns`, Npx in `L`, 16..`ne_max` energies in [-0.4,0.4], flux in
+#        [0,2], W in [0,1], own seed; scaled down from the configuration's
+#        Npy=Npz in {8..24}, Npx in [50,300], 16..256 energies so that the stream
+#        fits about a minute of one GPU (the full distribution's mean request is
+#        ~75 s at FP64 peak).
+SERVER_WORKLOADS = {
+    "c3": dict(instances=16, energies_per=8, ns=(8, 12, 16, 20), L=(40, 120), seed=3000),
+    "c5": dict(requests=20, rate=0.3, ns=(8, 12, 16), L=(50, 150), ne_max=64, seed=5000),
+}
 ETA, SR_TOL, SR_MAX = 1e-6, 1e-10, 200
 MU, MUT = (-0.1, 0.0), 0.0
 OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", "RhoN", "RhoP",
@@ -201,12 +216,95 @@ def errors_vs(acc, ref):
     return out
 
 
+def server_workload(name, args):
+    """Run a many-wire workload through capi.Server; returns the JSON line."""
+    import threading
+    wl = SERVER_WORKLOADS[name]
+    rng = np.random.default_rng(wl["seed"])
+    reqs = []  # (arrival_s, wire, energies)
+    if name == "c3":
+        grid = energy_grid(-0.4, 0.4, 32)
+        for r in range(wl["instances"]):
+            ny = int(rng.choice(wl["ns"]))
+            L = int(rng.integers(wl["L"][0], wl["L"][1] + 1))
+            w = build_wire(L, ny, ny, disorder=float(rng.uniform(0, 1)), seed=wl["seed"] + r,
+                           flux=float(rng.uniform(0, 2)), zeeman=float(rng.uniform(0, 0.05)))
+            sel = grid[(np.arange(wl["energies_per"]) * (32 // wl["energies_per"]) + r) % 32]
+            reqs.append((0.0, w, sel))
+        desc = (f"c3 sample: {wl['instances']} of the 256 seeded realisations x {wl['energies_per']} of "
+                f"their 32 energies in [-0.4,0.4]; Npy=Npz in {wl['ns']}, Npx in {wl['L']}, W in [0,1], "
+                f"g*B in [0,0.05], flux in [0,2]; all submitted at t=0")
+    else:
+        t = 0.0
+        for r in range(wl["requests"]):
+            t += float(rng.exponential(1.0 / wl["rate"]))
+            ny = int(rng.choice(wl["ns"]))
+            L = int(rng.integers(wl["L"][0], wl["L"][1] + 1))
+            ne = int(rng.integers(16, wl["ne_max"] + 1))
+            w = build_wire(L, ny, ny, disorder=float(rng.uniform(0, 1)), seed=wl["seed"] + r,
+                           flux=float(rng.uniform(0, 2)))
+            reqs.append((t, w, energy_grid(-0.4, 0.4, ne)))
+        desc = (f"c5 stream: {wl['requests']} requests, Poisson arrivals at {wl['rate']}/s; per request "
+                f"Npy=Npz in {wl['ns']}, Npx in {wl['L']}, 16..{wl['ne_max']} energies in [-0.4,0.4], "
+                f"flux in [0,2], W in [0,1], own seed (scaled down from Npy=Npz 8..24, Npx 50..300, "
+                f"16..256 energies)")
+    prec = capi.NEGF_PREC_MIXED if args.precision == "mixed" else capi.NEGF_PREC_FP64
+    npts = sum(len(e) for _, _, e in reqs)
+    print(f"[bench] {name}: {len(reqs)} requests, {npts} energy points", file=sys.stderr, flush=True)
+    srv = capi.Server(eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX, precision=prec, flags=args.flags)
+    ids = [None] * len(reqs)
+    capi.profile_begin(args.profile_every)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, (at, w, e) in enumerate(reqs):
+            dt = at - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit(w, e, e[1] - e[0] if len(e) > 1 else 0.0, MU, MUT)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    lat, stat = [], []
+    for i, (rid, acc) in enumerate(ids):
+        st, l = srv.wait(rid)
+        lat.append(l)
+        stat.append(st)
+        print(f"[bench] request {i} n={reqs[i][1].n} Npx={reqs[i][1].npx} energies={len(reqs[i][2])} "
+              f"latency {l / 1e3:.2f}s", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    prof = capi.profile_end()
+    srv.close()
+    dominant = "fp32_gemm_3m" if args.precision == "mixed" else "fp64_gemm_3m"
+    peak = PEAK_FP32_MFMA if args.precision == "mixed" else PEAK_FP64_MFMA
+    g_flops, g_ms, g_launches = prof[dominant]
+    gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    return {
+        "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
+        "value": npts / elapsed,
+        "unit": "energy-points/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * elapsed,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+c64 (mixed)" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded 4-band TI wires, host-built Hamiltonians)",
+        "config": {"workload": desc, "requests": len(reqs), "energy_points": npts,
+                   "precision": args.precision},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "note": "per request, submission to completion (queueing included)"},
+        "status_nonzero": int(sum(1 for s in stat if s)),
+        "roofline": {"bound": "mfma", "kernel": f"k_gemm 3M complex product ({dominant})",
+                     "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s", "frac": gemm_tflops / peak,
+                     "traffic": None, "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS) + sorted(SERVER_WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"))
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -229,6 +327,9 @@ def main():
                                    "final sums; only --gpus 1 is measured)"}))
         sys.exit(2)
 
+    if args.config in SERVER_WORKLOADS:
+        print(json.dumps(server_workload(args.config, args)), flush=True)
+        return
     cfg = CONFIGS[args.config]
     per = args.per_step or cfg["per_step"]
     n = cfg["npy"] * cfg["npz"] * 4

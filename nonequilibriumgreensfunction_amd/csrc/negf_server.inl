@@ -107,7 +107,9 @@ SrvRequest* next_admission(negf_server* s) {
   if (s->waiting.empty()) return nullptr;
   SrvRequest* r = s->waiting.front();
   if (r->admitting) return nullptr;
-  if (s->used + r->footprint > s->budget && !s->active.empty()) return nullptr;
+  // `used` includes admissions still uploading; every footprint fits the budget on
+  // its own (checked at submission), so an idle server always admits
+  if (s->used + r->footprint > s->budget) return nullptr;
   s->waiting.pop_front();
   r->admitting = true;
   s->used += r->footprint;
