@@ -6,7 +6,7 @@
  * subroutines take no arguments and read/write NEGF_Module globals
  * (NEGF_Module.F90:51-100); these entry points take the same arrays as plain
  * pointers, so a Fortran caller binds them with iso_c_binding and passes its
- * module arrays (see the shims in fortran/negf_shims.f90).
+ * module arrays (see the shims in fortran/negf_shims.F90).
  *
  * Conventions
  *  - Complex arrays are COMPLEX(8): interleaved (re, im) doubles, column-major,
@@ -43,7 +43,8 @@ extern "C" {
 
 /* precision forms */
 #define NEGF_PREC_FP64 0  /* plain complex128 on FP64 MFMA */
-#define NEGF_PREC_MIXED 1 /* complex64 MFMA LU/GEMM with FP64 refinement */
+#define NEGF_PREC_MIXED 1 /* FP64 contacts, inverses and T12 chain; complex64 MFMA for the
+                             first-column substitution and the site-block products */
 
 typedef struct negf_ctx negf_ctx;
 
@@ -102,6 +103,19 @@ int negf_finalize(negf_ctx* ctx);
  * Poisson iteration. */
 int negf_set_hamiltonian(negf_ctx* ctx, const double* Htotd, const double* HopX,
                          const double* gM);
+
+/* Device-side Hamiltonian from a compact description: every slice is the
+ * template H0 (n,n) -- the slice block with its y/z hoppings, Peierls phases and
+ * Zeeman terms (Build_Hamiltonian.F90:59-100, :186-207, :348-360), the same in all
+ * slices -- plus the real site potentials onsite (Npx,Npy,Npz), Fortran order,
+ * on every orbital of the site (Insert_Impurity / Anderson disorder,
+ * Build_Hamiltonian.F90:154-165, :278-296):
+ *     Htotd(a,a,x) = H0(a,a) + onsite(x, site(a)),  Htotd(a,b,x) = H0(a,b) otherwise.
+ * The slices are assembled on the device as they are used; n*n + Npx*Npy*Npz
+ * values stay resident instead of n*n*Npx.  Results are identical to
+ * negf_set_hamiltonian with the Htotd so defined. */
+int negf_set_hamiltonian_compact(negf_ctx* ctx, const double* H0, const double* HopX,
+                                 const double* gM, const double* onsite);
 
 /* Energy_Sweep (Energy_Sweep.F90:1-46) for one energy, fused on the device:
  * contacts, RGF, Gn/Gp site blocks and observables.  mu[2], muT give the step
@@ -223,6 +237,9 @@ typedef struct {
   double deltaE;
   double mu[2], muT;
   negf_accum acc; /* outputs; per-energy arrays indexed 0 .. n_energies-1 */
+  /* compact Hamiltonian (as negf_set_hamiltonian_compact), used when Htotd is NULL */
+  const double* H0;     /* (n, n) complex128 slice template */
+  const double* onsite; /* (npx, npy, npz) float64 site potentials */
 } negf_request;
 
 int negf_server_create(negf_server** srv, const negf_server_config* cfg);

@@ -33,7 +33,7 @@ EXPORTS = (
     "negf_energy_sweep", "negf_submit", "negf_flush", "negf_contact_self_energy",
     "negf_sancho_rubio", "negf_retarded", "negf_gnp", "negf_observables", "negf_invert_matrix",
     "negf_zgemm", "negf_profile_begin", "negf_profile_end", "negf_energy_sweep_f", "negf_flush_f",
-    "negf_observables_f", "negf_server_create", "negf_server_submit", "negf_server_poll",
+    "negf_observables_f", "negf_set_hamiltonian_compact", "negf_server_create", "negf_server_submit", "negf_server_poll",
     "negf_server_wait", "negf_server_destroy",
 )
 
@@ -71,7 +71,8 @@ class NegfRequest(ctypes.Structure):
     _fields_ = [("npx", ctypes.c_int), ("npy", ctypes.c_int), ("npz", ctypes.c_int),
                 ("norb", ctypes.c_int), ("Htotd", _dp), ("HopX", _dp), ("gM", _dp),
                 ("n_energies", ctypes.c_int), ("energies", _dp), ("deltaE", ctypes.c_double),
-                ("mu", ctypes.c_double * 2), ("muT", ctypes.c_double), ("acc", NegfAccum)]
+                ("mu", ctypes.c_double * 2), ("muT", ctypes.c_double), ("acc", NegfAccum),
+                ("H0", _dp), ("onsite", _dp)]
 
 
 class NegfError(RuntimeError):
@@ -112,6 +113,7 @@ def lib():
         L.negf_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _ip]
         L.negf_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _ip, _dp]
         L.negf_server_destroy.argtypes = [ctypes.c_void_p]
+        L.negf_set_hamiltonian_compact.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 8
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
@@ -237,6 +239,14 @@ class Context:
         H, X, g = self._wire_keep
         _check(lib().negf_set_hamiltonian(self._h, _p(H), _p(X), _p(g)))
 
+    def set_hamiltonian_compact(self, wire):
+        """Upload the compact description (wire.h0, wire.onsite): the slices are
+        assembled on the device."""
+        self._wire_keep = (_cf(wire.h0), _cf(wire.hopx), wire.gm_fortran(),
+                           np.asfortranarray(wire.onsite, dtype=float))
+        H0, X, g, V = self._wire_keep
+        _check(lib().negf_set_hamiltonian_compact(self._h, _p(H0), _p(X), _p(g), _p(V)))
+
     def energy_sweep(self, energy, eCtr, deltaE, mu, muT, acc: Accumulators):
         m = np.asarray(mu, float)
         s = acc.struct()
@@ -341,15 +351,22 @@ class Server:
         self._h = h
         self._keep = {}
 
-    def submit(self, wire, energies, deltaE, mu=(-0.1, 0.0), muT=0.0):
+    def submit(self, wire, energies, deltaE, mu=(-0.1, 0.0), muT=0.0, compact=False):
         """Queue one wire's energy list; returns (request id, Accumulators).  The
-        accumulators are complete once wait(id) has returned."""
-        H, X, g = _cf(wire.htotd), _cf(wire.hopx), wire.gm_fortran()
+        accumulators are complete once wait(id) has returned.  compact: send the
+        slice template and site potentials (wire.h0, wire.onsite) instead of Htotd."""
+        X, g = _cf(wire.hopx), wire.gm_fortran()
+        H = None if compact else _cf(wire.htotd)
+        H0 = _cf(wire.h0) if compact else None
+        V = np.asfortranarray(wire.onsite, dtype=float) if compact else None
         e = np.ascontiguousarray(energies, float)
         acc = Accumulators(wire.npx, wire.npy, wire.npz, len(e))
         req = NegfRequest()
         req.npx, req.npy, req.npz, req.norb = wire.npx, wire.npy, wire.npz, wire.norb
-        req.Htotd, req.HopX, req.gM = _p(H), _p(X), _p(g)
+        req.Htotd = _p(H) if H is not None else None
+        req.HopX, req.gM = _p(X), _p(g)
+        if compact:
+            req.H0, req.onsite = _p(H0), _p(V)
         req.n_energies = len(e)
         req.energies = _p(e)
         req.deltaE = float(deltaE)
@@ -358,7 +375,7 @@ class Server:
         req.acc = acc.struct()
         rid = ctypes.c_longlong()
         _check(lib().negf_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
-        self._keep[rid.value] = (H, X, g, e, acc, req)
+        self._keep[rid.value] = (H, H0, V, X, g, e, acc, req)
         return rid.value, acc
 
     def poll(self, rid):

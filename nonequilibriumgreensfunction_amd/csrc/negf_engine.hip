@@ -39,7 +39,8 @@ namespace negf {
 template <typename T>
 hipError_t launch_assemble(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, int, double,
                            const double*, double, const cx<T>*, long long, const cx<T>*, long long,
-                           const cx<T>*, long long, const cx<T>*, int, int, const int*);
+                           const cx<T>*, long long, const cx<T>*, int, int, const int*,
+                           const double*);
 template <typename T>
 hipError_t launch_bd_mul(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, long long,
                          const cx<T>*, int, int, const int*);
@@ -181,7 +182,11 @@ struct negf_ctx {
   bool have_h = false;
   bool blockdiag = true;
   bool ends_equal = false;  // H slice 1 == H slice L bitwise: one decimation serves both leads
-  zd* H = nullptr;       // Htotd (n,n,L)
+  zd* H = nullptr;       // Htotd (n,n,L), or null for a compact Hamiltonian:
+  bool compact = false;  //   slice x = H0 + diag(V(x, site) on every orbital of the site)
+  zd* H0 = nullptr;      // slice template (n,n)
+  double* V = nullptr;   // site potentials, (L, nsites) slice-major
+  zd* Hends = nullptr;   // slices 1 and L assembled (n,n,2), for the contacts
   zd* X = nullptr;       // HopX
   zd* betaL = nullptr;   // -HopX'
   zd* betaR = nullptr;   // -HopX
@@ -294,19 +299,15 @@ extern "C" int negf_finalize(negf_ctx* c) {
   free_arena(c);
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
   hipFree(c->X32);
+  hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
   for (Lane& l : c->lanes) hipStreamDestroy(l.st);
   delete c;
   return 0;
 }
 
-extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const double* HopX,
-                                    const double* gM) {
-  if (!c || !Htotd || !HopX) {
-    negf_set_error("null argument");
-    return NEGF_ERR_ARG;
-  }
-  HIPCHK(hipSetDevice(c->cfg.device));
-  const int n = c->n, L = c->L, no = c->cfg.norb;
+// HopX, the contact couplings -HopX' and -HopX, and the spin matrices.
+static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
+  const int n = c->n, no = c->cfg.norb;
   const long long nn = (long long)n * n;
   const zd* hx = (const zd*)HopX;
   // HopX structure: site-block-diagonal (Build_Hamiltonian.F90:62-64) or dense
@@ -318,14 +319,11 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
         break;
       }
   c->blockdiag = bd;
-  c->ends_equal = std::memcmp(Htotd, Htotd + 2 * (size_t)(L - 1) * nn, sizeof(zd) * nn) == 0;
-  if (!c->H) {
-    HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
+  if (!c->X) {
     HIPCHK(hipMalloc(&c->X, sizeof(zd) * nn));
     HIPCHK(hipMalloc(&c->betaL, sizeof(zd) * nn));
     HIPCHK(hipMalloc(&c->betaR, sizeof(zd) * nn));
   }
-  HIPCHK(hipMemcpy(c->H, Htotd, sizeof(zd) * nn * L, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->X, HopX, sizeof(zd) * nn, hipMemcpyHostToDevice));
   std::vector<zd> b(nn);
   for (int j = 0; j < n; ++j)
@@ -353,8 +351,74 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
     HIPCHK(to_c64(c->st, c->X32, 0, c->X, 0, nn, 1));
     HIPCHK(hipStreamSynchronize(c->st));
   }
+  return 0;
+}
+
+extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const double* HopX,
+                                    const double* gM) {
+  if (!c || !Htotd || !HopX) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const int L = c->L;
+  const long long nn = (long long)c->n * c->n;
+  int rc = upload_couplings(c, HopX, gM);
+  if (rc) return rc;
+  hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
+  c->H0 = c->Hends = nullptr;
+  c->V = nullptr;
+  c->compact = false;
+  c->ends_equal = std::memcmp(Htotd, Htotd + 2 * (size_t)(L - 1) * nn, sizeof(zd) * nn) == 0;
+  if (!c->H) HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
+  HIPCHK(hipMemcpy(c->H, Htotd, sizeof(zd) * nn * L, hipMemcpyHostToDevice));
   c->have_h = true;
   return 0;
+}
+
+extern "C" int negf_set_hamiltonian_compact(negf_ctx* c, const double* H0, const double* HopX,
+                                            const double* gM, const double* onsite) {
+  if (!c || !H0 || !HopX || !onsite) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const int n = c->n, L = c->L, ns = c->nsites, no = c->cfg.norb;
+  const long long nn = (long long)n * n;
+  int rc = upload_couplings(c, HopX, gM);
+  if (rc) return rc;
+  hipFree(c->H);
+  c->H = nullptr;
+  if (!c->H0) {
+    HIPCHK(hipMalloc(&c->H0, sizeof(zd) * nn));
+    HIPCHK(hipMalloc(&c->V, sizeof(double) * (size_t)L * ns));
+    HIPCHK(hipMalloc(&c->Hends, sizeof(zd) * nn * 2));
+  }
+  HIPCHK(hipMemcpy(c->H0, H0, sizeof(zd) * nn, hipMemcpyHostToDevice));
+  // onsite (Npx, Npy, Npz) Fortran order -> slice-major (x, site), site = y + Npy z
+  std::vector<double> v((size_t)L * ns);
+  for (int x = 0; x < L; ++x)
+    for (int st = 0; st < ns; ++st) v[(size_t)x * ns + st] = onsite[x + (size_t)L * st];
+  HIPCHK(hipMemcpy(c->V, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+  // the two end slices assembled for the contacts (H0(a,a) + V, as the slices are)
+  std::vector<zd> ends(2 * nn);
+  const zd* h0 = (const zd*)H0;
+  for (int e = 0; e < 2; ++e) {
+    const int x = e ? L - 1 : 0;
+    std::memcpy(&ends[e * nn], h0, sizeof(zd) * nn);
+    for (int a = 0; a < n; ++a) ends[e * nn + (long long)a * n + a].x += v[(size_t)x * ns + a / no];
+  }
+  c->ends_equal = std::memcmp(ends.data(), ends.data() + nn, sizeof(zd) * nn) == 0;
+  HIPCHK(hipMemcpy(c->Hends, ends.data(), sizeof(zd) * 2 * nn, hipMemcpyHostToDevice));
+  c->compact = true;
+  c->have_h = true;
+  return 0;
+}
+
+// Slice 1 (which = 0) or slice L (which = 1) of the Hamiltonian, for the contacts.
+static const zd* slice_end(negf_ctx* c, int which) {
+  if (c->compact) return c->Hends + (size_t)which * c->lay.nn;
+  return c->H + (size_t)(which ? c->L - 1 : 0) * c->lay.nn;
 }
 
 // Allocate the lanes' arenas for `want` energies in flight in total (capped by the
@@ -747,8 +811,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   TRY(hipMemsetAsync(ln.d_scal, 0, sizeof(double) * 8 * nb, s));
   TRY(hipMemsetAsync(ln.Ie, 0, sizeof(int) * (size_t)y.Is * nb, s));
   // ---- contacts: chains [0,nb) left lead, [nb,2nb) right lead
-  const zd* Hend0 = c->H;
-  const zd* Hend1 = c->H + (size_t)(L - 1) * nn;
+  const zd* Hend0 = slice_end(c, 0);
+  const zd* Hend1 = slice_end(c, 1);
   TRY(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, Hend0, Hend1,
                               ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
   std::vector<int> iters, sst;
@@ -765,7 +829,9 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   w.m3 = 1;
   const bool bd = c->blockdiag;
   auto slot = [&](int x) { return Q(y.o_slots + (long long)x * nn); };
-  auto hx = [&](int x) { return (const zd*)(c->H + (size_t)x * nn); };
+  // slice x of the Hamiltonian: Htotd(:,:,x), or the template plus the slice's potentials
+  auto hx = [&](int x) { return c->compact ? (const zd*)c->H0 : (const zd*)(c->H + (size_t)x * nn); };
+  auto hv = [&](int x) { return c->compact ? (const double*)(c->V + (size_t)x * ns) : nullptr; };
   // Assemble M = zI - H_x - [SigL if x==0] - [SigR if x==L-1] - sandwich(g) into o_M.
   auto assemble = [&](int x, const zd* g, int mode) -> int {
     const zd* S1 = (x == 0) ? Q(y.o_SigL) : nullptr;
@@ -782,11 +848,11 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         TRY(gemm_batched<double>(s, OP_N, OP_C, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
       }
       TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy, c->cfg.eta, S1,
-                                  Es, S2, Es, T2, Es, c->X, n, nb, nullptr));
+                                  Es, S2, Es, T2, Es, c->X, n, nb, nullptr, hv(x)));
       return 0;
     }
     TRY(launch_assemble<double>(s, no, g ? mode : 0, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy,
-                                c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr));
+                                c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr, hv(x)));
     return 0;
   };
   // Y = X G (conj=0) or X' G (conj=1)
@@ -1141,8 +1207,8 @@ extern "C" int negf_contact_self_energy(negf_ctx* c, double energy, double* Sigm
   const long long nn = y.nn;
   hipStream_t s = c->st;
   HIPCHK(hipMemcpyAsync(ln.d_energy, &energy, sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, c->H,
-                                 c->H + (size_t)(L - 1) * nn, ln.d_energy, 1, c->cfg.eta,
+  HIPCHK(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, slice_end(c, 0),
+                                 slice_end(c, 1), ln.d_energy, 1, c->cfg.eta,
                                  c->cfg.sr_extra_eta, n));
   std::vector<int> it, st;
   zd* SL = ln.E + y.o_SigL;
@@ -1316,8 +1382,8 @@ template <typename T>
 hipError_t launch_trace_prod(hipStream_t, const cx<T>*, const cx<T>*, int, double*);
 template <typename T>
 hipError_t launch_site_obs(hipStream_t, int, const cx<T>*, const cx<T>*, const cx<T>*,
-                           const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,
-                           int, int, int, int, const SpinMats&, double*);
+                           const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, long long,
+                           const cx<T>*, int, int, int, int, const SpinMats&, double*);
 template <typename T>
 hipError_t launch_add_eye(hipStream_t, cx<T>*, long long, int, int);
 template <typename T>
@@ -1416,10 +1482,10 @@ extern "C" int negf_retarded(negf_ctx* c, const double* InvGRd, double* GRd_h, d
       }
       if (e != hipSuccess) return e;
       return launch_assemble<double>(s, no, 3, M, 0, blkp(A, x), 0, 1.0, nullptr, 0.0, nullptr, 0,
-                                     nullptr, 0, P1, 0, c->X, n, 1, nullptr);
+                                     nullptr, 0, P1, 0, c->X, n, 1, nullptr, nullptr);
     }
     return launch_assemble<double>(s, no, g ? mode : 0, M, 0, blkp(A, x), 0, 1.0, nullptr, 0.0,
-                                   nullptr, 0, nullptr, 0, g, 0, c->X, n, 1, nullptr);
+                                   nullptr, 0, nullptr, 0, g, 0, c->X, n, 1, nullptr, nullptr);
   };
   auto mm = [&](const zd* Aop, const zd* Bop, zd* Cout) {
     return gemm_batched<double>(s, OP_N, OP_N, gb(Aop, 0, n, Bop, 0, n, Cout, 0, n, n), 1, kOne, kZero);
@@ -1618,7 +1684,8 @@ extern "C" int negf_observables(negf_ctx* c, int eCtr, double f1, double f2, dou
   // densities vs energy (:34-43), spin (:10-30) and bond currents (:111-168)
   HIPCHK(launch_diag_re<double>(s, dn, nn, n, L, diag));
   HIPCHK(launch_diag_re<double>(s, dp, nn, n, L, diag + (size_t)n * L));
-  HIPCHK(launch_site_obs<double>(s, no, dn, dp, du, dl, pu, pl, c->H, c->X, n, L, npy, npz, c->sp, site));
+  HIPCHK(launch_site_obs<double>(s, no, dn, dp, du, dl, pu, pl, c->compact ? c->H0 : c->H,
+                                 c->compact ? 0LL : (long long)n * n, c->X, n, L, npy, npz, c->sp, site));
   double sc[4];
   std::vector<double> so((size_t)6 * L * ns), dg((size_t)2 * n * L);
   HIPCHK(hipMemcpyAsync(sc, scal, sizeof(double) * 3, hipMemcpyDeviceToHost, s));

@@ -28,17 +28,20 @@ namespace negf {
 // with z_b = energy[b] + i*eta and, by MODE,
 //   MODE 0: D = 0,  MODE 1: D = X' g_b X,  MODE 2: D = X g_b X',  MODE 3: D = P_b.
 // X is site-block-diagonal with norb x norb blocks (MODE 1/2 read only those).
+// V (optional): per-site potentials of this slice, added to the diagonal of H on
+// every orbital of the site (compact Hamiltonian: slice = template + potentials).
 template <typename T, int NORB, int MODE>
 __global__ void k_assemble(cx<T>* M, long long sM, const cx<T>* H, int add_diag, T hsign,
                            const double* energy, double eta, const cx<T>* S1, long long sS1,
                            const cx<T>* S2, long long sS2, const cx<T>* g, long long sg,
-                           const cx<T>* X, int n, const int* active) {
+                           const cx<T>* X, int n, const int* active, const double* V) {
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
   const int c = blockIdx.y;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   const long long e = (long long)c * n + a;
   cx<T> v = H[e];
+  if (V && a == c) v.x += (T)V[a / NORB];
   v.x *= hsign;
   v.y *= hsign;
   if (add_diag && a == c) {
@@ -373,10 +376,12 @@ __global__ __launch_bounds__(1024) void k_trace_prod(const cx<T>* P, const cx<T>
 //   y: Re i [H(a,b)Gn(b,a) - Gn(a,b)H(b,a) - H(a,b)Gp(b,a) + Gp(a,b)H(b,a)], b on site y+1
 //   z: the same with b on site z+1;  x: HopX with Gnl/Gnu/Gpl/Gpu of slice x.
 // out[(q * L + x) * nsites + s], q = 0..2 spin, 3 Jx, 4 Jy, 5 Jz.
+// sH: element stride between the slices of H (n*n for Htotd, 0 when every slice's
+// off-diagonal part is one template -- only off-diagonal elements of H are read)
 template <typename T, int NORB>
 __global__ void k_site_obs(const cx<T>* Gnd, const cx<T>* Gpd, const cx<T>* Gnu, const cx<T>* Gnl,
-                           const cx<T>* Gpu, const cx<T>* Gpl, const cx<T>* H, const cx<T>* X,
-                           int n, int L, int npy, int npz, SpinMats sp, double* out) {
+                           const cx<T>* Gpu, const cx<T>* Gpl, const cx<T>* H, long long sH,
+                           const cx<T>* X, int n, int L, int npy, int npz, SpinMats sp, double* out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int x = blockIdx.y;
   const int ns = npy * npz;
@@ -399,7 +404,7 @@ __global__ void k_site_obs(const cx<T>* Gnd, const cx<T>* Gpd, const cx<T>* Gnu,
     for (int ii = 0; ii < NORB; ++ii)
       for (int jj = 0; jj < NORB; ++jj) {
         const int a = idx + ii, b = b0 + jj;
-        const cx<T> hab = at(H, bx, a, b), hba = at(H, bx, b, a);
+        const cx<T> hab = at(H, x * sH, a, b), hba = at(H, x * sH, b, a);
         cx<T> c = cmul(hab, at(Gnd, bx, b, a));
         c = csub(c, cmul(at(Gnd, bx, a, b), hba));
         c = csub(c, cmul(hab, at(Gpd, bx, b, a)));
@@ -479,13 +484,13 @@ hipError_t launch_trace_prod(hipStream_t s, const cx<T>* P, const cx<T>* Q, int 
 template <typename T>
 hipError_t launch_site_obs(hipStream_t s, int norb, const cx<T>* Gnd, const cx<T>* Gpd,
                            const cx<T>* Gnu, const cx<T>* Gnl, const cx<T>* Gpu, const cx<T>* Gpl,
-                           const cx<T>* H, const cx<T>* X, int n, int L, int npy, int npz,
-                           const SpinMats& sp, double* out) {
+                           const cx<T>* H, long long sH, const cx<T>* X, int n, int L, int npy,
+                           int npz, const SpinMats& sp, double* out) {
   dim3 grid((npy * npz + 63) / 64, L);
 #define NEGF_SO(NO)                                                                              \
   if (norb == NO) {                                                                              \
     hipLaunchKernelGGL((k_site_obs<T, NO>), grid, dim3(64), 0, s, Gnd, Gpd, Gnu, Gnl, Gpu, Gpl, H, \
-                       X, n, L, npy, npz, sp, out);                                              \
+                       sH, X, n, L, npy, npz, sp, out);                                          \
     return hipGetLastError();                                                                    \
   }
   NEGF_SO(1) NEGF_SO(2) NEGF_SO(4)
@@ -505,8 +510,9 @@ template hipError_t launch_trace_prod<double>(hipStream_t, const cx<double>*, co
                                               double*);
 template hipError_t launch_site_obs<double>(hipStream_t, int, const cx<double>*, const cx<double>*,
                                             const cx<double>*, const cx<double>*, const cx<double>*,
-                                            const cx<double>*, const cx<double>*, const cx<double>*,
-                                            int, int, int, int, const SpinMats&, double*);
+                                            const cx<double>*, const cx<double>*, long long,
+                                            const cx<double>*, int, int, int, int, const SpinMats&,
+                                            double*);
 template hipError_t launch_add_eye<double>(hipStream_t, cx<double>*, long long, int, int);
 
 // ---------------------------------------------------------------------------
@@ -522,13 +528,13 @@ hipError_t launch_assemble(hipStream_t s, int norb, int mode, cx<T>* M, long lon
                            const cx<T>* H, int add_diag, double hsign, const double* energy,
                            double eta, const cx<T>* S1, long long sS1, const cx<T>* S2,
                            long long sS2, const cx<T>* g, long long sg, const cx<T>* X, int n,
-                           int batch, const int* active) {
+                           int batch, const int* active, const double* V) {
   if (batch <= 0) return hipSuccess;
   dim3 grid((n + 255) / 256, n, batch);
 #define NEGF_ASM(NO, MO)                                                                    \
   if (norb == NO && mode == MO) {                                                           \
     hipLaunchKernelGGL((k_assemble<T, NO, MO>), grid, dim3(256), 0, s, M, sM, H, add_diag,   \
-                       (T)hsign, energy, eta, S1, sS1, S2, sS2, g, sg, X, n, active);        \
+                       (T)hsign, energy, eta, S1, sS1, S2, sS2, g, sg, X, n, active, V);     \
     return hipGetLastError();                                                               \
   }
 #define NEGF_ASM_N(NO) NEGF_ASM(NO, 0) NEGF_ASM(NO, 1) NEGF_ASM(NO, 2) NEGF_ASM(NO, 3)
@@ -646,7 +652,8 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
   template hipError_t launch_assemble<T>(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,   \
                                          int, double, const double*, double, const cx<T>*,         \
                                          long long, const cx<T>*, long long, const cx<T>*,         \
-                                         long long, const cx<T>*, int, int, const int*);           \
+                                         long long, const cx<T>*, int, int, const int*,            \
+                                         const double*);                                           \
   template hipError_t launch_bd_mul<T>(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,     \
                                        long long, const cx<T>*, int, int, const int*);             \
   template hipError_t launch_siteblocks<T>(hipStream_t, int, const cx<T>*, long long,              \

@@ -61,9 +61,13 @@ void bind_lane(Lane& ln, const Layout& y, int nb, char* pool) {
 }
 
 // Device footprint of an admitted request beyond the lane pools.
-size_t resident_bytes(int n, int L, int precision) {
+size_t resident_bytes(int n, int L, int nsites, int precision, bool compact) {
   const size_t nn = (size_t)n * n;
-  size_t b = sizeof(zd) * nn * (size_t)L + 4 * sizeof(zd) * nn;  // H, X, betaL, betaR, tmpP
+  size_t b = 4 * sizeof(zd) * nn;  // X, betaL, betaR, tmpP
+  if (compact)
+    b += 3 * sizeof(zd) * nn + sizeof(double) * (size_t)L * nsites;  // H0, two end slices, V
+  else
+    b += sizeof(zd) * nn * (size_t)L;  // Htotd
   if (precision == NEGF_PREC_MIXED) b += sizeof(cx<float>) * nn;
   return b;
 }
@@ -128,6 +132,8 @@ int admit(negf_server* s, SrvRequest* r) {
   cfg.flags = s->cfg.flags;
   int rc = negf_init(&r->ctx, &cfg);
   if (rc) return rc;
+  if (!r->req.Htotd)
+    return negf_set_hamiltonian_compact(r->ctx, r->req.H0, r->req.HopX, r->req.gM, r->req.onsite);
   return negf_set_hamiltonian(r->ctx, r->req.Htotd, r->req.HopX, r->req.gM);
 }
 
@@ -251,8 +257,8 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
 }
 
 extern "C" int negf_server_submit(negf_server* s, const negf_request* req, long long* id) {
-  if (!s || !req || !id || !req->Htotd || !req->HopX || req->n_energies < 0 ||
-      (req->n_energies > 0 && !req->energies)) {
+  if (!s || !req || !id || !req->HopX || (!req->Htotd && (!req->H0 || !req->onsite)) ||
+      req->n_energies < 0 || (req->n_energies > 0 && !req->energies)) {
     negf_set_error("null argument");
     return NEGF_ERR_ARG;
   }
@@ -270,7 +276,7 @@ extern "C" int negf_server_submit(negf_server* s, const negf_request* req, long 
   r->req = *req;
   r->lay.init(n, req->npx, req->npy * req->npz, !(s->cfg.flags & NEGF_FLAG_RGF_THREE_PASS),
               s->cfg.precision == NEGF_PREC_MIXED);
-  r->footprint = resident_bytes(n, req->npx, s->cfg.precision);
+  r->footprint = resident_bytes(n, req->npx, req->npy * req->npz, s->cfg.precision, !req->Htotd);
   if (lane_bytes_per_energy(r->lay) > s->pool_bytes || r->footprint > s->budget) {
     negf_set_error("request (n=%d, Npx=%d) needs %.1f GB per energy and %.1f GB resident; lane pool "
                    "%.1f GB, resident budget %.1f GB",

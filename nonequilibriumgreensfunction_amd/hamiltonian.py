@@ -59,6 +59,9 @@ class Wire:
     htotd: np.ndarray  # (n, n, npx) complex128, Fortran order
     hopx: np.ndarray   # (n, n) complex128, Fortran order
     gm: np.ndarray     # (3, norb, norb) complex128: spin matrices gMx, gMy, gMz
+    # compact description: htotd[:, :, x] = h0 + diag(onsite[x] on every orbital of its site)
+    h0: np.ndarray | None = None      # (n, n) complex128 slice template (flux, Zeeman included)
+    onsite: np.ndarray | None = None  # (npx, npy, npz) float64 site potentials, Fortran order
 
     @property
     def n(self) -> int:
@@ -132,31 +135,31 @@ def build_wire(npx: int, npy: int, npz: int, model: str = "ti4", params: dict | 
         hh2[0:NN, NN * (npz - 1):NN * npz] = hopz.conj().T
         hh2[NN * (npz - 1):NN * npz, 0:NN] = hopz
     hopx = np.kron(np.eye(npy * npz), hx)  # site-block-diagonal (Build_Hamiltonian.F90:62-64)
-    H = np.empty((n, n, npx), complex, order="F")
-    H[:, :, :] = hh2[:, :, None]
-    # Aharonov-Bohm phase on y-hops (Build_Hamiltonian.F90:186-207)
+    h0 = hh2
+    # Aharonov-Bohm phase on y-hops (Build_Hamiltonian.F90:186-207): the same in every slice
     if abs(flux) > 1e-10 and npy > 1 and npz > 1:
         for z in range(npz):
             ph = np.exp(1j * 2.0 * np.pi * z * flux / (npy - 1) / (npz - 1))
             for y in range(npy - 1):
                 i0 = y * norb + z * NN
-                blk = H[i0:i0 + norb, i0 + norb:i0 + 2 * norb, :] * ph
-                H[i0:i0 + norb, i0 + norb:i0 + 2 * norb, :] = blk
-                H[i0 + norb:i0 + 2 * norb, i0:i0 + norb, :] = np.conj(np.transpose(blk, (1, 0, 2)))
+                blk = h0[i0:i0 + norb, i0 + norb:i0 + 2 * norb] * ph
+                h0[i0:i0 + norb, i0 + norb:i0 + 2 * norb] = blk
+                h0[i0 + norb:i0 + 2 * norb, i0:i0 + norb] = np.conj(blk.T)
+    # Zeeman g*B*gMx on every site (Insert_Magnetic_Impurity, Build_Hamiltonian.F90:348-360);
+    # gMx has a zero diagonal, so it commutes bitwise with the potentials added below
+    if zeeman != 0.0 and norb == 4:
+        h0 = h0 + np.kron(np.eye(npy * npz), zeeman * GMX)
     # Anderson disorder on slices 2..Npx-1 (Build_Hamiltonian.F90:278-296), z fastest
+    onsite = np.zeros((npx, npy, npz), order="F")
     if abs(disorder) > 1e-20 and npx > 2:
         u = splitmix64_uniform(seed, (npx - 2) * npy * npz).reshape(npx - 2, npy, npz)
-        pot = (u - 0.5) * disorder
-        for xi in range(npx - 2):
-            for y in range(npy):
-                for z in range(npz):
-                    i0 = y * norb + z * NN
-                    d = np.arange(i0, i0 + norb)
-                    H[d, d, xi + 1] += pot[xi, y, z]
-    # Zeeman g*B*gMx on every site (Insert_Magnetic_Impurity, Build_Hamiltonian.F90:348-360)
-    if zeeman != 0.0 and norb == 4:
-        H += np.kron(np.eye(npy * npz), zeeman * GMX)[:, :, None]
-    return Wire(npx, npy, npz, norb, H, np.asfortranarray(hopx), gm)
+        onsite[1:npx - 1] = (u - 0.5) * disorder
+    H = np.empty((n, n, npx), complex, order="F")
+    H[:, :, :] = h0[:, :, None]
+    a = np.arange(n)
+    site = a // norb
+    H[a, a, :] += onsite[:, site % npy, site // npy].T
+    return Wire(npx, npy, npz, norb, H, np.asfortranarray(hopx), gm, np.asfortranarray(h0), onsite)
 
 
 def energy_grid(emin: float, emax: float, count: int) -> np.ndarray:
