@@ -228,12 +228,13 @@ def server_workload(name, args):
             ny = int(rng.choice(wl["ns"]))
             L = int(rng.integers(wl["L"][0], wl["L"][1] + 1))
             w = build_wire(L, ny, ny, disorder=float(rng.uniform(0, 1)), seed=wl["seed"] + r,
-                           flux=float(rng.uniform(0, 2)), zeeman=float(rng.uniform(0, 0.05)))
+                           flux=float(rng.uniform(0, 2)), zeeman=float(rng.uniform(0, 0.05)), dense=False)
             sel = grid[(np.arange(wl["energies_per"]) * (32 // wl["energies_per"]) + r) % 32]
             reqs.append((0.0, w, sel))
         desc = (f"c3 sample: {wl['instances']} of the 256 seeded realisations x {wl['energies_per']} of "
                 f"their 32 energies in [-0.4,0.4]; Npy=Npz in {wl['ns']}, Npx in {wl['L']}, W in [0,1], "
-                f"g*B in [0,0.05], flux in [0,2]; all submitted at t=0")
+                f"g*B in [0,0.05], flux in [0,2]; all submitted at t=0 as compact Hamiltonians "
+                f"(slice template + site potentials)")
     else:
         t = 0.0
         for r in range(wl["requests"]):
@@ -242,12 +243,12 @@ def server_workload(name, args):
             L = int(rng.integers(wl["L"][0], wl["L"][1] + 1))
             ne = int(rng.integers(16, wl["ne_max"] + 1))
             w = build_wire(L, ny, ny, disorder=float(rng.uniform(0, 1)), seed=wl["seed"] + r,
-                           flux=float(rng.uniform(0, 2)))
+                           flux=float(rng.uniform(0, 2)), dense=False)
             reqs.append((t, w, energy_grid(-0.4, 0.4, ne)))
         desc = (f"c5 stream: {wl['requests']} requests, Poisson arrivals at {wl['rate']}/s; per request "
                 f"Npy=Npz in {wl['ns']}, Npx in {wl['L']}, 16..{wl['ne_max']} energies in [-0.4,0.4], "
                 f"flux in [0,2], W in [0,1], own seed (scaled down from Npy=Npz 8..24, Npx 50..300, "
-                f"16..256 energies)")
+                f"16..256 energies); compact Hamiltonians (slice template + site potentials)")
     prec = capi.NEGF_PREC_MIXED if args.precision == "mixed" else capi.NEGF_PREC_FP64
     npts = sum(len(e) for _, _, e in reqs)
     print(f"[bench] {name}: {len(reqs)} requests, {npts} energy points", file=sys.stderr, flush=True)
@@ -261,7 +262,7 @@ def server_workload(name, args):
             dt = at - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            ids[i] = srv.submit(w, e, e[1] - e[0] if len(e) > 1 else 0.0, MU, MUT)
+            ids[i] = srv.submit(w, e, e[1] - e[0] if len(e) > 1 else 0.0, MU, MUT, compact=True)
 
     th = threading.Thread(target=submitter)
     th.start()

@@ -105,11 +105,13 @@ def _model_blocks(model: str, params: dict):
 
 def build_wire(npx: int, npy: int, npz: int, model: str = "ti4", params: dict | None = None,
                bcy: int = 0, bcz: int = 0, disorder: float = 0.0, seed: int = 1,
-               flux: float = 0.0, zeeman: float = 0.0) -> Wire:
+               flux: float = 0.0, zeeman: float = 0.0, dense: bool = True) -> Wire:
     """Build ``Htotd`` and ``HopX`` for an Npx x Npy x Npz wire.
 
     ``disorder`` is the Anderson width W, ``flux`` the Aharonov-Bohm phase
-    phi/phi0, ``zeeman`` the g*B prefactor of gMx.
+    phi/phi0, ``zeeman`` the g*B prefactor of gMx.  The compact description
+    (``h0``, ``onsite``) is always built; ``dense=False`` skips the n x n x Npx
+    ``Htotd`` (None) for callers that upload the compact form.
     """
     norb, onsite, hx, hy, hz, gm = _model_blocks(model, params or {})
     NN = npy * norb
@@ -154,11 +156,13 @@ def build_wire(npx: int, npy: int, npz: int, model: str = "ti4", params: dict | 
     if abs(disorder) > 1e-20 and npx > 2:
         u = splitmix64_uniform(seed, (npx - 2) * npy * npz).reshape(npx - 2, npy, npz)
         onsite[1:npx - 1] = (u - 0.5) * disorder
-    H = np.empty((n, n, npx), complex, order="F")
-    H[:, :, :] = h0[:, :, None]
-    a = np.arange(n)
-    site = a // norb
-    H[a, a, :] += onsite[:, site % npy, site // npy].T
+    H = None
+    if dense:
+        H = np.empty((n, n, npx), complex, order="F")
+        H[:, :, :] = h0[:, :, None]
+        a = np.arange(n)
+        site = a // norb
+        H[a, a, :] += onsite[:, site % npy, site // npy].T
     return Wire(npx, npy, npz, norb, H, np.asfortranarray(hopx), gm, np.asfortranarray(h0), onsite)
 
 
