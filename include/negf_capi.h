@@ -78,6 +78,14 @@ typedef struct {
  * concurrent lanes (for tools that cannot follow kernels dispatched from two
  * host threads, e.g. per-dispatch hardware counter collection). */
 #define NEGF_FLAG_SINGLE_LANE 4
+/* Keep each energy's contact self-energies (Sigma_L, Sigma_R, with the
+ * Sancho-Rubio iteration counts) on the device and reuse them in later sweeps of
+ * the same energies while HopX and the two end slices are bitwise unchanged --
+ * e.g. across Poisson iterations that only move the interior potential
+ * (NEGF_Main.F90:7-30).  Reused self-energies are the same bits, so results are
+ * unchanged; the Sancho-Rubio cost is paid once per energy.  Up to 15% of the
+ * free HBM holds the cache. */
+#define NEGF_FLAG_CACHE_CONTACTS 8
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
@@ -116,6 +124,14 @@ int negf_set_hamiltonian(negf_ctx* ctx, const double* Htotd, const double* HopX,
  * negf_set_hamiltonian with the Htotd so defined. */
 int negf_set_hamiltonian_compact(negf_ctx* ctx, const double* H0, const double* HopX,
                                  const double* gM, const double* onsite);
+
+/* Replace the site potentials of a compact Hamiltonian (the part a Poisson
+ * iteration changes: the electrostatic potential on the diagonal,
+ * Build_Hamiltonian.F90:104-112) without re-uploading the template. */
+int negf_set_onsite(negf_ctx* ctx, const double* onsite);
+/* Contact-cache state (NEGF_FLAG_CACHE_CONTACTS): stored energies, and energies
+ * served from the cache since it was last invalidated. */
+int negf_contact_cache_stats(negf_ctx* ctx, int* entries, long long* hits);
 
 /* Energy_Sweep (Energy_Sweep.F90:1-46) for one energy, fused on the device:
  * contacts, RGF, Gn/Gp site blocks and observables.  mu[2], muT give the step

@@ -33,7 +33,8 @@ EXPORTS = (
     "negf_energy_sweep", "negf_submit", "negf_flush", "negf_contact_self_energy",
     "negf_sancho_rubio", "negf_retarded", "negf_gnp", "negf_observables", "negf_invert_matrix",
     "negf_zgemm", "negf_profile_begin", "negf_profile_end", "negf_energy_sweep_f", "negf_flush_f",
-    "negf_observables_f", "negf_set_hamiltonian_compact", "negf_server_create", "negf_server_submit", "negf_server_poll",
+    "negf_observables_f", "negf_set_hamiltonian_compact", "negf_set_onsite", "negf_contact_cache_stats",
+    "negf_server_create", "negf_server_submit", "negf_server_poll",
     "negf_server_wait", "negf_server_destroy",
 )
 
@@ -52,6 +53,7 @@ class NegfConfig(ctypes.Structure):
 FLAG_RGF_THREE_PASS = 1
 FLAG_SR_SEPARATE_LEADS = 2
 FLAG_SINGLE_LANE = 4
+FLAG_CACHE_CONTACTS = 8
 
 
 class NegfAccum(ctypes.Structure):
@@ -114,6 +116,8 @@ def lib():
         L.negf_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _ip, _dp]
         L.negf_server_destroy.argtypes = [ctypes.c_void_p]
         L.negf_set_hamiltonian_compact.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.negf_set_onsite.argtypes = [ctypes.c_void_p, _dp]
+        L.negf_contact_cache_stats.argtypes = [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]
         L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 8
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
@@ -246,6 +250,18 @@ class Context:
                            np.asfortranarray(wire.onsite, dtype=float))
         H0, X, g, V = self._wire_keep
         _check(lib().negf_set_hamiltonian_compact(self._h, _p(H0), _p(X), _p(g), _p(V)))
+
+    def set_onsite(self, onsite):
+        """Replace the site potentials (Npx, Npy, Npz) of a compact Hamiltonian."""
+        V = np.asfortranarray(onsite, dtype=float)
+        self._onsite_keep = V
+        _check(lib().negf_set_onsite(self._h, _p(V)))
+
+    def contact_cache_stats(self):
+        """(stored energies, energies served from the contact cache)."""
+        e, h = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().negf_contact_cache_stats(self._h, ctypes.byref(e), ctypes.byref(h)))
+        return e.value, h.value
 
     def energy_sweep(self, energy, eCtr, deltaE, mu, muT, acc: Accumulators):
         m = np.asarray(mu, float)

@@ -34,6 +34,7 @@
 #include <atomic>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 
 namespace negf {
 template <typename T>
@@ -173,6 +174,30 @@ struct Lane {
   double* d_site = nullptr;    // cap * site_per_e
   double* d_mc = nullptr;      // 2cap
   int* d_active = nullptr;     // 4cap: Sancho-Rubio active lists
+  // contact-cache entries written by this lane's current batch: (energy, slot,
+  // iters L/R, status L/R), published once the batch's stream has drained
+  struct CachePending { double energy; int slot, it0, it1, st0, st1; };
+  std::vector<CachePending> cc_pending;
+};
+
+// Contact self-energies kept across sweeps (NEGF_FLAG_CACHE_CONTACTS): per energy
+// value, Sigma_L and Sigma_R (two n x n slots) with the Sancho-Rubio iteration
+// counts and status.  Valid while HopX and the two end slices are bitwise
+// unchanged, the inputs Contact_Self_Energy.F90:68-101 reads.
+struct ContactCache {
+  std::mutex mu;
+  std::unordered_map<unsigned long long, int> slot;  // energy bits -> slot
+  std::vector<int> meta;                               // per slot: it0, it1, st0, st1
+  zd* buf = nullptr;
+  int cap = 0, used = 0;
+  std::vector<zd> key;  // HopX and the two end slices the entries belong to
+  long long hits = 0;   // energies served from the cache since the last clear
+  void clear() {
+    std::lock_guard<std::mutex> lk(mu);
+    slot.clear();
+    used = 0;
+    hits = 0;
+  }
 };
 
 struct negf_ctx {
@@ -182,6 +207,8 @@ struct negf_ctx {
   bool have_h = false;
   bool blockdiag = true;
   bool ends_equal = false;  // H slice 1 == H slice L bitwise: one decimation serves both leads
+  ContactCache cc;
+  std::vector<zd> h0_host, hopx_host;  // compact form: template and HopX (for set_onsite)
   zd* H = nullptr;       // Htotd (n,n,L), or null for a compact Hamiltonian:
   bool compact = false;  //   slice x = H0 + diag(V(x, site) on every orbital of the site)
   zd* H0 = nullptr;      // slice template (n,n)
@@ -300,6 +327,7 @@ extern "C" int negf_finalize(negf_ctx* c) {
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
   hipFree(c->X32);
   hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
+  hipFree(c->cc.buf);
   for (Lane& l : c->lanes) hipStreamDestroy(l.st);
   delete c;
   return 0;
@@ -354,6 +382,20 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
   return 0;
 }
 
+// Keep the contact cache only if HopX and both end slices are bitwise what its
+// entries were computed from.
+static void contact_key_update(negf_ctx* c, const double* HopX, const zd* end0, const zd* end1) {
+  if (!(c->cfg.flags & NEGF_FLAG_CACHE_CONTACTS)) return;
+  const size_t nn = (size_t)c->n * c->n;
+  std::vector<zd> key(3 * nn);
+  std::memcpy(key.data(), HopX, sizeof(zd) * nn);
+  std::memcpy(key.data() + nn, end0, sizeof(zd) * nn);
+  std::memcpy(key.data() + 2 * nn, end1, sizeof(zd) * nn);
+  if (c->cc.key.size() != key.size() || std::memcmp(c->cc.key.data(), key.data(), sizeof(zd) * key.size()))
+    c->cc.clear();
+  c->cc.key.swap(key);
+}
+
 extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const double* HopX,
                                     const double* gM) {
   if (!c || !Htotd || !HopX) {
@@ -370,11 +412,14 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
   c->V = nullptr;
   c->compact = false;
   c->ends_equal = std::memcmp(Htotd, Htotd + 2 * (size_t)(L - 1) * nn, sizeof(zd) * nn) == 0;
+  contact_key_update(c, HopX, (const zd*)Htotd, (const zd*)Htotd + (size_t)(L - 1) * nn);
   if (!c->H) HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
   HIPCHK(hipMemcpy(c->H, Htotd, sizeof(zd) * nn * L, hipMemcpyHostToDevice));
   c->have_h = true;
   return 0;
 }
+
+static int set_potentials(negf_ctx* c, const double* onsite);
 
 extern "C" int negf_set_hamiltonian_compact(negf_ctx* c, const double* H0, const double* HopX,
                                             const double* gM, const double* onsite) {
@@ -395,6 +440,17 @@ extern "C" int negf_set_hamiltonian_compact(negf_ctx* c, const double* H0, const
     HIPCHK(hipMalloc(&c->Hends, sizeof(zd) * nn * 2));
   }
   HIPCHK(hipMemcpy(c->H0, H0, sizeof(zd) * nn, hipMemcpyHostToDevice));
+  c->h0_host.assign((const zd*)H0, (const zd*)H0 + nn);
+  c->hopx_host.assign((const zd*)HopX, (const zd*)HopX + nn);
+  c->compact = true;
+  return set_potentials(c, onsite);
+}
+
+// Site potentials of a compact Hamiltonian: the device array, the two end slices
+// and the contact-cache key.
+static int set_potentials(negf_ctx* c, const double* onsite) {
+  const int n = c->n, L = c->L, ns = c->nsites, no = c->cfg.norb;
+  const long long nn = (long long)n * n;
   // onsite (Npx, Npy, Npz) Fortran order -> slice-major (x, site), site = y + Npy z
   std::vector<double> v((size_t)L * ns);
   for (int x = 0; x < L; ++x)
@@ -402,17 +458,38 @@ extern "C" int negf_set_hamiltonian_compact(negf_ctx* c, const double* H0, const
   HIPCHK(hipMemcpy(c->V, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
   // the two end slices assembled for the contacts (H0(a,a) + V, as the slices are)
   std::vector<zd> ends(2 * nn);
-  const zd* h0 = (const zd*)H0;
+  const zd* h0 = c->h0_host.data();
   for (int e = 0; e < 2; ++e) {
     const int x = e ? L - 1 : 0;
     std::memcpy(&ends[e * nn], h0, sizeof(zd) * nn);
     for (int a = 0; a < n; ++a) ends[e * nn + (long long)a * n + a].x += v[(size_t)x * ns + a / no];
   }
   c->ends_equal = std::memcmp(ends.data(), ends.data() + nn, sizeof(zd) * nn) == 0;
+  contact_key_update(c, (const double*)c->hopx_host.data(), ends.data(), ends.data() + nn);
   HIPCHK(hipMemcpy(c->Hends, ends.data(), sizeof(zd) * 2 * nn, hipMemcpyHostToDevice));
-  c->compact = true;
   c->have_h = true;
   return 0;
+}
+
+extern "C" int negf_contact_cache_stats(negf_ctx* c, int* entries, long long* hits) {
+  if (!c) return NEGF_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->cc.mu);
+  if (entries) *entries = (int)c->cc.slot.size();
+  if (hits) *hits = c->cc.hits;
+  return 0;
+}
+
+extern "C" int negf_set_onsite(negf_ctx* c, const double* onsite) {
+  if (!c || !onsite) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  if (!c->compact) {
+    negf_set_error("negf_set_onsite needs a compact Hamiltonian (negf_set_hamiltonian_compact)");
+    return NEGF_ERR_STATE;
+  }
+  HIPCHK(hipSetDevice(c->cfg.device));
+  return set_potentials(c, onsite);
 }
 
 // Slice 1 (which = 0) or slice L (which = 1) of the Hamiltonian, for the contacts.
@@ -785,6 +862,17 @@ static int collect(negf_ctx* c, Lane& ln, int nb, const std::vector<int>& iters,
   TRY(hipMemcpyAsync(site.data(), ln.d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
   TRY(hipMemcpyAsync(ints.data(), ln.Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
+  if (!ln.cc_pending.empty()) {  // the batch's Sigma copies are complete: publish them
+    std::lock_guard<std::mutex> lk(c->cc.mu);
+    for (const Lane::CachePending& p : ln.cc_pending) {
+      unsigned long long key;
+      std::memcpy(&key, &p.energy, sizeof(key));
+      int* m = &c->cc.meta[4 * p.slot];
+      m[0] = p.it0; m[1] = p.it1; m[2] = p.st0; m[3] = p.st1;
+      c->cc.slot[key] = p.slot;
+    }
+    ln.cc_pending.clear();
+  }
   for (int b = 0; b < nb; ++b) {
     EnergyResult& e = res[b];
     std::memcpy(e.scal, &scal[8 * b], sizeof(e.scal));
@@ -811,17 +899,74 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   TRY(hipMemsetAsync(ln.d_scal, 0, sizeof(double) * 8 * nb, s));
   TRY(hipMemsetAsync(ln.Ie, 0, sizeof(int) * (size_t)y.Is * nb, s));
   // ---- contacts: chains [0,nb) left lead, [nb,2nb) right lead
-  const zd* Hend0 = slice_end(c, 0);
-  const zd* Hend1 = slice_end(c, 1);
-  TRY(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, Hend0, Hend1,
-                              ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
   std::vector<int> iters, sst;
-  // left: beta = -HopX', betad = -HopX;  right: beta = -HopX, betad = -HopX'
-  const bool paired = c->ends_equal && !(c->cfg.flags & NEGF_FLAG_SR_SEPARATE_LEADS);
-  int rc = paired ? run_sr_paired(c, ln, nb, Q(y.o_SigL), Q(y.o_SigR), Es, iters, sst)
-                  : run_sr(c, ln, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL),
-                           Q(y.o_SigR), Es, iters, sst);
-  if (rc) return rc;
+  ln.cc_pending.clear();
+  const bool use_cache = (c->cfg.flags & NEGF_FLAG_CACHE_CONTACTS) != 0;
+  std::vector<int> hit(nb, -1);
+  bool all_hit = use_cache;
+  if (use_cache) {
+    std::lock_guard<std::mutex> lk(c->cc.mu);
+    for (int b = 0; b < nb; ++b) {
+      unsigned long long key;
+      std::memcpy(&key, &energies[b], sizeof(key));
+      auto it = c->cc.slot.find(key);
+      if (it == c->cc.slot.end()) all_hit = false;
+      else hit[b] = it->second;
+    }
+  }
+  if (all_hit) {
+    // every energy's Sigma_L, Sigma_R from an earlier sweep with the same leads
+    {
+      std::lock_guard<std::mutex> lk(c->cc.mu);
+      c->cc.hits += nb;
+    }
+    iters.assign(2 * nb, 0);
+    sst.assign(2 * nb, 0);
+    for (int b = 0; b < nb; ++b) {
+      const int sl = hit[b];
+      TRY(hipMemcpyAsync(Q(y.o_SigL) + (size_t)b * Es, c->cc.buf + (size_t)sl * 2 * nn, sizeof(zd) * nn,
+                         hipMemcpyDeviceToDevice, s));
+      TRY(hipMemcpyAsync(Q(y.o_SigR) + (size_t)b * Es, c->cc.buf + ((size_t)sl * 2 + 1) * nn,
+                         sizeof(zd) * nn, hipMemcpyDeviceToDevice, s));
+      const int* m = &c->cc.meta[4 * sl];
+      iters[b] = m[0]; iters[nb + b] = m[1]; sst[b] = m[2]; sst[nb + b] = m[3];
+    }
+  } else {
+    const zd* Hend0 = slice_end(c, 0);
+    const zd* Hend1 = slice_end(c, 1);
+    TRY(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, Hend0, Hend1,
+                                ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
+    // left: beta = -HopX', betad = -HopX;  right: beta = -HopX, betad = -HopX'
+    const bool paired = c->ends_equal && !(c->cfg.flags & NEGF_FLAG_SR_SEPARATE_LEADS);
+    int rc = paired ? run_sr_paired(c, ln, nb, Q(y.o_SigL), Q(y.o_SigR), Es, iters, sst)
+                    : run_sr(c, ln, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL),
+                             Q(y.o_SigR), Es, iters, sst);
+    if (rc) return rc;
+    if (use_cache) {
+      std::lock_guard<std::mutex> lk(c->cc.mu);
+      if (!c->cc.buf) {  // sized once: up to 15% of the free HBM
+        size_t freeb = 0, totalb = 0;
+        TRY(hipMemGetInfo(&freeb, &totalb));
+        const size_t per = sizeof(zd) * 2 * (size_t)nn;
+        c->cc.cap = (int)std::min<size_t>(0.15 * (double)freeb / per, 1 << 16);
+        if (c->cc.cap > 0) TRY(hipMalloc(&c->cc.buf, per * c->cc.cap));
+        c->cc.meta.assign(4 * (size_t)c->cc.cap, 0);
+      }
+      for (int b = 0; b < nb && c->cc.used < c->cc.cap; ++b) {
+        if (sst[b] || sst[nb + b]) continue;  // only clean contacts are reused
+        const int sl = c->cc.used++;
+        ln.cc_pending.push_back({energies[b], sl, iters[b], iters[nb + b], sst[b], sst[nb + b]});
+      }
+    }
+    for (const Lane::CachePending& p : ln.cc_pending) {
+      const int b = (int)(std::find_if(energies, energies + nb, [&](double e) { return e == p.energy; }) -
+                          energies);
+      TRY(hipMemcpyAsync(c->cc.buf + (size_t)p.slot * 2 * nn, Q(y.o_SigL) + (size_t)b * Es,
+                         sizeof(zd) * nn, hipMemcpyDeviceToDevice, s));
+      TRY(hipMemcpyAsync(c->cc.buf + ((size_t)p.slot * 2 + 1) * nn, Q(y.o_SigR) + (size_t)b * Es,
+                         sizeof(zd) * nn, hipMemcpyDeviceToDevice, s));
+    }
+  }
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
 
