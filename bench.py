@@ -45,9 +45,10 @@ CONFIGS = {
     # plain FP64 mid-size: block 1024, Npx=100, W=0.5
     "c2": dict(npx=100, npy=16, npz=16, ne=512, emin=-0.4, emax=0.4, W=0.5, seed=1, per_step=64,
                budget_gb=0.0),
-    # precision target size: block 2304, Npx=200, W=0.3
+    # precision target size: block 2304, Npx=200, W=0.3.  245 GB of energies in flight
+    # (the library caps it at 90% of free HBM) holds two energies per lane on four lanes
     "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=8,
-               budget_gb=220.0),
+               budget_gb=245.0),
 }
 # Many-wire workloads run through the serving / batching layer (negf_server):
 #  c3 -- a seeded sample of the batching configuration (256 disorder realisations x
@@ -66,6 +67,7 @@ SERVER_WORKLOADS = {
 }
 ETA, SR_TOL, SR_MAX = 1e-6, 1e-10, 200
 MU, MUT = (-0.1, 0.0), 0.0
+LANES = [0]  # set from --lanes
 OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", "RhoN", "RhoP",
            "SigXExp", "SigYExp", "SigZExp")
 
@@ -167,7 +169,8 @@ def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmu
     L = cfg["npx"]
     prec = capi.NEGF_PREC_MIXED if precision == "mixed" else capi.NEGF_PREC_FP64
     ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
-                       max_batch=per, flags=flags, precision=prec, hbm_budget_gb=cfg["budget_gb"])
+                       max_batch=per, flags=flags, precision=prec, hbm_budget_gb=cfg["budget_gb"],
+                       lanes=LANES[0])
     ctx.set_hamiltonian(w)
 
     def run_step(k, acc):
@@ -307,6 +310,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS) + sorted(SERVER_WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"))
+    ap.add_argument("--lanes", type=int, default=0, help="concurrent device lanes (0 = library default)")
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -328,6 +332,7 @@ def main():
                                    "final sums; only --gpus 1 is measured)"}))
         sys.exit(2)
 
+    LANES[0] = args.lanes
     if args.config in SERVER_WORKLOADS:
         print(json.dumps(server_workload(args.config, args)), flush=True)
         return
@@ -413,8 +418,8 @@ def main():
                      "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
                      "note": ("achieved = algorithmic 8 n^3 flops per complex product x batch / HIP-event "
                               "duration on the launching stream (one launch in --profile-every sampled); the "
-                              "3M form executes 6 n^3 on the matrix cores; two lanes run concurrently, so "
-                              "durations include the other lane's kernels sharing the CUs")},
+                              "3M form executes 6 n^3 on the matrix cores; the lanes run concurrently, so "
+                              "durations include the other lanes' kernels sharing the CUs")},
         "kernels": kernels,
     }
     if compare is not None:

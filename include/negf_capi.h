@@ -62,6 +62,7 @@ typedef struct {
   double hbm_budget_gb;/* device memory the context may use for in-flight energies; 0 = auto */
   int max_batch;       /* cap on energies in flight; 0 = auto */
   int flags;           /* NEGF_FLAG_* (0 = default algorithms) */
+  int lanes;           /* concurrent device lanes of negf_flush, 1..4 (0 = 4) */
 } negf_config;
 
 /* Algorithm selection (results agree to rounding either way).

@@ -48,7 +48,7 @@ class NegfConfig(ctypes.Structure):
                 ("sr_max_iter", ctypes.c_int), ("sr_extra_eta", ctypes.c_int),
                 ("precision", ctypes.c_int), ("device", ctypes.c_int),
                 ("hbm_budget_gb", ctypes.c_double), ("max_batch", ctypes.c_int),
-                ("flags", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("lanes", ctypes.c_int)]
 
 FLAG_RGF_THREE_PASS = 1
 FLAG_SR_SEPARATE_LEADS = 2
@@ -212,11 +212,11 @@ class Context:
 
     def __init__(self, npx, npy, npz, norb=4, eta=1e-6, sr_tol=1e-10, sr_max_iter=200,
                  sr_extra_eta=True, precision=NEGF_PREC_FP64, device=0, hbm_budget_gb=0.0,
-                 max_batch=0, flags=0):
+                 max_batch=0, flags=0, lanes=0):
         self.dims = (npx, npy, npz, norb)
         self.n = npy * npz * norb
         cfg = NegfConfig(npx, npy, npz, norb, eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)),
-                         precision, device, hbm_budget_gb, max_batch, flags)
+                         precision, device, hbm_budget_gb, max_batch, flags, lanes)
         h = ctypes.c_void_p()
         _check(lib().negf_init(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h

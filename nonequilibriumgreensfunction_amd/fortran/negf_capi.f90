@@ -23,6 +23,7 @@ module negf_capi
     real(c_double) :: hbm_budget_gb
     integer(c_int) :: max_batch
     integer(c_int) :: flags
+    integer(c_int) :: lanes
   end type negf_config
 
   interface

@@ -68,6 +68,7 @@ subroutine NEGF_Prepare_Device
     cfg%hbm_budget_gb = 0d0
     cfg%max_batch = 0
     cfg%flags = negf_shim_flags
+    cfg%lanes = 0
     rc = negf_init(negf_shim_ctx, cfg)
     if (rc /= 0) call negf_shim_fail("negf_init", rc)
     negf_shim_h_ite = -huge(1)
