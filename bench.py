@@ -312,6 +312,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"))
     ap.add_argument("--lanes", type=int, default=0, help="concurrent device lanes (0 = library default)")
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
+    ap.add_argument("--budget-gb", type=float, default=-1.0,
+                    help="HBM for energies in flight (default per config; 0 = library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain-FP64 comparison step of the mixed form")
@@ -336,7 +338,9 @@ def main():
     if args.config in SERVER_WORKLOADS:
         print(json.dumps(server_workload(args.config, args)), flush=True)
         return
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config])
+    if args.budget_gb >= 0:
+        cfg["budget_gb"] = args.budget_gb
     per = args.per_step or cfg["per_step"]
     n = cfg["npy"] * cfg["npz"] * 4
     L = cfg["npx"]

@@ -14,7 +14,7 @@
 // which the final unpermute kernel applies while copying to the output.
 //
 // Blocking.  Columns are processed in outer blocks of kNBO and inner panels of
-// kNBI.  A panel kernel (one 768- or 1024-thread workgroup per matrix, panel rows
+// kNBI.  A panel kernel (one 512-thread workgroup per matrix, panel rows
 // held in registers) factors kNBI columns; a gather kernel then copies the pivot
 // rows' values in all other columns into Prow.  A rank-kNBI update brings the rest of the outer
 // block up to date; after the whole outer block a rank-kNBO update (the GEMM
@@ -28,7 +28,7 @@ template <typename T>
 hipError_t gemm_gj_update(hipStream_t s, const GemmBatch& g, int batch);
 
 constexpr int kNBI = 8;    // panel width
-constexpr int kNBO = kGjNBO;  // outer block width (= GEMM tile width)
+constexpr int kNBO = kGjNBO;  // outer block width (two GEMM tiles; the update skips its columns)
 
 __global__ void k_gj_init(int* pivstep, int* pivrow, int* info, long long sInt, int n,
                           const int* active) {
@@ -41,12 +41,39 @@ __global__ void k_gj_init(int* pivstep, int* pivrow, int* info, long long sInt, 
   }
 }
 
-// Two barriers per pivot column: (1) after each wave has published its partial
-// (max, row), every wave reduces the NT/64 partials itself -- no broadcast barrier
-// -- and the thread holding the pivot row publishes the row scaled by 1/pivot;
-// (2) after that, every thread eliminates.  The LDS scratch alternates between two
-// buffers by column parity, so column jj+1 never overwrites what a slower wave
-// still reads for column jj.
+// Pivot ranking.  A candidate row i is ranked by a 64-bit key: the bit pattern of
+// its magnitude |re|+|im| as a double (monotonic for non-negative values) with
+// the low 12 bits replaced by 4095 - i, so the larger magnitude wins and ties go
+// to the lower row (izamax's rule; magnitudes within 2^-40 relative count as
+// ties).  Key 0 means no candidate (a real key is at least 4095 - 3071).
+__device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
+  return a > b ? a : b;
+}
+template <int CTRL>
+__device__ inline unsigned long long dpp_u64(unsigned long long x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, 0xf, 0xf, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+// Max over the wave, in every lane: within each row of 16 lanes by DPP (quad
+// xor 1, quad xor 2, half-row mirror, row mirror), then across rows by xor 16
+// and xor 32 shuffles.
+__device__ inline unsigned long long wave_max_u64(unsigned long long x) {
+  x = umax64(x, dpp_u64<0xb1>(x));   // quad_perm [1,0,3,2]
+  x = umax64(x, dpp_u64<0x4e>(x));   // quad_perm [2,3,0,1]
+  x = umax64(x, dpp_u64<0x141>(x));  // row_half_mirror
+  x = umax64(x, dpp_u64<0x140>(x));  // row_mirror
+  x = umax64(x, __shfl_xor(x, 16));
+  x = umax64(x, __shfl_xor(x, 32));
+  return x;
+}
+
+// Two barriers per pivot column: (1) after each wave has published its best key,
+// every thread reduces the NT/64 keys itself -- no broadcast barrier -- and the
+// thread holding the pivot row publishes the row scaled by 1/pivot; (2) after
+// that, every thread eliminates.  The LDS scratch alternates between two buffers
+// by column parity, so column jj+1 never overwrites what a slower wave still
+// reads for column jj.  512 threads (two waves per SIMD) hold R rows each.
 template <typename T, int R, int NT>
 __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, int ld,
                                                              int c0, int nb, int k0, GjWork w,
@@ -58,10 +85,8 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
   int* pivstep = w.pivstep + (long long)b * w.sInt;
   int* pivrow = w.pivrow + (long long)b * w.sInt;
   int* info = w.info + (long long)b * w.sInt;
-  V* prow = (V*)w.prow + (long long)b * w.sProw;
 
-  __shared__ T red_v[2][NW];
-  __shared__ int red_i[2][NW];
+  __shared__ unsigned long long red_k[2][NW];
   __shared__ V psc[2][kNBI + 1];  // pivot row scaled by 1/pivot; [kNBI] = 1/pivot
 
   const int tid = threadIdx.x, lane = tid & 63;
@@ -85,41 +110,29 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
   for (int jj = 0; jj < kNBI; ++jj) {
     if (jj < nb) {
       const int par = jj & 1;
-      // pivot search: largest |re|+|im| over unused rows, ties to the lower row
-      T best = (T)-1;
-      int bi = 0x7fffffff;
+      unsigned long long best = 0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int i = tid + r * NT;
         if (i < n && !used[r]) {
-          T m = fabs(v[r][jj].x) + fabs(v[r][jj].y);
-          if (m != m) m = (T)INFINITY;
-          if (m > best || (m == best && i < bi)) { best = m; bi = i; }
+          double m = (double)(fabs(v[r][jj].x) + fabs(v[r][jj].y));
+          if (m != m) m = (double)INFINITY;
+          best = umax64(best, ((unsigned long long)__double_as_longlong(m) & ~0xfffull) |
+                                  (unsigned long long)(4095 - i));
         }
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        T ob = __shfl_xor(best, off);
-        int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-      }
-      if (lane == 0) { red_v[par][tid >> 6] = best; red_i[par][tid >> 6] = bi; }
+      best = wave_max_u64(best);
+      if (lane == 0) red_k[par][tid >> 6] = best;
       __syncthreads();  // (1)
-      best = lane < NW ? red_v[par][lane] : (T)-1;
-      bi = lane < NW ? red_i[par][lane] : 0x7fffffff;
+      unsigned long long g = red_k[par][0];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        T ob = __shfl_xor(best, off);
-        int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-      }
-      const int p = bi;  // the same in every wave
+      for (int q = 1; q < NW; ++q) g = umax64(g, red_k[par][q]);
+      if (g == 0) continue;  // no unused row left (the same in every thread)
+      const int p = 4095 - (int)(g & 0xfffull);
       if (tid == 0) {
-        if (best == (T)0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
-        if (p < n) {
-          pivstep[p] = c0 + jj;
-          pivrow[c0 + jj] = p;
-        }
+        if ((g & ~0xfffull) == 0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
+        pivstep[p] = c0 + jj;
+        pivrow[c0 + jj] = p;
       }
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -162,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
 }
 
 // Gathers the pivot rows of panel [c0, c0 + nb) in every column outside the panel
-// into Prow (row c0 - k0 + m of the outer block's 64 x n panel).  In a
+// into Prow (row c0 - k0 + m of the outer block's kNBO x n panel).  In a
 // column-major matrix each pivot-row element is its own cache line, so the gather
 // is spread over many workgroups (one per 64 columns of each matrix) instead of
 // being done by the single panel workgroup of that matrix.
@@ -217,19 +230,25 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
   if (n > 3072) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_gj_init, dim3((n + 255) / 256, batch), dim3(256), 0, s, w.pivstep, w.pivrow,
                      w.info, w.sInt, n, active);
-  // panel rows live in registers: R rows per thread, 1024 threads up to n = 2048,
-  // then 768 threads (larger register budget per thread) up to n = 3072.
-  const int R = (n <= 2048) ? (n + 1023) / 1024 : (n + 767) / 768;
+  // panel rows live in registers: 512 threads x R rows, R = ceil(n / 512) <= 6
+  constexpr int NT = 512;
+  const int R = (n + NT - 1) / NT;
   for (int k0 = 0; k0 < n; k0 += kNBO) {
     const int kb = (n - k0 < kNBO) ? n - k0 : kNBO;
     for (int c0 = k0; c0 < k0 + kb; c0 += kNBI) {
       const int nb = (k0 + kb - c0 < kNBI) ? k0 + kb - c0 : kNBI;
+#define NEGF_PANEL(RR)                                                                             \
+  hipLaunchKernelGGL((k_gj_panel<T, RR, NT>), dim3(batch), dim3(NT), 0, s, S, sS, n, ld, c0, nb, k0, \
+                     w, active)
       switch (R) {
-        case 1: hipLaunchKernelGGL((k_gj_panel<T, 1, 1024>), dim3(batch), dim3(1024), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
-        case 2: hipLaunchKernelGGL((k_gj_panel<T, 2, 1024>), dim3(batch), dim3(1024), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
-        case 3: hipLaunchKernelGGL((k_gj_panel<T, 3, 768>), dim3(batch), dim3(768), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
-        default: hipLaunchKernelGGL((k_gj_panel<T, 4, 768>), dim3(batch), dim3(768), 0, s, S, sS, n, ld, c0, nb, k0, w, active); break;
+        case 1: NEGF_PANEL(1); break;
+        case 2: NEGF_PANEL(2); break;
+        case 3: NEGF_PANEL(3); break;
+        case 4: NEGF_PANEL(4); break;
+        case 5: NEGF_PANEL(5); break;
+        default: NEGF_PANEL(6); break;
       }
+#undef NEGF_PANEL
       hipLaunchKernelGGL((k_gj_gather<T>), dim3((n + 63) / 64, batch), dim3(64), 0, s, S, sS, n, ld, c0,
                          nb, k0, w, active);
       if (kb > nb) {  // bring the rest of the outer block up to date (rank nb)

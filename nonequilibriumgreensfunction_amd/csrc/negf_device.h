@@ -26,15 +26,22 @@ template <typename V> __host__ __device__ inline V cmulc(V a, V b) {
 }
 template <typename V> __host__ __device__ inline V cconj(V a) { V z; z.x = a.x; z.y = -a.y; return z; }
 template <typename V> __host__ __device__ inline V cneg(V a) { V z; z.x = -a.x; z.y = -a.y; return z; }
-// 1/a = conj(a)/|a|^2, scaled by max(|re|,|im|) so |a|^2 cannot overflow.
+// 1/a = conj(a)/|a|^2 with one division: a is first scaled by the power of two
+// 2^-e that brings max(|re|,|im|) into [1/2, 1) (exact), so |a|^2 can neither
+// overflow nor underflow, and the result is scaled back by 2^-e.  Zero gives
+// (inf, 0); inf or NaN parts give NaN.
 template <typename V> __device__ inline V cinv(V a) {
   typedef decltype(a.x) R;
-  R s = fmax(fabs(a.x), fabs(a.y));
+  const R s = fmax(fabs(a.x), fabs(a.y));
   V z;
   if (s == (R)0) { z.x = (R)1 / s; z.y = (R)0; return z; }
-  R ar = a.x / s, ai = a.y / s;
-  R d = (ar * ar + ai * ai) * s;
-  z.x = ar / d; z.y = -ai / d;
+  if (!(s - s == (R)0)) { z.x = z.y = s - s; return z; }  // inf or NaN
+  int e;
+  (void)frexp(s, &e);
+  const R ar = ldexp(a.x, -e), ai = ldexp(a.y, -e);
+  const R rd = (R)1 / (ar * ar + ai * ai);
+  z.x = ldexp(ar * rd, -e);
+  z.y = ldexp(-ai * rd, -e);
   return z;
 }
 
@@ -87,7 +94,7 @@ hipError_t gemm_batched(hipStream_t s, int opa, int opb, const GemmBatch& g, int
 template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
-constexpr int kGjNBO = 64;  // Gauss-Jordan outer block width (columns per trailing update)
+constexpr int kGjNBO = 128;  // Gauss-Jordan outer block width (columns per trailing update)
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
 // every launch is bracketed by HIP events on its own stream and its algorithmic

@@ -261,15 +261,24 @@ __global__ void k_sr_alpha(cx<T>* alpha, cx<T>* work, long long stride, const cx
   }
 }
 
-// out[c] = max_ij |X_ij| (NaN if any entry is NaN), one workgroup per chain.
+// out[c] = max_ij |X_ij| (NaN if any entry is NaN).  A (chunks x chains) grid:
+// each workgroup reduces its chunk and folds the result into out[c] with a 64-bit
+// unsigned atomic max on the double's bit pattern, which orders non-negative
+// doubles as their values; NaN is stored as the positive quiet NaN, whose pattern
+// exceeds +inf's.  k_maxabs_clear zeroes out[c] first (stream order).
+__global__ void k_maxabs_clear(double* out, const int* active, int batch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < batch) out[active ? active[t] : t] = 0.0;
+}
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_maxabs(const cx<T>* X, long long sX, long long nn,
-                                                 double* out, const int* active) {
-  const int c = active ? active[blockIdx.x] : (int)blockIdx.x;
+__global__ __launch_bounds__(256) void k_maxabs(const cx<T>* X, long long sX, long long nn,
+                                                double* out, const int* active) {
+  const int c = active ? active[blockIdx.y] : (int)blockIdx.y;
   const cx<T>* Xc = X + (long long)c * sX;
   double m = 0.0;
   bool nan = false;
-  for (long long i = threadIdx.x; i < nn; i += 1024) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn; i += (long long)gridDim.x * 256) {
     const double v = hypot((double)Xc[i].x, (double)Xc[i].y);
     if (v != v) nan = true;
     m = fmax(m, v);
@@ -278,21 +287,19 @@ __global__ __launch_bounds__(1024) void k_maxabs(const cx<T>* X, long long sX, l
     m = fmax(m, __shfl_xor(m, off));
     nan = nan || __shfl_xor((int)nan, off);
   }
-  __shared__ double red[16];
-  __shared__ int rn[16];
+  __shared__ double red[4];
+  __shared__ int rn[4];
   if ((threadIdx.x & 63) == 0) {
     red[threadIdx.x >> 6] = m;
     rn[threadIdx.x >> 6] = nan;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    int an = 0;
-    for (int w = 0; w < 16; ++w) {
-      t = fmax(t, red[w]);
-      an |= rn[w];
-    }
-    out[c] = an ? (double)NAN : t;
+    double t = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    const bool an = rn[0] | rn[1] | rn[2] | rn[3];
+    const unsigned long long bits =
+        an ? 0x7ff8000000000000ull : (unsigned long long)__double_as_longlong(t);
+    atomicMax((unsigned long long*)(out + c), bits);
   }
 }
 
@@ -625,7 +632,10 @@ template <typename T>
 hipError_t launch_maxabs(hipStream_t s, const cx<T>* X, long long sX, long long nn, double* out,
                          int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_maxabs<T>), dim3(batch), dim3(1024), 0, s, X, sX, nn, out, active);
+  hipLaunchKernelGGL(k_maxabs_clear, dim3((batch + 63) / 64), dim3(64), 0, s, out, active, batch);
+  const long long chunks = std::min<long long>(512, std::max<long long>(1, (nn + 4095) / 4096));
+  hipLaunchKernelGGL((k_maxabs<T>), dim3((unsigned)chunks, batch), dim3(256), 0, s, X, sX, nn, out,
+                     active);
   return hipGetLastError();
 }
 

@@ -302,7 +302,11 @@ static bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
     int* kd = (int*)realloc(t.kind, sizeof(int) * nc);
     if (!kd) return false;
     t.kind = kd;
-    for (size_t i = 2 * t.cap; i < 2 * nc; ++i) hipEventCreate(&ev[i]);
+    for (size_t i = 2 * t.cap; i < 2 * nc; ++i)
+      if (hipEventCreate(&ev[i]) != hipSuccess) {
+        for (size_t k = 2 * t.cap; k < i; ++k) (void)hipEventDestroy(ev[k]);
+        return false;  // sampling stops; the launch itself is unaffected
+      }
     t.cap = nc;
   }
   t.flops[t.used] = flops;
@@ -335,7 +339,10 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   hipEvent_t ev0, ev1;
   const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
   const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, kind, &ev0, &ev1);
-  if (timed) hipEventRecord(ev0, s);
+  if (timed) {
+    const hipError_t er = hipEventRecord(ev0, s);
+    if (er != hipSuccess) return er;
+  }
   int cfg = (EPI == EPI_GJ) ? g_tile_gj : g_tile_std;
   if (cfg < 0) cfg = 0;
   hipError_t e;
@@ -352,7 +359,7 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
       default: e = launch_cfg<T, OPA, OPB, EPI, 2, 2, 0>(s, g, batch, alpha, beta); break;
     }
   }
-  if (timed) hipEventRecord(ev1, s);
+  if (timed && e == hipSuccess) e = hipEventRecord(ev1, s);
   return e;
 }
 

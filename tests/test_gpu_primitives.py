@@ -29,7 +29,10 @@ def test_zgemm_matches_numpy(rng, n, ops):
     assert np.all(np.abs(got - ref) <= bound + 1e-300)
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 63, 64, 65, 100, 144, 256, 300, 1024, 2100])
+# sizes cover every panel shape of the inverse (512 threads x ceil(n/512) rows, 1..6)
+# at both ends, and partial outer blocks
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 63, 64, 65, 100, 144, 256, 300, 512, 513, 1024, 1025,
+                               1600, 2048, 2100, 2304, 2561, 2800])
 def test_invert_matches_lapack(rng, n):
     A = _rand(rng, n) + 2.0 * np.sqrt(n) * np.eye(n)  # well conditioned
     X, info = capi.invert_matrix(A)
@@ -40,9 +43,9 @@ def test_invert_matches_lapack(rng, n):
     assert np.abs(A @ X - np.eye(n)).max() < 1e-11
 
 
-def test_invert_needs_pivoting(rng):
+@pytest.mark.parametrize("n", [96, 1500, 2500])
+def test_invert_needs_pivoting(rng, n):
     # zero diagonal: elimination without row pivoting would divide by zero
-    n = 96
     P = np.roll(np.eye(n), 7, axis=1)
     A = P * (1.0 + 0.3j) + 1e-3 * _rand(rng, n)
     np.fill_diagonal(A, 0.0)
@@ -51,8 +54,21 @@ def test_invert_needs_pivoting(rng):
     assert np.abs(A @ X - np.eye(n)).max() < 1e-10
 
 
-def test_invert_reports_singular(rng):
-    n = 70
+@pytest.mark.parametrize("n", [64, 1024, 2048])
+def test_invert_hadamard_ties(n):
+    # every column offers many pivots of exactly equal magnitude; inv(c H) = H / (c n)
+    H = np.ones((1, 1))
+    while H.shape[0] < n:
+        H = np.block([[H, H], [H, -H]])
+    c = 1.0 + 1.0j
+    X, info = capi.invert_matrix(c * H)
+    assert info == 0
+    # entries are 1/(|c| n); LAPACK's getrf/getri reproduces them exactly here
+    assert np.abs(X - H.T / (c * n)).max() * n < 1e-10
+
+
+@pytest.mark.parametrize("n", [70, 1100, 2400])
+def test_invert_reports_singular(rng, n):
     A = _rand(rng, n)
     A[:, 33] = 0.0  # exactly singular: some pivot column is all zeros
     X, info = capi.invert_matrix(A)
