@@ -14,12 +14,15 @@
 // which the final unpermute kernel applies while copying to the output.
 //
 // Blocking.  Columns are processed in outer blocks of kNBO and inner panels of
-// kNBI.  A panel kernel (one 512-thread workgroup per matrix, panel rows
-// held in registers) factors kNBI columns; a gather kernel then copies the pivot
-// rows' values in all other columns into Prow.  A rank-kNBI update brings the rest of the outer
-// block up to date; after the whole outer block a rank-kNBO update (the GEMM
-// kernel's Gauss-Jordan epilogue) is applied to every other column:
+// kNBI.  A panel kernel (one 512-thread workgroup per matrix, panel rows held in
+// registers) factors kNBI columns and copies its pivot rows' values in the other
+// columns of the outer block into Prow; a rank-kNBI update then brings the rest of
+// the outer block up to date.  After the whole outer block, a gather kernel copies
+// the block's pivot rows in every column outside it into Prow, and a rank-kNBO
+// update (the GEMM kernel's Gauss-Jordan epilogue) is applied to those columns:
 //     A[i][j] <- [p(i) not in block] A[i][j] + sum_m S[i][m] Prow[m][j].
+// Two launches per panel: under concurrent streams each dependent launch costs
+// tens of microseconds of dispatch latency, more than the panel's own work.
 #include "negf_device.h"
 
 namespace negf {
@@ -76,7 +79,7 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long x) {
 // reads for column jj.  512 threads (two waves per SIMD) hold R rows each.
 template <typename T, int R, int NT>
 __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, int ld,
-                                                             int c0, int nb, int k0, GjWork w,
+                                                             int c0, int nb, int k0, int kb, GjWork w,
                                                              const int* active) {
   typedef cx<T> V;
   constexpr int NW = NT / 64;
@@ -92,6 +95,9 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
   const int tid = threadIdx.x, lane = tid & 63;
   V v[R][kNBI];
   bool used[R];
+  int pp[kNBI];  // pivot row of each column (the same in every thread), -1 if none
+#pragma unroll
+  for (int j = 0; j < kNBI; ++j) pp[j] = -1;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = tid + r * NT;
@@ -129,6 +135,7 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
       for (int q = 1; q < NW; ++q) g = umax64(g, red_k[par][q]);
       if (g == 0) continue;  // no unused row left (the same in every thread)
       const int p = 4095 - (int)(g & 0xfffull);
+      pp[jj] = p;
       if (tid == 0) {
         if ((g & ~0xfffull) == 0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
         pivstep[p] = c0 + jj;
@@ -172,33 +179,52 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
         if (j < nb) A[(long long)(c0 + j) * ld + i] = v[r][j];
     }
   }
+  // The pivot rows' values in the other columns of the outer block [k0, k0 + kb)
+  // -> Prow rows c0 - k0 + m, for the rank-nb update of the block that follows.
+  // Those columns are up to date (the previous update has completed) and this
+  // kernel does not write them.
+  V* prow = (V*)w.prow + (long long)b * w.sProw;
+#pragma unroll
+  for (int m = 0; m < kNBI; ++m) {
+    const int p = pp[m];
+    if (m < nb && p >= 0) {
+      for (int jc = tid; jc < kb; jc += NT) {
+        const int j = k0 + jc;
+        if (j < c0 || j >= c0 + nb) prow[(long long)(c0 - k0 + m) * n + j] = A[(long long)j * ld + p];
+      }
+    }
+  }
 }
 
-// Gathers the pivot rows of panel [c0, c0 + nb) in every column outside the panel
-// into Prow (row c0 - k0 + m of the outer block's kNBO x n panel).  In a
+// Gathers the pivot rows of the outer block [k0, k0 + kb) in every column outside
+// the block into Prow rows 0..kb-1, for the rank-kb update of those columns.  The
+// columns outside the block are not modified while the block is factored, so one
+// gather after its last panel reads the same values as one per panel would.  In a
 // column-major matrix each pivot-row element is its own cache line, so the gather
-// is spread over many workgroups (one per 64 columns of each matrix) instead of
-// being done by the single panel workgroup of that matrix.
+// is spread over many small workgroups: (64 columns) x (8 pivot rows) x matrix.
 template <typename T>
-__global__ __launch_bounds__(64) void k_gj_gather(const cx<T>* S, long long sS, int n, int ld, int c0,
-                                                  int nb, int k0, GjWork w, const int* active) {
+__global__ __launch_bounds__(64) void k_gj_gather(const cx<T>* S, long long sS, int n, int ld, int k0,
+                                                  int kb, GjWork w, const int* active) {
   typedef cx<T> V;
-  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
   const V* A = S + (long long)b * sS;
   const int* pivrow = w.pivrow + (long long)b * w.sInt;
   V* prow = (V*)w.prow + (long long)b * w.sProw;
   const int j = blockIdx.x * 64 + threadIdx.x;
-  if (j >= n || (j >= c0 && j < c0 + nb)) return;
-  V t[kNBI];
+  const int m0 = blockIdx.y * 8;
+  if (j >= n || (j >= k0 && j < k0 + kb)) return;
+  V t[8];
 #pragma unroll
-  for (int m = 0; m < kNBI; ++m)
-    if (m < nb) {
-      const int p = pivrow[c0 + m];
-      t[m] = A[(long long)j * ld + (p < n ? p : 0)];
+  for (int u = 0; u < 8; ++u) {
+    t[u] = mk<T>(0, 0);
+    if (m0 + u < kb) {
+      const int p = pivrow[k0 + m0 + u];
+      t[u] = A[(long long)j * ld + (p < n ? p : 0)];
     }
+  }
 #pragma unroll
-  for (int m = 0; m < kNBI; ++m)
-    if (m < nb) prow[(long long)(c0 - k0 + m) * n + j] = t[m];
+  for (int u = 0; u < 8; ++u)
+    if (m0 + u < kb) prow[(long long)(m0 + u) * n + j] = t[u];
 }
 
 // out[a][b] = S[pivrow[a]][pivstep[b]]
@@ -239,7 +265,7 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
       const int nb = (k0 + kb - c0 < kNBI) ? k0 + kb - c0 : kNBI;
 #define NEGF_PANEL(RR)                                                                             \
   hipLaunchKernelGGL((k_gj_panel<T, RR, NT>), dim3(batch), dim3(NT), 0, s, S, sS, n, ld, c0, nb, k0, \
-                     w, active)
+                     kb, w, active)
       switch (R) {
         case 1: NEGF_PANEL(1); break;
         case 2: NEGF_PANEL(2); break;
@@ -249,8 +275,6 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
         default: NEGF_PANEL(6); break;
       }
 #undef NEGF_PANEL
-      hipLaunchKernelGGL((k_gj_gather<T>), dim3((n + 63) / 64, batch), dim3(64), 0, s, S, sS, n, ld, c0,
-                         nb, k0, w, active);
       if (kb > nb) {  // bring the rest of the outer block up to date (rank nb)
         GemmBatch g = {};
         g.A = S + (long long)c0 * ld; g.sA = sS; g.lda = ld;
@@ -266,6 +290,8 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
       }
     }
     if (n > kb) {  // rank-kb update of every column outside the outer block
+      hipLaunchKernelGGL((k_gj_gather<T>), dim3((n + 63) / 64, (kb + 7) / 8, batch), dim3(64), 0, s, S,
+                         sS, n, ld, k0, kb, w, active);
       GemmBatch g = {};
       g.A = S + (long long)k0 * ld; g.sA = sS; g.lda = ld;
       g.B = (const cx<T>*)w.prow; g.sB = w.sProw; g.ldb = n;
