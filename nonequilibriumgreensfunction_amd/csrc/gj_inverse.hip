@@ -16,8 +16,8 @@
 // Blocking.  Columns are processed in outer blocks of kNBO and inner panels of
 // kNBI.  A panel kernel (one 512-thread workgroup per matrix, panel rows held in
 // registers) factors kNBI columns and copies its pivot rows' values in the other
-// columns of the outer block into Prow; a rank-kNBI update then brings the rest of
-// the outer block up to date.  After the whole outer block, a gather kernel copies
+// columns of the outer block into Prow; a rank-kNBI update (vector kernel) then
+// brings the rest of the outer block up to date.  After the whole outer block, a gather kernel copies
 // the block's pivot rows in every column outside it into Prow, and a rank-kNBO
 // update (the GEMM kernel's Gauss-Jordan epilogue) is applied to those columns:
 //     A[i][j] <- [p(i) not in block] A[i][j] + sum_m S[i][m] Prow[m][j].
@@ -196,6 +196,47 @@ __global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, 
   }
 }
 
+// Rank-nb update of the outer block after panel [c0, c0 + nb):
+//     A[i][j] <- [p(i) not in panel] A[i][j] + sum_m A[i][c0 + m] Prow[c0 - k0 + m][j]
+// for every column j of [k0, k0 + kb) outside the panel.  It is memory-bound
+// (nb = 8), so it runs on the vector units: one thread per row, 16 columns per
+// workgroup, the panel row in registers and the 8 x 16 pivot-row block in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gj_update_block(cx<T>* S, long long sS, int n, int ld, int c0,
+                                                         int nb, int k0, int kb, GjWork w,
+                                                         const int* active) {
+  typedef cx<T> V;
+  constexpr int CW = 16;  // columns per workgroup
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  V* A = S + (long long)b * sS;
+  const int* pivstep = w.pivstep + (long long)b * w.sInt;
+  const V* prow = (const V*)w.prow + (long long)b * w.sProw;
+  const int jc0 = blockIdx.y * CW;
+  __shared__ V P[kNBI][CW];
+  if (threadIdx.x < kNBI * CW) {
+    const int m = threadIdx.x / CW, u = threadIdx.x % CW, jc = jc0 + u;
+    P[m][u] = (m < nb && jc < kb) ? prow[(long long)(c0 - k0 + m) * n + k0 + jc] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  V sv[kNBI];
+#pragma unroll
+  for (int m = 0; m < kNBI; ++m) sv[m] = m < nb ? A[(long long)(c0 + m) * ld + i] : mk<T>(0, 0);
+  const int ps = pivstep[i];
+  const bool keep = !(ps >= c0 && ps < c0 + nb);
+#pragma unroll 4
+  for (int u = 0; u < CW; ++u) {
+    const int jc = jc0 + u, j = k0 + jc;
+    if (jc >= kb) break;
+    if (j >= c0 && j < c0 + nb) continue;
+    V acc = keep ? A[(long long)j * ld + i] : mk<T>(0, 0);
+#pragma unroll
+    for (int m = 0; m < kNBI; ++m) acc = cadd(acc, cmul(sv[m], P[m][u]));
+    A[(long long)j * ld + i] = acc;
+  }
+}
+
 // Gathers the pivot rows of the outer block [k0, k0 + kb) in every column outside
 // the block into Prow rows 0..kb-1, for the rank-kb update of those columns.  The
 // columns outside the block are not modified while the block is factored, so one
@@ -275,19 +316,9 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
         default: NEGF_PANEL(6); break;
       }
 #undef NEGF_PANEL
-      if (kb > nb) {  // bring the rest of the outer block up to date (rank nb)
-        GemmBatch g = {};
-        g.A = S + (long long)c0 * ld; g.sA = sS; g.lda = ld;
-        g.B = (const cx<T>*)w.prow + (long long)(c0 - k0) * n + k0; g.sB = w.sProw; g.ldb = n;
-        g.C = S + (long long)k0 * ld; g.sC = sS; g.ldc = ld;
-        g.m = n; g.n = kb; g.k = nb;
-        g.active = active; g.pivstep = w.pivstep; g.sPiv = w.sInt;
-        g.pv_lo = c0; g.pv_hi = c0 + nb;
-        g.skip_lo = c0 - k0; g.skip_hi = c0 - k0 + nb;
-        g.m3 = w.m3;
-        hipError_t e = gemm_gj_update<T>(s, g, batch);
-        if (e != hipSuccess) return e;
-      }
+      if (kb > nb)  // bring the rest of the outer block up to date (rank nb)
+        hipLaunchKernelGGL((k_gj_update_block<T>), dim3((n + 255) / 256, (kb + 15) / 16, batch), dim3(256),
+                           0, s, S, sS, n, ld, c0, nb, k0, kb, w, active);
     }
     if (n > kb) {  // rank-kb update of every column outside the outer block
       hipLaunchKernelGGL((k_gj_gather<T>), dim3((n + 63) / 64, (kb + 7) / 8, batch), dim3(64), 0, s, S,
