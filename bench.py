@@ -255,7 +255,8 @@ def server_workload(name, args):
     prec = capi.NEGF_PREC_MIXED if args.precision == "mixed" else capi.NEGF_PREC_FP64
     npts = sum(len(e) for _, _, e in reqs)
     print(f"[bench] {name}: {len(reqs)} requests, {npts} energy points", file=sys.stderr, flush=True)
-    srv = capi.Server(eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX, precision=prec, flags=args.flags)
+    srv = capi.Server(eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX, precision=prec, flags=args.flags,
+                      lanes=args.lanes)
     ids = [None] * len(reqs)
     capi.profile_begin(args.profile_every)
     t0 = time.perf_counter()

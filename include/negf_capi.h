@@ -241,7 +241,8 @@ typedef struct {
   double eta, sr_tol;
   int sr_max_iter, sr_extra_eta, precision, device, flags;
   double hbm_budget_gb; /* device memory for resident Hamiltonians + lane pools; 0 = 85% of free */
-  double lane_pool_gb;  /* working pool per lane; 0 = a quarter of the budget */
+  double lane_pool_gb;  /* working pool per lane; 0 = half the budget split over the lanes */
+  int lanes;            /* worker lanes (HIP streams), 1..4; 0 = 4 */
 } negf_server_config;
 
 typedef struct {

@@ -66,7 +66,7 @@ class NegfServerConfig(ctypes.Structure):
     _fields_ = [("eta", ctypes.c_double), ("sr_tol", ctypes.c_double), ("sr_max_iter", ctypes.c_int),
                 ("sr_extra_eta", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
                 ("flags", ctypes.c_int), ("hbm_budget_gb", ctypes.c_double),
-                ("lane_pool_gb", ctypes.c_double)]
+                ("lane_pool_gb", ctypes.c_double), ("lanes", ctypes.c_int)]
 
 
 class NegfRequest(ctypes.Structure):
@@ -359,9 +359,10 @@ class Server:
     concurrently, each accumulated in energy order."""
 
     def __init__(self, eta=1e-6, sr_tol=1e-10, sr_max_iter=200, sr_extra_eta=True,
-                 precision=NEGF_PREC_FP64, device=0, flags=0, hbm_budget_gb=0.0, lane_pool_gb=0.0):
+                 precision=NEGF_PREC_FP64, device=0, flags=0, hbm_budget_gb=0.0, lane_pool_gb=0.0,
+                 lanes=0):
         cfg = NegfServerConfig(eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)), precision, device,
-                               flags, hbm_budget_gb, lane_pool_gb)
+                               flags, hbm_budget_gb, lane_pool_gb, lanes)
         h = ctypes.c_void_p()
         _check(lib().negf_server_create(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h

@@ -5,11 +5,11 @@
 //
 // Scheduling.  Requests wait in arrival order until their resident footprint
 // (Htotd and the coupling matrices, uploaded through a per-request negf_ctx) fits
-// the HBM budget next to the admitted ones.  Two worker lanes, each with its own
-// HIP stream and a fixed working pool, repeatedly take the next chunk of energies
-// from the oldest admitted request that still has unassigned energies; the pool
-// is carved for that request's layout, so one lane can run a block-2304 chunk
-// while the other runs a block-256 chunk of a later request.  A request finishes
+// the HBM budget next to the admitted ones.  Up to four worker lanes, each with its
+// own HIP stream and a fixed working pool, repeatedly take the next chunk of
+// energies from the oldest admitted request that still has unassigned energies; the
+// pool is carved for that request's layout, so one lane can run a block-2304 chunk
+// while another runs a block-256 chunk of a later request.  A request finishes
 // when all its energies are accumulated; its device state is released and the
 // next waiting requests are admitted.
 //
@@ -97,9 +97,10 @@ struct negf_server {
   std::map<long long, SrvRequest*> all;
   long long next_id = 1;
   size_t budget = 0, used = 0, pool_bytes = 0;
-  Lane lanes[2];
-  char* pools[2] = {nullptr, nullptr};
-  std::thread workers[2];
+  int nl = 4;  // worker lanes
+  Lane lanes[kMaxLanes];
+  char* pools[kMaxLanes] = {};
+  std::thread workers[kMaxLanes];
   bool stop = false;
 };
 
@@ -223,6 +224,10 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
     return NEGF_ERR_ARG;
   }
   *out = nullptr;
+  if (cfg->lanes < 0 || cfg->lanes > kMaxLanes) {
+    negf_set_error("server lanes must be 1..%d (0 = %d)", kMaxLanes, kMaxLanes);
+    return NEGF_ERR_ARG;
+  }
   int rc = require_device(cfg->device);
   if (rc) return rc;
   negf_server* s = new negf_server();
@@ -235,11 +240,12 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
   }
   double budget = cfg->hbm_budget_gb > 0 ? cfg->hbm_budget_gb * 1e9 : 0.85 * (double)freeb;
   budget = std::min(budget, 0.92 * (double)freeb);
-  double pool = cfg->lane_pool_gb > 0 ? cfg->lane_pool_gb * 1e9 : 0.25 * budget;
-  pool = std::min(pool, 0.45 * budget);
+  s->nl = cfg->lanes == 0 ? kMaxLanes : cfg->lanes;
+  double pool = cfg->lane_pool_gb > 0 ? cfg->lane_pool_gb * 1e9 : 0.5 * budget / s->nl;
+  pool = std::min(pool, 0.9 * budget / s->nl);
   s->pool_bytes = (size_t)pool;
-  s->budget = (size_t)(budget - 2 * pool);
-  for (int li = 0; li < 2; ++li) {
+  s->budget = (size_t)(budget - s->nl * pool);
+  for (int li = 0; li < s->nl; ++li) {
     if (hipMalloc(&s->pools[li], s->pool_bytes) != hipSuccess ||
         hipStreamCreateWithFlags(&s->lanes[li].st, hipStreamNonBlocking) != hipSuccess) {
       for (int k = 0; k <= li; ++k) {
@@ -251,7 +257,7 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
       return NEGF_ERR_MEMORY;
     }
   }
-  for (int li = 0; li < 2; ++li) s->workers[li] = std::thread(worker, s, li);
+  for (int li = 0; li < s->nl; ++li) s->workers[li] = std::thread(worker, s, li);
   *out = s;
   return 0;
 }
@@ -338,7 +344,7 @@ extern "C" int negf_server_destroy(negf_server* s) {
   }
   for (std::thread& t : s->workers)
     if (t.joinable()) t.join();
-  for (int li = 0; li < 2; ++li) {
+  for (int li = 0; li < s->nl; ++li) {
     hipStreamSynchronize(s->lanes[li].st);
     hipFree(s->pools[li]);
     hipStreamDestroy(s->lanes[li].st);

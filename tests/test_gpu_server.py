@@ -81,7 +81,7 @@ def test_server_queues_by_hbm_footprint():
     wires = [build_wire(20, 8, 8, disorder=0.5, seed=s) for s in (1, 2, 3)]
     energies = [-0.2, 0.1, 0.3]
     print("server: queued admissions", flush=True)
-    with capi.Server(hbm_budget_gb=0.36, lane_pool_gb=0.16) as srv:
+    with capi.Server(hbm_budget_gb=0.36, lane_pool_gb=0.16, lanes=2) as srv:
         subs = [srv.submit(w, energies, DE, MU, MUT) for w in wires]
         res = [srv.wait(rid) for rid, _ in subs]
     assert all(st == 0 for st, _ in res)
