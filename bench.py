@@ -150,6 +150,24 @@ def traffic_from_profiles(config, kind):
     return None
 
 
+def gemm_only(args):
+    """The dominant kernel alone: the 3M complex product (complex64 in the mixed form,
+    complex128 in the plain form) with the block size of the workload and the batch
+    one lane launches it with (the energies in flight per lane)."""
+    cfg = CONFIGS[args.config]
+    n = cfg["npy"] * cfg["npz"] * 4
+    per = args.per_step or cfg["per_step"]
+    lanes = args.lanes or 4
+    batch = max(1, per // lanes)
+    reps = 8
+    ms = capi.bench_gemm(n, batch, complex64=(args.precision == "mixed"), m3=True, reps=reps)
+    flops = 8.0 * n ** 3 * batch
+    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.precision == "mixed" else "double"),
+                      "n": n, "batch": batch, "launches": reps + 1, "ms_per_launch": ms,
+                      "algorithmic_flops_per_launch": flops, "tflops": flops / ms / 1e9,
+                      "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.precision == "mixed" else 16)}))
+
+
 def copy_calibration(mib):
     """A plain copy of known size: reads and writes mib MiB per launch."""
     import torch
@@ -322,12 +340,18 @@ def main():
     ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
     ap.add_argument("--profile-every", type=int, default=32,
                     help="time one MFMA GEMM launch in this many with HIP events")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="run only the dominant kernel (the 3M complex product) at the shape the "
+                         "sweep launches it with, for rocprofv3 --pmc counter runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 8 times (the known-size "
                          "reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.gemm_only:
+        gemm_only(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: this benchmark runs on exactly one GPU "

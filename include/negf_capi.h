@@ -285,6 +285,15 @@ int negf_server_destroy(negf_server* srv);
 int negf_profile_begin(int sample_every);
 int negf_profile_end(double* flops, double* ms, long long* launches);
 
+/* Instrumentation, no counterpart in the reference: the batched complex product
+ * kernel of the sweep alone, C_b = A_b B_b for `batch` n x n matrices (complex64 if
+ * complex64 != 0, else complex128; 3M form if m3 != 0), `reps` timed launches after
+ * one untimed one; the operands are a constant byte pattern.  For hardware-counter
+ * (rocprofv3 --pmc) runs of that kernel at the shape the sweep launches it with.
+ * *ms_per_launch gets the mean launch time. */
+int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
+                    double* ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

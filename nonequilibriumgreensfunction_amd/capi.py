@@ -35,7 +35,7 @@ EXPORTS = (
     "negf_zgemm", "negf_profile_begin", "negf_profile_end", "negf_energy_sweep_f", "negf_flush_f",
     "negf_observables_f", "negf_set_hamiltonian_compact", "negf_set_onsite", "negf_contact_cache_stats",
     "negf_server_create", "negf_server_submit", "negf_server_poll",
-    "negf_server_wait", "negf_server_destroy",
+    "negf_server_wait", "negf_server_destroy", "negf_bench_gemm",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -162,6 +162,15 @@ def profile_end():
     n = (ctypes.c_longlong * 8)()
     _check(lib().negf_profile_end(f, m, n))
     return {k: (f[i], m[i], n[i]) for i, k in enumerate(PROFILE_KINDS)}
+
+
+def bench_gemm(n, batch, complex64=True, m3=True, reps=8, device=0):
+    """Mean launch time (ms) of the batched complex product kernel alone at one shape
+    (negf_bench_gemm): the kernel that bounds the sweep, for counter runs."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().negf_bench_gemm(int(device), int(bool(complex64)), int(n), int(batch), int(reps),
+                                 int(bool(m3)), ctypes.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

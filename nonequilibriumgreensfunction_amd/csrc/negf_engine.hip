@@ -1529,6 +1529,61 @@ extern "C" int negf_profile_end(double* flops, double* ms, long long* launches) 
   return 0;
 }
 
+// The batched complex product kernel alone at one shape: C_b = A_b B_b for `batch`
+// n x n matrices (complex64 or complex128, 3M or 4-product form), `reps` launches
+// on a stream of its own after one untimed launch.  For hardware-counter runs of
+// the kernel that bounds the sweep; the operands are a constant byte pattern.
+extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
+                               double* ms_per_launch) {
+  if (n < 1 || n > 8192 || batch < 1 || reps < 1 || !ms_per_launch) {
+    negf_set_error("negf_bench_gemm: bad arguments");
+    return NEGF_ERR_ARG;
+  }
+  int rc = require_device(device);
+  if (rc) return rc;
+  const size_t bytes = (complex64 ? 8 : 16) * (size_t)n * n * batch;
+  void *A = nullptr, *B = nullptr, *C = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipSuccess;
+  auto step = [&](hipError_t r) { if (e == hipSuccess) e = r; return e == hipSuccess; };
+  if (step(hipMalloc(&A, bytes)) && step(hipMalloc(&B, bytes)) && step(hipMalloc(&C, bytes)) &&
+      step(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) && step(hipEventCreate(&e0)) &&
+      step(hipEventCreate(&e1)) && step(hipMemsetAsync(A, 0x3f, bytes, s)) &&
+      step(hipMemsetAsync(B, 0x3f, bytes, s))) {
+    GemmBatch g = {};
+    g.A = A; g.B = B; g.C = C;
+    g.sA = g.sB = g.sC = (long long)n * n;
+    g.lda = g.ldb = g.ldc = n;
+    g.m = g.n = g.k = n;
+    g.m3 = m3 ? 1 : 0;
+    auto launch = [&]() {
+      return complex64 ? gemm_batched<float>(s, OP_N, OP_N, g, batch, mk<float>(1, 0), mk<float>(0, 0))
+                       : gemm_batched<double>(s, OP_N, OP_N, g, batch, mk<double>(1, 0), mk<double>(0, 0));
+    };
+    if (step(launch()) && step(hipStreamSynchronize(s)) && step(hipEventRecord(e0, s))) {
+      for (int r = 0; r < reps && step(launch()); ++r) {
+      }
+      float ms = 0.f;
+      if (step(hipEventRecord(e1, s)) && step(hipEventSynchronize(e1)) &&
+          step(hipEventElapsedTime(&ms, e0, e1)))
+        *ms_per_launch = ms / reps;
+    }
+  }
+  if (s) (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(A);
+  (void)hipFree(B);
+  (void)hipFree(C);
+  if (e != hipSuccess) {
+    negf_set_error("negf_bench_gemm: %s", hipGetErrorString(e));
+    return NEGF_ERR_HIP;
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Stage-wise entry points on materialised module arrays.  These serve the
 // reference's per-subroutine interface (and parity checks at small sizes); the

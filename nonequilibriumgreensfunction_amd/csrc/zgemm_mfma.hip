@@ -64,12 +64,23 @@ __device__ inline cx<T> load_op(const cx<T>* p, bool ok) {
   return v;
 }
 
-// Maps a linear workgroup id to a tile so that the tiles of one XCD (ids
-// congruent mod 8 are dispatched to the same XCD) form a contiguous run of
-// column-major tiles: neighbouring tiles share A/B panels in that XCD's L2.
-__device__ inline int xcd_tile(int bid, int ntiles) {
-  int per = (ntiles + 7) >> 3;
-  return (bid & 7) * per + (bid >> 3);
+// Maps a linear workgroup id to a (column-major) tile index.  Workgroup ids
+// congruent mod 8 are dispatched to the same XCD, so each XCD gets a contiguous run
+// o of a grouped order: runs of G = 8 tile rows, column-major inside a run.  The
+// ~64 workgroups an XCD holds at once then cover about 8 x 8 tiles, which share 8
+// A row panels and 8 B column panels in that XCD's L2 (plain column-major runs
+// cover all tile rows and two columns: ~38 panels for the same 64 tiles).
+// Returns ntiles for ids past the last tile.
+__device__ inline int xcd_tile(int bid, int tiles_m, int ntiles) {
+  constexpr int G = 8;
+  const int per = (ntiles + 7) >> 3;
+  const int o = (bid & 7) * per + (bid >> 3);
+  if (o >= ntiles) return ntiles;
+  const int tiles_n = ntiles / tiles_m;
+  const int g = o / (G * tiles_n);
+  const int r = o - g * G * tiles_n;
+  const int rows = min(G, tiles_m - g * G);
+  return (r / rows) * tiles_m + g * G + r % rows;
 }
 
 // Workgroup tile (32 SM) x (32 SN): 2 x 2 waves, each owning SM x SN MFMA blocks of
@@ -89,7 +100,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
   constexpr int BM = 32 * SM, BN = 32 * SN;
   constexpr int LA = BM * kBK / kThreads, LB = BN * kBK / kThreads;
   const int b = g.active ? g.active[blockIdx.z] : (int)blockIdx.z;
-  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int tile = xcd_tile(blockIdx.x, tiles_m, ntiles);
   if (tile >= ntiles) return;
   const int m0 = (tile % tiles_m) * BM;
   const int n0 = (tile / tiles_m) * BN;
