@@ -136,12 +136,18 @@ def cpu_baseline(cfg, energy, threads, k_mean):
     return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port", sample=sample)
 
 
+# Committed rocprofv3 --pmc summaries (FETCH_SIZE and WRITE_SIZE passes of the
+# dominant kernel at the shape the bench launches it with, each scaled by the same
+# counter's reading for a copy kernel of known size), per (workload, kernel kind).
+TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md"}
+
+
 def traffic_from_profiles(config, kind):
-    """HBM bytes per launch of the dominant kernel, from a committed rocprofv3
-    --pmc summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy-kernel
-    calibration) for this workload and form, if one exists; else None."""
-    path = os.path.join(ROOT, "profiles", f"k_gemm_{kind}_traffic_{config}.md")
-    if not os.path.exists(path):
+    """HBM bytes per launch of the dominant kernel from its committed counter
+    summary, or None when no summary covers this workload and form."""
+    name = TRAFFIC_PROFILES.get((config, kind))
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
