@@ -52,6 +52,20 @@ def test_bad_config_rejected_before_device():
     assert capi.lib().negf_init(ctypes.byref(h), ctypes.byref(cfg)) == capi.NEGF_ERR_ARG
 
 
+def test_server_lanes_out_of_range_rejected_before_device():
+    for lanes in (-1, 5):
+        cfg = capi.NegfServerConfig(1e-6, 1e-10, 200, 1, 0, 0, 0, 0.0, 0.0, lanes)
+        h = ctypes.c_void_p()
+        assert capi.lib().negf_server_create(ctypes.byref(h), ctypes.byref(cfg)) == capi.NEGF_ERR_ARG
+        assert "lanes" in capi.lib().negf_last_error().decode()
+
+
+def test_bench_gemm_bad_shape_rejected_before_device():
+    ms = ctypes.c_double(0.0)
+    for n, batch, reps in ((0, 1, 1), (64, 0, 1), (64, 1, 0), (9000, 1, 1)):
+        assert capi.lib().negf_bench_gemm(0, 1, n, batch, reps, 1, ctypes.byref(ms)) == capi.NEGF_ERR_ARG
+
+
 def test_splitmix64_stream_is_pinned():
     # first outputs of SplitMix64 seeded with 1 (published reference values of the generator)
     u = splitmix64_uniform(1, 3)

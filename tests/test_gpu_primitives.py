@@ -73,3 +73,12 @@ def test_invert_reports_singular(rng, n):
     A[:, 33] = 0.0  # exactly singular: some pivot column is all zeros
     X, info = capi.invert_matrix(A)
     assert info > 0
+
+
+@pytest.mark.parametrize("complex64", [True, False])
+@pytest.mark.parametrize("m3", [True, False])
+def test_bench_gemm_instrumentation_runs(complex64, m3):
+    # negf_bench_gemm times the product kernel alone (counter runs); it must run
+    # and report a positive launch time in every form
+    ms = capi.bench_gemm(256, 3, complex64=complex64, m3=m3, reps=2)
+    assert ms > 0.0
