@@ -142,18 +142,23 @@ def cpu_baseline(cfg, energy, threads, k_mean):
 TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md"}
 
 
-def traffic_from_profiles(config, kind):
-    """HBM bytes per launch of the dominant kernel from its committed counter
-    summary, or None when no summary covers this workload and form."""
+def profile_value(config, kind, key):
+    """A `key: value` line of the dominant kernel's committed counter summary, or
+    None when no summary covers this workload and form."""
     name = TRAFFIC_PROFILES.get((config, kind))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def traffic_from_profiles(config, kind):
+    """HBM bytes per launch of the dominant kernel (FETCH_SIZE + WRITE_SIZE, scaled)."""
+    return profile_value(config, kind, "traffic_bytes_per_launch")
 
 
 def gemm_only(args):
@@ -454,9 +459,16 @@ def main():
                      "note": ("achieved = algorithmic 8 n^3 flops per complex product x batch / HIP-event "
                               "duration on the launching stream (one launch in --profile-every sampled); the "
                               "3M form executes 6 n^3 on the matrix cores; the lanes run concurrently, so "
-                              "durations include the other lanes' kernels sharing the CUs")},
+                              "durations include the other lanes' kernels sharing the CUs; traffic: HBM-side "
+                              "bytes per launch from FETCH_SIZE/WRITE_SIZE counter runs of the kernel alone "
+                              "(bench.py --gemm-only), scaled by a copy-kernel calibration")},
         "kernels": kernels,
     }
+    alone = profile_value(args.config, dominant, "alone_tflops")
+    if alone is not None:
+        # the same kernel and shape with the GPU to itself (bench.py --gemm-only)
+        result["roofline"]["alone"] = {"achieved": alone, "frac": alone / peak,
+                                       "source": "profiles/" + TRAFFIC_PROFILES[(args.config, dominant)]}
     if compare is not None:
         result["compare"] = compare
     if not args.no_cpu_baseline:
