@@ -123,7 +123,7 @@ def check_all(got, ref, noise, jx=False):
         check(k, got[k], ref[k], noise[k], nm)
     for k in ("DOSc1", "DOSc2", "EDensity", "HDensity", "RhoN", "RhoP", "ContactCurrent"):
         check(k, got[k], ref[k], noise[k])
-    # |Tr(Gn_site sigma)| <= Tr(Gn_site): spin is judged on the density's scale
+    # |Tr(Gn_site sigma)| <= Tr(Gn_site): spin is checked on the density's scale
     rho = float(np.abs(ref["RhoN"]).max())
     for k in ("SigX", "SigY", "SigZ"):
         check(k, got[k], ref[k], noise[k], rho)

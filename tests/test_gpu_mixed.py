@@ -55,7 +55,7 @@ def test_mixed_matches_oracle(case):
     assert np.all(st == 0), st
     assert np.array_equal(it, ref["iters"])
     sc = P.scales(ref)
-    # T12: the FP64 chain, judged like the plain form but at 1e-8
+    # T12: the FP64 chain, checked like the plain form but at 1e-8
     lim = np.maximum(T_RTOL * sc["T12"], P.FLOOR * noise["T12"])
     err = np.abs(got["T12"] - ref["T12"])
     print("T12 rel err", (err / sc["T12"]).max(), flush=True)

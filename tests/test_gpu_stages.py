@@ -73,7 +73,7 @@ def test_gnp_and_observables_match_oracle(case):
     for k, rk in (("SigXExp", "SigX"), ("SigYExp", "SigY"), ("SigZExp", "SigZ")):
         _close(k, ogot[k], oref[rk], scale=rho)
     # a bond current is a difference of H*Gn products (Calculate_Observables.F90:119-154)
-    # that cancels to ~0 in a gap: judge it on the scale of those products
+    # that cancels to ~0 in a gap: check it on the scale of those products
     q, hbar = 1.602e-19, 6.626e-34 / (2 * np.pi)
     terms = (q * q / hbar) * dE / (2 * np.pi) * w.norb**2 * max(
         np.abs(w.htotd).max(), np.abs(w.hopx).max()) * max(np.abs(ref["Gnd"]).max(), np.abs(ref["Gpd"]).max())

@@ -125,7 +125,7 @@ def test_contact_self_energy_matches_oracle():
         for k in ("SigmaL", "SigmaR"):
             P.check(k, got[k], ref[k], 0.0)
         # Gamma = i(Sigma - Sigma') cancels to O(eta) inside the gap: its error is
-        # judged on the scale of the Sigma it is formed from
+        # checked on the scale of the Sigma it is formed from
         for k, sk in (("GammaL", "SigmaL"), ("GammaR", "SigmaR")):
             P.check(k, got[k], ref[k], 0.0, float(np.abs(ref[sk]).max()))
         P.check("DOSc1", got["DOSc1"], ref["DOSc1"], 0.0)
