@@ -597,6 +597,19 @@ static GjWork gjw(zd* prow_base, long long sProw, int* ints, long long sInt, int
 // chains [0, nl) use betaA/betadA, [nl, nchains) use betaB/betadB).  Expects
 // chain alpha and work (= seed matrix) filled.  Writes Sigma for chain c to
 // sig0 + c*sSig (c < nl) or sig1 + (c-nl)*sSig.  iters/status per chain.
+// Sancho-Rubio products: the 4-product form in the plain FP64 form, whose
+// componentwise accuracy keeps the O(eta) anti-Hermitian part of Sigma in a gap to
+// 1e-10 relative; the 3M form in the mixed form, where the normwise error it adds
+// to Sigma (~1e-16 |Sigma|, ~1e-10 of an eta-scale DOSc) is far inside that form's
+// bounds.
+static int sr_m3(const negf_ctx* c) { return c->cfg.precision == NEGF_PREC_MIXED ? 1 : 0; }
+static GemmBatch gbs(const negf_ctx* c, const zd* A, long long sA, int lda, const zd* B, long long sB,
+                     int ldb, zd* C, long long sC, int ldc, int n, const int* active = nullptr) {
+  GemmBatch g = gb(A, sA, lda, B, sB, ldb, C, sC, ldc, n, active);
+  g.m3 = sr_m3(c);
+  return g;
+}
+
 static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const zd* betadA, const zd* betaB,
                   const zd* betadB, zd* sig0, zd* sig1, long long sSig, std::vector<int>& iters,
                   std::vector<int>& status) {
@@ -608,6 +621,7 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
   zd* C0 = ln.C;
   auto P = [&](long long off) { return C0 + off; };
   GjWork w = gjw(P(y.c_prow), Cs, ln.Ic, y.Is, n);
+  w.m3 = sr_m3(c);
   iters.assign(nc, 0);
   status.assign(nc, 0);
   // clear INFO of every chain slot
@@ -619,9 +633,9 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
     const int cnt = grp ? nr : nl;
     if (!cnt) continue;
     const long long off = grp ? (long long)nl * Cs : 0;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(off + y.c_tmp), Cs, n, grp ? betadB : betadA, 0, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(off + y.c_tmp), Cs, n, grp ? betadB : betadA, 0, n,
                                                P(off + y.c_t), Cs, n, n), cnt, kMone, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(off + y.c_tmp), Cs, n, grp ? betaB : betaA, 0, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(off + y.c_tmp), Cs, n, grp ? betaB : betaA, 0, n,
                                                P(off + y.c_tt), Cs, n, n), cnt, kMone, kZero));
   }
   TRY(launch_copy3<double>(s, P(y.c_CapT), P(y.c_t), nullptr, nullptr, nullptr, nullptr, Cs, nn, nc,
@@ -638,27 +652,27 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
     int* A = ln.d_active;
     TRY(hipMemcpyAsync(A, act.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
     // W1 = Toldt*tt   (:157)
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs,
                                                n, n, A), na, kOne, kZero));
     // work = I - t*tt - tt*t   (:160-162)
     TRY(launch_eye<double>(s, P(y.c_work), Cs, n, na, A));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n,
                                                n, A), na, kMone, kOne));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n,
                                                n, A), na, kMone, kOne));
     // tmp = inv(work)   (:163)
     TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, na, w, A));
     // t' = tmp*t*t, tt' = tmp*tt*tt   (:170-176)
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n,
                                                n, A), na, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n,
                                                n, A), na, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n,
                                                n, A), na, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n,
                                                n, A), na, kOne, kZero));
     // change = Toldt*t'   (:178);  CapT += change   (:180)
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
                                                n, A), na, kOne, kZero));
     TRY(launch_add<double>(s, P(y.c_CapT), Cs, P(y.c_CapT), Cs, P(y.c_W2), Cs, nn, na, A));
     TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, ln.d_mc, na, A));
@@ -685,7 +699,7 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
     if (!cnt) continue;
     const long long off = grp ? (long long)nl * Cs : 0;
     const zd* beta = grp ? betaB : betaA;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(off + y.c_CapT), Cs, n, P(off + y.c_W1),
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, beta, 0, n, P(off + y.c_CapT), Cs, n, P(off + y.c_W1),
                                                Cs, n, n), cnt, kOne, kZero));
   }
   TRY(launch_add<double>(s, P(y.c_work), Cs, P(y.c_W1), Cs, P(y.c_alpha), Cs, nn, nc, nullptr));
@@ -695,9 +709,9 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
     if (!cnt) continue;
     const long long off = grp ? (long long)nl * Cs : 0;
     const zd* beta = grp ? betaB : betaA;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(off + y.c_W2), Cs, n, P(off + y.c_W3),
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, beta, 0, n, P(off + y.c_W2), Cs, n, P(off + y.c_W3),
                                                Cs, n, n), cnt, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_C, gb(P(off + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
+    TRY(gemm_batched<double>(s, OP_N, OP_C, gbs(c, P(off + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
                                                sSig, n, n), cnt, kOne, kZero));
   }
   // LAPACK INFO of the chain inverses
@@ -732,14 +746,15 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   zd* C0 = ln.C;
   auto P = [&](long long off) { return C0 + off; };
   GjWork w = gjw(P(y.c_prow), Cs, ln.Ic, y.Is, n);
+  w.m3 = sr_m3(c);
   iters.assign(2 * nb, 0);
   status.assign(2 * nb, 0);
   TRY(hipMemsetAsync(ln.Ic, 0, sizeof(int) * (size_t)y.Is * 2 * nb, s));
   // tmp = inv(seed); t = -tmp*betad_L = tmp X, tt = -tmp*beta_L = tmp X'  (:141-149)
   TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nb, w, nullptr));
-  TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, c->betaR, 0, n, P(y.c_t), Cs, n, n), nb,
+  TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, c->betaR, 0, n, P(y.c_t), Cs, n, n), nb,
                            kMone, kZero));
-  TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, c->betaL, 0, n, P(y.c_tt), Cs, n, n),
+  TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, c->betaL, 0, n, P(y.c_tt), Cs, n, n),
                            nb, kMone, kZero));
   // CapT_L = t_L = t, CapT_R = t_R = tt; Toldt = I for both
   TRY(launch_copy3<double>(s, P(y.c_CapT), P(y.c_t), P(R + y.c_CapT), P(y.c_tt), nullptr, nullptr, Cs,
@@ -765,30 +780,30 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
     TRY(hipMemcpyAsync(ln.d_active, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice, s));
     // W1_L = Toldt_L*tt, W1_R = Toldt_R*t   (:157)
     if (nL)
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs, n,
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs, n,
                                                  n, AL), nL, kOne, kZero));
     if (nR)
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(R + y.c_Toldt), Cs, n, P(y.c_t), Cs, n,
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + y.c_Toldt), Cs, n, P(y.c_t), Cs, n,
                                                  P(R + y.c_W1), Cs, n, n, AR), nR, kOne, kZero));
     // work = I - t*tt - tt*t; tmp = inv(work)   (:160-163)
     TRY(launch_eye<double>(s, P(y.c_work), Cs, n, nU, AU));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n, n,
                                                AU), nU, kMone, kOne));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n, n,
                                                AU), nU, kMone, kOne));
     TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nU, w, AU));
     // t' = tmp*t*t -> W3, tt' = tmp*tt*tt -> W4   (:170-176)
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n, n,
                                                AU), nU, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n, n,
                                                AU), nU, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n, n,
                                                AU), nU, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n, n,
                                                AU), nU, kOne, kZero));
     // change_L = Toldt_L*t', change_R = Toldt_R*tt'; CapT += change   (:178-180)
     if (nL) {
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
                                                  n, AL), nL, kOne, kZero));
       TRY(launch_add<double>(s, P(y.c_CapT), Cs, P(y.c_CapT), Cs, P(y.c_W2), Cs, nn, nL, AL));
       TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, ln.d_mc, nL, AL));
@@ -796,7 +811,7 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
                                nL, AL));
     }
     if (nR) {
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gb(P(R + y.c_W1), Cs, n, P(y.c_W4), Cs, n, P(R + y.c_W2),
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + y.c_W1), Cs, n, P(y.c_W4), Cs, n, P(R + y.c_W2),
                                                  Cs, n, n, AR), nR, kOne, kZero));
       TRY(launch_add<double>(s, P(R + y.c_CapT), Cs, P(R + y.c_CapT), Cs, P(R + y.c_W2), Cs, nn, nR, AR));
       TRY(launch_maxabs<double>(s, P(R + y.c_W2), Cs, nn, ln.d_mc + nb, nR, AR));
@@ -828,16 +843,16 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   // g = inv(alpha + beta*CapT), Sigma = beta*g*beta' for each lead   (:195-209)
   for (int grp = 0; grp < 2; ++grp) {
     const zd* beta = grp ? c->betaR : c->betaL;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(grp * R + y.c_CapT), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, beta, 0, n, P(grp * R + y.c_CapT), Cs, n,
                                                P(grp * R + y.c_W1), Cs, n, n), nb, kOne, kZero));
   }
   TRY(launch_add<double>(s, P(y.c_work), Cs, P(y.c_W1), Cs, P(y.c_alpha), Cs, nn, 2 * nb, nullptr));
   TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_W2), Cs, n, n, 2 * nb, w, nullptr));
   for (int grp = 0; grp < 2; ++grp) {
     const zd* beta = grp ? c->betaR : c->betaL;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gb(beta, 0, n, P(grp * R + y.c_W2), Cs, n,
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, beta, 0, n, P(grp * R + y.c_W2), Cs, n,
                                                P(grp * R + y.c_W3), Cs, n, n), nb, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_C, gb(P(grp * R + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
+    TRY(gemm_batched<double>(s, OP_N, OP_C, gbs(c, P(grp * R + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
                                                sSig, n, n), nb, kOne, kZero));
   }
   std::vector<int> info(2 * nb);
