@@ -100,27 +100,32 @@ __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, c
   Y[(long long)b * sY + (long long)col * n + a] = v;
 }
 
-// Site-diagonal blocks D_s = (B G')[site s] for 64 rows per workgroup (4 waves
-// split the k range; partial sums are combined in LDS in a fixed order).
+// Site-diagonal blocks D_s = (B G')[site s].
 // Writes, per site s of slice x:  out[0] = Re tr D,  out[1+a] = Re tr(D sigma_a).
 // out layout: out_b + q*qstride + s  (q = 0..3).
+// A workgroup takes 16 rows (16 / NORB sites) and splits the sum over k into 16
+// chunks, one per 16-thread row group: with 16 rows per workgroup there are n / 16
+// workgroups per matrix (144 at n = 2304), enough to spread the stream of B and G
+// over the whole GPU.  The 16 chunk sums are added in chunk order by one thread per
+// site, so the result does not depend on the batch.
 template <typename T, int NORB>
 __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long sB,
                                                     const cx<T>* G, long long sG, int n,
                                                     double* out, long long sOut, long long qstride,
                                                     SpinMats sp, const int* active) {
+  constexpr int RW = 16, KQ = 16;
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
   const cx<T>* Bb = Bm + (long long)b * sB;
   const cx<T>* Gb = G + (long long)b * sG;
-  const int row0 = blockIdx.x * 64;
-  const int tr = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * RW;
+  const int tr = threadIdx.x % RW, kq = threadIdx.x / RW;
   const int a = row0 + tr;
   const int a0 = (a / NORB) * NORB;
   double accr[NORB], acci[NORB];
 #pragma unroll
   for (int q = 0; q < NORB; ++q) accr[q] = acci[q] = 0.0;
   if (a < n) {
-    const int kbeg = (n * kq) / 4, kend = (n * (kq + 1)) / 4;
+    const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
     for (int k = kbeg; k < kend; ++k) {
       const cx<T> bv = Bb[(long long)k * n + a];
 #pragma unroll
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
       }
     }
   }
-  __shared__ double sr[4][64][NORB], si[4][64][NORB];
+  __shared__ double sr[KQ][RW][NORB], si[KQ][RW][NORB];
 #pragma unroll
   for (int q = 0; q < NORB; ++q) {
     sr[kq][tr][q] = accr[q];
@@ -139,7 +144,7 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
   }
   __syncthreads();
   // one thread per site of this row group
-  const int nsite_grp = 64 / NORB;
+  constexpr int nsite_grp = RW / NORB;
   if (threadIdx.x < nsite_grp) {
     const int sl = threadIdx.x;
     const int s = (row0 / NORB) + sl;
@@ -150,8 +155,13 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 #pragma unroll
         for (int q = 0; q < NORB; ++q) {
           const int t = sl * NORB + r;
-          dr[r][q] = ((sr[0][t][q] + sr[1][t][q]) + sr[2][t][q]) + sr[3][t][q];
-          di[r][q] = ((si[0][t][q] + si[1][t][q]) + si[2][t][q]) + si[3][t][q];
+          double xr = 0.0, xi = 0.0;
+          for (int c = 0; c < KQ; ++c) {
+            xr += sr[c][t][q];
+            xi += si[c][t][q];
+          }
+          dr[r][q] = xr;
+          di[r][q] = xi;
         }
       double* o = out + (long long)b * sOut + s;
       double trace = 0.0;
@@ -572,7 +582,7 @@ hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long
                              const cx<T>* G, long long sG, int n, double* out, long long sOut,
                              long long qstride, const SpinMats& sp, int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
-  dim3 grid((n + 63) / 64, 1, batch);
+  dim3 grid((n + 15) / 16, 1, batch);
 #define NEGF_SB(NO)                                                                            \
   if (norb == NO) {                                                                            \
     hipLaunchKernelGGL((k_siteblocks<T, NO>), grid, dim3(256), 0, s, Bm, sB, G, sG, n, out, sOut, \
