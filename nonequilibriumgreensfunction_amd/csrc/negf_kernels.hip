@@ -396,7 +396,7 @@ __global__ __launch_bounds__(1024) void k_trace_prod(const cx<T>* P, const cx<T>
 // sH: element stride between the slices of H (n*n for Htotd, 0 when every slice's
 // off-diagonal part is one template -- only off-diagonal elements of H are read)
 template <typename T, int NORB>
-__global__ void k_site_obs(const cx<T>* Gnd, const cx<T>* Gpd, const cx<T>* Gnu, const cx<T>* Gnl,
+__global__ __launch_bounds__(64) void k_site_obs(const cx<T>* Gnd, const cx<T>* Gpd, const cx<T>* Gnu, const cx<T>* Gnl,
                            const cx<T>* Gpu, const cx<T>* Gpl, const cx<T>* H, long long sH,
                            const cx<T>* X, int n, int L, int npy, int npz, SpinMats sp, double* out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
