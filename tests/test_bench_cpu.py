@@ -1,0 +1,40 @@
+"""CPU checks of bench.py's contract pieces that need no GPU: the single-GPU
+refusal for --gpus N > 1, and the roofline traffic read from the committed
+counter summary."""
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_multi_gpu_is_refused_with_a_message():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300, cwd=ROOT)
+    assert p.returncode == 2
+    msg = json.loads(p.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
+
+
+def test_traffic_summary_parses_and_exceeds_algorithmic_bytes():
+    b = _bench()
+    for (config, kind), name in b.TRAFFIC_PROFILES.items():
+        assert os.path.exists(os.path.join(ROOT, "profiles", name)), name
+        t = b.traffic_from_profiles(config, kind)
+        cfg = b.CONFIGS[config]
+        n = cfg["npy"] * cfg["npz"] * 4
+        # two energies per lane: A and B read once, C written once (complex64)
+        alg = 3 * 2 * n * n * 8
+        assert t is not None and t >= alg, (t, alg)
+        alone = b.profile_value(config, kind, "alone_tflops")
+        assert alone is not None and 0 < alone < 157.3
+    assert b.traffic_from_profiles("c2", "fp64_gemm_3m") is None
