@@ -16,8 +16,15 @@ pinned to the complex128 CPU oracle by the test suite).
 
 Prints one JSON line.  `roofline` is for the dominant kernel, timed with HIP
 events on the stream it runs on (a uniform sample of its launches inside the
-timed region); `cpu_baseline` times the complex128 CPU oracle (MKL) on the host
-on a bounded sample of the same wire.
+timed region); its `traffic` (and the kernel's stand-alone rate, `alone`) come
+from the committed rocprofv3 counter summary of that kernel when one covers the
+workload.  `cpu_baseline` times the complex128 CPU oracle (MKL) on the host on a
+bounded sample of the same wire.
+
+Counter runs: `--gemm-only` runs only the dominant kernel at the shape the sweep
+launches it with; `--copy-calibration MIB` runs a plain copy of known size, whose
+counter readings scale the kernel's.  `--config c3` / `--config c5` run the
+batched and serving workloads through the server interface.
 """
 from __future__ import annotations
 
