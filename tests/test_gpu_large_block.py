@@ -1,0 +1,71 @@
+"""GPU parity of the fused per-energy solve at a block above 1024 against the
+complex128 CPU oracle, in both precision forms.
+
+n = 16 x 17 sites x 4 orbitals = 1088: the inverse's panels hold three rows per
+thread, the products run over 17 x 17 tiles in several 8-row groups per XCD, and
+the cross-section is not square.  Tolerances as in test_gpu_sweep.py (plain FP64
+form: 1e-10 of each output's natural scale or 10x the oracle's own spread under a
+different summation order) and test_gpu_mixed.py (mixed form: T12 to 1e-8, the
+complex64-path outputs to 2e-5)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("nonequilibriumgreensfunction_amd.capi")
+from nonequilibriumgreensfunction_amd.hamiltonian import build_wire  # noqa: E402
+
+import parity as P  # noqa: E402
+
+MU, MUT = (-0.1, 0.0), 0.0
+DE = 0.01
+T_RTOL = 1e-8
+C64_RTOL = 2e-5
+
+
+@pytest.fixture(scope="module")
+def case():
+    w = build_wire(3, 16, 17, disorder=0.4, seed=5, flux=0.3)
+    assert w.n == 1088
+    energies = [-0.2]
+    print("large block: oracle at n=1088 (three sweeps for the noise floor)", flush=True)
+    ref, noise = P.reference(w, energies, DE, mu=MU, muT=MUT)
+    return w, energies, ref, noise
+
+
+def _run(w, energies, precision):
+    acc = capi.Accumulators(w.npx, w.npy, w.npz, len(energies))
+    with capi.Context(w.npx, w.npy, w.npz, w.norb, eta=1e-6, sr_tol=1e-10, sr_max_iter=200,
+                      precision=precision) as ctx:
+        ctx.set_hamiltonian(w)
+        ctx.submit(energies, np.arange(len(energies)), np.full(len(energies), DE))
+        st, it = ctx.flush(MU, MUT, acc)
+    d = {k: np.array(getattr(acc, k)) for k in P.PER_E + ("ContactCurrent",)}
+    for k, a in (("RhoN", "RhoN"), ("RhoP", "RhoP"), ("SigX", "SigXExp"), ("SigY", "SigYExp"),
+                 ("SigZ", "SigZExp")):
+        d[k] = np.asarray(getattr(acc, a)).ravel(order="F")
+    return d, np.asarray(st), np.asarray(it)
+
+
+def test_plain_fp64_large_block(case):
+    w, energies, ref, noise = case
+    print("large block: plain FP64 form", flush=True)
+    got, st, it = _run(w, energies, capi.NEGF_PREC_FP64)
+    assert np.all(st == 0), st
+    assert np.array_equal(it, ref["iters"]), (it, ref["iters"])
+    P.check_all(got, ref, noise)
+
+
+def test_mixed_large_block(case):
+    w, energies, ref, noise = case
+    print("large block: mixed form", flush=True)
+    got, st, it = _run(w, energies, capi.NEGF_PREC_MIXED)
+    assert np.all(st == 0), st
+    assert np.array_equal(it, ref["iters"]), (it, ref["iters"])
+    sc = P.scales(ref)
+    lim = np.maximum(T_RTOL * sc["T12"], P.FLOOR * noise["T12"])
+    assert np.all(np.abs(got["T12"] - ref["T12"]) <= lim)
+    for k in ("Modec1", "Modec2", "EDensity", "HDensity", "RhoN", "RhoP", "SigX", "SigY", "SigZ",
+              "ContactCurrent"):
+        err = float(np.abs(np.asarray(got[k]) - np.asarray(ref[k])).max()) / sc[k]
+        assert err <= C64_RTOL, (k, err)
