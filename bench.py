@@ -80,9 +80,11 @@ OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", 
 
 
 def f_useful(n, L, k_left, k_right):
-    """Irreducible dense flops of one energy point: 2L inverses, 2(L-1) column
-    GEMMs, 2L G*Gamma GEMMs, 1 for T12 and (2 + 9K) per lead for Sancho-Rubio,
-    each n^3 complex multiply-adds = 8 n^3 flops."""
+    """A fixed yardstick of dense flops per energy point, the three-pass RGF's
+    count: 2L inverses, 2(L-1) column GEMMs, 2L full G*Gamma GEMMs, 1 for T12 and
+    (2 + 9K) per lead for Sancho-Rubio, each n^3 complex multiply-adds = 8 n^3
+    flops.  The solve does less than this (L inverses, half products with the
+    triangular half of Gamma), so the rate it gives rises with such savings."""
     return 8.0 * n**3 * (6 * L + 3 + 9 * (k_left + k_right))
 
 

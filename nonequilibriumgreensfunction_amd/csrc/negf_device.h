@@ -69,6 +69,8 @@ struct GemmBatch {
   int skip_lo, skip_hi;
   int m3;  // 1: 3-product complex MFMA form (normwise-accurate, 3/4 of the work);
            // 0: 4-product form (each of Re, Im accurate relative to its own terms)
+  int tri_b;  // 1: op(B) is upper triangular (zero below the diagonal, EPI_STD only):
+              // a column tile starting at j0 sums K only over [0, j0 + tile width)
 };
 
 // Spin matrices gMx, gMy, gMz (norb <= 4) as kernel arguments.

@@ -71,6 +71,9 @@ hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long l
 template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
                           const int*);
+// U = 2 triu(Gam, 1) + diag(Gam) for `batch` strided n x n matrices
+template <typename Td, typename Ts>
+hipError_t launch_tri_half(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, int, int);
 // complex128 -> complex64 copies of nn elements for `batch` strided matrices
 static inline hipError_t to_c64(hipStream_t s, cx<float>* Y, long long sY, const cx<double>* X,
                                 long long sX, long long nn, int batch) {
@@ -108,10 +111,11 @@ static const double kH = 6.626e-34;   // Constants.F90:6
 struct Layout {
   long long nn;
   // per-energy region (complex elements)
+  // o_UL, o_UR: the upper-triangular halves 2 triu(Gam, 1) + diag(Gam) of GamL, GamR
   long long Es, o_slots, o_z, o_M, o_colA, o_colB, o_Y, o_Bm, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
-      o_prow;
+      o_UL, o_UR, o_prow;
   // mixed-precision form: complex64 region (offsets in complex64 elements from o_f32)
-  long long o_f32, f_z, f_g, f_G, f_Y, f_Bm, f_P, f_GamL, f_GamR;
+  long long o_f32, f_z, f_g, f_G, f_Y, f_Bm, f_P, f_GamL, f_GamR, f_UL, f_UR;
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
@@ -126,8 +130,8 @@ struct Layout {
     o_z = o; if (zslots && !mixed) o += (long long)L * nn;
     o_f32 = o;
     f_z = 0; f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_Bm = f_Y + nn;
-    f_P = f_Bm + nn; f_GamL = f_P + nn; f_GamR = f_GamL + nn;
-    if (mixed) o += (f_GamR + nn + 1) / 2;  // two complex64 per complex128 slot
+    f_P = f_Bm + nn; f_GamL = f_P + nn; f_GamR = f_GamL + nn; f_UL = f_GamR + nn; f_UR = f_UL + nn;
+    if (mixed) o += (f_UR + nn + 1) / 2;  // two complex64 per complex128 slot
     o_M = o; o += nn;
     o_colA = o; o += nn;
     o_colB = o; o += nn;
@@ -138,6 +142,8 @@ struct Layout {
     o_SigR = o; o += nn;
     o_GamL = o; o += nn;
     o_GamR = o; o += nn;
+    o_UL = o; if (!mixed) o += nn;  // the mixed form keeps them in complex64
+    o_UR = o; if (!mixed) o += nn;
     o_prow = o; o += (long long)kGjNBO * n;
     Es = (o + 31) & ~31LL;
     o = 0;
@@ -224,6 +230,7 @@ struct negf_ctx {
   zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
   cx<float>* X32 = nullptr;  // HopX in complex64 (mixed-precision form)
   SpinMats sp;
+  bool sp_herm = true;  // gMx, gMy, gMz Hermitian: site blocks may use the half of Gamma
   Layout lay;
   Lane lanes[kMaxLanes];
   int nlanes = kMaxLanes;
@@ -376,6 +383,7 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
   for (long long k = 0; k < nn; ++k) b[k] = mk<double>(-hx[k].x, -hx[k].y);
   HIPCHK(hipMemcpy(c->betaR, b.data(), sizeof(zd) * nn, hipMemcpyHostToDevice));
   std::memset(&c->sp, 0, sizeof(c->sp));
+  c->sp_herm = true;
   if (gM) {
     const zd* g = (const zd*)gM;
     for (int m = 0; m < 3; ++m)
@@ -385,6 +393,12 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
           c->sp.re[m][row][col] = v.x;
           c->sp.im[m][row][col] = v.y;
         }
+    for (int m = 0; m < 3; ++m)
+      for (int col = 0; col < no; ++col)
+        for (int row = 0; row < no; ++row)
+          if (c->sp.re[m][row][col] != c->sp.re[m][col][row] ||
+              c->sp.im[m][row][col] != -c->sp.im[m][col][row])
+            c->sp_herm = false;
   }
   if (!bd && !c->tmpP) HIPCHK(hipMalloc(&c->tmpP, sizeof(zd) * nn * 2));
   if (c->cfg.precision == NEGF_PREC_MIXED) {
@@ -999,6 +1013,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   }
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
+  if (c->cfg.precision != NEGF_PREC_MIXED) {
+    TRY((launch_tri_half<double, double>(s, Q(y.o_UL), Es, Q(y.o_GamL), Es, n, nb)));
+    TRY((launch_tri_half<double, double>(s, Q(y.o_UR), Es, Q(y.o_GamR), Es, n, nb)));
+  }
 
   GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
   w.m3 = 1;
@@ -1040,17 +1058,20 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     }
     return 0;
   };
-  // Gn/Gp site blocks of column block G (lead 0 = left/Gamma_L, 1 = right/Gamma_R) at slice x
+  // Gn/Gp site blocks of column block G (lead 0 = left/Gamma_L, 1 = right/Gamma_R) at
+  // slice x, from Bm = G U with U the upper-triangular half of Gamma (same real traces)
   auto siteblocks = [&](int lead, int x, const zd* G) -> int {
-    const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(G, Es, n, Gam, Es, n, Q(y.o_Bm), Es, n, n), nb, kOne,
-                             kZero));
+    const bool half = c->sp_herm;  // the spin traces need Hermitian spin matrices
+    const zd* Gu = half ? Q(lead ? y.o_UR : y.o_UL) : Q(lead ? y.o_GamR : y.o_GamL);
+    GemmBatch gu = gbr(G, Es, n, Gu, Es, n, Q(y.o_Bm), Es, n, n);
+    gu.tri_b = half;
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gu, nb, kOne, kZero));
     double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
     TRY(launch_siteblocks<double>(s, no, Q(y.o_Bm), Es, G, Es, n, out, y.site_per_e,
                                   (long long)L * ns, c->sp, nb, nullptr));
     return 0;
   };
-  // scal[idx] = Re <Gam_lead * Bm, G>
+  // scal[idx] = Re <Gam_lead * Bm, G>  (= Re tr(Gam_lead G Gam' G') for Bm = G U')
   auto trace_term = [&](int lead, const zd* G, int idx) -> int {
     const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
     TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(Gam, Es, n, Q(y.o_Bm), Es, n, Q(y.o_P), Es, n, n), nb,
@@ -1082,10 +1103,15 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     };
     TRY(to_c64(s, F(y.f_GamL), Es32, Q(y.o_GamL), Es, nn, nb));
     TRY(to_c64(s, F(y.f_GamR), Es32, Q(y.o_GamR), Es, nn, nb));
-    // Bm32 = G32 Gam_lead and the site blocks of Bm32 G32' for one lead at slice x
+    TRY((launch_tri_half<float, double>(s, F(y.f_UL), Es32, Q(y.o_GamL), Es, n, nb)));
+    TRY((launch_tri_half<float, double>(s, F(y.f_UR), Es32, Q(y.o_GamR), Es, n, nb)));
+    // Bm32 = G32 U_lead (U the upper-triangular half of Gamma) and the site blocks of
+    // Bm32 G32' for one lead at slice x
     auto site32 = [&](int lead, int x, const zf* G32) -> int {
-      TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(G32, F(lead ? y.f_GamR : y.f_GamL), F(y.f_Bm)), nb,
-                              kOnef, kZerof));
+      const bool half = c->sp_herm;
+      GemmBatch gu = gbf(G32, half ? F(lead ? y.f_UR : y.f_UL) : F(lead ? y.f_GamR : y.f_GamL), F(y.f_Bm));
+      gu.tri_b = half;
+      TRY(gemm_batched<float>(s, OP_N, OP_N, gu, nb, kOnef, kZerof));
       double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
       TRY(launch_siteblocks<float>(s, no, F(y.f_Bm), Es32, G32, Es32, n, out, y.site_per_e,
                                    (long long)L * ns, c->sp, nb, nullptr));

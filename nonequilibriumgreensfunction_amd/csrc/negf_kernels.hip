@@ -328,6 +328,23 @@ __global__ void k_gamma(const cx<T>* Sig, long long sS, cx<T>* Gam, long long sG
   Gam[(long long)c * sG + (long long)col * n + r] = mk<T>(-di, dr);
 }
 
+// U = 2 triu(Gam, 1) + diag(Gam): upper triangular, with U + U' = 2 Gam for a
+// Hermitian Gam.  For any G, G U G' = G Gam G' + A with A = G (U - Gam) G'
+// anti-Hermitian, so on every diagonal block D the real traces Re tr D and
+// Re tr(D sigma) (sigma Hermitian) of G U G' equal those of G Gam G', and
+// Re tr(Gam2 G U G') = Re tr(Gam2 G Gam G') for a Hermitian Gam2: the site blocks
+// and the trace terms take B = G U, a product with half the work of G Gam.
+template <typename Td, typename Ts>
+__global__ void k_tri_half(cx<Td>* U, long long sU, const cx<Ts>* Gam, long long sG, int n) {
+  const int b = blockIdx.z;
+  const int col = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const cx<Ts> v = Gam[(long long)b * sG + (long long)col * n + r];
+  const Ts f = r < col ? (Ts)2 : (r == col ? (Ts)1 : (Ts)0);
+  U[(long long)b * sU + (long long)col * n + r] = mk<Td>((Td)(f * v.x), (Td)(f * v.y));
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_dos(const cx<T>* Sig, long long sS, int n, double* out,
                                              long long sOut) {
@@ -660,6 +677,15 @@ hipError_t launch_gamma_dos(hipStream_t s, const cx<T>* Sig, long long sS, cx<T>
 }
 
 template <typename Td, typename Ts>
+hipError_t launch_tri_half(hipStream_t s, cx<Td>* U, long long sU, const cx<Ts>* Gam, long long sG,
+                           int n, int batch) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_tri_half<Td, Ts>), dim3((n + 255) / 256, n, batch), dim3(256), 0, s, U, sU,
+                     Gam, sG, n);
+  return hipGetLastError();
+}
+
+template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX,
                           long long nn, int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
@@ -695,6 +721,10 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
 NEGF_INST(double)
 NEGF_INST(float)
 #undef NEGF_INST
+template hipError_t launch_tri_half<double, double>(hipStream_t, cx<double>*, long long,
+                                                    const cx<double>*, long long, int, int);
+template hipError_t launch_tri_half<float, double>(hipStream_t, cx<float>*, long long,
+                                                   const cx<double>*, long long, int, int);
 template hipError_t launch_convert<float, double>(hipStream_t, cx<float>*, long long,
                                                   const cx<double>*, long long, long long, int,
                                                   const int*);

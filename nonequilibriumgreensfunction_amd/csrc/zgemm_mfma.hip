@@ -70,8 +70,11 @@ __device__ inline cx<T> load_op(const cx<T>* p, bool ok) {
 // ~64 workgroups an XCD holds at once then cover about 8 x 8 tiles, which share 8
 // A row panels and 8 B column panels in that XCD's L2 (plain column-major runs
 // cover all tile rows and two columns: ~38 panels for the same 64 tiles).
+// With fold (a triangular B, whose tile columns carry work growing with the
+// column index), the columns of a run alternate from both ends (last, first,
+// second to last, ...), so each XCD's contiguous share has the average load.
 // Returns ntiles for ids past the last tile.
-__device__ inline int xcd_tile(int bid, int tiles_m, int ntiles) {
+__device__ inline int xcd_tile(int bid, int tiles_m, int ntiles, bool fold) {
   constexpr int G = 8;
   const int per = (ntiles + 7) >> 3;
   const int o = (bid & 7) * per + (bid >> 3);
@@ -80,7 +83,9 @@ __device__ inline int xcd_tile(int bid, int tiles_m, int ntiles) {
   const int g = o / (G * tiles_n);
   const int r = o - g * G * tiles_n;
   const int rows = min(G, tiles_m - g * G);
-  return (r / rows) * tiles_m + g * G + r % rows;
+  int col = r / rows;
+  if (fold) col = (col & 1) ? (col >> 1) : tiles_n - 1 - (col >> 1);
+  return col * tiles_m + g * G + r % rows;
 }
 
 // Workgroup tile (32 SM) x (32 SN): 2 x 2 waves, each owning SM x SN MFMA blocks of
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
   constexpr int BM = 32 * SM, BN = 32 * SN;
   constexpr int LA = BM * kBK / kThreads, LB = BN * kBK / kThreads;
   const int b = g.active ? g.active[blockIdx.z] : (int)blockIdx.z;
-  const int tile = xcd_tile(blockIdx.x, tiles_m, ntiles);
+  const int tile = xcd_tile(blockIdx.x, tiles_m, ntiles, EPI == EPI_STD && g.tri_b);
   if (tile >= ntiles) return;
   const int m0 = (tile % tiles_m) * BM;
   const int n0 = (tile / tiles_m) * BN;
@@ -194,7 +199,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
     }
   };
 
-  const int nk = (K + kBK - 1) / kBK;
+  // an upper-triangular op(B) is zero in rows K >= n0 + BN of this tile's columns
+  const int Ke = (EPI == EPI_STD && g.tri_b) ? min(K, n0 + BN) : K;
+  const int nk = (Ke + kBK - 1) / kBK;
   if (nk > 0) {
     fetch(0);
     stash(0);
@@ -347,7 +354,14 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
   const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
-  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, kind, &ev0, &ev1);
+  // multiply-adds per row: k per column, or for an upper-triangular op(B) the
+  // sum over columns j of min(k, j + 1)
+  double kcols = (double)ncols * g.k;
+  if (EPI == EPI_STD && g.tri_b) {
+    const double kk = g.k, nc = g.n;
+    kcols = kk >= nc ? nc * (nc + 1) / 2 : kk * (kk + 1) / 2 + (nc - kk) * kk;
+  }
+  const bool timed = timer_slot(8.0 * g.m * kcols * batch, kind, &ev0, &ev1);
   if (timed) {
     const hipError_t er = hipEventRecord(ev0, s);
     if (er != hipSuccess) return er;

@@ -6,7 +6,12 @@ thread, the products run over 17 x 17 tiles in several 8-row groups per XCD, and
 the cross-section is not square.  Tolerances as in test_gpu_sweep.py (plain FP64
 form: 1e-10 of each output's natural scale or 10x the oracle's own spread under a
 different summation order) and test_gpu_mixed.py (mixed form: T12 to 1e-8, the
-complex64-path outputs to 2e-5)."""
+complex64-path outputs to 2e-5).  The oracle's outputs and noise floor for this
+wire are the committed fixture tests/golden/large_block_n1088.npz
+(tests/golden/make_large_block.py), checked against the wire by a digest."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -23,13 +28,21 @@ T_RTOL = 1e-8
 C64_RTOL = 2e-5
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLDEN)
+import make_large_block as MK  # noqa: E402
+
+
 @pytest.fixture(scope="module")
 def case():
     w = build_wire(3, 16, 17, disorder=0.4, seed=5, flux=0.3)
     assert w.n == 1088
-    energies = [-0.2]
-    print("large block: oracle at n=1088 (three sweeps for the noise floor)", flush=True)
-    ref, noise = P.reference(w, energies, DE, mu=MU, muT=MUT)
+    gold = np.load(os.path.join(GOLDEN, "large_block_n1088.npz"))
+    assert str(gold["wire_sha256"]) == MK.wire_digest(w), "fixture was made for another wire"
+    energies = [float(e) for e in gold["energies"]]
+    assert energies == MK.ENERGIES and MK.DE == DE and MK.MU == MU and MK.MUT == MUT
+    ref = {k[4:]: gold[k] for k in gold.files if k.startswith("ref_")}
+    noise = {k[6:]: gold[k] for k in gold.files if k.startswith("noise_")}
     return w, energies, ref, noise
 
 

@@ -129,3 +129,17 @@ def test_contact_self_energy_matches_oracle():
         for k, sk in (("GammaL", "SigmaL"), ("GammaR", "SigmaR")):
             P.check(k, got[k], ref[k], 0.0, float(np.abs(ref[sk]).max()))
         P.check("DOSc1", got["DOSc1"], ref["DOSc1"], 0.0)
+
+
+def test_sweep_non_hermitian_spin_matrices():
+    """Site blocks are formed from the upper-triangular half of Gamma only when the
+    spin matrices are Hermitian (then the real traces agree); other spin matrices
+    take the full Gamma products, still equal to the oracle's Re tr(Gn sigma)."""
+    w = build_wire(5, 2, 2, disorder=0.3, seed=4, zeeman=0.02)
+    w.gm = w.gm + 0.5j * np.roll(w.gm, 1, axis=0)  # Hermitian + i Hermitian
+    assert not np.allclose(w.gm[0], w.gm[0].conj().T)
+    energies = [-0.9, 0.1, 0.7]
+    ref, noise = P.reference(w, energies, 0.01, mu=MU, muT=MUT)
+    acc, st, it = _gpu_sweep(w, energies, 0.01, 1e-6, 1e-10, 200)
+    assert np.all(st == 0), st
+    _compare(acc, ref, noise, it)
