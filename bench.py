@@ -465,12 +465,14 @@ def main():
                      "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": gemm_tflops / peak, "traffic": traffic_from_profiles(args.config, dominant),
                      "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
-                     "note": ("achieved = algorithmic 8 n^3 flops per complex product x batch / HIP-event "
-                              "duration on the launching stream (one launch in --profile-every sampled); the "
-                              "3M form executes 6 n^3 on the matrix cores; the lanes run concurrently, so "
-                              "durations include the other lanes' kernels sharing the CUs; traffic: HBM-side "
-                              "bytes per launch from FETCH_SIZE/WRITE_SIZE counter runs of the kernel alone "
-                              "(bench.py --gemm-only), scaled by a copy-kernel calibration")},
+                     "note": ("achieved = algorithmic flops / HIP-event duration on the launching stream "
+                              "(one launch in --profile-every sampled): 8 n^3 per full complex product x "
+                              "batch, 8 n^2 (n+1)/2 per site-block product with the upper-triangular half "
+                              "of Gamma; the 3M form executes 3/4 of that on the matrix cores; the lanes "
+                              "run concurrently, so durations include the other lanes' kernels sharing the "
+                              "CUs; traffic: HBM-side bytes per launch of the full n x n x n product from "
+                              "FETCH_SIZE/WRITE_SIZE counter runs of the kernel alone (bench.py "
+                              "--gemm-only), scaled by a copy-kernel calibration")},
         "kernels": kernels,
     }
     alone = profile_value(args.config, dominant, "alone_tflops")

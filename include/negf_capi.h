@@ -109,7 +109,9 @@ int negf_finalize(negf_ctx* ctx);
 /* Upload Htotd (n,n,Npx), HopX (n,n) and the spin matrices gMx,gMy,gMz (three
  * column-major Norb x Norb blocks, Read_In_Allocate.F90:186-196).  Call after
  * Build_Hamiltonian / MPI_Broadcast (NEGF_Main.F90:15-19), i.e. once per
- * Poisson iteration. */
+ * Poisson iteration.  With Hermitian spin matrices (the reference's are) the
+ * density and spin products use the upper-triangular half of Gamma, half the
+ * work; other spin matrices take the full products, with the same results. */
 int negf_set_hamiltonian(negf_ctx* ctx, const double* Htotd, const double* HopX,
                          const double* gM);
 
