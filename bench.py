@@ -182,7 +182,7 @@ def gemm_only(args):
     reps = 8
     ms = capi.bench_gemm(n, batch, complex64=(args.precision == "mixed"), m3=True, reps=reps)
     flops = 8.0 * n ** 3 * batch
-    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.precision == "mixed" else "double"),
+    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1,0>" % ("float" if args.precision == "mixed" else "double"),
                       "n": n, "batch": batch, "launches": reps + 1, "ms_per_launch": ms,
                       "algorithmic_flops_per_launch": flops, "tflops": flops / ms / 1e9,
                       "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.precision == "mixed" else 16)}))
@@ -432,9 +432,9 @@ def main():
     kernels = {k: {"tflops": (f / (m * 1e-3) / 1e12 if m > 0 else 0.0), "launches": c,
                    "avg_launch_ms": m / max(1, c)}
                for k, (f, m, c) in prof.items() if c}
-    kname = ("k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form, n x n x n per matrix)"
+    kname = ("k_gemm<float,0,0,0,2,2,0,1,0> (complex64 MFMA product, 3M form, n x n x n per matrix)"
              if args.precision == "mixed" else
-             "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
+             "k_gemm<double,0,0,0,2,2,0,1,0> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
     result = {
         "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
         "value": value,

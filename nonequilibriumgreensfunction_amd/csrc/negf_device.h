@@ -100,9 +100,10 @@ constexpr int kGjNBO = 128;  // Gauss-Jordan outer block width (columns per trai
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
 // every launch is bracketed by HIP events on its own stream and its algorithmic
-// flop count (8 m n k per complex product) is recorded with its kind
-// = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing update).
-constexpr int kTimerKinds = 8;
+// flop count (8 m n k per complex product, 8 m sum_j min(k, j + 1) with a
+// triangular B) is recorded with its kind = 8 * (triangular B) + 4 * (3M form)
+// + 2 * (FP32) + (Gauss-Jordan trailing update).
+constexpr int kTimerKinds = 16;
 struct GemmTimer {
   bool on = false;
   int every = 1;         // time one launch in `every` (uniform sample)
