@@ -73,6 +73,19 @@ struct GemmBatch {
               // a column tile starting at j0 sums K only over [0, j0 + tile width)
 };
 
+// Pivoted Cholesky factors Gam = W W' of both leads' broadening for a batch of
+// energies (negf_kernels.hip): lead l's matrices at gam[l] + b*sG (overwritten
+// with what the factor leaves out), factors at w[l] + b*sW (complex elements),
+// state at st[l] + b*sSt (bytes, 12n + 40 needed).
+struct PcholArgs {
+  double2* gam[2];
+  double2* w[2];
+  char* st[2];
+  long long sG, sW, sSt;
+  int n;
+};
+hipError_t launch_pchol(hipStream_t s, const PcholArgs& a, int batch);
+
 // Spin matrices gMx, gMy, gMz (norb <= 4) as kernel arguments.
 struct SpinMats {
   double re[3][4][4];  // [matrix][row][col]

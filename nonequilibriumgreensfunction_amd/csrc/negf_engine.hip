@@ -71,9 +71,6 @@ hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long l
 template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
                           const int*);
-// U = 2 triu(Gam, 1) + diag(Gam) for `batch` strided n x n matrices
-template <typename Td, typename Ts>
-hipError_t launch_tri_half(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, int, int);
 // complex128 -> complex64 copies of nn elements for `batch` strided matrices
 static inline hipError_t to_c64(hipStream_t s, cx<float>* Y, long long sY, const cx<double>* X,
                                 long long sX, long long nn, int batch) {
@@ -111,11 +108,12 @@ static const double kH = 6.626e-34;   // Constants.F90:6
 struct Layout {
   long long nn;
   // per-energy region (complex elements)
-  // o_UL, o_UR: the upper-triangular halves 2 triu(Gam, 1) + diag(Gam) of GamL, GamR
-  long long Es, o_slots, o_z, o_M, o_colA, o_colB, o_Y, o_Bm, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
-      o_UL, o_UR, o_prow;
+  // o_WL, o_WR: the pivoted Cholesky factors of GamL, GamR (Gam = W W')
+  // o_pst: the factorization's state, pst_bytes per lead
+  long long Es, o_slots, o_z, o_M, o_colA, o_colB, o_Y, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
+      o_WL, o_WR, o_pst, pst_bytes, o_prow;
   // mixed-precision form: complex64 region (offsets in complex64 elements from o_f32)
-  long long o_f32, f_z, f_g, f_G, f_Y, f_Bm, f_P, f_GamL, f_GamR, f_UL, f_UR;
+  long long o_f32, f_z, f_g, f_G, f_Y, f_P, f_WL, f_WR;
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
@@ -129,21 +127,22 @@ struct Layout {
     o_slots = o; o += (long long)L * nn;
     o_z = o; if (zslots && !mixed) o += (long long)L * nn;
     o_f32 = o;
-    f_z = 0; f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_Bm = f_Y + nn;
-    f_P = f_Bm + nn; f_GamL = f_P + nn; f_GamR = f_GamL + nn; f_UL = f_GamR + nn; f_UR = f_UL + nn;
-    if (mixed) o += (f_UR + nn + 1) / 2;  // two complex64 per complex128 slot
+    f_z = 0; f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_P = f_Y + nn;
+    f_WL = f_P + nn; f_WR = f_WL + nn;
+    if (mixed) o += (f_WR + nn + 1) / 2;  // two complex64 per complex128 slot
     o_M = o; o += nn;
     o_colA = o; o += nn;
     o_colB = o; o += nn;
     o_Y = o; o += nn;
-    o_Bm = o; o += nn;
     o_P = o; o += nn;
     o_SigL = o; o += nn;
     o_SigR = o; o += nn;
     o_GamL = o; o += nn;
     o_GamR = o; o += nn;
-    o_UL = o; if (!mixed) o += nn;  // the mixed form keeps them in complex64
-    o_UR = o; if (!mixed) o += nn;
+    o_WL = o; o += nn;
+    o_WR = o; o += nn;
+    pst_bytes = ((12LL * n + 7) & ~7LL) + 32;  // d[n], used[n], 3 pivot keys, threshold
+    o_pst = o; o += (2 * pst_bytes + 15) / 16;
     o_prow = o; o += (long long)kGjNBO * n;
     Es = (o + 31) & ~31LL;
     o = 0;
@@ -230,7 +229,6 @@ struct negf_ctx {
   zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
   cx<float>* X32 = nullptr;  // HopX in complex64 (mixed-precision form)
   SpinMats sp;
-  bool sp_herm = true;  // gMx, gMy, gMz Hermitian: site blocks may use the half of Gamma
   Layout lay;
   Lane lanes[kMaxLanes];
   int nlanes = kMaxLanes;
@@ -383,7 +381,6 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
   for (long long k = 0; k < nn; ++k) b[k] = mk<double>(-hx[k].x, -hx[k].y);
   HIPCHK(hipMemcpy(c->betaR, b.data(), sizeof(zd) * nn, hipMemcpyHostToDevice));
   std::memset(&c->sp, 0, sizeof(c->sp));
-  c->sp_herm = true;
   if (gM) {
     const zd* g = (const zd*)gM;
     for (int m = 0; m < 3; ++m)
@@ -393,12 +390,6 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
           c->sp.re[m][row][col] = v.x;
           c->sp.im[m][row][col] = v.y;
         }
-    for (int m = 0; m < 3; ++m)
-      for (int col = 0; col < no; ++col)
-        for (int row = 0; row < no; ++row)
-          if (c->sp.re[m][row][col] != c->sp.re[m][col][row] ||
-              c->sp.im[m][row][col] != -c->sp.im[m][col][row])
-            c->sp_herm = false;
   }
   if (!bd && !c->tmpP) HIPCHK(hipMalloc(&c->tmpP, sizeof(zd) * nn * 2));
   if (c->cfg.precision == NEGF_PREC_MIXED) {
@@ -1013,9 +1004,15 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   }
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
   TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
-  if (c->cfg.precision != NEGF_PREC_MIXED) {
-    TRY((launch_tri_half<double, double>(s, Q(y.o_UL), Es, Q(y.o_GamL), Es, n, nb)));
-    TRY((launch_tri_half<double, double>(s, Q(y.o_UR), Es, Q(y.o_GamR), Es, n, nb)));
+  {  // Gamma_L = W_L W_L', Gamma_R = W_R W_R' (pivoted Cholesky)
+    PcholArgs pa = {};
+    pa.gam[0] = Q(y.o_GamL); pa.gam[1] = Q(y.o_GamR);
+    pa.w[0] = Q(y.o_WL); pa.w[1] = Q(y.o_WR);
+    pa.st[0] = (char*)Q(y.o_pst); pa.st[1] = pa.st[0] + y.pst_bytes;
+    pa.sG = pa.sW = Es;
+    pa.sSt = Es * (long long)sizeof(zd);
+    pa.n = n;
+    TRY(launch_pchol(s, pa, nb));
   }
 
   GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
@@ -1058,25 +1055,29 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     }
     return 0;
   };
-  // Gn/Gp site blocks of column block G (lead 0 = left/Gamma_L, 1 = right/Gamma_R) at
-  // slice x, from Bm = G U with U the upper-triangular half of Gamma (same real traces)
-  auto siteblocks = [&](int lead, int x, const zd* G) -> int {
-    const bool half = c->sp_herm;  // the spin traces need Hermitian spin matrices
-    const zd* Gu = half ? Q(lead ? y.o_UR : y.o_UL) : Q(lead ? y.o_GamR : y.o_GamL);
-    GemmBatch gu = gbr(G, Es, n, Gu, Es, n, Q(y.o_Bm), Es, n, n);
-    gu.tri_b = half;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gu, nb, kOne, kZero));
+  // The column chains run on V = G W (W the lead's factor, Gamma = W W'): both
+  // recursions act on the left, so they carry a right factor along, and
+  //   Gn site blocks:  (G Gamma G')_ss = (V V')_ss,
+  //   trace terms:     Re tr(Gam_a G Gam_b G') = |W_a' V_b|_F^2.
+  // Gn/Gp site blocks of V = G W_lead (lead 0 = left, 1 = right) at slice x
+  auto siteblocks = [&](int lead, int x, const zd* V) -> int {
     double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
-    TRY(launch_siteblocks<double>(s, no, Q(y.o_Bm), Es, G, Es, n, out, y.site_per_e,
-                                  (long long)L * ns, c->sp, nb, nullptr));
+    TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, out, y.site_per_e, (long long)L * ns, c->sp,
+                                  nb, nullptr));
     return 0;
   };
-  // scal[idx] = Re <Gam_lead * Bm, G>  (= Re tr(Gam_lead G Gam' G') for Bm = G U')
-  auto trace_term = [&](int lead, const zd* G, int idx) -> int {
-    const zd* Gam = Q(lead ? y.o_GamR : y.o_GamL);
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(Gam, Es, n, Q(y.o_Bm), Es, n, Q(y.o_P), Es, n, n), nb,
-                             kOne, kZero));
-    TRY(launch_inner<double>(s, Q(y.o_P), Es, G, Es, n, ln.d_scal + idx, 8, nb, nullptr));
+  // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b
+  auto trace_term = [&](int lead, const zd* V, int idx) -> int {
+    const zd* Wa = Q(lead ? y.o_WR : y.o_WL);
+    TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(Wa, Es, n, V, Es, n, Q(y.o_P), Es, n, n), nb, kOne,
+                             kZero));
+    TRY(launch_inner<double>(s, Q(y.o_P), Es, Q(y.o_P), Es, n, ln.d_scal + idx, 8, nb, nullptr));
+    return 0;
+  };
+  // V = G W_lead into dst
+  auto times_w = [&](const zd* G, int lead, zd* dst) -> int {
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(G, Es, n, Q(lead ? y.o_WR : y.o_WL), Es, n, dst, Es, n, n),
+                             nb, kOne, kZero));
     return 0;
   };
   int r;
@@ -1101,45 +1102,41 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       g.m3 = 1;
       return g;
     };
-    TRY(to_c64(s, F(y.f_GamL), Es32, Q(y.o_GamL), Es, nn, nb));
-    TRY(to_c64(s, F(y.f_GamR), Es32, Q(y.o_GamR), Es, nn, nb));
-    TRY((launch_tri_half<float, double>(s, F(y.f_UL), Es32, Q(y.o_GamL), Es, n, nb)));
-    TRY((launch_tri_half<float, double>(s, F(y.f_UR), Es32, Q(y.o_GamR), Es, n, nb)));
-    // Bm32 = G32 U_lead (U the upper-triangular half of Gamma) and the site blocks of
-    // Bm32 G32' for one lead at slice x
-    auto site32 = [&](int lead, int x, const zf* G32) -> int {
-      const bool half = c->sp_herm;
-      GemmBatch gu = gbf(G32, half ? F(lead ? y.f_UR : y.f_UL) : F(lead ? y.f_GamR : y.f_GamL), F(y.f_Bm));
-      gu.tri_b = half;
-      TRY(gemm_batched<float>(s, OP_N, OP_N, gu, nb, kOnef, kZerof));
+    TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, nn, nb));
+    TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, nn, nb));
+    // site blocks of V32 = G32 W_lead for one lead at slice x
+    auto site32 = [&](int lead, int x, const zf* V32) -> int {
       double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
-      TRY(launch_siteblocks<float>(s, no, F(y.f_Bm), Es32, G32, Es32, n, out, y.site_per_e,
-                                   (long long)L * ns, c->sp, nb, nullptr));
+      TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, out, y.site_per_e, (long long)L * ns,
+                                   c->sp, nb, nullptr));
       return 0;
     };
-    // scal[idx] = Re <Gam_lead Bm32, G32> (Bm32 from the last site32 call)
-    auto trace32 = [&](int lead, const zf* G32, int idx) -> int {
-      TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(lead ? y.f_GamR : y.f_GamL), F(y.f_Bm), F(y.f_P)), nb,
-                              kOnef, kZerof));
-      TRY(launch_inner<float>(s, F(y.f_P), Es32, G32, Es32, n, ln.d_scal + idx, 8, nb, nullptr));
+    // scal[idx] = |W_lead' V32|_F^2
+    auto trace32 = [&](int lead, const zf* V32, int idx) -> int {
+      GemmBatch g = gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P));
+      TRY(gemm_batched<float>(s, OP_C, OP_N, g, nb, kOnef, kZerof));
+      TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, ln.d_scal + idx, 8, nb, nullptr));
       return 0;
     };
-    // forward: gL_x in FP64, z_x = gL_x X' z_{x-1} in complex64
+    // forward: gL_x in FP64, z_x = gL_x X' z_{x-1} in complex64 (z_1 = gL_1 W_L)
     for (int x = 0; x < L; ++x) {
       if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
       TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
       if (x == 0) {
-        TRY(to_c64(s, zs(0), Es32, slot(0), Es, nn, nb));
+        TRY(to_c64(s, F(y.f_g), Es32, slot(0), Es, nn, nb));
+        TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0)), nb, kOnef, kZerof));
       } else {
         TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
         TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, nb, nullptr));
         TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kZerof));
       }
     }
-    // backward: G_xL in FP64; G_x1 = z_x + gL_x X G_{x+1,1} in complex64 (in place of z_x)
-    zd* colA = slot(L - 1);
-    zd* colB = nullptr;
+    // backward: G_xL W_R in FP64; G_x1 W_L = z_x + gL_x X G_{x+1,1} W_L in complex64
+    // (in place of z_x)
     zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
+    if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
+    zd* colA = bufs[1];
+    zd* colB = nullptr;
     int pb = 0;
     for (int x = L - 1; x >= 0; --x) {
       if (x < L - 1) {
@@ -1152,11 +1149,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kOnef));
       }
       const zd* GxL = (x < L - 1) ? colB : colA;
-      if (x == 0) {  // T12 = Re tr(GamL G1L GamR G1L') from FP64 products
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(GxL, Es, n, Q(y.o_GamR), Es, n, Q(y.o_Bm), Es, n, n),
-                                 nb, kOne, kZero));
-        if ((r = trace_term(0, GxL, 0))) return r;
-      }
+      if (x == 0 && (r = trace_term(0, GxL, 0))) return r;  // T12 = |W_L' G_1L W_R|^2 in FP64
       TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, nn, nb));
       if ((r = site32(1, x, F(y.f_G)))) return r;
       if (x == L - 1 && (r = trace32(1, F(y.f_G), 3))) return r;  // Modec2, right-lead term
@@ -1178,7 +1171,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
       TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
       if (x == 0) {
-        TRY(launch_copy3<double>(s, zs(0), slot(0), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
+        if ((r = times_w(slot(0), 0, zs(0)))) return r;  // z_1 W_L = gL_1 W_L
       } else {
         if ((r = xmul(1, Q(y.o_Y), zs(x - 1)))) return r;
         TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
@@ -1189,9 +1182,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     // back substitution G_x1 = z_x + gL_x X G_{x+1,1}, G_L1 = z_L, in place of z_x.
     // Slice x then has both columns: the Gn/Gp site blocks of both leads, T12 at x = 1
     // (Calculate_Observables.F90:86-100) and the Modec terms at the ends (:48-53).
-    zd* colA = slot(L - 1);  // G_{x+1,L}
-    zd* colB = nullptr;      // G_{x,L}
     zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
+    if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
+    zd* colA = bufs[1];  // G_{x+1,L} W_R
+    zd* colB = nullptr;  // G_{x,L} W_R
     int pb = 0;
     for (int x = L - 1; x >= 0; --x) {
       if (x < L - 1) {
@@ -1225,7 +1219,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   // ---- pass 2: last column and gR
   zd* colA = Q(y.o_colA);
   zd* colB = Q(y.o_colB);
-  TRY(launch_copy3<double>(s, colA, slot(L - 1), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
+  if ((r = times_w(slot(L - 1), 1, colA))) return r;  // G_LL W_R
   if ((r = siteblocks(1, L - 1, colA))) return r;
   if ((r = trace_term(1, colA, 3))) return r;          // Modec2, right-lead term
   if (L == 1 && (r = trace_term(0, colA, 0))) return r; // T12
@@ -1244,7 +1238,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   // ---- pass 3: first column
   colA = Q(y.o_colA);
   colB = Q(y.o_colB);
-  TRY(launch_copy3<double>(s, colA, slot(0), nullptr, nullptr, nullptr, nullptr, Es, nn, nb, nullptr));
+  if ((r = times_w(slot(0), 0, colA))) return r;  // G_11 W_L
   if ((r = siteblocks(0, 0, colA))) return r;
   if ((r = trace_term(0, colA, 1))) return r;            // Modec1, left-lead term
   if (L == 1 && (r = trace_term(1, colA, 2))) return r;  // Modec2, left-lead term

@@ -328,21 +328,134 @@ __global__ void k_gamma(const cx<T>* Sig, long long sS, cx<T>* Gam, long long sG
   Gam[(long long)c * sG + (long long)col * n + r] = mk<T>(-di, dr);
 }
 
-// U = 2 triu(Gam, 1) + diag(Gam): upper triangular, with U + U' = 2 Gam for a
-// Hermitian Gam.  For any G, G U G' = G Gam G' + A with A = G (U - Gam) G'
-// anti-Hermitian, so on every diagonal block D the real traces Re tr D and
-// Re tr(D sigma) (sigma Hermitian) of G U G' equal those of G Gam G', and
-// Re tr(Gam2 G U G') = Re tr(Gam2 G Gam G') for a Hermitian Gam2: the site blocks
-// and the trace terms take B = G U, a product with half the work of G Gam.
-template <typename Td, typename Ts>
-__global__ void k_tri_half(cx<Td>* U, long long sU, const cx<Ts>* Gam, long long sG, int n) {
-  const int b = blockIdx.z;
-  const int col = blockIdx.y;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const cx<Ts> v = Gam[(long long)b * sG + (long long)col * n + r];
-  const Ts f = r < col ? (Ts)2 : (r == col ? (Ts)1 : (Ts)0);
-  U[(long long)b * sU + (long long)col * n + r] = mk<Td>((Td)(f * v.x), (Td)(f * v.y));
+// ---------------------------------------------------------------------------
+// Pivoted Cholesky factor Gam = W W' of a Hermitian positive semidefinite matrix
+// (the contact broadening), one launch per column, with implicit pivoting: W keeps
+// the original row order and column j belongs to pivot p_j, the unused row of
+// largest remaining diagonal d_i = Gam_ii - sum_{l<j} |W_il|^2 (ties to the lower
+// row):
+//     W_pj = sqrt(d_p),  W_ij = (Gam_ip - sum_{l<j} W_il conj(W_pl)) / W_pj  (i unused),
+//     W_ij = 0 for rows already used.
+// Blocked by panels of kPcholNB columns: after each panel Gam -= W_panel W_panel'
+// (one MFMA product, in place), so a column sums only over its own panel.
+// Once the largest remaining d_p is at most kPcholTol * max_i Gam_ii the remaining
+// columns are zero (the dropped part is positive semidefinite with trace below
+// n * kPcholTol * max Gam_ii).  blockIdx.y selects the lead, blockIdx.z the matrix.
+// Per-matrix state (bytes from st + z * sSt): d[n] doubles, used[n] ints, then three
+// 64-bit pivot keys (key of column j in slot j % 3) and the stop threshold.
+constexpr double kPcholTol = 1e-15;
+constexpr int kPcholNB = 32;
+
+__device__ inline long long pchol_key_off(int n) { return ((12LL * n) + 7) & ~7LL; }
+
+__device__ inline unsigned long long pchol_key(double d, int i) {
+  const double c = d > 0.0 ? d : 0.0;  // rounding below zero counts as zero; NaN too
+  return ((unsigned long long)__double_as_longlong(c) & ~0xfffull) | (unsigned long long)(4095 - i);
+}
+
+__global__ void k_pchol_clear(PcholArgs a) {
+  char* st = a.st[blockIdx.y] + (long long)blockIdx.z * a.sSt;
+  unsigned long long* key = (unsigned long long*)(st + pchol_key_off(a.n));
+  if (threadIdx.x < 3) key[threadIdx.x] = 0ull;
+  if (threadIdx.x == 3) *(double*)(key + 3) = 0.0;
+}
+
+__device__ inline unsigned long long wave_umax64(unsigned long long x) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long y = __shfl_xor(x, off);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_pchol_init(PcholArgs a) {
+  const int lead = blockIdx.y;
+  const double2* G = a.gam[lead] + (long long)blockIdx.z * a.sG;
+  char* st = a.st[lead] + (long long)blockIdx.z * a.sSt;
+  double* d = (double*)st;
+  int* used = (int*)(st + 8LL * a.n);
+  unsigned long long* key = (unsigned long long*)(st + pchol_key_off(a.n));
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long k = 0ull;
+  if (i < a.n) {
+    const double di = G[(long long)i * a.n + i].x;
+    d[i] = di;
+    used[i] = 0;
+    k = pchol_key(di, i);
+  }
+  k = wave_umax64(k);
+  if ((threadIdx.x & 63) == 0 && k) atomicMax(&key[0], k);
+}
+
+__global__ __launch_bounds__(256) void k_pchol_col(PcholArgs a, int j, int j0) {
+  const int lead = blockIdx.y, n = a.n;
+  const double2* G = a.gam[lead] + (long long)blockIdx.z * a.sG;
+  double2* W = a.w[lead] + (long long)blockIdx.z * a.sW;
+  char* st = a.st[lead] + (long long)blockIdx.z * a.sSt;
+  double* d = (double*)st;
+  int* used = (int*)(st + 8LL * n);
+  unsigned long long* key = (unsigned long long*)(st + pchol_key_off(n));
+  double* thr = (double*)(key + 3);
+  const unsigned long long kj = key[j % 3];
+  // slot (j + 2) % 3 held the key of column j - 1, read by the previous launch; it
+  // takes column j + 2's key, written by the next launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) key[(j + 2) % 3] = 0ull;
+  const int p = 4095 - (int)(kj & 0xfffull);
+  const double dp = kj ? d[p] : 0.0;
+  const double t = j == 0 ? dp * kPcholTol : *thr;
+  if (j == 0 && blockIdx.x == 0 && threadIdx.x == 0) *thr = t;
+  const bool stop = kj == 0ull || !(dp > t);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long k = 0ull;
+  if (i < n) {
+    double2* Wj = W + (long long)j * n;
+    if (stop || (used[i] && i != p)) {
+      Wj[i] = make_double2(0.0, 0.0);
+    } else if (i == p) {
+      Wj[i] = make_double2(sqrt(dp), 0.0);
+      used[i] = 1;
+    } else {
+      double2 acc = G[(long long)p * n + i];  // Gam minus the earlier panels' part
+      for (int l = j0; l < j; ++l) {
+        const double2 wi = W[(long long)l * n + i], wp = W[(long long)l * n + p];
+        // acc -= wi * conj(wp)
+        acc.x -= wi.x * wp.x + wi.y * wp.y;
+        acc.y -= wi.y * wp.x - wi.x * wp.y;
+      }
+      const double s = sqrt(dp);
+      const double2 w = make_double2(acc.x / s, acc.y / s);
+      Wj[i] = w;
+      const double di = d[i] - (w.x * w.x + w.y * w.y);
+      d[i] = di;
+      k = pchol_key(di, i);
+    }
+  }
+  k = wave_umax64(k);
+  if ((threadIdx.x & 63) == 0 && k) atomicMax(&key[(j + 1) % 3], k);
+}
+
+hipError_t launch_pchol(hipStream_t s, const PcholArgs& a, int batch) {
+  if (batch <= 0 || a.n <= 0) return hipSuccess;
+  if (a.n > 4095) return hipErrorInvalidValue;  // 12-bit row index in the pivot keys
+  const dim3 grid((a.n + 255) / 256, 2, batch);
+  hipLaunchKernelGGL(k_pchol_clear, dim3(1, 2, batch), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_pchol_init, grid, dim3(256), 0, s, a);
+  for (int j0 = 0; j0 < a.n; j0 += kPcholNB) {
+    const int j1 = min(a.n, j0 + kPcholNB);
+    for (int j = j0; j < j1; ++j) hipLaunchKernelGGL(k_pchol_col, grid, dim3(256), 0, s, a, j, j0);
+    if (j1 == a.n) break;
+    for (int lead = 0; lead < 2; ++lead) {  // Gam -= W(:, j0:j1) W(:, j0:j1)'
+      GemmBatch g = {};
+      g.A = a.w[lead] + (long long)j0 * a.n; g.sA = a.sW; g.lda = a.n;
+      g.B = a.w[lead] + (long long)j0 * a.n; g.sB = a.sW; g.ldb = a.n;
+      g.C = a.gam[lead]; g.sC = a.sG; g.ldc = a.n;
+      g.m = a.n; g.n = a.n; g.k = j1 - j0;
+      g.m3 = 1;
+      const hipError_t e = gemm_batched<double>(s, OP_N, OP_C, g, batch, mk<double>(-1, 0), mk<double>(1, 0));
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -677,15 +790,6 @@ hipError_t launch_gamma_dos(hipStream_t s, const cx<T>* Sig, long long sS, cx<T>
 }
 
 template <typename Td, typename Ts>
-hipError_t launch_tri_half(hipStream_t s, cx<Td>* U, long long sU, const cx<Ts>* Gam, long long sG,
-                           int n, int batch) {
-  if (batch <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_tri_half<Td, Ts>), dim3((n + 255) / 256, n, batch), dim3(256), 0, s, U, sU,
-                     Gam, sG, n);
-  return hipGetLastError();
-}
-
-template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX,
                           long long nn, int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
@@ -721,10 +825,6 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
 NEGF_INST(double)
 NEGF_INST(float)
 #undef NEGF_INST
-template hipError_t launch_tri_half<double, double>(hipStream_t, cx<double>*, long long,
-                                                    const cx<double>*, long long, int, int);
-template hipError_t launch_tri_half<float, double>(hipStream_t, cx<float>*, long long,
-                                                   const cx<double>*, long long, int, int);
 template hipError_t launch_convert<float, double>(hipStream_t, cx<float>*, long long,
                                                   const cx<double>*, long long, long long, int,
                                                   const int*);

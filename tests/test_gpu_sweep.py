@@ -132,9 +132,9 @@ def test_contact_self_energy_matches_oracle():
 
 
 def test_sweep_non_hermitian_spin_matrices():
-    """Site blocks are formed from the upper-triangular half of Gamma only when the
-    spin matrices are Hermitian (then the real traces agree); other spin matrices
-    take the full Gamma products, still equal to the oracle's Re tr(Gn sigma)."""
+    """Spin matrices that are not Hermitian: the site blocks (V V')_ss with V = G W
+    are the blocks of Gn itself, so Re tr(Gn sigma) matches the oracle for any
+    sigma."""
     w = build_wire(5, 2, 2, disorder=0.3, seed=4, zeeman=0.02)
     w.gm = w.gm + 0.5j * np.roll(w.gm, 1, axis=0)  # Hermitian + i Hermitian
     assert not np.allclose(w.gm[0], w.gm[0].conj().T)
