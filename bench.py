@@ -81,10 +81,11 @@ OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", 
 
 def f_useful(n, L, k_left, k_right):
     """A fixed yardstick of dense flops per energy point, the three-pass RGF's
-    count: 2L inverses, 2(L-1) column GEMMs, 2L full G*Gamma GEMMs, 1 for T12 and
+    count: 2L inverses, 2(L-1) column GEMMs, 2L G*Gamma GEMMs, 1 for T12 and
     (2 + 9K) per lead for Sancho-Rubio, each n^3 complex multiply-adds = 8 n^3
-    flops.  The solve does less than this (L inverses, half products with the
-    triangular half of Gamma), so the rate it gives rises with such savings."""
+    flops.  The solve does less than this (L inverses, no G*Gamma products: the
+    column chains carry the factor of Gamma), so the rate it gives rises with
+    such savings."""
     return 8.0 * n**3 * (6 * L + 3 + 9 * (k_left + k_right))
 
 
@@ -182,7 +183,7 @@ def gemm_only(args):
     reps = 8
     ms = capi.bench_gemm(n, batch, complex64=(args.precision == "mixed"), m3=True, reps=reps)
     flops = 8.0 * n ** 3 * batch
-    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1,0>" % ("float" if args.precision == "mixed" else "double"),
+    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.precision == "mixed" else "double"),
                       "n": n, "batch": batch, "launches": reps + 1, "ms_per_launch": ms,
                       "algorithmic_flops_per_launch": flops, "tflops": flops / ms / 1e9,
                       "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.precision == "mixed" else 16)}))
@@ -432,9 +433,9 @@ def main():
     kernels = {k: {"tflops": (f / (m * 1e-3) / 1e12 if m > 0 else 0.0), "launches": c,
                    "avg_launch_ms": m / max(1, c)}
                for k, (f, m, c) in prof.items() if c}
-    kname = ("k_gemm<float,0,0,0,2,2,0,1,0> (complex64 MFMA product, 3M form, n x n x n per matrix)"
+    kname = ("k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form, n x n x n per matrix)"
              if args.precision == "mixed" else
-             "k_gemm<double,0,0,0,2,2,0,1,0> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
+             "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
     result = {
         "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
         "value": value,
@@ -465,14 +466,12 @@ def main():
                      "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": gemm_tflops / peak, "traffic": traffic_from_profiles(args.config, dominant),
                      "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
-                     "note": ("achieved = algorithmic flops / HIP-event duration on the launching stream "
-                              "(one launch in --profile-every sampled): 8 n^3 per full complex product x "
-                              "batch, 8 n^2 (n+1)/2 per site-block product with the upper-triangular half "
-                              "of Gamma; the 3M form executes 3/4 of that on the matrix cores; the lanes "
-                              "run concurrently, so durations include the other lanes' kernels sharing the "
-                              "CUs; traffic: HBM-side bytes per launch of the full n x n x n product from "
-                              "FETCH_SIZE/WRITE_SIZE counter runs of the kernel alone (bench.py "
-                              "--gemm-only), scaled by a copy-kernel calibration")},
+                     "note": ("achieved = algorithmic 8 n^3 flops per complex product x batch / HIP-event "
+                              "duration on the launching stream (one launch in --profile-every sampled); the "
+                              "3M form executes 6 n^3 on the matrix cores; the lanes run concurrently, so "
+                              "durations include the other lanes' kernels sharing the CUs; traffic: HBM-side "
+                              "bytes per launch from FETCH_SIZE/WRITE_SIZE counter runs of the kernel alone "
+                              "(bench.py --gemm-only), scaled by a copy-kernel calibration")},
         "kernels": kernels,
     }
     alone = profile_value(args.config, dominant, "alone_tflops")

@@ -109,9 +109,7 @@ int negf_finalize(negf_ctx* ctx);
 /* Upload Htotd (n,n,Npx), HopX (n,n) and the spin matrices gMx,gMy,gMz (three
  * column-major Norb x Norb blocks, Read_In_Allocate.F90:186-196).  Call after
  * Build_Hamiltonian / MPI_Broadcast (NEGF_Main.F90:15-19), i.e. once per
- * Poisson iteration.  With Hermitian spin matrices (the reference's are) the
- * density and spin products use the upper-triangular half of Gamma, half the
- * work; other spin matrices take the full products, with the same results. */
+ * Poisson iteration. */
 int negf_set_hamiltonian(negf_ctx* ctx, const double* Htotd, const double* HopX,
                          const double* gM);
 
@@ -281,22 +279,19 @@ int negf_server_destroy(negf_server* srv);
  * launch).  end() synchronises the
  * device and returns, per kind, the launches' algorithmic flops (8 m n k per
  * complex product), their summed duration in ms and the launch count; each
- * argument is an array of 16 indexed by 8 * (upper-triangular B: the density
- * and spin products with the half of Gamma) + 4 * (3M complex form) + 2 * (FP32)
- * + (Gauss-Jordan trailing update): [0] FP64 product, [1] FP64 update, [2] FP32
- * product, [3] FP32 update, [4..7] the same in the 3M form, [8..15] the same with
- * a triangular B (updates never have one). */
+ * argument is an array of 8 indexed by 4 * (3M complex form) + 2 * (FP32) +
+ * (Gauss-Jordan trailing update): [0] FP64 product, [1] FP64 update, [2] FP32
+ * product, [3] FP32 update, [4..7] the same in the 3M form. */
 int negf_profile_begin(int sample_every);
 int negf_profile_end(double* flops, double* ms, long long* launches);
 
 /* Instrumentation, no counterpart in the reference: the batched complex product
  * kernel of the sweep alone, C_b = A_b B_b for `batch` n x n matrices (complex64 if
- * complex64 != 0, else complex128; form: bit 0 the 3M form, bit 1 B taken as upper
- * triangular, as in the density and spin products), `reps` timed launches after
+ * complex64 != 0, else complex128; 3M form if m3 != 0), `reps` timed launches after
  * one untimed one; the operands are a constant byte pattern.  For hardware-counter
  * (rocprofv3 --pmc) runs of that kernel at the shape the sweep launches it with.
  * *ms_per_launch gets the mean launch time. */
-int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int form,
+int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
                     double* ms_per_launch);
 
 #ifdef __cplusplus

@@ -118,7 +118,7 @@ def lib():
         L.negf_set_hamiltonian_compact.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.negf_set_onsite.argtypes = [ctypes.c_void_p, _dp]
         L.negf_contact_cache_stats.argtypes = [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]
-        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 16
+        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 8
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -150,10 +150,8 @@ def profile_begin(sample_every=1):
     _check(lib().negf_profile_begin(int(sample_every)))
 
 
-_KINDS8 = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update",
-           "fp64_gemm_3m", "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
-# [8..15]: the same kinds with an upper-triangular B (the half-Gamma site-block products)
-PROFILE_KINDS = _KINDS8 + tuple(k + "_tri" for k in _KINDS8)
+PROFILE_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update",
+                 "fp64_gemm_3m", "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
 
 
 def profile_end():
@@ -167,14 +165,12 @@ def profile_end():
     return {k: (f[i], m[i], n[i]) for i, k in enumerate(PROFILE_KINDS)}
 
 
-def bench_gemm(n, batch, complex64=True, m3=True, reps=8, device=0, tri=False):
+def bench_gemm(n, batch, complex64=True, m3=True, reps=8, device=0):
     """Mean launch time (ms) of the batched complex product kernel alone at one shape
-    (negf_bench_gemm): the kernel that bounds the sweep, for counter runs.  tri: B
-    taken as upper triangular (the density and spin products)."""
+    (negf_bench_gemm): the kernel that bounds the sweep, for counter runs."""
     ms = ctypes.c_double(0.0)
-    form = int(bool(m3)) | (int(bool(tri)) << 1)
     _check(lib().negf_bench_gemm(int(device), int(bool(complex64)), int(n), int(batch), int(reps),
-                                 form, ctypes.byref(ms)))
+                                 int(bool(m3)), ctypes.byref(ms)))
     return ms.value
 
 

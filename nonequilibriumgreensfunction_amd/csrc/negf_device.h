@@ -69,8 +69,6 @@ struct GemmBatch {
   int skip_lo, skip_hi;
   int m3;  // 1: 3-product complex MFMA form (normwise-accurate, 3/4 of the work);
            // 0: 4-product form (each of Re, Im accurate relative to its own terms)
-  int tri_b;  // 1: op(B) is upper triangular (zero below the diagonal, EPI_STD only):
-              // a column tile starting at j0 sums K only over [0, j0 + tile width)
 };
 
 // Pivoted Cholesky factors Gam = W W' of both leads' broadening for a batch of
@@ -113,10 +111,9 @@ constexpr int kGjNBO = 128;  // Gauss-Jordan outer block width (columns per trai
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
 // every launch is bracketed by HIP events on its own stream and its algorithmic
-// flop count (8 m n k per complex product, 8 m sum_j min(k, j + 1) with a
-// triangular B) is recorded with its kind = 8 * (triangular B) + 4 * (3M form)
-// + 2 * (FP32) + (Gauss-Jordan trailing update).
-constexpr int kTimerKinds = 16;
+// flop count (8 m n k per complex product) is recorded with its kind
+// = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing update).
+constexpr int kTimerKinds = 8;
 struct GemmTimer {
   bool on = false;
   int every = 1;         // time one launch in `every` (uniform sample)

@@ -1568,9 +1568,9 @@ extern "C" int negf_profile_end(double* flops, double* ms, long long* launches) 
 // n x n matrices (complex64 or complex128, 3M or 4-product form), `reps` launches
 // on a stream of its own after one untimed launch.  For hardware-counter runs of
 // the kernel that bounds the sweep; the operands are a constant byte pattern.
-extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int form,
+extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
                                double* ms_per_launch) {
-  if (n < 1 || n > 8192 || batch < 1 || reps < 1 || !ms_per_launch || (form & ~3)) {
+  if (n < 1 || n > 8192 || batch < 1 || reps < 1 || !ms_per_launch) {
     negf_set_error("negf_bench_gemm: bad arguments");
     return NEGF_ERR_ARG;
   }
@@ -1591,8 +1591,7 @@ extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int 
     g.sA = g.sB = g.sC = (long long)n * n;
     g.lda = g.ldb = g.ldc = n;
     g.m = g.n = g.k = n;
-    g.m3 = form & 1;
-    g.tri_b = (form >> 1) & 1;  // B read as upper triangular (its lower part is skipped)
+    g.m3 = m3 ? 1 : 0;
     auto launch = [&]() {
       return complex64 ? gemm_batched<float>(s, OP_N, OP_N, g, batch, mk<float>(1, 0), mk<float>(0, 0))
                        : gemm_batched<double>(s, OP_N, OP_N, g, batch, mk<double>(1, 0), mk<double>(0, 0));

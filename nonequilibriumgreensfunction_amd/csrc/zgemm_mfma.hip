@@ -70,11 +70,8 @@ __device__ inline cx<T> load_op(const cx<T>* p, bool ok) {
 // ~64 workgroups an XCD holds at once then cover about 8 x 8 tiles, which share 8
 // A row panels and 8 B column panels in that XCD's L2 (plain column-major runs
 // cover all tile rows and two columns: ~38 panels for the same 64 tiles).
-// With fold (a triangular B, whose tile columns carry work growing with the
-// column index), the columns of a run alternate from both ends (last, first,
-// second to last, ...), so each XCD's contiguous share has the average load.
 // Returns ntiles for ids past the last tile.
-__device__ inline int xcd_tile(int bid, int tiles_m, int ntiles, bool fold) {
+__device__ inline int xcd_tile(int bid, int tiles_m, int ntiles) {
   constexpr int G = 8;
   const int per = (ntiles + 7) >> 3;
   const int o = (bid & 7) * per + (bid >> 3);
@@ -83,9 +80,7 @@ __device__ inline int xcd_tile(int bid, int tiles_m, int ntiles, bool fold) {
   const int g = o / (G * tiles_n);
   const int r = o - g * G * tiles_n;
   const int rows = min(G, tiles_m - g * G);
-  int col = r / rows;
-  if (fold) col = (col & 1) ? (col >> 1) : tiles_n - 1 - (col >> 1);
-  return col * tiles_m + g * G + r % rows;
+  return (r / rows) * tiles_m + g * G + r % rows;
 }
 
 // Workgroup tile (32 SM) x (32 SN): 2 x 2 waves, each owning SM x SN MFMA blocks of
@@ -98,16 +93,14 @@ __device__ inline int xcd_tile(int bid, int tiles_m, int ntiles, bool fold) {
 // with no AGPR accumulators and no spills, and the second workgroup hides the
 // LDS/global latency that one wave per SIMD leaves exposed (measured 53 -> 70 TF/s
 // algorithmic at n = 1024, 55 -> 75 TF/s at n = 2304).
-// TRI = 1 (EPI_STD only): op(B) is upper triangular (GemmBatch::tri_b), a separate
-// instance so that its launches are timed and profiled apart from full products.
-template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int PRE, int M3, int TRI>
+template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int PRE, int M3>
 __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, int ntiles,
                                                    cx<T> alpha, cx<T> beta) {
   typedef typename Mfma<T>::acc acc_t;
   constexpr int BM = 32 * SM, BN = 32 * SN;
   constexpr int LA = BM * kBK / kThreads, LB = BN * kBK / kThreads;
   const int b = g.active ? g.active[blockIdx.z] : (int)blockIdx.z;
-  const int tile = xcd_tile(blockIdx.x, tiles_m, ntiles, TRI);
+  const int tile = xcd_tile(blockIdx.x, tiles_m, ntiles);
   if (tile >= ntiles) return;
   const int m0 = (tile % tiles_m) * BM;
   const int n0 = (tile / tiles_m) * BN;
@@ -201,9 +194,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
     }
   };
 
-  // an upper-triangular op(B) is zero in rows K >= n0 + BN of this tile's columns
-  const int Ke = TRI ? min(K, n0 + BN) : K;
-  const int nk = (Ke + kBK - 1) / kBK;
+  const int nk = (K + kBK - 1) / kBK;
   if (nk > 0) {
     fetch(0);
     stash(0);
@@ -329,7 +320,7 @@ static bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
   return true;
 }
 
-template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int M3, int TRI>
+template <typename T, int OPA, int OPB, int EPI, int SM, int SN, int M3>
 static hipError_t launch_cfg(hipStream_t s, const GemmBatch& g, int batch, cx<T> alpha, cx<T> beta) {
   constexpr int BM = 32 * SM, BN = 32 * SN;
   int tm = (g.m + BM - 1) / BM, tn = (g.n + BN - 1) / BN;
@@ -340,51 +331,31 @@ static hipError_t launch_cfg(hipStream_t s, const GemmBatch& g, int batch, cx<T>
   constexpr bool kPre = EPI == EPI_GJ && !(M3 && sizeof(T) == sizeof(double));
   if constexpr (kPre) {
     if (g.k <= 32) {
-      hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 1, M3, 0>), grid, dim3(kThreads), 0, s, g, tm,
+      hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 1, M3>), grid, dim3(kThreads), 0, s, g, tm,
                          nt, alpha, beta);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 0, M3, TRI>), grid, dim3(kThreads), 0, s, g, tm,
-                     nt, alpha, beta);
+  hipLaunchKernelGGL((k_gemm<T, OPA, OPB, EPI, SM, SN, 0, M3>), grid, dim3(kThreads), 0, s, g, tm, nt,
+                     alpha, beta);
   return hipGetLastError();
 }
 
 template <typename T, int OPA, int OPB, int EPI>
 static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alpha, cx<T> beta) {
   if (batch <= 0 || g.m <= 0 || g.n <= 0) return hipSuccess;
-  // a triangular op(B) is supported for the plain product C = A B only
-  const bool tri = g.tri_b != 0;
-  if (tri && !(EPI == EPI_STD && OPA == OP_N && OPB == OP_N)) return hipErrorInvalidValue;
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
-  const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0) +
-                   (tri ? 8 : 0);
-  // multiply-adds per row: k per column, or for an upper-triangular op(B) the
-  // sum over columns j of min(k, j + 1)
-  double kcols = (double)ncols * g.k;
-  if (tri) {
-    const double kk = g.k, nc = g.n;
-    kcols = kk >= nc ? nc * (nc + 1) / 2 : kk * (kk + 1) / 2 + (nc - kk) * kk;
-  }
-  const bool timed = timer_slot(8.0 * g.m * kcols * batch, kind, &ev0, &ev1);
+  const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
+  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, kind, &ev0, &ev1);
   if (timed) {
     const hipError_t er = hipEventRecord(ev0, s);
     if (er != hipSuccess) return er;
   }
   // 64x64 workgroup tiles: 128x64 and 64x128 measured within 3% of it for these
   // shapes, and their FP64 variants do not fit two workgroups per CU without spills
-  hipError_t e;
-  if constexpr (EPI == EPI_STD && OPA == OP_N && OPB == OP_N) {
-    if (tri) {
-      e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 2, 2, 1, 1>(s, g, batch, alpha, beta)
-               : launch_cfg<T, OPA, OPB, EPI, 2, 2, 0, 1>(s, g, batch, alpha, beta);
-      if (timed && e == hipSuccess) return hipEventRecord(ev1, s);
-      return e;
-    }
-  }
-  e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 2, 2, 1, 0>(s, g, batch, alpha, beta)
-           : launch_cfg<T, OPA, OPB, EPI, 2, 2, 0, 0>(s, g, batch, alpha, beta);
+  const hipError_t e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 2, 2, 1>(s, g, batch, alpha, beta)
+                            : launch_cfg<T, OPA, OPB, EPI, 2, 2, 0>(s, g, batch, alpha, beta);
   if (timed && e == hipSuccess) return hipEventRecord(ev1, s);
   return e;
 }

@@ -64,8 +64,6 @@ def test_bench_gemm_bad_shape_rejected_before_device():
     ms = ctypes.c_double(0.0)
     for n, batch, reps in ((0, 1, 1), (64, 0, 1), (64, 1, 0), (9000, 1, 1)):
         assert capi.lib().negf_bench_gemm(0, 1, n, batch, reps, 1, ctypes.byref(ms)) == capi.NEGF_ERR_ARG
-    # form has two bits (3M, triangular B)
-    assert capi.lib().negf_bench_gemm(0, 1, 64, 1, 1, 4, ctypes.byref(ms)) == capi.NEGF_ERR_ARG
 
 
 def test_splitmix64_stream_is_pinned():
