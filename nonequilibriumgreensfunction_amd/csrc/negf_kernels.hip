@@ -450,7 +450,7 @@ hipError_t launch_pchol(hipStream_t s, const PcholArgs& a, int batch) {
       g.B = a.w[lead] + (long long)j0 * a.n; g.sB = a.sW; g.ldb = a.n;
       g.C = a.gam[lead]; g.sC = a.sG; g.ldc = a.n;
       g.m = a.n; g.n = a.n; g.k = j1 - j0;
-      g.m3 = 1;
+      g.m3 = 0;  // 4-product form: componentwise accurate, and timed apart from the 3M products
       const hipError_t e = gemm_batched<double>(s, OP_N, OP_C, g, batch, mk<double>(-1, 0), mk<double>(1, 0));
       if (e != hipSuccess) return e;
     }
