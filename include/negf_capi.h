@@ -44,7 +44,7 @@ extern "C" {
 /* precision forms */
 #define NEGF_PREC_FP64 0  /* plain complex128 on FP64 MFMA */
 #define NEGF_PREC_MIXED 1 /* FP64 contacts, inverses and T12 chain; complex64 MFMA for the
-                             first-column substitution and the site-block products */
+                             first-column chain, whose site blocks feed the densities */
 
 typedef struct negf_ctx negf_ctx;
 

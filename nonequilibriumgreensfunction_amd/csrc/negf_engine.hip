@@ -120,7 +120,7 @@ struct Layout {
   long long site_per_e;  // doubles of site output per energy
   // zslots: L more blocks per energy for the forward substitution chain of the
   // two-pass RGF (first block column from the left-connected inverses)
-  // mixed: the first-column chain and the site-block products run in complex64
+  // mixed: the first-column chain runs in complex64
   void init(int n, int L, int nsites, bool zslots, bool mixed) {
     nn = (long long)n * n;
     long long o = 0;
@@ -1083,10 +1083,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   int r;
   if (c->cfg.precision == NEGF_PREC_MIXED) {
     // ---- mixed-precision two-pass RGF.  FP64: the inverses gL_x and the last block
-    // column G_xL (T12 = Re tr(GamL G1L GamR G1L') stays an FP64 product chain).
+    // column G_xL W_R (T12 = |W_L' G_1L W_R|^2 stays an FP64 product chain).
     // complex64 (3M MFMA): the forward/back substitution for the first block column
-    // and every G*Gamma site-block product -- they feed densities, spin and the
-    // injected modes only, at complex64 accuracy.
+    // G_x1 W_L and the site blocks of both columns -- they feed densities, spin and
+    // the injected modes only, at complex64 accuracy.
     typedef cx<float> zf;
     zf* F0 = (zf*)Q(y.o_f32);
     auto F = [&](long long off) { return F0 + off; };

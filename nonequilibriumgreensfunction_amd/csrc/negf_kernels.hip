@@ -10,7 +10,8 @@
 //                    norb^2 multiply-adds per element instead of 2n;
 //  * k_bd_mul     -- HopX * G and HopX' * G for the block-column recursions
 //                    (Recursive_Greens.F90:61-62, :94-95);
-//  * k_siteblocks -- the norb x norb site-diagonal blocks of B G' (B = G Gamma),
+//  * k_siteblocks -- the norb x norb site-diagonal blocks of B G' (in the solve
+//                    B = G = V = G W with Gamma = W W', so B G' = G Gamma G'),
 //                    reduced to the traces the density (Calculate_Gnp.F90:118-124)
 //                    and spin (Calculate_Observables.F90:10-30) need;
 //  * k_inner      -- Re sum_ij P_ij conj(Q_ij), the trace form of T12 and the
