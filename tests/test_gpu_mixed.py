@@ -45,6 +45,7 @@ def _rel_err(got, ref, scale):
     ((8, 3, 2), dict(disorder=0.5, seed=7, flux=0.4, zeeman=0.03), [-1.7, -0.3, 0.05, 0.25, 1.1]),
     ((12, 4, 4), dict(disorder=0.5, seed=1), [-0.4, -0.1, 0.2, 0.4]),
     ((1, 2, 3), dict(zeeman=0.02), [0.15, 0.9]),
+    ((6, 1, 1), dict(model="metal", params=dict(t=1.0, eps=0.0), disorder=0.3, seed=2), [-0.7, 0.4]),
 ])
 def test_mixed_matches_oracle(case):
     dims, kw, energies = case

@@ -143,3 +143,16 @@ def test_sweep_non_hermitian_spin_matrices():
     acc, st, it = _gpu_sweep(w, energies, 0.01, 1e-6, 1e-10, 200)
     assert np.all(st == 0), st
     _compare(acc, ref, noise, it)
+
+
+@pytest.mark.parametrize("npy", [1, 2])
+def test_sweep_single_orbital_chain(npy):
+    """A one-orbital chain (n = 1) and ladder (n = 2): the smallest blocks, where the
+    factorization of Gamma has one or two columns, in and out of the band."""
+    w = build_wire(6, npy, 1, model="metal", params=dict(t=1.0, eps=0.0), disorder=0.3, seed=2)
+    assert w.n == npy
+    energies = [-2.5, -0.7, 0.4, 1.9]
+    ref, noise = P.reference(w, energies, 0.01, mu=MU, muT=MUT)
+    acc, st, it = _gpu_sweep(w, energies, 0.01, 1e-6, 1e-10, 200)
+    assert np.all(st == 0), st
+    _compare(acc, ref, noise, it)
