@@ -146,10 +146,13 @@ def cpu_baseline(cfg, energy, threads, k_mean):
     return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port", sample=sample)
 
 
-# Committed rocprofv3 --pmc summaries (FETCH_SIZE and WRITE_SIZE passes of the
-# dominant kernel at the shape the bench launches it with, each scaled by the same
-# counter's reading for a copy kernel of known size), per (workload, kernel kind).
-TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md"}
+# Committed summaries per (workload, kernel kind) at the shape the bench launches
+# the kernel with: `alone_tflops` (the kernel alone, bench.py --gemm-only) and,
+# where counters were collected, `traffic_bytes_per_launch` (rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading for
+# a copy kernel of known size).
+TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md",
+                    ("c4", "fp64_gemm_3m"): "alone_fp64_gemm3m_n2304_b2.md"}
 
 
 def profile_value(config, kind, key):
@@ -433,6 +436,10 @@ def main():
     kernels = {k: {"tflops": (f / (m * 1e-3) / 1e12 if m > 0 else 0.0), "launches": c,
                    "avg_launch_ms": m / max(1, c)}
                for k, (f, m, c) in prof.items() if c}
+    for k, v in kernels.items():  # the kernel alone at the bench's shape, where measured
+        a = profile_value(args.config, k, "alone_tflops")
+        if a is not None:
+            v["alone_tflops"] = a
     kname = ("k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form, n x n x n per matrix)"
              if args.precision == "mixed" else
              "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
