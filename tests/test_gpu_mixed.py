@@ -1,9 +1,10 @@
 """GPU parity of the mixed-precision form (NEGF_PREC_MIXED) against the complex128
 oracle.
 
-The mixed form keeps the Sancho-Rubio contacts, the inverses gL_x and the last
-block column -- everything T12 depends on -- in FP64 and runs the first-column
-substitution and the G*Gamma site-block products in complex64.  Tolerances:
+The mixed form keeps the Sancho-Rubio contacts, the factors of Gamma, the inverses
+gL_x and the last block column -- everything T12 depends on -- in FP64 and runs
+the first-column chain (G_x1 W_L) and the site blocks of both columns in
+complex64.  Tolerances:
 T12 to a relative 1e-8 of its natural scale (the open-channel count), or within
 the oracle's own summation-order spread as for the plain form; densities, spin,
 the injected modes and E/HDensity to a relative 2e-5 of their natural scales
