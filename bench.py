@@ -14,12 +14,16 @@ the plain FP64 form is timed on the same energies right after, and the mixed
 form's error against it is reported for every output (the plain form itself is
 pinned to the complex128 CPU oracle by the test suite).
 
-Prints one JSON line.  `roofline` is for the dominant kernel, timed with HIP
-events on the stream it runs on (a uniform sample of its launches inside the
-timed region); its `traffic` (and the kernel's stand-alone rate, `alone`) come
-from the committed rocprofv3 counter summary of that kernel when one covers the
-workload.  `cpu_baseline` times the complex128 CPU oracle (MKL) on the host on a
-bounded sample of the same wire.
+Prints one JSON line.  `roofline` is for the dominant kernel -- the MFMA kernel
+kind with the largest summed duration among the sampled launches (the FP64 3M
+complex product: last-column chain and decimation) -- timed with HIP events on
+the stream it runs on (a uniform sample of its launches inside the timed
+region); its `traffic` (and the kernel's stand-alone rate, `alone`) come from
+the committed rocprofv3 counter summary of that kernel when one covers the
+workload.  `kernels` lists every sampled kind, the Gauss-Jordan panel with its
+achieved bytes/s.  `cpu_baseline` runs the complex128 CPU oracle in the
+reference's layout -- one process per host core on MKL sequential, energies
+round-robin -- on a bounded sample of the same wire (oracle/cpu_baseline.py).
 
 Counter runs: `--gemm-only` runs only the dominant kernel at the shape the sweep
 launches it with; `--copy-calibration MIB` runs a plain copy of known size, whose
@@ -79,71 +83,21 @@ OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", 
            "SigXExp", "SigYExp", "SigZExp")
 
 
-def f_useful(n, L, k_left, k_right):
-    """A fixed yardstick of dense flops per energy point, the three-pass RGF's
-    count: 2L inverses, 2(L-1) column GEMMs, 2L G*Gamma GEMMs, 1 for T12 and
-    (2 + 9K) per lead for Sancho-Rubio, each n^3 complex multiply-adds = 8 n^3
-    flops.  The solve does less than this (L inverses, no G*Gamma products: the
-    column chains carry the factor of Gamma), so the rate it gives rises with
-    such savings."""
-    return 8.0 * n**3 * (6 * L + 3 + 9 * (k_left + k_right))
-
-
-def cpu_baseline(cfg, energy, threads, k_mean):
-    """The complex128 CPU oracle (MKL, the reference's operation order) timed on the
-    host on a bounded sample of one energy of the same wire, extrapolated to the
-    workload's Npx.  The time of one energy is 2 x (Sancho-Rubio of one lead) +
-    Npx x (per-slice RGF, Gn/Gp and observables).  For blocks up to 1024 a whole
-    energy at Npx = 2 and 4 is timed and the line through them extrapolated; for
-    larger blocks the pieces are timed: one lead's decimation at 1 and 3
-    iterations (per-iteration and fixed cost, scaled to the GPU run's mean
-    iteration count k_mean) and the RGF/Gnp/observables stages at Npx = 2."""
+def cpu_baseline(mp_ctx, cfg, energies, k_mean, ranks, mode):
+    """The complex128 CPU oracle in the reference's run layout (oracle/cpu_baseline.py):
+    `ranks` processes on MKL sequential, energies round-robin.  mode "auto" solves
+    whole energies end to end when one takes at most ~30 s per core (blocks up to
+    256), and otherwise times the pieces of one energy at the workload's block
+    size and composes the energy from them."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # noqa: E402  (checker only)
-    O.set_threads(threads)
+    import cpu_baseline as CB  # noqa: E402  (checker only)
     n = cfg["npy"] * cfg["npz"] * 4
-
-    def wire(L):
-        return build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
-
-    if n <= 1024:
-        t = {}
-        for L in (2, 4):
-            w = wire(L)
-            t0 = time.perf_counter()
-            O.energy_sweep(w, energy, 0.0, MU, MUT, ETA, SR_TOL, SR_MAX, True)
-            t[L] = time.perf_counter() - t0
-        per_slice = max(t[4] - t[2], 0.0) / 2.0
-        t_full = t[2] + per_slice * (cfg["npx"] - 2)
-        sample = (f"oracle, 1 energy (E={energy:.4f}), Npy=Npz={cfg['npy']}, MKL {threads} threads, "
-                  f"at Npx=2 ({t[2]:.1f}s) and 4 ({t[4]:.1f}s); extrapolated affinely to "
-                  f"Npx={cfg['npx']}: {t_full:.1f}s per energy")
-        return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port",
-                    sample=sample)
-    w = wire(2)
-    alpha = (energy + 1j * ETA) * np.eye(n) - w.htotd[:, :, 0]
-    beta = -w.hopx.conj().T
-    ts = {}
-    for it in (1, 3):
-        t0 = time.perf_counter()
-        O.sancho_rubio(alpha, beta, ETA, SR_TOL, it, True)
-        ts[it] = time.perf_counter() - t0
-    per_it = max(ts[3] - ts[1], 0.0) / 2.0
-    fixed = max(ts[1] - per_it, 0.0)
-    c = O.contact_self_energy(w, energy, ETA, SR_TOL, 1, True)  # Sigma values do not change the cost
-    t0 = time.perf_counter()
-    A = O.assemble_invg(w, c["SigmaL"], c["SigmaR"], energy, ETA)
-    r = O.retarded(A, w.hopx)
-    g = O.gnp(w, 1.0, 0.0, 0.01, r["GRLef"], r["GRRgt"], c["GammaL"], c["GammaR"])
-    O.observables(w, 1.0, 0.0, 0.01, g, r["GRRgt"], c["GammaL"], c["GammaR"])
-    t_st = time.perf_counter() - t0
-    t_full = 2.0 * (fixed + per_it * k_mean) + t_st / 2.0 * cfg["npx"]
-    sample = (f"oracle pieces at Npy=Npz={cfg['npy']} (n={n}), E={energy:.4f}, MKL {threads} threads: "
-              f"one lead's Sancho-Rubio at 1 and 3 iterations ({ts[1]:.1f}s, {ts[3]:.1f}s -> "
-              f"{per_it:.2f}s/iteration), RGF+Gnp+observables at Npx=2 ({t_st:.1f}s); "
-              f"{ts[1] + ts[3] + t_st:.1f}s of CPU work; one energy = 2 leads x ({fixed:.2f}s + "
-              f"{k_mean:.1f} GPU-measured iterations) + {cfg['npx']} slices = {t_full:.0f}s")
-    return dict(value=1.0 / t_full, unit="energy-points/s", cores=threads, kind="port", sample=sample)
+    if mode == "auto":
+        mode = "full" if n <= 256 else "pieces"
+    spec = dict(npx=cfg["npx"], npy=cfg["npy"], npz=cfg["npz"], W=cfg["W"], seed=cfg["seed"])
+    if mode == "full":
+        energies = energies[:ranks * max(1, len(energies) // ranks)]
+    return CB.run(mp_ctx, spec, ranks, mode, energies, k_mean, ETA, SR_TOL, SR_MAX, MU, MUT)
 
 
 # Committed summaries per (workload, kernel kind) at the shape the bench launches
@@ -153,6 +107,15 @@ def cpu_baseline(cfg, energy, threads, k_mean):
 # a copy kernel of known size).
 TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md",
                     ("c4", "fp64_gemm_3m"): "alone_fp64_gemm3m_n2304_b2.md"}
+GEMM_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update", "fp64_gemm_3m",
+              "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
+KERNEL_NAMES = {
+    "fp64_gemm_3m": "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form)",
+    "fp64_gemm": "k_gemm<double,0,0,0,2,2,0,0> (FP64 MFMA complex product, 4-product form)",
+    "fp64_gj_update_3m": "k_gemm<double,0,1,1,2,2,0,1> (FP64 Gauss-Jordan trailing update, 3M form)",
+    "fp64_gj_update": "k_gemm<double,0,1,1,2,2,*,0> (FP64 Gauss-Jordan trailing update)",
+    "fp32_gemm_3m": "k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form)",
+}
 
 
 def profile_value(config, kind, key):
@@ -175,21 +138,21 @@ def traffic_from_profiles(config, kind):
 
 
 def gemm_only(args):
-    """The dominant kernel alone: the 3M complex product (complex64 in the mixed form,
-    complex128 in the plain form) with the block size of the workload and the batch
-    one lane launches it with (the energies in flight per lane)."""
+    """A product kernel alone: the 3M complex product (FP64, or complex64 with
+    --gemm-c64) with the block size of the workload and the batch one lane
+    launches it with (the energies in flight per lane)."""
     cfg = CONFIGS[args.config]
     n = cfg["npy"] * cfg["npz"] * 4
     per = args.per_step or cfg["per_step"]
     lanes = args.lanes or 4
     batch = max(1, per // lanes)
     reps = 8
-    ms = capi.bench_gemm(n, batch, complex64=(args.precision == "mixed"), m3=True, reps=reps)
+    ms = capi.bench_gemm(n, batch, complex64=args.gemm_c64, m3=True, reps=reps)
     flops = 8.0 * n ** 3 * batch
-    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.precision == "mixed" else "double"),
+    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.gemm_c64 else "double"),
                       "n": n, "batch": batch, "launches": reps + 1, "ms_per_launch": ms,
                       "algorithmic_flops_per_launch": flops, "tflops": flops / ms / 1e9,
-                      "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.precision == "mixed" else 16)}))
+                      "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.gemm_c64 else 16)}))
 
 
 def copy_calibration(mib):
@@ -205,9 +168,10 @@ def copy_calibration(mib):
 
 
 def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmup, profile_every,
-             label):
+             label, keep=1):
     """Warm up, then time `steps` steps of one form; returns timings, the
-    accumulators of the last timed step and the GEMM profile."""
+    accumulators of the last `keep` timed steps (by step index) and the launch
+    profile."""
     L = cfg["npx"]
     prec = capi.NEGF_PREC_MIXED if precision == "mixed" else capi.NEGF_PREC_FP64
     ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
@@ -226,8 +190,7 @@ def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmu
         print(f"[bench] {label} warmup step {k} {time.perf_counter() - t0:.2f}s", file=sys.stderr,
               flush=True)
     capi.profile_begin(profile_every)
-    times, iters, status = [], [], []
-    acc = None
+    times, iters, status, accs = [], [], [], {}
     t_start = time.perf_counter()
     for k in range(warmup, warmup + steps):
         acc = capi.Accumulators(L, cfg["npy"], cfg["npz"], cfg["ne"])
@@ -236,12 +199,15 @@ def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmu
         times.append(time.perf_counter() - t0)
         iters.append(it)
         status.append(st)
+        accs[k] = acc
+        if len(accs) > keep:
+            del accs[min(accs)]
         print(f"[bench] {label} step {k} {times[-1]:.2f}s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     prof = capi.profile_end()
     ctx.close()
-    return dict(times=times, elapsed=elapsed, iters=np.concatenate(iters), status=status, acc=acc,
-                prof=prof, last_step=warmup + steps - 1)
+    return dict(times=times, elapsed=elapsed, iters=np.concatenate(iters), status=status, accs=accs,
+                prof=prof)
 
 
 def errors_vs(acc, ref):
@@ -323,9 +289,9 @@ def server_workload(name, args):
     elapsed = time.perf_counter() - t0
     prof = capi.profile_end()
     srv.close()
-    dominant = "fp32_gemm_3m" if args.precision == "mixed" else "fp64_gemm_3m"
-    peak = PEAK_FP32_MFMA if args.precision == "mixed" else PEAK_FP64_MFMA
-    g_flops, g_ms, g_launches = prof[dominant]
+    dominant = dominant_kind(prof)
+    peak = PEAK_FP32_MFMA if dominant and dominant.startswith("fp32") else PEAK_FP64_MFMA
+    g_flops, g_ms, g_launches = prof[dominant] if dominant else (0.0, 0.0, 0)
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     return {
         "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
@@ -340,10 +306,38 @@ def server_workload(name, args):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "note": "per request, submission to completion (queueing included)"},
         "status_nonzero": int(sum(1 for s in stat if s)),
-        "roofline": {"bound": "mfma", "kernel": f"k_gemm 3M complex product ({dominant})",
+        "roofline": {"bound": "mfma", "kernel": KERNEL_NAMES.get(dominant, dominant),
                      "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s", "frac": gemm_tflops / peak,
                      "traffic": None, "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches)},
+        "kernels": kernel_table(prof, name),
     }
+
+
+def dominant_kind(prof):
+    """The MFMA kernel kind with the largest summed duration among the sampled launches."""
+    best = max(GEMM_KINDS, key=lambda k: prof[k][1])
+    return best if prof[best][2] else None
+
+
+def kernel_table(prof, config):
+    """Per sampled kind: achieved rate (TFLOP/s algorithmic for GEMMs, GB/s for the
+    Gauss-Jordan panels), launches and mean launch time."""
+    out = {}
+    for k, (work, ms, cnt) in prof.items():
+        if not cnt:
+            continue
+        if k.endswith("gj_panel"):
+            out[k] = {"gbytes_per_s": work / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
+                      "hbm_frac": work / (ms * 1e-3) / 8e12 if ms > 0 else 0.0,
+                      "launches": cnt, "avg_launch_ms": ms / cnt,
+                      "note": "bytes: the panel's columns and pivot rows read and written once"}
+            continue
+        out[k] = {"tflops": work / (ms * 1e-3) / 1e12 if ms > 0 else 0.0, "launches": cnt,
+                  "avg_launch_ms": ms / cnt, "summed_sampled_ms": ms}
+        a = profile_value(config, k, "alone_tflops")
+        if a is not None:
+            out[k]["alone_tflops"] = a
+    return out
 
 
 def main():
@@ -359,14 +353,21 @@ def main():
                     help="HBM for energies in flight (default per config; 0 = library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
-                    help="skip the plain-FP64 comparison step of the mixed form")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+                    help="skip the plain-FP64 comparison leg of the mixed form")
+    ap.add_argument("--compare-steps", type=int, default=3,
+                    help="timed steps of the plain-FP64 comparison leg (after the same warmup, "
+                         "capped at one step, on the energies of the last timed mixed steps)")
+    ap.add_argument("--cpu-ranks", type=int, default=0,
+                    help="CPU baseline ranks (0 = one per host core, at most 16)")
+    ap.add_argument("--cpu-baseline-mode", default="auto", choices=("auto", "full", "pieces"))
     ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
     ap.add_argument("--profile-every", type=int, default=32,
-                    help="time one MFMA GEMM launch in this many with HIP events")
+                    help="time one launch in this many of each kind with HIP events")
     ap.add_argument("--gemm-only", action="store_true",
-                    help="run only the dominant kernel (the 3M complex product) at the shape the "
-                         "sweep launches it with, for rocprofv3 --pmc counter runs")
+                    help="run only the 3M complex product kernel at the shape the sweep launches "
+                         "it with, for rocprofv3 --pmc counter runs")
+    ap.add_argument("--gemm-c64", action="store_true",
+                    help="with --gemm-only: the complex64 product (default FP64)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 8 times (the known-size "
                          "reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -387,6 +388,15 @@ def main():
     if args.config in SERVER_WORKLOADS:
         print(json.dumps(server_workload(args.config, args)), flush=True)
         return
+    mp_ctx = None
+    if not args.no_cpu_baseline:
+        # CPU-baseline ranks are forked from a server process started now, before
+        # this process touches the GPU
+        import multiprocessing as mp
+        mp_ctx = mp.get_context("forkserver")
+        mp_ctx.set_forkserver_preload(["numpy"])
+        from multiprocessing import forkserver
+        forkserver.ensure_running()
     cfg = dict(CONFIGS[args.config])
     if args.budget_gb >= 0:
         cfg["budget_gb"] = args.budget_gb
@@ -405,44 +415,43 @@ def main():
     print(f"[bench] building wire {args.config}: Npx={L} Npy=Npz={cfg['npy']} n={n}", file=sys.stderr,
           flush=True)
     w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
+    ncmp = max(1, min(args.compare_steps, args.steps))
     run = run_form(w, cfg, args.precision, args.flags, per, grid, dE, step_indices, args.steps,
-                   args.warmup, args.profile_every, args.precision)
+                   args.warmup, args.profile_every, args.precision, keep=ncmp)
     npts = per * args.steps
     elapsed = run["elapsed"]
     value = npts / elapsed
+    timed_idx = np.unique(np.concatenate([step_indices(k) for k in range(args.warmup, nsteps)]))
     compare = None
     if args.precision == "mixed" and not args.no_compare:
-        # the plain FP64 form on the same energies as the last timed step
-        last = run["last_step"]
-        base = run_form(w, cfg, "fp64", args.flags, per, grid, dE, lambda k: step_indices(last), 1, 0,
-                        args.profile_every, "fp64")
-        plain_rate = per / base["elapsed"]
+        # the plain FP64 form, warmed up the same way (one step at most), on the
+        # energies of the last `ncmp` timed mixed steps
+        last = sorted(run["accs"])
+        base = run_form(w, cfg, "fp64", args.flags, per, grid, dE, lambda k: step_indices(last[k - 1]),
+                        ncmp, min(args.warmup, 1), args.profile_every, "fp64", keep=ncmp)
+        plain_rate = per * ncmp / base["elapsed"]
+        err = {}
+        for j, k in enumerate(last):
+            for o, e in errors_vs(run["accs"][k], base["accs"][min(base["accs"]) + j]).items():
+                err[o] = max(err.get(o, 0.0), e)
         compare = {
             "plain_fp64_energy_points_per_s": plain_rate,
+            "plain_fp64_steps": ncmp,
             "speedup_vs_plain_fp64": value / plain_rate,
-            "error_vs_plain_fp64": errors_vs(run["acc"], base["acc"]),
-            "note": ("errors: max |mixed - fp64| over the last timed step's energies / the output's "
-                     "natural scale; the plain FP64 form agrees with the complex128 CPU oracle to "
-                     "1e-10 in the test suite"),
+            "error_vs_plain_fp64": err,
+            "note": (f"plain FP64 form after {min(args.warmup, 1)} warmup step, {ncmp} timed steps on "
+                     "the energies of the last timed mixed steps; errors: max over those steps of "
+                     "|mixed - fp64| / the output's natural scale; the plain FP64 form agrees with "
+                     "the complex128 CPU oracle to 1e-10 in the test suite"),
         }
 
     iters = run["iters"]
-    useful = sum(f_useful(n, L, a, b) for a, b in iters)
     prof = run["prof"]
-    dominant = "fp32_gemm_3m" if args.precision == "mixed" else "fp64_gemm_3m"
-    peak = PEAK_FP32_MFMA if args.precision == "mixed" else PEAK_FP64_MFMA
-    g_flops, g_ms, g_launches = prof[dominant]
+    dom = dominant_kind(prof)
+    peak = PEAK_FP32_MFMA if dom and dom.startswith("fp32") else PEAK_FP64_MFMA
+    g_flops, g_ms, g_launches = prof[dom] if dom else (0.0, 0.0, 0)
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
-    kernels = {k: {"tflops": (f / (m * 1e-3) / 1e12 if m > 0 else 0.0), "launches": c,
-                   "avg_launch_ms": m / max(1, c)}
-               for k, (f, m, c) in prof.items() if c}
-    for k, v in kernels.items():  # the kernel alone at the bench's shape, where measured
-        a = profile_value(args.config, k, "alone_tflops")
-        if a is not None:
-            v["alone_tflops"] = a
-    kname = ("k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form, n x n x n per matrix)"
-             if args.precision == "mixed" else
-             "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form, n x n x n per matrix)")
+    executed = 0.75 if dom and dom.endswith("_3m") else 1.0
     result = {
         "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
         "value": value,
@@ -459,40 +468,45 @@ def main():
         "config": {"workload": (f"{args.config}: Npx={L}, Npy=Npz={cfg['npy']} (block {n}), "
                                 f"{cfg['ne']} energies in [{cfg['emin']},{cfg['emax']}], "
                                 f"Anderson W={cfg['W']} seed {cfg['seed']}, eta={ETA}, "
-                                f"SR tol {SR_TOL} max {SR_MAX}; {args.precision} form"),
-                   "energies_per_step": per, "precision": args.precision, "flags": args.flags},
+                                f"SR tol {SR_TOL} max {SR_MAX}; {args.precision} form; timed: "
+                                f"{len(timed_idx)} of the {cfg['ne']} grid energies, {per} per step "
+                                f"(strided slices of the grid, shifted each step)"),
+                   "energies_per_step": per, "energies_timed": int(len(timed_idx)),
+                   "precision": args.precision, "flags": args.flags, "lanes": args.lanes or 4},
         "latency_ms": {"p50": 1e3 * float(np.percentile(run["times"], 50)),
                        "p99": 1e3 * float(np.percentile(run["times"], 99)),
                        "note": ("a step's energies complete together: per-energy solve latency = "
                                 "step time")},
         "sr_iters_mean": float(iters.mean()),
         "status_nonzero": int(sum(int(np.count_nonzero(s)) for s in run["status"])),
-        "path_tflops_useful": useful / elapsed / 1e12,
-        "path_frac_fp64_peak": useful / elapsed / 1e12 / PEAK_FP64_MFMA,
-        "roofline": {"bound": "mfma", "kernel": kname,
+        "roofline": {"bound": "mfma", "kernel": KERNEL_NAMES.get(dom, dom),
                      "achieved": gemm_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / peak, "traffic": traffic_from_profiles(args.config, dominant),
+                     "frac": gemm_tflops / peak, "executed_frac": executed * gemm_tflops / peak,
+                     "traffic": traffic_from_profiles(args.config, dom),
                      "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches),
-                     "note": ("achieved = algorithmic 8 n^3 flops per complex product x batch / HIP-event "
-                              "duration on the launching stream (one launch in --profile-every sampled); the "
-                              "3M form executes 6 n^3 on the matrix cores; the lanes run concurrently, so "
-                              "durations include the other lanes' kernels sharing the CUs; traffic: HBM-side "
-                              "bytes per launch from FETCH_SIZE/WRITE_SIZE counter runs of the kernel alone "
-                              "(bench.py --gemm-only), scaled by a copy-kernel calibration")},
-        "kernels": kernels,
+                     "note": ("kernel: the MFMA kind with the largest summed sampled duration; achieved "
+                              "= algorithmic 8 n^3 flops per complex product x batch / HIP-event duration "
+                              "on the launching stream (one launch in --profile-every of each kind); "
+                              "executed_frac counts the 6 n^3 the 3M form runs on the matrix cores; the "
+                              "lanes run concurrently, so durations include other lanes' kernels sharing "
+                              "the CUs; traffic: HBM-side bytes per launch from FETCH_SIZE/WRITE_SIZE "
+                              "counter runs of the kernel alone at this shape (bench.py --gemm-only), "
+                              "scaled by a copy-kernel calibration")},
+        "kernels": kernel_table(prof, args.config),
     }
-    alone = profile_value(args.config, dominant, "alone_tflops")
+    alone = profile_value(args.config, dom, "alone_tflops") if dom else None
     if alone is not None:
         # the same kernel and shape with the GPU to itself (bench.py --gemm-only)
         result["roofline"]["alone"] = {"achieved": alone, "frac": alone / peak,
-                                       "source": "profiles/" + TRAFFIC_PROFILES[(args.config, dominant)]}
+                                       "executed_frac": executed * alone / peak,
+                                       "source": "profiles/" + TRAFFIC_PROFILES[(args.config, dom)]}
     if compare is not None:
         result["compare"] = compare
-    if not args.no_cpu_baseline:
+    if mp_ctx is not None:
         t0 = time.perf_counter()
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(cfg, float(grid[cfg["ne"] // 3]), threads,
-                                              float(iters.mean()))
+        ranks = args.cpu_ranks or min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(mp_ctx, cfg, grid[timed_idx], float(iters.mean()), ranks,
+                                              args.cpu_baseline_mode)
         print(f"[bench] cpu baseline {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     print(json.dumps(result), flush=True)
 

@@ -217,6 +217,19 @@ int negf_invert_matrix(int n, const double* G, double* invG, int* info);
 int negf_zgemm(char transa, char transb, int n, const double* alpha, const double* A, int lda,
                const double* B, int ldb, const double* beta, double* C, int ldc);
 
+/* Parity checks of the sweep's kernels in each form, no counterpart in the
+ * reference (the reference's single form is MKL zgemm / zgetrf+zgetri,
+ * Recursive_Greens.F90:18-95, MKL_Routines_Cluster.F90:19,23).  `batch` n x n
+ * column-major matrices, contiguous; complex64 != 0: COMPLEX(4) pairs of floats,
+ * else COMPLEX(8); m3 != 0: the 3M complex product form (3 real products, normwise
+ * accurate), else the 4-product form.
+ *   negf_gemm_batch:   C_b = A_b B_b
+ *   negf_invert_batch: invG_b = G_b^-1 (the sweep's Gauss-Jordan inverse; m3 selects
+ *                      the form of its trailing updates); info[b] = first zero pivot
+ *                      (1-based) or 0. */
+int negf_gemm_batch(int complex64, int m3, int n, int batch, const void* A, const void* B, void* C);
+int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG, int* info);
+
 /* --- Serving and batching of whole transport solves (no single counterpart in
  * the reference: it runs one wire per process, NEGF_Main.F90:21-38) ----------
  * A server owns one GPU.  Each request is one wire -- its own dimensions,
@@ -272,16 +285,18 @@ int negf_server_wait(negf_server* srv, long long id, int* status, double* latenc
 int negf_server_destroy(negf_server* srv);
 
 /* --- Instrumentation (no counterpart in the reference) ---------------------
- * Time MFMA GEMM launches (the dominant kernel) with HIP events on the stream
- * each is launched on, between begin and end: one launch in sample_every, in
- * launch order (an event pair per launch costs host time and stream markers, so
- * a uniform sample keeps the timed run close to an untimed one; 1 = every
- * launch).  end() synchronises the
- * device and returns, per kind, the launches' algorithmic flops (8 m n k per
- * complex product), their summed duration in ms and the launch count; each
- * argument is an array of 8 indexed by 4 * (3M complex form) + 2 * (FP32) +
- * (Gauss-Jordan trailing update): [0] FP64 product, [1] FP64 update, [2] FP32
- * product, [3] FP32 update, [4..7] the same in the 3M form. */
+ * Time the MFMA GEMM launches (the dominant kernels) and the Gauss-Jordan panel
+ * launches with HIP events on the stream each is launched on, between begin and
+ * end: one launch in sample_every of each kind, in launch order (an event pair
+ * per launch costs host time and stream markers, so a uniform sample keeps the
+ * timed run close to an untimed one; 1 = every launch).  end() synchronises the
+ * device and returns, per kind, the sampled launches' algorithmic work, their
+ * summed duration in ms and their count; each argument is an array of 10:
+ * [0..7] GEMMs indexed by 4 * (3M complex form) + 2 * (FP32) + (Gauss-Jordan
+ * trailing update) -- [0] FP64 product, [1] FP64 update, [2] FP32 product,
+ * [3] FP32 update, [4..7] the same in the 3M form -- with work = 8 m n k flops per
+ * complex product; [8] FP64 and [9] FP32 Gauss-Jordan panels, with work = the
+ * bytes the panel's columns and pivot rows move (read and written once). */
 int negf_profile_begin(int sample_every);
 int negf_profile_end(double* flops, double* ms, long long* launches);
 

@@ -35,7 +35,8 @@ EXPORTS = (
     "negf_zgemm", "negf_profile_begin", "negf_profile_end", "negf_energy_sweep_f", "negf_flush_f",
     "negf_observables_f", "negf_set_hamiltonian_compact", "negf_set_onsite", "negf_contact_cache_stats",
     "negf_server_create", "negf_server_submit", "negf_server_poll",
-    "negf_server_wait", "negf_server_destroy", "negf_bench_gemm",
+    "negf_server_wait", "negf_server_destroy", "negf_bench_gemm", "negf_gemm_batch",
+    "negf_invert_batch",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -109,6 +110,8 @@ def lib():
         L.negf_zgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                  _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.negf_profile_begin.argtypes = [ctypes.c_int]
+        L.negf_gemm_batch.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3
+        L.negf_invert_batch.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [_ip]
         L.negf_server_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(NegfServerConfig)]
         L.negf_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(NegfRequest),
                                          ctypes.POINTER(ctypes.c_longlong)]
@@ -151,12 +154,14 @@ def profile_begin(sample_every=1):
 
 
 PROFILE_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update",
-                 "fp64_gemm_3m", "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
+                 "fp64_gemm_3m", "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m",
+                 "fp64_gj_panel", "fp32_gj_panel")
 
 
 def profile_end():
-    """{kind: (flops, ms, launches)} of the MFMA GEMM launches since profile_begin(),
-    kinds as in PROFILE_KINDS."""
+    """{kind: (work, ms, launches)} of the sampled launches since profile_begin(),
+    kinds as in PROFILE_KINDS; work is algorithmic flops for the GEMM kinds and
+    algorithmic bytes for the Gauss-Jordan panel kinds."""
     nk = len(PROFILE_KINDS)
     f = (ctypes.c_double * nk)()
     m = (ctypes.c_double * nk)()
@@ -172,6 +177,35 @@ def bench_gemm(n, batch, complex64=True, m3=True, reps=8, device=0):
     _check(lib().negf_bench_gemm(int(device), int(bool(complex64)), int(n), int(batch), int(reps),
                                  int(bool(m3)), ctypes.byref(ms)))
     return ms.value
+
+
+def gemm_batch(A, B, complex64=False, m3=True):
+    """C_b = A_b B_b for a stack A, B of shape (batch, n, n) with the sweep's product
+    kernel in the given form (negf_gemm_batch); returns the stack of C_b."""
+    dt = np.complex64 if complex64 else np.complex128
+    batch, n, _ = A.shape
+    # (batch, n, n) with each matrix column-major: transpose the last two axes
+    a = np.ascontiguousarray(np.swapaxes(A, 1, 2), dtype=dt)
+    b = np.ascontiguousarray(np.swapaxes(B, 1, 2), dtype=dt)
+    c = np.zeros_like(a)
+    _check(lib().negf_gemm_batch(int(bool(complex64)), int(bool(m3)), n, batch,
+                                 a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                                 c.ctypes.data_as(ctypes.c_void_p)))
+    return np.swapaxes(c, 1, 2)
+
+
+def invert_batch(G, complex64=False, m3=True):
+    """(inverses, info) of a stack G of shape (batch, n, n) with the sweep's
+    Gauss-Jordan inverse in the given form (negf_invert_batch)."""
+    dt = np.complex64 if complex64 else np.complex128
+    batch, n, _ = G.shape
+    g = np.ascontiguousarray(np.swapaxes(G, 1, 2), dtype=dt)
+    out = np.zeros_like(g)
+    info = np.zeros(batch, np.int32)
+    _check(lib().negf_invert_batch(int(bool(complex64)), int(bool(m3)), n, batch,
+                                   g.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                                   info.ctypes.data_as(_ip)))
+    return np.swapaxes(out, 1, 2), info
 
 
 def device_count() -> int:

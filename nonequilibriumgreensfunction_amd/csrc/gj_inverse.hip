@@ -307,6 +307,12 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
 #define NEGF_PANEL(RR)                                                                             \
   hipLaunchKernelGGL((k_gj_panel<T, RR, NT>), dim3(batch), dim3(NT), 0, s, S, sS, n, ld, c0, nb, k0, \
                      kb, w, active)
+      // sampled timing: the panel's columns read and written, and its pivot rows'
+      // values in the outer block copied out (nb x kb elements read and written)
+      hipEvent_t ev0, ev1;
+      const double bytes = 2.0 * sizeof(cx<T>) * nb * ((double)n + kb) * batch;
+      const bool timed = timer_slot(bytes, kTimerPanel64 + (sizeof(T) == sizeof(float) ? 1 : 0), &ev0, &ev1);
+      if (timed && hipEventRecord(ev0, s) != hipSuccess) return hipGetLastError();
       switch (R) {
         case 1: NEGF_PANEL(1); break;
         case 2: NEGF_PANEL(2); break;
@@ -316,6 +322,7 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
         default: NEGF_PANEL(6); break;
       }
 #undef NEGF_PANEL
+      if (timed && hipEventRecord(ev1, s) != hipSuccess) return hipGetLastError();
       if (kb > nb)  // bring the rest of the outer block up to date (rank nb)
         hipLaunchKernelGGL((k_gj_update_block<T>), dim3((n + 255) / 256, (kb + 15) / 16, batch), dim3(256),
                            0, s, S, sS, n, ld, c0, nb, k0, kb, w, active);

@@ -109,15 +109,18 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
                      int ldo, int n, int batch, const GjWork& w, const int* active);
 constexpr int kGjNBO = 128;  // Gauss-Jordan outer block width (columns per trailing update)
 
-// Opt-in timing of the MFMA GEMM launches (the dominant kernel): when enabled,
-// every launch is bracketed by HIP events on its own stream and its algorithmic
-// flop count (8 m n k per complex product) is recorded with its kind
-// = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing update).
-constexpr int kTimerKinds = 8;
+// Opt-in timing of the MFMA GEMM launches (the dominant kernels) and of the
+// Gauss-Jordan panel launches: when enabled, one launch in `every` of each kind is
+// bracketed by HIP events on its own stream and its algorithmic work is recorded
+// with its kind.  Kinds 0..7 = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing
+// update), work = 8 m n k flops per complex product; kinds 8, 9 = the FP64 / FP32
+// Gauss-Jordan panel, work = the bytes its columns move (read and written once).
+constexpr int kTimerKinds = 10;
+constexpr int kTimerPanel64 = 8;
 struct GemmTimer {
   bool on = false;
-  int every = 1;         // time one launch in `every` (uniform sample)
-  unsigned long long seen = 0;
+  int every = 1;         // time one launch in `every` of each kind (uniform sample)
+  unsigned long long seen[kTimerKinds] = {};
   size_t used = 0;
   hipEvent_t* ev = nullptr;  // 2 per launch
   double* flops = nullptr;
@@ -125,6 +128,9 @@ struct GemmTimer {
   size_t cap = 0;
 };
 GemmTimer& gemm_timer();
+// Reserves an event pair for a sampled launch of `kind` with algorithmic `work`;
+// false when this launch is not sampled (or timing is off).
+bool timer_slot(double work, int kind, hipEvent_t* e0, hipEvent_t* e1);
 
 }  // namespace negf
 

@@ -303,6 +303,10 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
     negf_set_error("unknown precision form %d", cfg->precision);
     return NEGF_ERR_ARG;
   }
+  if (cfg->precision == NEGF_PREC_MIXED && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    negf_set_error("the mixed-precision form runs on the two-pass RGF only");
+    return NEGF_ERR_ARG;
+  }
   int rc = require_device(cfg->device);
   if (rc) return rc;
   negf_ctx* c = new negf_ctx();
@@ -311,10 +315,6 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
   c->n = n;
   c->L = cfg->npx;
   c->nsites = cfg->npy * cfg->npz;
-  if (cfg->precision == NEGF_PREC_MIXED && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
-    negf_set_error("the mixed-precision form runs on the two-pass RGF only");
-    return NEGF_ERR_ARG;
-  }
   c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS),
               cfg->precision == NEGF_PREC_MIXED);
   // Four lanes by default: at n >= 1024 the inverse chains are latency-bound
@@ -1530,12 +1530,95 @@ extern "C" int negf_zgemm(char transa, char transb, int n, const double* alpha, 
   return 0;
 }
 
+// Batched product / inverse of the sweep's kernels on host matrices, in a chosen
+// form (complex64 or complex128; 3M or 4-product complex products).  One-shot
+// device buffers; for parity checks of each form at the sweep's block sizes.
+template <typename T>
+static int gemm_batch_host(int m3, int n, int batch, const void* A, const void* B, void* C) {
+  typedef cx<T> V;
+  const size_t bytes = sizeof(V) * (size_t)n * n * batch;
+  V *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  HIPCHK(hipMalloc(&dA, bytes));
+  HIPCHK(hipMalloc(&dB, bytes));
+  HIPCHK(hipMalloc(&dC, bytes));
+  HIPCHK(hipMemcpy(dA, A, bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice));
+  GemmBatch g = {};
+  g.A = dA; g.B = dB; g.C = dC;
+  g.sA = g.sB = g.sC = (long long)n * n;
+  g.lda = g.ldb = g.ldc = n;
+  g.m = g.n = g.k = n;
+  g.m3 = m3 ? 1 : 0;
+  HIPCHK(gemm_batched<T>(nullptr, OP_N, OP_N, g, batch, mk<T>(1, 0), mk<T>(0, 0)));
+  HIPCHK(hipMemcpy(C, dC, bytes, hipMemcpyDeviceToHost));
+  hipFree(dA); hipFree(dB); hipFree(dC);
+  return 0;
+}
+
+template <typename T>
+static int invert_batch_host(int m3, int n, int batch, const void* G, void* invG, int* info) {
+  typedef cx<T> V;
+  const long long nn = (long long)n * n;
+  const size_t bytes = sizeof(V) * (size_t)nn * batch;
+  const long long sInt = 2LL * n + 1;
+  V *dA = nullptr, *dO = nullptr, *prow = nullptr;
+  int* ints = nullptr;
+  HIPCHK(hipMalloc(&dA, bytes));
+  HIPCHK(hipMalloc(&dO, bytes));
+  HIPCHK(hipMalloc(&prow, sizeof(V) * (size_t)kGjNBO * n * batch));
+  HIPCHK(hipMalloc(&ints, sizeof(int) * (size_t)sInt * batch));
+  HIPCHK(hipMemcpy(dA, G, bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(ints, 0, sizeof(int) * (size_t)sInt * batch));
+  GjWork w;
+  w.prow = prow;
+  w.sProw = (long long)kGjNBO * n;
+  w.pivstep = ints;
+  w.pivrow = ints + n;
+  w.info = ints + 2 * n;
+  w.sInt = sInt;
+  w.m3 = m3 ? 1 : 0;
+  HIPCHK(gj_invert<T>(nullptr, dA, nn, n, dO, nn, n, n, batch, w, nullptr));
+  HIPCHK(hipMemcpy(invG, dO, bytes, hipMemcpyDeviceToHost));
+  std::vector<int> h((size_t)sInt * batch);
+  HIPCHK(hipMemcpy(h.data(), ints, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+  for (int b = 0; b < batch; ++b)
+    if (info) info[b] = h[(size_t)b * sInt + 2 * n];
+  hipFree(dA); hipFree(dO); hipFree(prow); hipFree(ints);
+  return 0;
+}
+
+static int batch_args_ok(int n, int batch, const void* a, const void* b, const void* c) {
+  if (n <= 0 || n > 3072 || batch <= 0 || !a || !b || !c) {
+    negf_set_error("bad arguments (n=%d must be 1..3072, batch=%d >= 1, non-null arrays)", n, batch);
+    return NEGF_ERR_ARG;
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  return require_device(dev);
+}
+
+extern "C" int negf_gemm_batch(int complex64, int m3, int n, int batch, const void* A, const void* B,
+                               void* C) {
+  int rc = batch_args_ok(n, batch, A, B, C);
+  if (rc) return rc;
+  return complex64 ? gemm_batch_host<float>(m3, n, batch, A, B, C)
+                   : gemm_batch_host<double>(m3, n, batch, A, B, C);
+}
+
+extern "C" int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG,
+                                 int* info) {
+  int rc = batch_args_ok(n, batch, G, invG, G);
+  if (rc) return rc;
+  return complex64 ? invert_batch_host<float>(m3, n, batch, G, invG, info)
+                   : invert_batch_host<double>(m3, n, batch, G, invG, info);
+}
+
 // ---------------------------------------------------------------------------
 // Instrumentation: MFMA GEMM launch timing (no counterpart in the reference).
 extern "C" int negf_profile_begin(int sample_every) {
   GemmTimer& t = gemm_timer();
   t.used = 0;
-  t.seen = 0;
+  for (unsigned long long& c : t.seen) c = 0;
   t.every = sample_every > 1 ? sample_every : 1;
   t.on = true;
   return 0;

@@ -63,7 +63,9 @@ void bind_lane(Lane& ln, const Layout& y, int nb, char* pool) {
 // Device footprint of an admitted request beyond the lane pools.
 size_t resident_bytes(int n, int L, int nsites, int precision, bool compact) {
   const size_t nn = (size_t)n * n;
-  size_t b = 4 * sizeof(zd) * nn;  // X, betaL, betaR, tmpP
+  // X, betaL, betaR, and tmpP (2 n x n, allocated only for a dense HopX; counted
+  // always, since admission runs before HopX is inspected)
+  size_t b = 5 * sizeof(zd) * nn;
   if (compact)
     b += 3 * sizeof(zd) * nn + sizeof(double) * (size_t)L * nsites;  // H0, two end slices, V
   else
@@ -131,6 +133,7 @@ int admit(negf_server* s, SrvRequest* r) {
   cfg.precision = s->cfg.precision;
   cfg.device = s->cfg.device;
   cfg.flags = s->cfg.flags;
+  cfg.lanes = 1;  // the request context only carries state: the server's lanes run its energies
   int rc = negf_init(&r->ctx, &cfg);
   if (rc) return rc;
   if (!r->req.Htotd)
@@ -226,6 +229,20 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
   *out = nullptr;
   if (cfg->lanes < 0 || cfg->lanes > kMaxLanes) {
     negf_set_error("server lanes must be 1..%d (0 = %d)", kMaxLanes, kMaxLanes);
+    return NEGF_ERR_ARG;
+  }
+  if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED) {
+    negf_set_error("unknown precision form %d", cfg->precision);
+    return NEGF_ERR_ARG;
+  }
+  if (cfg->precision == NEGF_PREC_MIXED && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    negf_set_error("the mixed-precision form runs on the two-pass RGF only");
+    return NEGF_ERR_ARG;
+  }
+  // A request's contact cache would live outside the footprint admission counts,
+  // and a request is solved once: the cache has nothing to serve.
+  if (cfg->flags & NEGF_FLAG_CACHE_CONTACTS) {
+    negf_set_error("NEGF_FLAG_CACHE_CONTACTS is not supported by the server");
     return NEGF_ERR_ARG;
   }
   int rc = require_device(cfg->device);

@@ -16,7 +16,7 @@
 // Tiles of op(A) (64x16) and op(B) (16x64) are staged through LDS as split
 // real/imaginary planes, k-major, so that each MFMA operand read is 16
 // consecutive LDS words; the next K slab is fetched into registers while the
-// current one is multiplied.
+// current one is multiplied (LDS: 34 KiB for complex64, 51 KiB for FP64).
 //
 // This replaces the reference's MKL zgemm call sites
 // (Recursive_Greens.F90:18-95, Contact_Self_Energy.F90:147-209): square
@@ -86,7 +86,8 @@ __device__ inline int xcd_tile(int bid, int tiles_m, int ntiles) {
 // Workgroup tile (32 SM) x (32 SN): 2 x 2 waves, each owning SM x SN MFMA blocks of
 // 16 x 16.  K advances in slabs of 16 through a two-stage LDS ring: the next slab
 // is fetched into registers and written to the idle stage while the current
-// stage feeds the MFMAs, so each slab costs one barrier.
+// stage feeds the MFMAs, so each slab costs one barrier (two in FP64, whose B
+// tile has one stage).
 // PRE = 1 (Gauss-Jordan epilogue with short K): the old C values and pivot flags
 // are loaded before the K loop, so their latency overlaps the A/B traffic.
 // Two workgroups per CU (launch bound): the FP64 64x64 variant then fits 199 VGPRs
@@ -112,8 +113,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
   cx<T>* C = (cx<T>*)g.C + (long long)b * g.sC;
   const int K = g.k, M = g.m, N = g.n;
 
-  __shared__ T sA[2][2][kBK][BM + kPad];  // [stage][re/im][k][m]
-  __shared__ T sB[2][2][kBK][BN + kPad];  // [stage][re/im][k][n]
+  // B is single-buffered in the FP64 variant: two stages of both tiles would take
+  // 68 KiB of LDS; one B stage keeps the kernel at 51 KiB (under 64 KiB) for the
+  // cost of a second barrier per slab.
+  constexpr int SB = sizeof(T) == sizeof(double) ? 1 : 2;
+  __shared__ T sA[2][2][kBK][BM + kPad];   // [stage][re/im][k][m]
+  __shared__ T sB[SB][2][kBK][BN + kPad];  // [stage][re/im][k][n]
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -176,6 +181,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
     }
   };
   auto stash = [&](int st) {
+    const int sb = SB == 1 ? 0 : st;
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int c = tid + kThreads * i;
@@ -189,8 +195,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
       const int c = tid + kThreads * i;
       int nn, kk;
       if (OPB == OP_N) { kk = c % kBK; nn = c / kBK; } else { nn = c % BN; kk = c / BN; }
-      sB[st][0][kk][nn] = rb[i].x;
-      sB[st][1][kk][nn] = rb[i].y;
+      sB[sb][0][kk][nn] = rb[i].x;
+      sB[sb][1][kk][nn] = rb[i].y;
     }
   };
 
@@ -202,6 +208,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int st = kt & 1;
+    const int sb = SB == 1 ? 0 : st;
     if (kt + 1 < nk) fetch((kt + 1) * kBK);
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
@@ -215,8 +222,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
       }
 #pragma unroll
       for (int s = 0; s < SN; ++s) {
-        br[s] = sB[st][0][kk][wn + 16 * s + lr];
-        bi[s] = sB[st][1][kk][wn + 16 * s + lr];
+        br[s] = sB[sb][0][kk][wn + 16 * s + lr];
+        bi[s] = sB[sb][1][kk][wn + 16 * s + lr];
         bs[s] = br[s] + bi[s];
       }
       if (M3) {
@@ -246,6 +253,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
           }
       }
     }
+    if (SB == 1 && kt + 1 < nk) __syncthreads();  // every wave is done with the B stage
     if (kt + 1 < nk) stash(st ^ 1);
     __syncthreads();
   }
@@ -289,11 +297,11 @@ static std::mutex g_timer_mu;
 
 // Reserves an event pair for one launch; the handles are copied out under the lock
 // (the event array may grow while another host thread launches).
-static bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
+bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
   GemmTimer& t = gemm_timer();
   if (!t.on) return false;
   std::lock_guard<std::mutex> lock(g_timer_mu);
-  if (t.seen++ % (unsigned long long)t.every != 0) return false;
+  if (t.seen[kind]++ % (unsigned long long)t.every != 0) return false;
   if (t.used == t.cap) {
     size_t nc = t.cap ? 2 * t.cap : 4096;
     hipEvent_t* ev = (hipEvent_t*)realloc(t.ev, sizeof(hipEvent_t) * 2 * nc);

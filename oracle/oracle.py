@@ -34,7 +34,10 @@ def _p(a):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "_build", "libnegf_oracle.so")
+        # NEGF_ORACLE_MKL=sequential: the build linked with MKL sequential, as the
+        # reference links it (Implement/Makefile:3), for one-rank-per-core runs
+        seq = os.environ.get("NEGF_ORACLE_MKL", "") == "sequential"
+        path = os.path.join(_HERE, "_build", "libnegf_oracle_seq.so" if seq else "libnegf_oracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
         _LIB = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)

@@ -87,10 +87,11 @@ def scales(ref):
     return sc
 
 
-def reference(w, energies, dE, **kw):
-    """(oracle result, absolute noise floor per output element)."""
+def reference(w, energies, dE, nperm=2, **kw):
+    """(oracle result, absolute noise floor per output element); the floor is the
+    spread over `nperm` (1 or 2) orbital permutations of the wire."""
     a = oracle_sweep(w, energies, dE, **kw)
-    others = [oracle_sweep(permuted(w, i), energies, dE, **kw) for i in range(2)]
+    others = [oracle_sweep(permuted(w, i), energies, dE, **kw) for i in range(nperm)]
     sc = scales(a)
     ne = len(energies)
     rel_e = np.zeros(ne)

@@ -42,3 +42,15 @@ def test_traffic_summary_parses_and_exceeds_algorithmic_bytes():
         assert t >= alg, (t, alg)
     assert b.traffic_from_profiles("c4", "fp32_gemm_3m") is not None
     assert b.traffic_from_profiles("c2", "fp64_gemm_3m") is None
+
+
+def test_dominant_kind_is_largest_summed_duration():
+    b = _bench()
+    prof = {k: (0.0, 0.0, 0) for k in b.GEMM_KINDS}
+    prof.update(fp64_gj_panel=(1e9, 500.0, 100), fp32_gj_panel=(0.0, 0.0, 0))
+    prof["fp32_gemm_3m"] = (1e15, 900.0, 300)
+    prof["fp64_gemm_3m"] = (1e15, 1800.0, 200)
+    assert b.dominant_kind(prof) == "fp64_gemm_3m"
+    t = b.kernel_table(prof, "c4")
+    assert "gbytes_per_s" in t["fp64_gj_panel"] and "tflops" in t["fp64_gemm_3m"]
+    assert "fp32_gj_panel" not in t

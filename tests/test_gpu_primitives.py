@@ -82,3 +82,44 @@ def test_bench_gemm_instrumentation_runs(complex64, m3):
     # and report a positive launch time in every form
     ms = capi.bench_gemm(256, 3, complex64=complex64, m3=m3, reps=2)
     assert ms > 0.0
+
+
+# The sweep's product kernel in each of its forms at the sweep's block sizes and
+# batches: FP64 3M (RGF chains, trailing updates of the inverse, the mixed form's
+# decimation), complex64 3M (the mixed form's first-column chain) and complex64
+# 4-product, against numpy complex128.  Bounds: normwise for the 3M form, whose
+# error is O(eps (|Re A|+|Im A|)(|Re B|+|Im B|)) per term, i.e. c * n * eps * |A||B|
+# with |.| the elementwise sums |Re|+|Im|; componentwise for the 4-product form.
+@pytest.mark.parametrize("n,batch", [(1024, 2), (1024, 8), (2304, 2), (2304, 8)])
+@pytest.mark.parametrize("form", ["fp64_3m", "c64_3m", "c64_4p"])
+def test_gemm_forms_at_sweep_sizes(rng, n, batch, form):
+    print(f"gemm form {form} n={n} batch={batch}", flush=True)
+    c64 = form.startswith("c64")
+    m3 = form.endswith("3m")
+    dt = np.complex64 if c64 else np.complex128
+    A = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))).astype(dt)
+    B = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))).astype(dt)
+    got = capi.gemm_batch(A, B, complex64=c64, m3=m3)
+    eps = np.finfo(np.float32 if c64 else np.float64).eps
+    for b in (0, batch - 1):  # first and last matrix of the batch
+        a, bb = A[b].astype(np.complex128), B[b].astype(np.complex128)
+        ref = a @ bb
+        mag = (np.abs(a.real) + np.abs(a.imag)) @ (np.abs(bb.real) + np.abs(bb.imag))
+        err = np.abs(got[b] - ref)
+        assert np.all(err <= 4 * n * eps * mag), (form, b, float((err / mag).max() / eps))
+
+
+@pytest.mark.parametrize("n,batch", [(1024, 2), (2304, 2)])
+@pytest.mark.parametrize("m3", [1, 0], ids=["3m", "4p"])
+def test_invert_forms_at_sweep_sizes(rng, n, batch, m3):
+    # the FP64 Gauss-Jordan inverse with 3M (RGF, mixed-form decimation) and 4-product
+    # (plain-form decimation) trailing updates, against LAPACK
+    print(f"invert m3={m3} n={n} batch={batch}", flush=True)
+    G = np.stack([_rand(rng, n) + 2.0 * np.sqrt(n) * np.eye(n) for _ in range(batch)])
+    X, info = capi.invert_batch(G, complex64=False, m3=bool(m3))
+    assert np.all(info == 0)
+    for b in range(batch):
+        ref = np.linalg.inv(G[b])
+        rel = np.abs(X[b] - ref).max() / np.abs(ref).max()
+        assert rel < 1e-12, (b, rel)
+        assert np.abs(G[b] @ X[b] - np.eye(n)).max() < 1e-11
