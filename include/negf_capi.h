@@ -90,12 +90,23 @@ typedef struct {
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
- * reference does. */
+ * reference does.  The bond currents and the energy-/orbital-resolved arrays
+ * (Calculate_Observables.F90:111-168, Calculate_Gnp.F90:125-130; the reference
+ * fills the resolved ones when erCD / orCD = 1) are filled by the fused entry
+ * points (negf_energy_sweep, negf_flush, the server); negf_observables takes its
+ * bond currents as separate arguments.  Currents accumulate in REAL(8) (the
+ * reference passes each term through a REAL(4), Calculate_Observables.F90:7).
+ * Orbital pair (ii, jj), 0-based, of the OR current arrays is index ii*Norb + jj
+ * ((ii-1)*Norb + jj 1-based in the reference). */
 typedef struct {
   double *T12, *Modec1, *Modec2, *DOSc1, *DOSc2, *EDensity, *HDensity; /* [eStpnum] at eCtr */
   double *RhoN, *RhoP;                                                  /* (Npx,Npy,Npz) */
   double *SigXExp, *SigYExp, *SigZExp;                                  /* (Npx,Npy,Npz) */
   double *ContactCurrent;                                               /* [2] */
+  double *Jx, *Jy, *Jz;                                                 /* (Npx,Npy,Npz) */
+  double *ERJx, *ERJy, *ERJz, *ERRhon;  /* (Npx,Npy,Npz,eStpnum), at eCtr; unweighted by dE/2pi */
+  double *ORJx, *ORJy, *ORJz;           /* (Npx,Npy,Npz,Norb*Norb) */
+  double *ORRhon;                       /* (Npx,Npy,Npz,Norb) */
 } negf_accum;
 
 const char* negf_last_error(void);
@@ -187,16 +198,21 @@ int negf_observables(negf_ctx* ctx, int eCtr, double f1, double f2, double delta
  * The same operations with the accumulators passed as separate arrays, so a
  * Fortran caller can pass its (non-TARGET) module arrays by reference.  Bit k of
  * `mask` marks the k-th array argument present, in negf_accum's field order
- * (0 T12 ... 12 ContactCurrent); for negf_observables_f bits 13-15 mark Jx, Jy, Jz. */
+ * (0 T12 ... 12 ContactCurrent, 13-15 Jx Jy Jz, 16-19 ERJx ERJy ERJz ERRhon,
+ * 20-23 ORJx ORJy ORJz ORRhon); negf_observables_f takes fields 0-15. */
 int negf_energy_sweep_f(negf_ctx* ctx, double energy, int eCtr, double deltaE, const double* mu,
                         double muT, int mask, double* T12, double* Modec1, double* Modec2,
                         double* DOSc1, double* DOSc2, double* EDensity, double* HDensity,
                         double* RhoN, double* RhoP, double* SigXExp, double* SigYExp,
-                        double* SigZExp, double* ContactCurrent);
+                        double* SigZExp, double* ContactCurrent, double* Jx, double* Jy,
+                        double* Jz, double* ERJx, double* ERJy, double* ERJz, double* ERRhon,
+                        double* ORJx, double* ORJy, double* ORJz, double* ORRhon);
 int negf_flush_f(negf_ctx* ctx, const double* mu, double muT, int mask, double* T12,
                  double* Modec1, double* Modec2, double* DOSc1, double* DOSc2, double* EDensity,
                  double* HDensity, double* RhoN, double* RhoP, double* SigXExp, double* SigYExp,
-                 double* SigZExp, double* ContactCurrent);
+                 double* SigZExp, double* ContactCurrent, double* Jx, double* Jy, double* Jz,
+                 double* ERJx, double* ERJy, double* ERJz, double* ERRhon, double* ORJx,
+                 double* ORJy, double* ORJz, double* ORRhon);
 int negf_observables_f(negf_ctx* ctx, int eCtr, double f1, double f2, double deltaE,
                        const double* Gnd, const double* Gpd, const double* Gnu, const double* Gnl,
                        const double* Gpu, const double* Gpl, const double* GRRgt,

@@ -60,7 +60,8 @@ FLAG_CACHE_CONTACTS = 8
 class NegfAccum(ctypes.Structure):
     _fields_ = [(k, _dp) for k in ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity",
                                    "HDensity", "RhoN", "RhoP", "SigXExp", "SigYExp", "SigZExp",
-                                   "ContactCurrent")]
+                                   "ContactCurrent", "Jx", "Jy", "Jz", "ERJx", "ERJy", "ERJz",
+                                   "ERRhon", "ORJx", "ORJy", "ORJz", "ORRhon")]
 
 
 class NegfServerConfig(ctypes.Structure):
@@ -232,22 +233,35 @@ def zgemm(ta, tb, A, B, alpha=1.0, beta=0.0, C=None):
 
 
 class Accumulators:
-    """The reference's energy-integrated outputs (NEGF_Module.F90:51-97)."""
+    """The reference's energy-integrated outputs (NEGF_Module.F90:51-100).
+
+    currents: the bond currents Jx, Jy, Jz (Calculate_Observables.F90:111-168), filled
+    by the fused entry points; resolved: also the energy-resolved (ERJx.., ERRhon,
+    shape (Npx,Npy,Npz,eStpnum)) and orbital-resolved (ORJx.. (Npx,Npy,Npz,Norb^2),
+    ORRhon (Npx,Npy,Npz,Norb)) arrays the reference fills with erCD / orCD = 1."""
 
     PER_ENERGY = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity")
     SPATIAL = ("RhoN", "RhoP", "SigXExp", "SigYExp", "SigZExp")
+    CURRENTS = ("Jx", "Jy", "Jz")
+    RESOLVED = ("ERJx", "ERJy", "ERJz", "ERRhon", "ORJx", "ORJy", "ORJz", "ORRhon")
 
-    def __init__(self, npx, npy, npz, estpnum):
+    def __init__(self, npx, npy, npz, estpnum, currents=True, resolved=False, norb=4):
         for k in self.PER_ENERGY:
             setattr(self, k, np.zeros(estpnum))
         for k in self.SPATIAL:
             setattr(self, k, np.zeros((npx, npy, npz), order="F"))
         self.ContactCurrent = np.zeros(2)
+        for k in self.CURRENTS:
+            setattr(self, k, np.zeros((npx, npy, npz), order="F") if currents else None)
+        for k in self.RESOLVED:
+            last = estpnum if k.startswith("ER") else (norb if k == "ORRhon" else norb * norb)
+            setattr(self, k, np.zeros((npx, npy, npz, last), order="F") if resolved else None)
 
     def struct(self):
         s = NegfAccum()
-        for k in self.PER_ENERGY + self.SPATIAL + ("ContactCurrent",):
-            setattr(s, k, _p(getattr(self, k)))
+        for k in self.PER_ENERGY + self.SPATIAL + ("ContactCurrent",) + self.CURRENTS + self.RESOLVED:
+            a = getattr(self, k)
+            setattr(s, k, _p(a) if a is not None else None)
         return s
 
 
@@ -384,7 +398,7 @@ class Context:
         """Calculate_Observables for one energy (eCtr 0): dict of T12, Modec1, Modec2,
         EDensity, HDensity, SigXExp.., Jx/Jy/Jz and ContactCurrent increments."""
         npx, npy, npz, norb = self.dims
-        acc = Accumulators(npx, npy, npz, 1)
+        acc = Accumulators(npx, npy, npz, 1, currents=False)
         J = [np.zeros((npx, npy, npz), order="F") for _ in range(3)]
         ins = [_cf(g[k]) for k in ("Gnd", "Gpd", "Gnu", "Gnl", "Gpu", "Gpl")]
         s = acc.struct()

@@ -49,6 +49,10 @@ template <typename T>
 hipError_t launch_siteblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              int, double*, long long, long long, const SpinMats&, int, const int*);
 template <typename T>
+hipError_t launch_bondblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
+                             const double2*, const double2*, int, int, int, double*, long long,
+                             long long, int, const int*);
+template <typename T>
 hipError_t launch_inner(hipStream_t, const cx<T>*, long long, const cx<T>*, long long, int,
                         double*, long long, int, const int*);
 template <typename T>
@@ -105,6 +109,8 @@ static const double kQ = 1.602e-19;   // Constants.F90:4
 static const double kH = 6.626e-34;   // Constants.F90:6
 
 // ---------------------------------------------------------------------------
+constexpr int kSiteQ = 56;      // site-output planes per (lead, slice, site)
+constexpr int kSiteQBasic = 8;  // planes without the bond-current terms
 struct Layout {
   long long nn;
   // per-energy region (complex elements)
@@ -117,7 +123,12 @@ struct Layout {
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
+  // Site outputs per energy: kSiteQ planes, each (lead, slice, site), at
+  // ((q * 2 + lead) * L + x) * nsites + s.  Planes: 0 Re tr of the site block,
+  // 1..3 its spin traces, 4..7 its diagonal per orbital, 8..55 the bond-current
+  // terms per orbital pair (16 each for x, y, z).
   long long site_per_e;  // doubles of site output per energy
+  long long site_q;      // doubles per plane (2 L nsites)
   // zslots: L more blocks per energy for the forward substitution chain of the
   // two-pass RGF (first block column from the left-connected inverses)
   // mixed: the first-column chain runs in complex64
@@ -160,7 +171,8 @@ struct Layout {
     c_prow = o; o += (long long)kGjNBO * n;
     Cs = (o + 31) & ~31LL;
     Is = ((2LL * n + 1) + 63) & ~63LL;
-    site_per_e = 2LL * 4 * L * nsites;
+    site_q = 2LL * L * nsites;
+    site_per_e = kSiteQ * site_q;
   }
 };
 
@@ -216,6 +228,7 @@ struct negf_ctx {
   bool have_h = false;
   bool blockdiag = true;
   bool ends_equal = false;  // H slice 1 == H slice L bitwise: one decimation serves both leads
+  bool want_currents = false;  // the caller's accumulators take bond currents (set per flush)
   ContactCache cc;
   std::vector<zd> h0_host, hopx_host;  // compact form: template and HopX (for set_onsite)
   zd* H = nullptr;       // Htotd (n,n,L), or null for a compact Hamiltonian:
@@ -891,10 +904,13 @@ static int collect(negf_ctx* c, Lane& ln, int nb, const std::vector<int>& iters,
   hipStream_t s = ln.st;
   res.assign(nb, EnergyResult());
   std::vector<double> scal(8 * nb);
+  // the planes this flush fills: the bond-current terms only when they were formed
+  const size_t used = (size_t)(c->want_currents ? kSiteQ : kSiteQBasic) * y.site_q;
   std::vector<double> site((size_t)y.site_per_e * nb);
   std::vector<int> ints((size_t)y.Is * nb);
   TRY(hipMemcpyAsync(scal.data(), ln.d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
-  TRY(hipMemcpyAsync(site.data(), ln.d_site, sizeof(double) * site.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpy2DAsync(site.data(), sizeof(double) * y.site_per_e, ln.d_site, sizeof(double) * y.site_per_e,
+                       sizeof(double) * used, nb, hipMemcpyDeviceToHost, s));
   TRY(hipMemcpyAsync(ints.data(), ln.Ie, sizeof(int) * ints.size(), hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
   if (!ln.cc_pending.empty()) {  // the batch's Sigma copies are complete: publish them
@@ -1060,10 +1076,19 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   //   Gn site blocks:  (G Gamma G')_ss = (V V')_ss,
   //   trace terms:     Re tr(Gam_a G Gam_b G') = |W_a' V_b|_F^2.
   // Gn/Gp site blocks of V = G W_lead (lead 0 = left, 1 = right) at slice x
+  auto site_out = [&](int lead, int x) { return ln.d_site + ((long long)lead * L + x) * ns; };
   auto siteblocks = [&](int lead, int x, const zd* V) -> int {
-    double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
-    TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, out, y.site_per_e, (long long)L * ns, c->sp,
-                                  nb, nullptr));
+    TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, site_out(lead, x), y.site_per_e, y.site_q,
+                                  c->sp, nb, nullptr));
+    return 0;
+  };
+  // Bond-current terms of slice x for one lead (Calculate_Observables.F90:111-168)
+  // from V_x = G_x. W and V_{x+1} (null at the last slice: no x bond there).
+  const int npy = c->cfg.npy, npz = c->cfg.npz;
+  auto bonds = [&](int lead, int x, const zd* V, const zd* Vn) -> int {
+    if (!c->want_currents) return 0;
+    TRY(launch_bondblocks<double>(s, no, V, Es, Vn, Es, hx(x), c->X, n, npy, npz, site_out(lead, x),
+                                  y.site_per_e, y.site_q, nb, nullptr));
     return 0;
   };
   // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b
@@ -1106,9 +1131,14 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, nn, nb));
     // site blocks of V32 = G32 W_lead for one lead at slice x
     auto site32 = [&](int lead, int x, const zf* V32) -> int {
-      double* out = ln.d_site + ((long long)lead * 4 * L + x) * ns;
-      TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, out, y.site_per_e, (long long)L * ns,
-                                   c->sp, nb, nullptr));
+      TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, site_out(lead, x), y.site_per_e,
+                                   y.site_q, c->sp, nb, nullptr));
+      return 0;
+    };
+    auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
+      if (!c->want_currents) return 0;
+      TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
+                                   site_out(lead, x), y.site_per_e, y.site_q, nb, nullptr));
       return 0;
     };
     // scal[idx] = |W_lead' V32|_F^2
@@ -1152,8 +1182,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       if (x == 0 && (r = trace_term(0, GxL, 0))) return r;  // T12 = |W_L' G_1L W_R|^2 in FP64
       TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, nn, nb));
       if ((r = site32(1, x, F(y.f_G)))) return r;
+      if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
       if (x == L - 1 && (r = trace32(1, F(y.f_G), 3))) return r;  // Modec2, right-lead term
       if ((r = site32(0, x, zs(x)))) return r;
+      if ((r = bonds32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
       if (x == L - 1 && (r = trace32(1, zs(x), 2))) return r;  // Modec2, left-lead term
       if (x == 0 && (r = trace32(0, zs(x), 1))) return r;      // Modec1, left-lead term
       if (x < L - 1) {
@@ -1199,9 +1231,11 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       }
       const zd* GxL = (x < L - 1) ? colB : colA;
       if ((r = siteblocks(1, x, GxL))) return r;
+      if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
       if (x == L - 1 && (r = trace_term(1, GxL, 3))) return r;  // Modec2, right-lead term
       if (x == 0 && (r = trace_term(0, GxL, 0))) return r;      // T12
       if ((r = siteblocks(0, x, zs(x)))) return r;
+      if ((r = bonds(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
       if (x == L - 1 && (r = trace_term(1, zs(x), 2))) return r;  // Modec2, left-lead term
       if (x == 0 && (r = trace_term(0, zs(x), 1))) return r;      // Modec1, left-lead term
       if (x < L - 1) {
@@ -1221,6 +1255,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   zd* colB = Q(y.o_colB);
   if ((r = times_w(slot(L - 1), 1, colA))) return r;  // G_LL W_R
   if ((r = siteblocks(1, L - 1, colA))) return r;
+  if ((r = bonds(1, L - 1, colA, nullptr))) return r;
   if ((r = trace_term(1, colA, 3))) return r;          // Modec2, right-lead term
   if (L == 1 && (r = trace_term(0, colA, 0))) return r; // T12
   if ((r = assemble(L - 1, nullptr, 2))) return r;
@@ -1230,6 +1265,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
                              kOne, kZero));
     if ((r = siteblocks(1, x, colB))) return r;
+    if ((r = bonds(1, x, colB, colA))) return r;
     if (x == 0 && (r = trace_term(0, colB, 0))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
     if ((r = assemble(x, slot(x + 1), 2))) return r;
     TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
@@ -1247,24 +1283,41 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
                              kOne, kZero));
     if ((r = siteblocks(0, x, colB))) return r;
+    if ((r = bonds(0, x - 1, colA, colB))) return r;
     if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;
     std::swap(colA, colB);
   }
+  if ((r = bonds(0, L - 1, colA, nullptr))) return r;
   return collect(c, ln, nb, iters, sst, res);
 }
 
 // Accumulate one energy's results in the reference's output layout.
+// Per lead, Gn = f A and Gp = (1 - f) A with A = G Gamma G' (Calculate_Gnp.F90:12-114),
+// so Gn - Gp = (2f - 1) A: the bond-current terms combine with those weights.
+static bool wants_currents(const negf_accum* a) {
+  return a && (a->Jx || a->Jy || a->Jz || a->ERJx || a->ERJy || a->ERJz || a->ORJx || a->ORJy ||
+               a->ORJz);
+}
+
 static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eCtr, double deltaE,
                        const double* mu, double muT, negf_accum* acc) {
   if (!acc) return;
-  const int L = c->L, ns = c->nsites;
+  const int L = c->L, ns = c->nsites, no = c->cfg.norb;
   const double f1 = (energy <= muT - mu[0]) ? 1.0 : 0.0;  // Energy_Sweep.F90:11-23
   const double f2 = (energy <= muT - mu[1]) ? 1.0 : 0.0;
   const double w = deltaE / (2.0 * kPi);
   const double T = e.scal[0];
-  const size_t q = (size_t)L * ns;
-  const double* sl = e.site.data();          // left lead: tr, spin x, y, z
-  const double* sr = e.site.data() + 4 * q;  // right lead
+  const size_t q = (size_t)L * ns;  // one (lead, plane) block
+  // plane p of lead l
+  auto pl = [&](int p, int l) { return e.site.data() + ((size_t)p * 2 + l) * q; };
+  const double* sl = pl(0, 0);
+  const double* sr = pl(0, 1);
+  const bool cur = wants_currents(acc) && e.site.size() >= (size_t)kSiteQ * 2 * q;
+  const double qq_hbar = kQ * kQ / (kH / (2.0 * kPi));  // q^2/hbar (Constants.F90:4-6)
+  const double wl = 2.0 * f1 - 1.0, wr = 2.0 * f2 - 1.0;
+  double* Jd[3] = {acc->Jx, acc->Jy, acc->Jz};
+  double* ERJd[3] = {acc->ERJx, acc->ERJy, acc->ERJz};
+  double* ORJd[3] = {acc->ORJx, acc->ORJy, acc->ORJz};
   double ed = 0.0, hd = 0.0;
   for (int x = 0; x < L; ++x)
     for (int sidx = 0; sidx < ns; ++sidx) {
@@ -1278,7 +1331,26 @@ static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eC
       if (acc->RhoP) acc->RhoP[o] += trP * w;
       double* sg[3] = {acc->SigXExp, acc->SigYExp, acc->SigZExp};
       for (int a = 0; a < 3; ++a)
-        if (sg[a]) sg[a][o] += (f1 * sl[(1 + a) * q + k] + f2 * sr[(1 + a) * q + k]) * w;
+        if (sg[a]) sg[a][o] += (f1 * pl(1 + a, 0)[k] + f2 * pl(1 + a, 1)[k]) * w;
+      // Calculate_Gnp.F90:125-130: energy-resolved Re(Gnd - Gpd), orbital-resolved Re Gnd
+      if (acc->ERRhon && eCtr >= 0) acc->ERRhon[o + q * (size_t)eCtr] += trN - trP;
+      if (acc->ORRhon)
+        for (int r = 0; r < no; ++r)
+          acc->ORRhon[o + q * r] += (f1 * pl(4 + r, 0)[k] + f2 * pl(4 + r, 1)[k]) * w;
+      if (!cur) continue;
+      // Calculate_Observables.F90:111-168 (accumulated in REAL(8) here)
+      for (int d = 0; d < 3; ++d) {
+        if (!Jd[d] && !ERJd[d] && !ORJd[d]) continue;
+        double sum = 0.0;
+        for (int p = 0; p < no * no; ++p) {
+          const int plane = 8 + 16 * d + p;
+          const double v = qq_hbar * (wl * pl(plane, 0)[k] + wr * pl(plane, 1)[k]);
+          sum += v;
+          if (ORJd[d]) ORJd[d][o + q * p] += v * w;
+        }
+        if (Jd[d]) Jd[d][o] += sum * w;
+        if (ERJd[d] && eCtr >= 0) ERJd[d][o + q * (size_t)eCtr] += sum;
+      }
     }
   if (eCtr >= 0) {
     if (acc->T12) acc->T12[eCtr] += T;
@@ -1322,6 +1394,7 @@ extern "C" int negf_flush(negf_ctx* c, const double* mu, double muT, negf_accum*
   HIPCHK(hipSetDevice(c->cfg.device));
   const int total = (int)c->qE.size();
   if (total == 0) return 0;
+  c->want_currents = wants_currents(acc);
   if ((rc = ensure_arena(c, total))) {
     c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
     return rc;
@@ -2062,12 +2135,13 @@ extern "C" int negf_observables(negf_ctx* c, int eCtr, double f1, double f2, dou
 // Fortran-friendly forms: the accumulators as separate array arguments (a Fortran
 // caller passes its module arrays by reference, no TARGET attribute needed); bit k
 // of `mask` marks the k-th array as present, in the field order of negf_accum.
-static negf_accum accum_from(int mask, double* const* a) {
-  negf_accum acc;
-  double** f[13] = {&acc.T12, &acc.Modec1, &acc.Modec2, &acc.DOSc1, &acc.DOSc2, &acc.EDensity,
+static negf_accum accum_from(int mask, double* const* a, int count) {
+  negf_accum acc = {};
+  double** f[24] = {&acc.T12, &acc.Modec1, &acc.Modec2, &acc.DOSc1, &acc.DOSc2, &acc.EDensity,
                     &acc.HDensity, &acc.RhoN, &acc.RhoP, &acc.SigXExp, &acc.SigYExp,
-                    &acc.SigZExp, &acc.ContactCurrent};
-  for (int k = 0; k < 13; ++k) *f[k] = (mask >> k & 1) ? a[k] : nullptr;
+                    &acc.SigZExp, &acc.ContactCurrent, &acc.Jx, &acc.Jy, &acc.Jz, &acc.ERJx,
+                    &acc.ERJy, &acc.ERJz, &acc.ERRhon, &acc.ORJx, &acc.ORJy, &acc.ORJz, &acc.ORRhon};
+  for (int k = 0; k < count; ++k) *f[k] = (mask >> k & 1) ? a[k] : nullptr;
   return acc;
 }
 
@@ -2076,10 +2150,13 @@ extern "C" int negf_energy_sweep_f(negf_ctx* c, double energy, int eCtr, double 
                                    double* Modec1, double* Modec2, double* DOSc1, double* DOSc2,
                                    double* EDensity, double* HDensity, double* RhoN, double* RhoP,
                                    double* SigXExp, double* SigYExp, double* SigZExp,
-                                   double* ContactCurrent) {
-  double* a[13] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
-                   SigYExp, SigZExp, ContactCurrent};
-  negf_accum acc = accum_from(mask, a);
+                                   double* ContactCurrent, double* Jx, double* Jy, double* Jz,
+                                   double* ERJx, double* ERJy, double* ERJz, double* ERRhon,
+                                   double* ORJx, double* ORJy, double* ORJz, double* ORRhon) {
+  double* a[24] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
+                   SigYExp, SigZExp, ContactCurrent, Jx, Jy, Jz, ERJx, ERJy, ERJz, ERRhon,
+                   ORJx, ORJy, ORJz, ORRhon};
+  negf_accum acc = accum_from(mask, a, 24);
   return negf_energy_sweep(c, energy, eCtr, deltaE, mu, muT, &acc);
 }
 
@@ -2087,10 +2164,13 @@ extern "C" int negf_flush_f(negf_ctx* c, const double* mu, double muT, int mask,
                             double* Modec1, double* Modec2, double* DOSc1, double* DOSc2,
                             double* EDensity, double* HDensity, double* RhoN, double* RhoP,
                             double* SigXExp, double* SigYExp, double* SigZExp,
-                            double* ContactCurrent) {
-  double* a[13] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
-                   SigYExp, SigZExp, ContactCurrent};
-  negf_accum acc = accum_from(mask, a);
+                            double* ContactCurrent, double* Jx, double* Jy, double* Jz,
+                            double* ERJx, double* ERJy, double* ERJz, double* ERRhon, double* ORJx,
+                            double* ORJy, double* ORJz, double* ORRhon) {
+  double* a[24] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
+                   SigYExp, SigZExp, ContactCurrent, Jx, Jy, Jz, ERJx, ERJy, ERJz, ERRhon,
+                   ORJx, ORJy, ORJz, ORRhon};
+  negf_accum acc = accum_from(mask, a, 24);
   return negf_flush(c, mu, muT, &acc, nullptr, nullptr);
 }
 
@@ -2105,7 +2185,7 @@ extern "C" int negf_observables_f(negf_ctx* c, int eCtr, double f1, double f2, d
                                   double* Jz) {
   double* a[13] = {T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp,
                    SigYExp, SigZExp, ContactCurrent};
-  negf_accum acc = accum_from(mask, a);
+  negf_accum acc = accum_from(mask, a, 13);
   return negf_observables(c, eCtr, f1, f2, deltaE, Gnd, Gpd, Gnu, Gnl, Gpu, Gpl, GRRgt, GammaL,
                           GammaR, &acc, (mask >> 13 & 1) ? Jx : nullptr,
                           (mask >> 14 & 1) ? Jy : nullptr, (mask >> 15 & 1) ? Jz : nullptr);

@@ -102,8 +102,8 @@ __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, c
 }
 
 // Site-diagonal blocks D_s = (B G')[site s].
-// Writes, per site s of slice x:  out[0] = Re tr D,  out[1+a] = Re tr(D sigma_a).
-// out layout: out_b + q*qstride + s  (q = 0..3).
+// Writes, per site s of slice x:  out[0] = Re tr D,  out[1+a] = Re tr(D sigma_a),
+// out[4+r] = Re D_rr (r < NORB).  out layout: out_b + q*qstride + s.
 // A workgroup takes 16 rows (16 / NORB sites) and splits the sum over k into 16
 // chunks, one per 16-thread row group: with 16 rows per workgroup there are n / 16
 // workgroups per matrix (144 at n = 2304), enough to spread the stream of B and G
@@ -167,7 +167,10 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
       double* o = out + (long long)b * sOut + s;
       double trace = 0.0;
 #pragma unroll
-      for (int r = 0; r < NORB; ++r) trace += dr[r][r];
+      for (int r = 0; r < NORB; ++r) {
+        trace += dr[r][r];
+        o[(4 + r) * qstride] = dr[r][r];  // per orbital (the orbital-resolved density)
+      }
       o[0] = trace;
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
@@ -179,6 +182,112 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
             acc += dr[r][q] * sp.re[m][q][r] - di[r][q] * sp.im[m][q][r];
         o[(m + 1) * qstride] = acc;
       }
+    }
+  }
+}
+
+// Bond-current terms of one slice from the neighbour site blocks of A = V V'
+// (V = G W, Gamma = W W', so A = G Gamma G' is one lead's Gn/Gp up to the
+// occupation factor), Calculate_Observables.F90:111-168 before the q^2/hbar and
+// occupation factors.  For site s (orbitals a = NORB s + ii) and a bond to b:
+//   y: b on site s + 1 (y < Npy-1), z: b on site s + Npy (z < Npz-1), both inside
+//      the slice, H = this slice's block;
+//   x: b the same orbitals of slice x+1 (Vn, when given), H_ab = HopX(a, b'),
+//      H_ba = conj(HopX(a, b')) as the reference writes it;
+// with M = A_ba (row b, column a):  out = Re i (H_ab M - conj(M) H_ba) = -Im(...),
+// per orbital pair at out_b + (8 + 16 d + NORB ii + jj) * qstride + s, d = 0 x,
+// 1 y, 2 z (0 where the bond does not exist).  The host sums the pairs for the
+// bond currents and keeps them for the orbital-resolved ones.
+// Reduction as in k_siteblocks: 16 rows per workgroup, the k sum in 16 chunks
+// added in chunk order, so results do not depend on the batch.
+template <typename T, int NORB>
+__global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV, const cx<T>* Vn,
+                                                    long long sVn, const double2* H, const double2* X,
+                                                    int n, int npy, int npz, double* out,
+                                                    long long sOut, long long qstride,
+                                                    const int* active) {
+  constexpr int RW = 16, KQ = 16, NQ = 3 * NORB;
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const cx<T>* Vb = V + (long long)b * sV;
+  const cx<T>* Vnb = Vn ? Vn + (long long)b * sVn : nullptr;
+  const int NN = NORB * npy;
+  const int row0 = blockIdx.x * RW;
+  const int tr = threadIdx.x % RW, kq = threadIdx.x / RW;
+  const int a = row0 + tr;
+  const int site = a / NORB, a0 = site * NORB;
+  const int y = site % npy, z = site / npy;
+  const bool hy = y < npy - 1, hz = z < npz - 1, hx = Vnb != nullptr;
+  // acc[d * NORB + q] = sum_k V_nb(b0_d + q, k) conj(V(a, k))
+  double accr[NQ], acci[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) accr[q] = acci[q] = 0.0;
+  if (a < n) {
+    const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
+    for (int k = kbeg; k < kend; ++k) {
+      const long long col = (long long)k * n;
+      const cx<T> va = Vb[col + a];
+#pragma unroll
+      for (int q = 0; q < NORB; ++q) {
+        if (hx) {
+          const cx<T> w = Vnb[col + a0 + q];
+          accr[q] += (double)w.x * va.x + (double)w.y * va.y;
+          acci[q] += (double)w.y * va.x - (double)w.x * va.y;
+        }
+        if (hy) {
+          const cx<T> w = Vb[col + a0 + NORB + q];
+          accr[NORB + q] += (double)w.x * va.x + (double)w.y * va.y;
+          acci[NORB + q] += (double)w.y * va.x - (double)w.x * va.y;
+        }
+        if (hz) {
+          const cx<T> w = Vb[col + a0 + NN + q];
+          accr[2 * NORB + q] += (double)w.x * va.x + (double)w.y * va.y;
+          acci[2 * NORB + q] += (double)w.y * va.x - (double)w.x * va.y;
+        }
+      }
+    }
+  }
+  __shared__ double sr[KQ][RW][NQ], si[KQ][RW][NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    sr[kq][tr][q] = accr[q];
+    si[kq][tr][q] = acci[q];
+  }
+  __syncthreads();
+  constexpr int nsite_grp = RW / NORB;
+  if (threadIdx.x < nsite_grp) {
+    const int sl = threadIdx.x;
+    const int s = (row0 / NORB) + sl;
+    if (s * NORB < n) {
+      const int sy = s % npy, sz = s / npy;
+      const bool has[3] = {hx, sy < npy - 1, sz < npz - 1};
+      const int sa0 = s * NORB;
+      double* o = out + (long long)b * sOut + s;
+      for (int d = 0; d < 3; ++d)
+        for (int ii = 0; ii < NORB; ++ii)
+          for (int jj = 0; jj < NORB; ++jj) {
+            double val = 0.0;
+            if (has[d]) {
+              double mr = 0.0, mi = 0.0;
+              for (int c = 0; c < KQ; ++c) {
+                mr += sr[c][sl * NORB + ii][d * NORB + jj];
+                mi += si[c][sl * NORB + ii][d * NORB + jj];
+              }
+              const int ai = sa0 + ii;
+              double2 hab, hba;
+              if (d == 0) {
+                hab = X[(long long)(sa0 + jj) * n + ai];
+                hba = make_double2(hab.x, -hab.y);
+              } else {
+                const int bj = (d == 1 ? sa0 + NORB : sa0 + NN) + jj;
+                hab = H[(long long)bj * n + ai];
+                hba = H[(long long)ai * n + bj];
+              }
+              // Im(H_ab M - conj(M) H_ba)
+              const double im = (hab.x * mi + hab.y * mr) - (mr * hba.y - mi * hba.x);
+              val = -im;
+            }
+            o[(8 + 16 * d + NORB * ii + jj) * qstride] = val;
+          }
     }
   }
 }
@@ -709,6 +818,24 @@ hipError_t launch_bd_mul(hipStream_t s, int norb, int conj, cx<T>* Y, long long 
 }
 
 template <typename T>
+hipError_t launch_bondblocks(hipStream_t s, int norb, const cx<T>* V, long long sV, const cx<T>* Vn,
+                             long long sVn, const double2* H, const double2* X, int n, int npy,
+                             int npz, double* out, long long sOut, long long qstride, int batch,
+                             const int* active) {
+  if (batch <= 0) return hipSuccess;
+  dim3 grid((n + 15) / 16, 1, batch);
+#define NEGF_BB(NO)                                                                                \
+  if (norb == NO) {                                                                                \
+    hipLaunchKernelGGL((k_bondblocks<T, NO>), grid, dim3(256), 0, s, V, sV, Vn, sVn, H, X, n, npy, npz, \
+                       out, sOut, qstride, active);                                                \
+    return hipGetLastError();                                                                      \
+  }
+  NEGF_BB(1) NEGF_BB(2) NEGF_BB(4)
+#undef NEGF_BB
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
 hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long sB,
                              const cx<T>* G, long long sG, int n, double* out, long long sOut,
                              long long qstride, const SpinMats& sp, int batch, const int* active) {
@@ -810,6 +937,9 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
   template hipError_t launch_siteblocks<T>(hipStream_t, int, const cx<T>*, long long,              \
                                            const cx<T>*, long long, int, double*, long long,       \
                                            long long, const SpinMats&, int, const int*);           \
+  template hipError_t launch_bondblocks<T>(hipStream_t, int, const cx<T>*, long long, const cx<T>*, \
+                                           long long, const double2*, const double2*, int, int,    \
+                                           int, double*, long long, long long, int, const int*);   \
   template hipError_t launch_inner<T>(hipStream_t, const cx<T>*, long long, const cx<T>*,          \
                                       long long, int, double*, long long, int, const int*);        \
   template hipError_t launch_eye<T>(hipStream_t, cx<T>*, long long, int, int, const int*);         \

@@ -136,6 +136,7 @@ int admit(negf_server* s, SrvRequest* r) {
   cfg.lanes = 1;  // the request context only carries state: the server's lanes run its energies
   int rc = negf_init(&r->ctx, &cfg);
   if (rc) return rc;
+  r->ctx->want_currents = wants_currents(&r->req.acc);
   if (!r->req.Htotd)
     return negf_set_hamiltonian_compact(r->ctx, r->req.H0, r->req.HopX, r->req.gM, r->req.onsite);
   return negf_set_hamiltonian(r->ctx, r->req.Htotd, r->req.HopX, r->req.gM);

@@ -48,15 +48,20 @@ module negf_capi
     end function
 
     function negf_energy_sweep_f(ctx, energy, eCtr, deltaE, mu, muT, mask, T12, Modec1, Modec2, &
-        DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigX, SigY, SigZ, ContactCurrent) &
+        DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigX, SigY, SigZ, ContactCurrent, Jx, Jy, Jz, &
+        ERJx, ERJy, ERJz, ERRhon, ORJx, ORJy, ORJz, ORRhon) &
         result(rc) bind(C, name="negf_energy_sweep_f")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: energy, deltaE, muT
       integer(c_int), value :: eCtr, mask
       real(c_double), intent(in) :: mu(2)
-      real(c_double), intent(inout) :: T12(*), Modec1(*), Modec2(*), DOSc1(*), DOSc2(*), &
-          EDensity(*), HDensity(*), RhoN(*), RhoP(*), SigX(*), SigY(*), SigZ(*), ContactCurrent(*)
+      ! optional: an absent (or unallocated allocatable) actual argument passes NULL,
+      ! which the library treats as an output not wanted
+      real(c_double), optional, intent(inout) :: T12(*), Modec1(*), Modec2(*), DOSc1(*), DOSc2(*), &
+          EDensity(*), HDensity(*), RhoN(*), RhoP(*), SigX(*), SigY(*), SigZ(*), ContactCurrent(*), &
+          Jx(*), Jy(*), Jz(*), ERJx(*), ERJy(*), ERJz(*), ERRhon(*), ORJx(*), ORJy(*), ORJz(*), &
+          ORRhon(*)
       integer(c_int) :: rc
     end function
 
@@ -70,14 +75,19 @@ module negf_capi
     end function
 
     function negf_flush_f(ctx, mu, muT, mask, T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, &
-        HDensity, RhoN, RhoP, SigX, SigY, SigZ, ContactCurrent) result(rc) bind(C, name="negf_flush_f")
+        HDensity, RhoN, RhoP, SigX, SigY, SigZ, ContactCurrent, Jx, Jy, Jz, ERJx, ERJy, ERJz, &
+        ERRhon, ORJx, ORJy, ORJz, ORRhon) result(rc) bind(C, name="negf_flush_f")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: muT
       integer(c_int), value :: mask
       real(c_double), intent(in) :: mu(2)
-      real(c_double), intent(inout) :: T12(*), Modec1(*), Modec2(*), DOSc1(*), DOSc2(*), &
-          EDensity(*), HDensity(*), RhoN(*), RhoP(*), SigX(*), SigY(*), SigZ(*), ContactCurrent(*)
+      ! optional: an absent (or unallocated allocatable) actual argument passes NULL,
+      ! which the library treats as an output not wanted
+      real(c_double), optional, intent(inout) :: T12(*), Modec1(*), Modec2(*), DOSc1(*), DOSc2(*), &
+          EDensity(*), HDensity(*), RhoN(*), RhoP(*), SigX(*), SigY(*), SigZ(*), ContactCurrent(*), &
+          Jx(*), Jy(*), Jz(*), ERJx(*), ERJy(*), ERJz(*), ERRhon(*), ORJx(*), ORJy(*), ORJz(*), &
+          ORRhon(*)
       integer(c_int) :: rc
     end function
 

@@ -120,19 +120,68 @@ subroutine negf_shim_occupations
   f2v = 1d0 - f2
 end subroutine negf_shim_occupations
 
-! Accumulator mask: every per-energy output, densities and contact current on the
-! observables iteration (Energy_Sweep.F90:44); otherwise only what Contact_Self_Energy
-! and Calculate_Gnp accumulate every iteration (DOSc, RhoN, RhoP).
-integer(c_int) function negf_shim_mask(obs, spin)
+! Accumulator mask: every per-energy output, densities, contact and bond currents on
+! the observables iteration (Energy_Sweep.F90:44); otherwise only what
+! Contact_Self_Energy and Calculate_Gnp accumulate every iteration (DOSc, RhoN, RhoP,
+! and the resolved densities ERRhon / ORRhon).  er / orb: erCD / orCD = 1 with the
+! resolved arrays allocated (Calculate_Gnp.F90:125-130, Calculate_Observables.F90:111-168).
+integer(c_int) function negf_shim_mask(obs, spin, jxyz, er, orb)
   use iso_c_binding
   implicit none
-  logical, intent(in) :: obs, spin
+  logical, intent(in) :: obs, spin, jxyz, er, orb
   negf_shim_mask = ibset(ibset(ibset(ibset(0, 3), 4), 7), 8)
+  if (er) negf_shim_mask = ibset(negf_shim_mask, 19)   ! ERRhon
+  if (orb) negf_shim_mask = ibset(negf_shim_mask, 23)  ! ORRhon
   if (obs) then
     negf_shim_mask = ior(negf_shim_mask, int(b'1000001100111', c_int))  ! T12, Modec1/2, E/HDensity, ContactCurrent
     if (spin) negf_shim_mask = ior(negf_shim_mask, int(b'0111000000000', c_int))
+    if (jxyz) negf_shim_mask = ior(negf_shim_mask, ishft(7_c_int, 13))  ! Jx, Jy, Jz
+    if (er) negf_shim_mask = ior(negf_shim_mask, ishft(7_c_int, 16))    ! ERJx, ERJy, ERJz
+    if (orb) negf_shim_mask = ior(negf_shim_mask, ishft(7_c_int, 20))   ! ORJx, ORJy, ORJz
   end if
 end function negf_shim_mask
+
+! Which outputs this sweep feeds: the observables gate (Energy_Sweep.F90:44), spin
+! (calcSigExp, Read_In_Allocate.F90:93), bond currents and the resolved arrays
+! (erCD / orCD), each only where its arrays are allocated.
+subroutine negf_shim_outputs(obs, spin, jxyz, er, orb)
+  use NEGF_STATE_MODULE
+  implicit none
+  logical, intent(out) :: obs, spin, jxyz, er, orb
+  obs = (ite == iteMax .or. poissonIsConverged == 1)
+  spin = (calcSigExp == 1)
+  if (spin) spin = allocated(SigXExp)
+  jxyz = allocated(Jx)
+  er = (erCD == 1)
+  if (er) er = allocated(ERJx)
+  orb = (orCD == 1)
+  if (orb) orb = allocated(ORJx)
+end subroutine negf_shim_outputs
+
+! The fused sweep / flush with the module's arrays: an unallocated one arrives as an
+! absent optional argument (NULL), and the mask keeps the library off the arrays
+! this iteration does not feed.
+subroutine negf_shim_sweep_call(mask, rc)
+  use NEGF_STATE_MODULE
+  use negf_shim_ctl
+  implicit none
+  integer(c_int), intent(in) :: mask
+  integer(c_int), intent(out) :: rc
+  rc = negf_energy_sweep_f(negf_shim_ctx, energy, eCtr, deltaE, mu, muT, mask, T12, Modec1, Modec2, &
+      DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp, SigYExp, SigZExp, ContactCurrent, &
+      Jx, Jy, Jz, ERJx, ERJy, ERJz, ERRhon, ORJx, ORJy, ORJz, ORRhon)
+end subroutine negf_shim_sweep_call
+
+subroutine negf_shim_flush_call(mask, rc)
+  use NEGF_STATE_MODULE
+  use negf_shim_ctl
+  implicit none
+  integer(c_int), intent(in) :: mask
+  integer(c_int), intent(out) :: rc
+  rc = negf_flush_f(negf_shim_ctx, mu, muT, mask, T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, &
+      HDensity, RhoN, RhoP, SigXExp, SigYExp, SigZExp, ContactCurrent, Jx, Jy, Jz, ERJx, ERJy, ERJz, &
+      ERRhon, ORJx, ORJy, ORJz, ORRhon)
+end subroutine negf_shim_flush_call
 
 subroutine Energy_Sweep
   use NEGF_STATE_MODULE
@@ -140,8 +189,7 @@ subroutine Energy_Sweep
   implicit none
   integer(c_int) :: rc, mask
   integer(c_int), external :: negf_shim_mask
-  logical :: obs, spin
-  real(c_double) :: dmy(1)
+  logical :: obs, spin, jxyz, er, orb
   if (mpiRank == mpiRoot) write(*, *) eCtr
   call negf_shim_occupations
   if (contactType /= 0 .or. isBdGHam /= 0) then
@@ -153,18 +201,9 @@ subroutine Energy_Sweep
     call negf_shim_staged_sweep
     return
   end if
-  obs = (ite == iteMax .or. poissonIsConverged == 1)
-  spin = (calcSigExp == 1)
-  if (spin) spin = allocated(SigXExp)
-  mask = negf_shim_mask(obs, spin)
-  dmy = 0d0
-  if (spin) then
-    rc = negf_energy_sweep_f(negf_shim_ctx, energy, eCtr, deltaE, mu, muT, mask, T12, Modec1, &
-        Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, SigXExp, SigYExp, SigZExp, ContactCurrent)
-  else
-    rc = negf_energy_sweep_f(negf_shim_ctx, energy, eCtr, deltaE, mu, muT, mask, T12, Modec1, &
-        Modec2, DOSc1, DOSc2, EDensity, HDensity, RhoN, RhoP, dmy, dmy, dmy, ContactCurrent)
-  end if
+  call negf_shim_outputs(obs, spin, jxyz, er, orb)
+  mask = negf_shim_mask(obs, spin, jxyz, er, orb)
+  call negf_shim_sweep_call(mask, rc)
   if (rc < 0) call negf_shim_fail("Energy_Sweep", rc)
   if (rc > 0) then
     write(*, *) "NEGF: energy status ", rc, " (SR not converged / NaN / singular block), Energy = ", energy
@@ -215,21 +254,11 @@ subroutine NEGF_Flush
   implicit none
   integer(c_int) :: rc, mask
   integer(c_int), external :: negf_shim_mask
-  logical :: obs, spin
-  real(c_double) :: dmy(1)
+  logical :: obs, spin, jxyz, er, orb
   if (.not. c_associated(negf_shim_ctx)) return
-  obs = (ite == iteMax .or. poissonIsConverged == 1)
-  spin = (calcSigExp == 1)
-  if (spin) spin = allocated(SigXExp)
-  mask = negf_shim_mask(obs, spin)
-  dmy = 0d0
-  if (spin) then
-    rc = negf_flush_f(negf_shim_ctx, mu, muT, mask, T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, &
-        HDensity, RhoN, RhoP, SigXExp, SigYExp, SigZExp, ContactCurrent)
-  else
-    rc = negf_flush_f(negf_shim_ctx, mu, muT, mask, T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, &
-        HDensity, RhoN, RhoP, dmy, dmy, dmy, ContactCurrent)
-  end if
+  call negf_shim_outputs(obs, spin, jxyz, er, orb)
+  mask = negf_shim_mask(obs, spin, jxyz, er, orb)
+  call negf_shim_flush_call(mask, rc)
   if (rc < 0) call negf_shim_fail("NEGF_Flush", rc)
   if (rc > 0) errorFlag = rc
 end subroutine NEGF_Flush

@@ -33,6 +33,9 @@ module negf_state
   complex(8), dimension(:,:), allocatable :: SigmaL, SigmaR, GammaL, GammaR
 
   real(8), dimension(:,:,:), allocatable :: RhoN, RhoP, SigXExp, SigYExp, SigZExp, Jx, Jy, Jz
+  ! energy- and orbital-resolved outputs, allocated when erCD / orCD = 1
+  ! (NEGF_Module.F90:100, Read_In_Allocate.F90:228-239)
+  real(8), dimension(:,:,:,:), allocatable :: ERJx, ERJy, ERJz, ERRhon, ORJx, ORJy, ORJz, ORRhon
   real(8), dimension(:), allocatable :: T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity
   real(8), dimension(:), allocatable :: ContactCurrent
 
@@ -65,6 +68,16 @@ contains
     T12 = 0d0; Modec1 = 0d0; Modec2 = 0d0; DOSc1 = 0d0; DOSc2 = 0d0; EDensity = 0d0; HDensity = 0d0
     EAxis = 0d0; ContactCurrent = 0d0
     gMx = c0; gMy = c0; gMz = c0
+    if (erCD == 1) then
+      allocate(ERJx(Npx, Npy, Npz, eStpnum), ERJy(Npx, Npy, Npz, eStpnum), &
+               ERJz(Npx, Npy, Npz, eStpnum), ERRhon(Npx, Npy, Npz, eStpnum))
+      ERJx = 0d0; ERJy = 0d0; ERJz = 0d0; ERRhon = 0d0
+    end if
+    if (orCD == 1) then
+      allocate(ORJx(Npx, Npy, Npz, Norb * Norb), ORJy(Npx, Npy, Npz, Norb * Norb), &
+               ORJz(Npx, Npy, Npz, Norb * Norb), ORRhon(Npx, Npy, Npz, Norb))
+      ORJx = 0d0; ORJy = 0d0; ORJz = 0d0; ORRhon = 0d0
+    end if
   end subroutine negf_state_allocate
 
   subroutine negf_state_deallocate()
@@ -72,6 +85,8 @@ contains
     deallocate(RhoN, RhoP, SigXExp, SigYExp, SigZExp, Jx, Jy, Jz, GammaL, GammaR, SigmaL, SigmaR)
     deallocate(T12, Modec1, Modec2, DOSc1, DOSc2, EDensity, HDensity, EAxis, ContactCurrent)
     deallocate(gMx, gMy, gMz)
+    if (allocated(ERJx)) deallocate(ERJx, ERJy, ERJz, ERRhon)
+    if (allocated(ORJx)) deallocate(ORJx, ORJy, ORJz, ORRhon)
   end subroutine negf_state_deallocate
 
 end module negf_state

@@ -217,3 +217,68 @@ def energy_sweep(wire, energy, deltaE, mu=(-0.1, 0.0), muT=0.0, eta=1e-6, sr_tol
     out.update(acc)
     out.update(ContactCurrent=cc, status=st, iters=(iters[0], iters[1]))
     return out
+
+
+def resolved(wire, f1, f2, deltaE, g, eCtr=0, estpnum=1):
+    """Energy- and orbital-resolved outputs of one energy from its Gn/Gp blocks (g:
+    dict with Gnd..Gpl, as gnp returns).  Restates, in plain loops (small wires only):
+      * Calculate_Gnp.F90:118-130 -- ERRhon(x,y,z,eCtr) += sum_ii Re Gnd(i,i) - Re Gpd(i,i),
+        ORRhon(x,y,z,ii) += Re Gnd(i,i) dE/2pi;
+      * Calculate_Observables.F90:111-168 -- per bond and orbital pair (ii, jj)
+        dumVar = Re i q^2/hbar (H_ab Gn_ba - Gn_ab H_ba - H_ab Gp_ba + Gp_ab H_ba)
+        (x bonds: HopX with Gnl/Gnu/Gpl/Gpu), ERJ(x,y,z,eCtr) += dumVar,
+        ORJ(x,y,z,(ii-1)*Norb+jj) += dumVar dE/2pi, kept in REAL(8) (the reference
+        rounds dumVar to REAL(4), Calculate_Observables.F90:7).
+    Returns a dict of those arrays for this energy alone."""
+    L, npy, npz, no = wire.npx, wire.npy, wire.npz, wire.norb
+    n = wire.n
+    NN = npy * no
+    H, X = wire.htotd, wire.hopx
+    Gnd, Gpd, Gnu, Gnl, Gpu, Gpl = (g[k] for k in ("Gnd", "Gpd", "Gnu", "Gnl", "Gpu", "Gpl"))
+    pref = 1j * 1.602e-19 ** 2 / (6.626e-34 / (2 * np.pi))
+    w = deltaE / (2 * np.pi)
+    out = {k: np.zeros((L, npy, npz, estpnum)) for k in ("ERJx", "ERJy", "ERJz", "ERRhon")}
+    out.update({k: np.zeros((L, npy, npz, no * no)) for k in ("ORJx", "ORJy", "ORJz")})
+    out["ORRhon"] = np.zeros((L, npy, npz, no))
+    for x in range(L):
+        for y in range(npy):
+            for z in range(npz):
+                idx = y * no + z * NN
+                for ii in range(no):
+                    i = idx + ii
+                    out["ERRhon"][x, y, z, eCtr] += Gnd[i, i, x].real - Gpd[i, i, x].real
+                    out["ORRhon"][x, y, z, ii] += Gnd[i, i, x].real * w
+                    for jj in range(no):
+                        p = ii * no + jj
+                        terms = []
+                        if y < npy - 1:
+                            a, b = idx + ii, idx + no + jj
+                            terms.append(("y", H[a, b, x] * Gnd[b, a, x] - Gnd[a, b, x] * H[b, a, x]
+                                          - H[a, b, x] * Gpd[b, a, x] + Gpd[a, b, x] * H[b, a, x]))
+                        if z < npz - 1:
+                            a, b = idx + ii, idx + NN + jj
+                            terms.append(("z", H[a, b, x] * Gnd[b, a, x] - Gnd[a, b, x] * H[b, a, x]
+                                          - H[a, b, x] * Gpd[b, a, x] + Gpd[a, b, x] * H[b, a, x]))
+                        if x < L - 1:
+                            a, b = idx + ii, idx + jj
+                            hx = X[a, b]
+                            terms.append(("x", hx * Gnl[b, a, x] - Gnu[a, b, x] * np.conj(hx)
+                                          - hx * Gpl[b, a, x] + Gpu[a, b, x] * np.conj(hx)))
+                        for d, t in terms:
+                            v = (pref * t).real
+                            out["ERJ" + d][x, y, z, eCtr] += v
+                            out["ORJ" + d][x, y, z, p] += v * w
+    return out
+
+
+def energy_resolved(wire, energy, deltaE, mu=(-0.1, 0.0), muT=0.0, eta=1e-6, sr_tol=1e-10,
+                    sr_max_iter=200, eCtr=0, estpnum=1):
+    """One Energy_Sweep's resolved outputs: contacts, InvGRd, RGF, Gn/Gp (the oracle's
+    stages), then `resolved`."""
+    f1 = 1.0 if energy <= muT - mu[0] else 0.0  # Energy_Sweep.F90:11-23
+    f2 = 1.0 if energy <= muT - mu[1] else 0.0
+    c = contact_self_energy(wire, energy, eta, sr_tol, sr_max_iter, True)
+    A = assemble_invg(wire, c["SigmaL"], c["SigmaR"], energy, eta)
+    r = retarded(A, wire.hopx)
+    g = gnp(wire, f1, f2, deltaE, r["GRLef"], r["GRRgt"], c["GammaL"], c["GammaR"])
+    return resolved(wire, f1, f2, deltaE, g, eCtr, estpnum)

@@ -57,3 +57,53 @@ def test_dump_round_trip(tmp_path, rng):
     full = np.load(os.path.join(out, "results.npz"))
     for k in ("T12", "RhoN", "SigZExp", "ContactCurrent"):
         assert np.array_equal(full[k], np.asarray(getattr(acc, k))), k
+
+
+def test_es11_4_matches_the_fortran_runtime(tmp_path, rng):
+    # the text fields against a Fortran program's own `FORMAT(ES11.4)` output (the
+    # format of Dump_Data.F90:152), written by amdflang's runtime
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        import pytest
+        pytest.skip("no Fortran compiler")
+    vals = np.concatenate([rng.standard_normal(20) * 10.0 ** rng.integers(-30, 30, 20),
+                           [0.0, 1.0, -1.0, 9.99995e-3, 1.23455e-7, -7.65435e12, 5e-310, -2.5e-300,
+                            1.5e300, np.inf, -np.inf, np.nan]])
+    src = tmp_path / "w.f90"
+    src.write_text("program w\n  real(8) :: v\n  integer :: i\n"
+                   "  open(10, file='in.txt'); open(13, file='out.txt')\n"
+                   f"  do i = 1, {len(vals)}\n    read(10, *) v\n    write(13, 43) v\n  end do\n"
+                   "  close(13)\n43 format(ES11.4)\nend program w\n")
+    (tmp_path / "in.txt").write_text("\n".join(repr(float(v)) for v in vals) + "\n")
+    subprocess.run([fc, str(src), "-o", str(tmp_path / "w")], check=True, capture_output=True)
+    subprocess.run([str(tmp_path / "w")], check=True, cwd=tmp_path, capture_output=True)
+    fortran = (tmp_path / "out.txt").read_text().splitlines()
+    assert fortran == [_es(v) for v in vals]
+
+
+def test_resolved_files_round_trip(tmp_path, rng):
+    dims = (2, 3, 2, 4)
+    ne = 3
+    npx, npy, npz, no = dims
+    acc = _filled(dims, ne, rng)
+    acc2 = Accumulators(npx, npy, npz, ne, resolved=True, norb=no)
+    for k in acc2.RESOLVED + acc2.CURRENTS:
+        a = getattr(acc2, k)
+        a[...] = rng.standard_normal(a.shape)
+        setattr(acc, k, a)
+    out = str(tmp_path / "res")
+    phi = rng.standard_normal((npx, npy, npz))
+    dump_data(out, dims, np.linspace(-0.1, 0.1, ne), acc, phi=phi, ro=phi * 2)
+    ns = npx * npy * npz
+    assert sum(1 for _ in open(os.path.join(out, "ERCD.txt"))) == 4 * ne * ns
+    assert sum(1 for _ in open(os.path.join(out, "ORCD.txt"))) == (3 * no * no + no) * ns
+    assert sum(1 for _ in open(os.path.join(out, "Poissons.txt"))) == 2 * ns
+    r = read_data(out)
+    for k in acc2.RESOLVED + acc2.CURRENTS:
+        b = getattr(acc, k)
+        assert np.all(np.abs(r[k] - b) <= 5e-5 * np.abs(b) + 1e-300), k
+    assert np.all(np.abs(r["phi"] - phi) <= 5e-5 * np.abs(phi))
+    # the accumulators' currents were used (no explicit currents argument)
+    assert np.all(r["Jy"] != 0)

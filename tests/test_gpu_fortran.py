@@ -60,10 +60,10 @@ def test_fortran_host_matches_oracle(tmp_path, staged, submit):
     ref, noise = P.reference(w, energies, energies[1] - energies[0], mu=MU, muT=MUT)
     assert got["errorFlag"] == 0
     P.check_all(got, ref, noise)
-    if staged:  # the per-stage path also produces the bond currents
-        q, hbar = 1.602e-19, 6.626e-34 / (2 * np.pi)
-        # current terms are H*Gn products: e^2/hbar * norb^2 * max|H| * max(Gn dE/2pi)
-        rho = float(np.abs(ref["RhoN"]).max())
-        jscale = q * q / hbar * w.norb**2 * np.abs(w.htotd).max() * rho
-        for k in ("Jx", "Jy", "Jz"):
-            P.check(k, got[k], ref[k], noise[k], jscale)
+    # every path produces the bond currents (the fused one from the V V' site blocks)
+    q, hbar = 1.602e-19, 6.626e-34 / (2 * np.pi)
+    # current terms are H*Gn products: e^2/hbar * norb^2 * max|H| * max(Gn dE/2pi)
+    rho = float(np.abs(ref["RhoN"]).max())
+    jscale = q * q / hbar * w.norb**2 * np.abs(w.htotd).max() * rho
+    for k in ("Jx", "Jy", "Jz"):
+        P.check(k, got[k], ref[k], noise[k], jscale)

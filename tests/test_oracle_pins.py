@@ -336,3 +336,44 @@ def test_zgemm_matches_numpy(rng, ta, tb):
         got = O.zgemm(ta, tb, A, B, alpha, beta, C)
         want = alpha * op[ta](A) @ op[tb](B) + beta * C
         assert rel(got, want) < 1e-13, (ta, tb, alpha, beta)
+
+
+@pytest.mark.parametrize("E", [-1.6, 0.9])
+def test_resolved_outputs_sum_to_the_integrated_ones(E):
+    # oracle.resolved restates the erCD/orCD branches (Calculate_Gnp.F90:125-130,
+    # Calculate_Observables.F90:111-168); summed over orbital pairs / orbitals, or
+    # weighted by dE/2pi, they are the integrated outputs of the C++ restatement
+    w = build_wire(4, 2, 3, disorder=0.5, seed=3, flux=0.3, zeeman=0.03)
+    dE = 0.02
+    r = O.energy_sweep(w, E, dE, (-0.1, 0.0), 0.0, ETA, 1e-10, 200, True)
+    res = O.energy_resolved(w, E, dE, (-0.1, 0.0), 0.0, ETA)
+    wgt = dE / (2 * np.pi)
+    for d in "xyz":
+        J = np.asarray(r["J" + d])
+        s = max(np.abs(J).max(), 1e-300)
+        assert np.abs(res["ORJ" + d].sum(axis=3) - J).max() <= 1e-12 * s, d
+        assert np.abs(res["ERJ" + d][..., 0] * wgt - J).max() <= 1e-12 * s, d
+    rn, rp = np.asarray(r["RhoN"]), np.asarray(r["RhoP"])
+    sr = max(np.abs(rn).max(), np.abs(rp).max())
+    assert np.abs(res["ORRhon"].sum(axis=3) - rn).max() <= 1e-12 * sr
+    assert np.abs(res["ERRhon"][..., 0] * wgt - (rn - rp)).max() <= 1e-12 * sr
+    assert np.abs(J).max() > 0
+
+
+def test_resolved_current_orbital_pair_index():
+    # only the (ii, jj) = (1, 3) element of every y hop is kept: the y current of a
+    # bond then lives in orbital pair (ii-1)*Norb + jj = 7 (0-based ii*Norb + jj) only
+    w = build_wire(3, 3, 1)
+    no, n = w.norb, w.n
+    for y in range(w.npy - 1):
+        a0, b0 = y * no, (y + 1) * no
+        blk = w.htotd[a0:a0 + no, b0:b0 + no, :].copy()
+        keep = np.zeros_like(blk)
+        keep[1, 3, :] = blk[1, 3, :] if np.any(blk[1, 3, :]) else 0.4 + 0.3j
+        w.htotd[a0:a0 + no, b0:b0 + no, :] = keep
+        w.htotd[b0:b0 + no, a0:a0 + no, :] = np.conj(np.transpose(keep, (1, 0, 2)))
+    res = O.energy_resolved(w, 1.1, 0.02, (-0.1, 0.0), 0.0, ETA)
+    jy = res["ORJy"]
+    assert np.abs(jy[..., 1 * no + 3]).max() > 0
+    others = np.delete(jy, 1 * no + 3, axis=3)
+    assert np.abs(others).max() == 0.0
