@@ -62,21 +62,28 @@ CONFIGS = {
                budget_gb=245.0),
 }
 # Many-wire workloads run through the serving / batching layer (negf_server):
-#  c3 -- a seeded sample of the batching configuration (256 disorder realisations x
+#  c3 -- a seeded fraction of the batching configuration (256 disorder realisations x
 #        32 energies, Npy=Npz in {8,12,16,20}, Npx in [40,120], W in [0,1], g*B in
-#        [0,0.05], flux in [0,2]): `instances` realisations x `energies_per` of their 32
-#        energies, all submitted at once;
-#  c5 -- a serving stream with Poisson arrivals at `rate` requests/s, per request
-#        Npy=Npz in `ns`, Npx in `L`, 16..`ne_max` energies in [-0.4,0.4], flux in
-#        [0,2], W in [0,1], own seed; scaled down from the configuration's
-#        Npy=Npz in {8..24}, Npx in [50,300], 16..256 energies so that the stream
-#        fits about a minute of one GPU (the full distribution's mean request is
-#        ~75 s at FP64 peak).
+#        [0,0.05], flux in [0,2]): --c3-instances realisations x --c3-energies of
+#        their 32 energies, all submitted at once; the fraction is in config.workload;
+#  c5 -- a serving stream drawn from the configured distribution: Poisson arrivals
+#        at --c5-rate requests/s, per request Npy=Npz in {8,...,24}, Npx in
+#        [50,300], 16..256 energies in [-0.4,0.4], flux in [0,2], W in [0,1], own
+#        seed.  One such request is ~100 s of GPU work, so --c5-requests bounds the
+#        stream and --c5-energy-stride thins every request's energy list (every
+#        k-th energy of its grid) to keep the run to minutes; the line reports the
+#        offered load rho = rate x mean service time for the thinned and the full
+#        lists, and for each configured rate.
 SERVER_WORKLOADS = {
     "c3": dict(instances=16, energies_per=8, ns=(8, 12, 16, 20), L=(40, 120), seed=3000),
-    "c5": dict(requests=20, rate=0.3, ns=(8, 12, 16), L=(50, 150), ne_max=64, seed=5000),
+    "c5": dict(requests=8, rate=10.0, ns=(8, 12, 16, 20, 24), L=(50, 300), ne=(16, 256), stride=16,
+               seed=5000, rates=(10, 20, 50, 100, 200, 500)),
 }
 ETA, SR_TOL, SR_MAX = 1e-6, 1e-10, 200
+PRECISIONS = {"fp64": capi.NEGF_PREC_FP64, "mixed": capi.NEGF_PREC_MIXED,
+              "refined": capi.NEGF_PREC_REFINED}
+DTYPES = {"fp64": "f64", "mixed": "f64+c64 (mixed)",
+          "refined": "f64+c64 (complex64 block inverses with FP64 refinement)"}
 MU, MUT = (-0.1, 0.0), 0.0
 LANES = [0]  # set from --lanes
 OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", "RhoN", "RhoP",
@@ -106,7 +113,7 @@ def cpu_baseline(mp_ctx, cfg, energies, k_mean, ranks, mode):
 # FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading for
 # a copy kernel of known size).
 TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md",
-                    ("c4", "fp64_gemm_3m"): "alone_fp64_gemm3m_n2304_b2.md"}
+                    ("c4", "fp64_gemm_3m"): "traffic_fp64_gemm3m_n2304_b2.md"}
 GEMM_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update", "fp64_gemm_3m",
               "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
 KERNEL_NAMES = {
@@ -173,7 +180,7 @@ def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmu
     accumulators of the last `keep` timed steps (by step index) and the launch
     profile."""
     L = cfg["npx"]
-    prec = capi.NEGF_PREC_MIXED if precision == "mixed" else capi.NEGF_PREC_FP64
+    prec = PRECISIONS[precision]
     ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
                        max_batch=per, flags=flags, precision=prec, hbm_budget_gb=cfg["budget_gb"],
                        lanes=LANES[0])
@@ -205,9 +212,10 @@ def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmu
         print(f"[bench] {label} step {k} {times[-1]:.2f}s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     prof = capi.profile_end()
+    refine = ctx.refine_stats()
     ctx.close()
     return dict(times=times, elapsed=elapsed, iters=np.concatenate(iters), status=status, accs=accs,
-                prof=prof)
+                prof=prof, refine=refine)
 
 
 def errors_vs(acc, ref):
@@ -234,6 +242,8 @@ def server_workload(name, args):
     rng = np.random.default_rng(wl["seed"])
     reqs = []  # (arrival_s, wire, energies)
     if name == "c3":
+        wl = dict(wl, instances=args.c3_instances or wl["instances"],
+                  energies_per=args.c3_energies or wl["energies_per"])
         grid = energy_grid(-0.4, 0.4, 32)
         for r in range(wl["instances"]):
             ny = int(rng.choice(wl["ns"]))
@@ -242,29 +252,37 @@ def server_workload(name, args):
                            flux=float(rng.uniform(0, 2)), zeeman=float(rng.uniform(0, 0.05)), dense=False)
             sel = grid[(np.arange(wl["energies_per"]) * (32 // wl["energies_per"]) + r) % 32]
             reqs.append((0.0, w, sel))
-        desc = (f"c3 sample: {wl['instances']} of the 256 seeded realisations x {wl['energies_per']} of "
-                f"their 32 energies in [-0.4,0.4]; Npy=Npz in {wl['ns']}, Npx in {wl['L']}, W in [0,1], "
-                f"g*B in [0,0.05], flux in [0,2]; all submitted at t=0 as compact Hamiltonians "
-                f"(slice template + site potentials)")
+        frac = wl["instances"] * wl["energies_per"] / (256 * 32)
+        desc = (f"c3 fraction {frac:.4f} ({wl['instances']} of the 256 seeded realisations x "
+                f"{wl['energies_per']} of their 32 energies in [-0.4,0.4]); Npy=Npz in {wl['ns']}, Npx in "
+                f"{wl['L']}, W in [0,1], g*B in [0,0.05], flux in [0,2]; all submitted at t=0 as compact "
+                f"Hamiltonians (slice template + site potentials)")
     else:
+        wl = dict(wl, requests=args.c5_requests or wl["requests"], rate=args.c5_rate or wl["rate"],
+                  stride=args.c5_energy_stride or wl["stride"])
         t = 0.0
+        full_points = 0
         for r in range(wl["requests"]):
             t += float(rng.exponential(1.0 / wl["rate"]))
             ny = int(rng.choice(wl["ns"]))
             L = int(rng.integers(wl["L"][0], wl["L"][1] + 1))
-            ne = int(rng.integers(16, wl["ne_max"] + 1))
+            ne = int(rng.integers(wl["ne"][0], wl["ne"][1] + 1))
             w = build_wire(L, ny, ny, disorder=float(rng.uniform(0, 1)), seed=wl["seed"] + r,
                            flux=float(rng.uniform(0, 2)), dense=False)
-            reqs.append((t, w, energy_grid(-0.4, 0.4, ne)))
-        desc = (f"c5 stream: {wl['requests']} requests, Poisson arrivals at {wl['rate']}/s; per request "
-                f"Npy=Npz in {wl['ns']}, Npx in {wl['L']}, 16..{wl['ne_max']} energies in [-0.4,0.4], "
-                f"flux in [0,2], W in [0,1], own seed (scaled down from Npy=Npz 8..24, Npx 50..300, "
-                f"16..256 energies); compact Hamiltonians (slice template + site potentials)")
-    prec = capi.NEGF_PREC_MIXED if args.precision == "mixed" else capi.NEGF_PREC_FP64
+            grid = energy_grid(-0.4, 0.4, ne)
+            full_points += ne
+            reqs.append((t, w, grid[::wl["stride"]]))
+        desc = (f"c5 stream: {wl['requests']} requests drawn from the configured distribution, Poisson "
+                f"arrivals at {wl['rate']}/s; per request Npy=Npz in {wl['ns']}, Npx in {wl['L']}, "
+                f"{wl['ne'][0]}..{wl['ne'][1]} energies in [-0.4,0.4] (every {wl['stride']}-th energy of "
+                f"its grid solved), flux in [0,2], W in [0,1], own seed; compact Hamiltonians")
+    prec = PRECISIONS[args.precision]
     npts = sum(len(e) for _, _, e in reqs)
     print(f"[bench] {name}: {len(reqs)} requests, {npts} energy points", file=sys.stderr, flush=True)
+    # lane pools that hold one energy of the largest request (block 2304, Npx 300:
+    # ~42 GB in the mixed forms)
     srv = capi.Server(eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX, precision=prec, flags=args.flags,
-                      lanes=args.lanes)
+                      lanes=args.lanes, lane_pool_gb=args.lane_pool_gb)
     ids = [None] * len(reqs)
     capi.profile_begin(args.profile_every)
     t0 = time.perf_counter()
@@ -293,13 +311,13 @@ def server_workload(name, args):
     peak = PEAK_FP32_MFMA if dominant and dominant.startswith("fp32") else PEAK_FP64_MFMA
     g_flops, g_ms, g_launches = prof[dominant] if dominant else (0.0, 0.0, 0)
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
-    return {
+    out = {
         "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
         "value": npts / elapsed,
         "unit": "energy-points/s",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * elapsed,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+c64 (mixed)" if args.precision == "mixed" else "f64",
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (seeded 4-band TI wires, host-built Hamiltonians)",
         "config": {"workload": desc, "requests": len(reqs), "energy_points": npts,
                    "precision": args.precision},
@@ -311,6 +329,20 @@ def server_workload(name, args):
                      "traffic": None, "launches": g_launches, "avg_launch_ms": g_ms / max(1, g_launches)},
         "kernels": kernel_table(prof, name),
     }
+    if name == "c5":
+        # the GPU is busy throughout (the stream arrives within ~requests/rate s), so the
+        # mean service time of a request is elapsed / requests
+        s_thin = elapsed / len(reqs)
+        s_full = s_thin * full_points / max(1, npts)
+        out["offered_load"] = {
+            "mean_service_s_thinned": s_thin, "mean_service_s_full_lists": s_full,
+            "rho_at_rate": {str(r): r * s_full for r in wl["rates"]},
+            "rho_this_run": wl["rate"] * s_thin,
+            "note": ("rho = arrival rate x mean service time of one request (full energy lists: the "
+                     "measured service time scaled by the energy points the thinning dropped); rho >> 1: "
+                     "the queue grows by ~(rate - 1/mean service) requests per second, so latencies are "
+                     "set by the backlog; p50/p99 above are for this finite stream")}
+    return out
 
 
 def dominant_kind(prof):
@@ -346,7 +378,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS) + sorted(SERVER_WORKLOADS))
-    ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"))
+    ap.add_argument("--precision", default="mixed", choices=tuple(PRECISIONS))
     ap.add_argument("--lanes", type=int, default=0, help="concurrent device lanes (0 = library default)")
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
     ap.add_argument("--budget-gb", type=float, default=-1.0,
@@ -360,6 +392,14 @@ def main():
     ap.add_argument("--cpu-ranks", type=int, default=0,
                     help="CPU baseline ranks (0 = one per host core, at most 16)")
     ap.add_argument("--cpu-baseline-mode", default="auto", choices=("auto", "full", "pieces"))
+    ap.add_argument("--c3-instances", type=int, default=0, help="c3: realisations (default 16 of 256)")
+    ap.add_argument("--c3-energies", type=int, default=0, help="c3: energies per realisation (default 8 of 32)")
+    ap.add_argument("--c5-requests", type=int, default=0, help="c5: requests in the stream (default 8)")
+    ap.add_argument("--c5-rate", type=float, default=0.0, help="c5: arrivals per second (default 10)")
+    ap.add_argument("--c5-energy-stride", type=int, default=0,
+                    help="c5: solve every k-th energy of each request's grid (default 16)")
+    ap.add_argument("--lane-pool-gb", type=float, default=45.0,
+                    help="server workloads: working pool per lane (GB)")
     ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
     ap.add_argument("--profile-every", type=int, default=32,
                     help="time one launch in this many of each kind with HIP events")
@@ -423,7 +463,7 @@ def main():
     value = npts / elapsed
     timed_idx = np.unique(np.concatenate([step_indices(k) for k in range(args.warmup, nsteps)]))
     compare = None
-    if args.precision == "mixed" and not args.no_compare:
+    if args.precision != "fp64" and not args.no_compare:
         # the plain FP64 form, warmed up the same way (one step at most), on the
         # energies of the last `ncmp` timed mixed steps
         last = sorted(run["accs"])
@@ -463,7 +503,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+c64 (mixed)" if args.precision == "mixed" else "f64",
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (seeded 4-band TI wire, host-built Hamiltonian)",
         "config": {"workload": (f"{args.config}: Npx={L}, Npy=Npz={cfg['npy']} (block {n}), "
                                 f"{cfg['ne']} energies in [{cfg['emin']},{cfg['emax']}], "
@@ -502,6 +542,10 @@ def main():
                                        "source": "profiles/" + TRAFFIC_PROFILES[(args.config, dom)]}
     if compare is not None:
         result["compare"] = compare
+    if args.precision == "refined":
+        b, st, e = run["refine"]
+        result["refine"] = {"block_inverses": b, "fp64_newton_steps": st, "escalated_to_fp64": e,
+                            "note": "counts since the context was created (warmup steps included)"}
     if mp_ctx is not None:
         t0 = time.perf_counter()
         ranks = args.cpu_ranks or min(16, os.cpu_count() or 1)

@@ -45,6 +45,11 @@ extern "C" {
 #define NEGF_PREC_FP64 0  /* plain complex128 on FP64 MFMA */
 #define NEGF_PREC_MIXED 1 /* FP64 contacts, inverses and T12 chain; complex64 MFMA for the
                              first-column chain, whose site blocks feed the densities */
+#define NEGF_PREC_REFINED 2 /* NEGF_PREC_MIXED with every RGF block inverse taken in complex64
+                               (Gauss-Jordan on FP32 MFMA) and refined in FP64 by Newton steps
+                               X <- X + X (I - M X) until max|I - M X| <= 1e-12 (at most two);
+                               a block whose complex64 inverse is singular or whose residual does
+                               not contract is inverted again in FP64 (escalation) */
 
 typedef struct negf_ctx negf_ctx;
 
@@ -62,7 +67,8 @@ typedef struct {
   double hbm_budget_gb;/* device memory the context may use for in-flight energies; 0 = auto */
   int max_batch;       /* cap on energies in flight; 0 = auto */
   int flags;           /* NEGF_FLAG_* (0 = default algorithms) */
-  int lanes;           /* concurrent device lanes of negf_flush, 1..4 (0 = 4) */
+  int lanes;           /* concurrent device lanes of negf_flush, 1..4 (0 = the tuning
+                          table's value for this block size, else 4) */
 } negf_config;
 
 /* Algorithm selection (results agree to rounding either way).
@@ -87,6 +93,12 @@ typedef struct {
  * unchanged; the Sancho-Rubio cost is paid once per energy.  Up to 15% of the
  * free HBM holds the cache. */
 #define NEGF_FLAG_CACHE_CONTACTS 8
+/* Capture each batch's launches after the contacts (assembly, block inverses,
+ * column chains, site and bond blocks, traces) into a HIP graph once per lane,
+ * context, Hamiltonian upload and batch size, and replay it for later batches of
+ * the same shape (plain and mixed forms; the refined form decides per block on
+ * the host).  Results are bitwise those of the launches issued one by one. */
+#define NEGF_FLAG_GRAPH 16
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
@@ -144,6 +156,14 @@ int negf_set_onsite(negf_ctx* ctx, const double* onsite);
 /* Contact-cache state (NEGF_FLAG_CACHE_CONTACTS): stored energies, and energies
  * served from the cache since it was last invalidated. */
 int negf_contact_cache_stats(negf_ctx* ctx, int* entries, long long* hits);
+/* NEGF_PREC_REFINED, no counterpart in the reference (whose Invert_Matrix is one
+ * FP64 zgetrf + zgetri, MKL_Routines_Cluster.F90:19,23): block inverses taken since
+ * negf_init, FP64 Newton steps applied to them, and blocks escalated to the FP64
+ * inverse. */
+int negf_refine_stats(negf_ctx* ctx, long long* blocks, long long* steps, long long* escalated);
+/* NEGF_FLAG_GRAPH, no counterpart in the reference: graphs captured and graph
+ * replays (batches launched from a graph) since negf_init. */
+int negf_graph_stats(negf_ctx* ctx, long long* built, long long* replays);
 
 /* Energy_Sweep (Energy_Sweep.F90:1-46) for one energy, fused on the device:
  * contacts, RGF, Gn/Gp site blocks and observables.  mu[2], muT give the step
@@ -246,6 +266,7 @@ int negf_zgemm(char transa, char transb, int n, const double* alpha, const doubl
 int negf_gemm_batch(int complex64, int m3, int n, int batch, const void* A, const void* B, void* C);
 int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG, int* info);
 
+
 /* --- Serving and batching of whole transport solves (no single counterpart in
  * the reference: it runs one wire per process, NEGF_Main.F90:21-38) ----------
  * A server owns one GPU.  Each request is one wire -- its own dimensions,
@@ -324,6 +345,20 @@ int negf_profile_end(double* flops, double* ms, long long* launches);
  * *ms_per_launch gets the mean launch time. */
 int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
                     double* ms_per_launch);
+/* The same with the workgroup tile fixed (0: 64x64, 1: 32x32), for the tile autotuner
+ * (nonequilibriumgreensfunction_amd/autotune.py). */
+int negf_bench_gemm_tile(int device, int complex64, int n, int batch, int reps, int m3, int tile,
+                         double* ms_per_launch);
+/* Tuning table of the complex product kernel (no counterpart in the reference): for
+ * complex64 (complex64 != 0) or complex128 products, `count` rules (n_max[i],
+ * tile[i]) with increasing n_max; a product whose larger dimension n is at most
+ * n_max[i] (first such rule) runs with tile[i] (0: 64x64, 1: 32x32 workgroup
+ * tiles), larger ones with 64x64.  Call before solves start.  The Python layer
+ * loads the table the autotuner wrote. */
+int negf_set_gemm_tiles(int complex64, int count, const int* n_max, const int* tile);
+/* Lanes a context gets when negf_config.lanes is 0: the first rule with block size
+ * n <= n_max[i] gives lanes[i] (1..4); no rule: 4.  Set by the autotuner's table. */
+int negf_set_lane_table(int count, const int* n_max, const int* lanes);
 
 #ifdef __cplusplus
 }

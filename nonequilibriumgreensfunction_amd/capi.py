@@ -26,6 +26,7 @@ NEGF_STATUS_SR_NOT_CONVERGED = 1 << 20
 NEGF_STATUS_SR_NAN = 1 << 21
 NEGF_PREC_FP64 = 0
 NEGF_PREC_MIXED = 1
+NEGF_PREC_REFINED = 2
 
 # Every entry point include/negf_capi.h declares (checked by the CPU test suite).
 EXPORTS = (
@@ -36,7 +37,8 @@ EXPORTS = (
     "negf_observables_f", "negf_set_hamiltonian_compact", "negf_set_onsite", "negf_contact_cache_stats",
     "negf_server_create", "negf_server_submit", "negf_server_poll",
     "negf_server_wait", "negf_server_destroy", "negf_bench_gemm", "negf_gemm_batch",
-    "negf_invert_batch",
+    "negf_invert_batch", "negf_refine_stats", "negf_graph_stats", "negf_bench_gemm_tile",
+    "negf_set_gemm_tiles", "negf_set_lane_table",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -55,6 +57,7 @@ FLAG_RGF_THREE_PASS = 1
 FLAG_SR_SEPARATE_LEADS = 2
 FLAG_SINGLE_LANE = 4
 FLAG_CACHE_CONTACTS = 8
+FLAG_GRAPH = 16
 
 
 class NegfAccum(ctypes.Structure):
@@ -122,13 +125,19 @@ def lib():
         L.negf_set_hamiltonian_compact.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.negf_set_onsite.argtypes = [ctypes.c_void_p, _dp]
         L.negf_contact_cache_stats.argtypes = [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]
-        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 8
+        L.negf_refine_stats.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3
+        L.negf_graph_stats.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2
+        L.negf_bench_gemm_tile.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
+        L.negf_set_gemm_tiles.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
+        L.negf_set_lane_table.argtypes = [ctypes.c_int, _ip, _ip]
+        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 10
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
             L.negf_observables.argtypes = ([ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 3
                                            + [_dp] * 9 + [ctypes.POINTER(NegfAccum)] + [_dp] * 3)
         _LIB = L
+        _load_tuning(L)
     return _LIB
 
 
@@ -169,6 +178,39 @@ def profile_end():
     n = (ctypes.c_longlong * nk)()
     _check(lib().negf_profile_end(f, m, n))
     return {k: (f[i], m[i], n[i]) for i, k in enumerate(PROFILE_KINDS)}
+
+
+TUNING_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "table.json")
+
+
+def _load_tuning(L, path=None):
+    """Install the autotuner's table (tuning/table.json) in the library: GEMM
+    workgroup tiles per element type and block size, and lanes per block size."""
+    path = path or os.environ.get("NEGF_TUNING_TABLE", TUNING_TABLE)
+    if not path or not os.path.exists(path):
+        return None
+    import json
+    with open(path) as f:
+        t = json.load(f)
+    for key, c64 in (("gemm_tile_fp64", 0), ("gemm_tile_c64", 1)):
+        rules = t.get(key, [])
+        nm = np.array([r[0] for r in rules], np.int32)
+        tl = np.array([r[1] for r in rules], np.int32)
+        _check(L.negf_set_gemm_tiles(c64, len(rules), nm.ctypes.data_as(_ip), tl.ctypes.data_as(_ip)))
+    rules = t.get("lanes", [])
+    nm = np.array([r[0] for r in rules], np.int32)
+    ln = np.array([r[1] for r in rules], np.int32)
+    _check(L.negf_set_lane_table(len(rules), nm.ctypes.data_as(_ip), ln.ctypes.data_as(_ip)))
+    return t
+
+
+def bench_gemm_tile(n, batch, tile, complex64=True, m3=True, reps=8, device=0):
+    """Mean launch time (ms) of the product kernel with workgroup tile `tile`
+    (0: 64x64, 1: 32x32), for the autotuner."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().negf_bench_gemm_tile(int(device), int(bool(complex64)), int(n), int(batch), int(reps),
+                                      int(bool(m3)), int(tile), ctypes.byref(ms)))
+    return ms.value
 
 
 def bench_gemm(n, batch, complex64=True, m3=True, reps=8, device=0):
@@ -314,6 +356,18 @@ class Context:
         V = np.asfortranarray(onsite, dtype=float)
         self._onsite_keep = V
         _check(lib().negf_set_onsite(self._h, _p(V)))
+
+    def graph_stats(self):
+        """NEGF_FLAG_GRAPH: (graphs captured, batches replayed from a graph)."""
+        b, r = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().negf_graph_stats(self._h, ctypes.byref(b), ctypes.byref(r)))
+        return b.value, r.value
+
+    def refine_stats(self):
+        """Refined form: (block inverses, FP64 Newton steps, blocks escalated to FP64)."""
+        b, st, e = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().negf_refine_stats(self._h, ctypes.byref(b), ctypes.byref(st), ctypes.byref(e)))
+        return b.value, st.value, e.value
 
     def contact_cache_stats(self):
         """(stored energies, energies served from the contact cache)."""

@@ -131,6 +131,14 @@ GemmTimer& gemm_timer();
 // Reserves an event pair for a sampled launch of `kind` with algorithmic `work`;
 // false when this launch is not sampled (or timing is off).
 bool timer_slot(double work, int kind, hipEvent_t* e0, hipEvent_t* e1);
+// While on, this host thread's launches are not sampled (it is capturing a graph).
+void timer_capture(bool on);
+// GEMM workgroup tile per element type and block size (0: 64x64, 1: 32x32), from
+// the tuning table set by negf_set_gemm_tiles; force_gemm_tile(t >= 0) overrides it
+// for the calling thread (-1 restores the table).
+int gemm_tile(int fp32, int n);
+int set_gemm_tiles(int fp32, int count, const int* n_max, const int* tile);
+void force_gemm_tile(int tile);
 
 }  // namespace negf
 

@@ -48,6 +48,7 @@ hipError_t launch_bd_mul(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,
 template <typename T>
 hipError_t launch_siteblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              int, double*, long long, long long, const SpinMats&, int, const int*);
+hipError_t launch_zero(hipStream_t, void*, size_t);
 template <typename T>
 hipError_t launch_bondblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              const double2*, const double2*, int, int, int, double*, long long,
@@ -109,6 +110,12 @@ static const double kQ = 1.602e-19;   // Constants.F90:4
 static const double kH = 6.626e-34;   // Constants.F90:6
 
 // ---------------------------------------------------------------------------
+// Refined form: Newton steps on a complex64 block inverse stop at max|I - M X| <=
+// kRefineDone; at most kRefineSteps; escalation to FP64 when the first residual
+// exceeds kRefineStart (|R| < 1 is needed for contraction; 0.25 leaves margin for
+// the max-entry norm) or the last one exceeds kRefineAccept.
+constexpr double kRefineDone = 1e-12, kRefineStart = 0.25, kRefineAccept = 1e-10;
+constexpr int kRefineSteps = 2;
 constexpr int kSiteQ = 56;      // site-output planes per (lead, slice, site)
 constexpr int kSiteQBasic = 8;  // planes without the bond-current terms
 struct Layout {
@@ -176,6 +183,20 @@ struct Layout {
   }
 };
 
+// What a captured RGF graph depends on besides device memory contents: the
+// context (its Hamiltonian buffers and spin matrices), the Hamiltonian upload
+// generation, the batch size, whether bond currents are formed, and the arena.
+struct GraphKey {
+  unsigned long long ctx;  // the context's unique id (never reused, unlike its address)
+  unsigned long long gen;
+  int nb;
+  bool currents;
+  const void* arena;
+  bool operator==(const GraphKey& o) const {
+    return ctx == o.ctx && gen == o.gen && nb == o.nb && currents == o.currents && arena == o.arena;
+  }
+};
+static std::atomic<unsigned long long> g_ctx_uid{1};
 // One independent stream of energy batches: its own HIP stream and arena.  The
 // lanes run concurrently (one host thread each), so the latency-bound panel
 // factorizations of some lanes overlap the MFMA GEMMs of the others.
@@ -195,6 +216,20 @@ struct Lane {
   // iters L/R, status L/R), published once the batch's stream has drained
   struct CachePending { double energy; int slot, it0, it1, st0, st1; };
   std::vector<CachePending> cc_pending;
+  // captured RGF launches (NEGF_FLAG_GRAPH), a few shapes at a time (a flush's
+  // full chunks and its last, shorter one alternate)
+  static constexpr int kGraphSlots = 4;
+  hipGraphExec_t gexec[kGraphSlots] = {};
+  GraphKey gkey[kGraphSlots] = {};
+  unsigned long long guse[kGraphSlots] = {};
+  unsigned long long gclock = 0;
+  void drop_graphs() {
+    for (int i = 0; i < kGraphSlots; ++i) {
+      if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+      gexec[i] = nullptr;
+      guse[i] = 0;
+    }
+  }
 };
 
 // Contact self-energies kept across sweeps (NEGF_FLAG_CACHE_CONTACTS): per energy
@@ -229,6 +264,12 @@ struct negf_ctx {
   bool blockdiag = true;
   bool ends_equal = false;  // H slice 1 == H slice L bitwise: one decimation serves both leads
   bool want_currents = false;  // the caller's accumulators take bond currents (set per flush)
+  int cap_budget = 0;                 // energies per lane the HBM budget allowed at the last allocation
+  unsigned long long uid = 0;         // unique per context (set by negf_init)
+  unsigned long long generation = 0;  // bumped by every Hamiltonian upload
+  std::atomic<long long> graphs_built{0}, graph_replays{0};
+  // refined form: block inverses taken, Newton steps applied, blocks escalated to FP64
+  std::atomic<long long> refine_blocks{0}, refine_steps{0}, refine_escalated{0};
   ContactCache cc;
   std::vector<zd> h0_host, hopx_host;  // compact form: template and HopX (for set_onsite)
   zd* H = nullptr;       // Htotd (n,n,L), or null for a compact Hamiltonian:
@@ -284,6 +325,7 @@ static int require_device(int dev) {
 
 static void free_arena(negf_ctx* c) {
   for (Lane& l : c->lanes) {
+    l.drop_graphs();
     hipFree(l.E); hipFree(l.C); hipFree(l.Ie); hipFree(l.Ic);
     hipFree(l.d_energy); hipFree(l.d_scal); hipFree(l.d_site); hipFree(l.d_mc);
     hipFree(l.d_active);
@@ -293,6 +335,34 @@ static void free_arena(negf_ctx* c) {
     l.d_active = nullptr;
     l.cap = 0;
   }
+}
+
+// Lanes when the config leaves them at 0: the tuning table's rule for the block
+// size (negf_set_lane_table), else four.
+static int g_lane_rules[16][2];
+static int g_nlane_rules = 0;
+static int default_lanes(int n) {
+  for (int i = 0; i < g_nlane_rules; ++i)
+    if (n <= g_lane_rules[i][0]) return g_lane_rules[i][1];
+  return kMaxLanes;
+}
+
+extern "C" int negf_set_lane_table(int count, const int* n_max, const int* lanes) {
+  if (count < 0 || count > 16 || (count && (!n_max || !lanes))) {
+    negf_set_error("negf_set_lane_table: at most 16 rules");
+    return NEGF_ERR_ARG;
+  }
+  for (int i = 0; i < count; ++i)
+    if (lanes[i] < 1 || lanes[i] > kMaxLanes || (i && n_max[i] <= n_max[i - 1])) {
+      negf_set_error("negf_set_lane_table: lanes must be 1..%d and n_max increasing", kMaxLanes);
+      return NEGF_ERR_ARG;
+    }
+  for (int i = 0; i < count; ++i) {
+    g_lane_rules[i][0] = n_max[i];
+    g_lane_rules[i][1] = lanes[i];
+  }
+  g_nlane_rules = count;
+  return 0;
 }
 
 extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
@@ -312,30 +382,32 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
     negf_set_error("block size n=%d exceeds 3072", n);
     return NEGF_ERR_ARG;
   }
-  if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED) {
+  if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED &&
+      cfg->precision != NEGF_PREC_REFINED) {
     negf_set_error("unknown precision form %d", cfg->precision);
     return NEGF_ERR_ARG;
   }
-  if (cfg->precision == NEGF_PREC_MIXED && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
-    negf_set_error("the mixed-precision form runs on the two-pass RGF only");
+  if (cfg->precision != NEGF_PREC_FP64 && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    negf_set_error("the mixed-precision forms run on the two-pass RGF only");
     return NEGF_ERR_ARG;
   }
   int rc = require_device(cfg->device);
   if (rc) return rc;
   negf_ctx* c = new negf_ctx();
   c->cfg = *cfg;
+  c->uid = g_ctx_uid.fetch_add(1);
   if (c->cfg.sr_max_iter < 0) c->cfg.sr_max_iter = 0;
   c->n = n;
   c->L = cfg->npx;
   c->nsites = cfg->npy * cfg->npz;
   c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS),
-              cfg->precision == NEGF_PREC_MIXED);
+              cfg->precision != NEGF_PREC_FP64);
   // Four lanes by default: at n >= 1024 the inverse chains are latency-bound
   // (panel kernels of one workgroup per matrix), and four concurrent chains keep
   // the matrix cores fed with the other lanes' GEMMs (measured: 11.8 -> 10.1 s per
   // 64-energy step at n=1024, 23.5 -> 20.7 s per 8-energy step at n=2304 against
   // two lanes).
-  c->nlanes = cfg->lanes > 0 ? std::min(cfg->lanes, kMaxLanes) : kMaxLanes;
+  c->nlanes = cfg->lanes > 0 ? std::min(cfg->lanes, kMaxLanes) : default_lanes(n);
   for (int li = 0; li < c->nlanes; ++li)
     if (hipStreamCreateWithFlags(&c->lanes[li].st, hipStreamNonBlocking) != hipSuccess) {
       negf_set_error("hipStreamCreate failed");
@@ -405,7 +477,7 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
         }
   }
   if (!bd && !c->tmpP) HIPCHK(hipMalloc(&c->tmpP, sizeof(zd) * nn * 2));
-  if (c->cfg.precision == NEGF_PREC_MIXED) {
+  if (c->cfg.precision != NEGF_PREC_FP64) {
     if (!c->X32) HIPCHK(hipMalloc(&c->X32, sizeof(cx<float>) * nn));
     HIPCHK(to_c64(c->st, c->X32, 0, c->X, 0, nn, 1));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -438,6 +510,7 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
   const long long nn = (long long)c->n * c->n;
   int rc = upload_couplings(c, HopX, gM);
   if (rc) return rc;
+  ++c->generation;
   hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
   c->H0 = c->Hends = nullptr;
   c->V = nullptr;
@@ -463,6 +536,7 @@ extern "C" int negf_set_hamiltonian_compact(negf_ctx* c, const double* H0, const
   const long long nn = (long long)n * n;
   int rc = upload_couplings(c, HopX, gM);
   if (rc) return rc;
+  ++c->generation;
   hipFree(c->H);
   c->H = nullptr;
   if (!c->H0) {
@@ -502,6 +576,22 @@ static int set_potentials(negf_ctx* c, const double* onsite) {
   return 0;
 }
 
+extern "C" int negf_graph_stats(negf_ctx* c, long long* built, long long* replays) {
+  if (!c) return NEGF_ERR_ARG;
+  if (built) *built = c->graphs_built.load();
+  if (replays) *replays = c->graph_replays.load();
+  return 0;
+}
+
+extern "C" int negf_refine_stats(negf_ctx* c, long long* blocks, long long* steps,
+                                 long long* escalated) {
+  if (!c) return NEGF_ERR_ARG;
+  if (blocks) *blocks = c->refine_blocks.load();
+  if (steps) *steps = c->refine_steps.load();
+  if (escalated) *escalated = c->refine_escalated.load();
+  return 0;
+}
+
 extern "C" int negf_contact_cache_stats(negf_ctx* c, int* entries, long long* hits) {
   if (!c) return NEGF_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->cc.mu);
@@ -533,9 +623,14 @@ static const zd* slice_end(negf_ctx* c, int which) {
 // HBM budget), split evenly over the lanes.
 static int ensure_arena(negf_ctx* c, int want) {
   const int nl = (c->cfg.flags & NEGF_FLAG_SINGLE_LANE) ? 1 : std::max(1, std::min(c->nlanes, want));
-  const int per_lane = (want + nl - 1) / nl;
+  int per_lane = (want + nl - 1) / nl;
+  if (c->cfg.max_batch > 0) per_lane = std::min(per_lane, std::max(1, c->cfg.max_batch / nl));
+  // the lanes in use hold per_lane energies each, or as many as the HBM budget allowed
+  // when they were allocated (then a larger arena would not fit either)
   bool ready = true;
-  for (int li = 0; li < nl; ++li) ready = ready && c->lanes[li].E && c->lanes[li].cap >= per_lane;
+  for (int li = 0; li < nl; ++li)
+    ready = ready && c->lanes[li].E &&
+            (c->lanes[li].cap >= per_lane || c->lanes[li].cap == c->cap_budget);
   if (ready) return 0;
   const Layout& y = c->lay;
   size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(3 * y.Is) +
@@ -545,8 +640,8 @@ static int ensure_arena(negf_ctx* c, int want) {
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
   double budget = c->cfg.hbm_budget_gb > 0 ? c->cfg.hbm_budget_gb * 1e9 : 0.6 * (double)freeb;
   if (budget > 0.9 * (double)freeb) budget = 0.9 * (double)freeb;
-  int cap = (int)std::min<double>(per_lane, budget / nl / (double)per_e);
-  if (c->cfg.max_batch > 0) cap = std::min(cap, std::max(1, c->cfg.max_batch / nl));
+  const int cap_budget = (int)std::min<double>(1 << 20, budget / nl / (double)per_e);
+  int cap = std::min(per_lane, cap_budget);
   if (cap < 1) {
     negf_set_error("one energy needs %.2f GB of device memory; budget %.2f GB", per_e / 1e9,
                    budget / 1e9);
@@ -565,6 +660,7 @@ static int ensure_arena(negf_ctx* c, int want) {
     HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 4 * cap));
     l.cap = cap;
   }
+  c->cap_budget = cap_budget;
   return 0;
 }
 
@@ -620,7 +716,7 @@ static GjWork gjw(zd* prow_base, long long sProw, int* ints, long long sInt, int
 // 1e-10 relative; the 3M form in the mixed form, where the normwise error it adds
 // to Sigma (~1e-16 |Sigma|, ~1e-10 of an eta-scale DOSc) is far inside that form's
 // bounds.
-static int sr_m3(const negf_ctx* c) { return c->cfg.precision == NEGF_PREC_MIXED ? 1 : 0; }
+static int sr_m3(const negf_ctx* c) { return c->cfg.precision != NEGF_PREC_FP64 ? 1 : 0; }
 static GemmBatch gbs(const negf_ctx* c, const zd* A, long long sA, int lda, const zd* B, long long sB,
                      int ldb, zd* C, long long sC, int ldc, int n, const int* active = nullptr) {
   GemmBatch g = gb(A, sA, lda, B, sB, ldb, C, sC, ldc, n, active);
@@ -643,7 +739,7 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
   iters.assign(nc, 0);
   status.assign(nc, 0);
   // clear INFO of every chain slot
-  TRY(hipMemsetAsync(ln.Ic, 0, sizeof(int) * (size_t)y.Is * nc, s));
+  TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nc));
   // tmp = inv(seed)   (Contact_Self_Energy.F90:141)
   TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nc, w, nullptr));
   // t = -tmp*betad, tt = -tmp*beta   (:147, :149)
@@ -767,7 +863,7 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   w.m3 = sr_m3(c);
   iters.assign(2 * nb, 0);
   status.assign(2 * nb, 0);
-  TRY(hipMemsetAsync(ln.Ic, 0, sizeof(int) * (size_t)y.Is * 2 * nb, s));
+  TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * 2 * nb));
   // tmp = inv(seed); t = -tmp*betad_L = tmp X, tt = -tmp*beta_L = tmp X'  (:141-149)
   TRY(gj_invert<double>(s, P(y.c_work), Cs, n, P(y.c_tmp), Cs, n, n, nb, w, nullptr));
   TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, c->betaR, 0, n, P(y.c_t), Cs, n, n), nb,
@@ -947,8 +1043,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   zd* E0 = ln.E;
   auto Q = [&](long long off) { return E0 + off; };
   TRY(hipMemcpyAsync(ln.d_energy, energies, sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  TRY(hipMemsetAsync(ln.d_scal, 0, sizeof(double) * 8 * nb, s));
-  TRY(hipMemsetAsync(ln.Ie, 0, sizeof(int) * (size_t)y.Is * nb, s));
+  TRY(launch_zero(s, ln.d_scal, sizeof(double) * 8 * nb));
+  TRY(launch_zero(s, ln.Ie, sizeof(int) * (size_t)y.Is * nb));
   // ---- contacts: chains [0,nb) left lead, [nb,2nb) right lead
   std::vector<int> iters, sst;
   ln.cc_pending.clear();
@@ -1018,276 +1114,393 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
                          sizeof(zd) * nn, hipMemcpyDeviceToDevice, s));
     }
   }
-  TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
-  TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
-  {  // Gamma_L = W_L W_L', Gamma_R = W_R W_R' (pivoted Cholesky)
-    PcholArgs pa = {};
-    pa.gam[0] = Q(y.o_GamL); pa.gam[1] = Q(y.o_GamR);
-    pa.w[0] = Q(y.o_WL); pa.w[1] = Q(y.o_WR);
-    pa.st[0] = (char*)Q(y.o_pst); pa.st[1] = pa.st[0] + y.pst_bytes;
-    pa.sG = pa.sW = Es;
-    pa.sSt = Es * (long long)sizeof(zd);
-    pa.n = n;
-    TRY(launch_pchol(s, pa, nb));
-  }
+  // Everything after the contacts issues device work only (no host syncs, no
+  // memsets) in the plain and mixed forms, so with NEGF_FLAG_GRAPH it is captured
+  // once into a HIP graph per (lane, context, Hamiltonian, batch size) and replayed
+  // for later batches of the same shape; the graph reads the energies, Sigma and
+  // the Hamiltonian from device memory, so a replay solves the new batch.
+  auto rgf = [&]() -> int {
+    TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
+    TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
+    {  // Gamma_L = W_L W_L', Gamma_R = W_R W_R' (pivoted Cholesky)
+      PcholArgs pa = {};
+      pa.gam[0] = Q(y.o_GamL); pa.gam[1] = Q(y.o_GamR);
+      pa.w[0] = Q(y.o_WL); pa.w[1] = Q(y.o_WR);
+      pa.st[0] = (char*)Q(y.o_pst); pa.st[1] = pa.st[0] + y.pst_bytes;
+      pa.sG = pa.sW = Es;
+      pa.sSt = Es * (long long)sizeof(zd);
+      pa.n = n;
+      TRY(launch_pchol(s, pa, nb));
+    }
 
-  GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
-  w.m3 = 1;
-  const bool bd = c->blockdiag;
-  auto slot = [&](int x) { return Q(y.o_slots + (long long)x * nn); };
-  // slice x of the Hamiltonian: Htotd(:,:,x), or the template plus the slice's potentials
-  auto hx = [&](int x) { return c->compact ? (const zd*)c->H0 : (const zd*)(c->H + (size_t)x * nn); };
-  auto hv = [&](int x) { return c->compact ? (const double*)(c->V + (size_t)x * ns) : nullptr; };
-  // Assemble M = zI - H_x - [SigL if x==0] - [SigR if x==L-1] - sandwich(g) into o_M.
-  auto assemble = [&](int x, const zd* g, int mode) -> int {
-    const zd* S1 = (x == 0) ? Q(y.o_SigL) : nullptr;
-    const zd* S2 = (x == L - 1) ? Q(y.o_SigR) : nullptr;
-    if (g && !bd) {
-      // dense HopX: P = X' g X (mode 1) or X g X' (mode 2) with two GEMMs, then subtract
-      zd* T1 = Q(y.o_P);
-      zd* T2 = Q(y.o_Y);
-      if (mode == 1) {
-        TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
-      } else {
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
-        TRY(gemm_batched<double>(s, OP_N, OP_C, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+    GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
+    w.m3 = 1;
+    const bool bd = c->blockdiag;
+    auto slot = [&](int x) { return Q(y.o_slots + (long long)x * nn); };
+    // slice x of the Hamiltonian: Htotd(:,:,x), or the template plus the slice's potentials
+    auto hx = [&](int x) { return c->compact ? (const zd*)c->H0 : (const zd*)(c->H + (size_t)x * nn); };
+    auto hv = [&](int x) { return c->compact ? (const double*)(c->V + (size_t)x * ns) : nullptr; };
+    // Assemble M = zI - H_x - [SigL if x==0] - [SigR if x==L-1] - sandwich(g) into o_M.
+    auto assemble = [&](int x, const zd* g, int mode) -> int {
+      const zd* S1 = (x == 0) ? Q(y.o_SigL) : nullptr;
+      const zd* S2 = (x == L - 1) ? Q(y.o_SigR) : nullptr;
+      if (g && !bd) {
+        // dense HopX: P = X' g X (mode 1) or X g X' (mode 2) with two GEMMs, then subtract
+        zd* T1 = Q(y.o_P);
+        zd* T2 = Q(y.o_Y);
+        if (mode == 1) {
+          TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+        } else {
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(c->X, 0, n, g, Es, n, T1, Es, n, n), nb, kOne, kZero));
+          TRY(gemm_batched<double>(s, OP_N, OP_C, gbr(T1, Es, n, c->X, 0, n, T2, Es, n, n), nb, kOne, kZero));
+        }
+        TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy, c->cfg.eta, S1,
+                                    Es, S2, Es, T2, Es, c->X, n, nb, nullptr, hv(x)));
+        return 0;
       }
-      TRY(launch_assemble<double>(s, no, 3, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy, c->cfg.eta, S1,
-                                  Es, S2, Es, T2, Es, c->X, n, nb, nullptr, hv(x)));
-      return 0;
-    }
-    TRY(launch_assemble<double>(s, no, g ? mode : 0, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy,
-                                c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr, hv(x)));
-    return 0;
-  };
-  // Y = X G (conj=0) or X' G (conj=1)
-  auto xmul = [&](int conj, zd* Y, const zd* G) -> int {
-    if (bd) {
-      TRY(launch_bd_mul<double>(s, no, conj, Y, Es, G, Es, c->X, n, nb, nullptr));
-    } else {
-      TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gbr(c->X, 0, n, G, Es, n, Y, Es, n, n), nb,
-                               kOne, kZero));
-    }
-    return 0;
-  };
-  // The column chains run on V = G W (W the lead's factor, Gamma = W W'): both
-  // recursions act on the left, so they carry a right factor along, and
-  //   Gn site blocks:  (G Gamma G')_ss = (V V')_ss,
-  //   trace terms:     Re tr(Gam_a G Gam_b G') = |W_a' V_b|_F^2.
-  // Gn/Gp site blocks of V = G W_lead (lead 0 = left, 1 = right) at slice x
-  auto site_out = [&](int lead, int x) { return ln.d_site + ((long long)lead * L + x) * ns; };
-  auto siteblocks = [&](int lead, int x, const zd* V) -> int {
-    TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, site_out(lead, x), y.site_per_e, y.site_q,
-                                  c->sp, nb, nullptr));
-    return 0;
-  };
-  // Bond-current terms of slice x for one lead (Calculate_Observables.F90:111-168)
-  // from V_x = G_x. W and V_{x+1} (null at the last slice: no x bond there).
-  const int npy = c->cfg.npy, npz = c->cfg.npz;
-  auto bonds = [&](int lead, int x, const zd* V, const zd* Vn) -> int {
-    if (!c->want_currents) return 0;
-    TRY(launch_bondblocks<double>(s, no, V, Es, Vn, Es, hx(x), c->X, n, npy, npz, site_out(lead, x),
-                                  y.site_per_e, y.site_q, nb, nullptr));
-    return 0;
-  };
-  // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b
-  auto trace_term = [&](int lead, const zd* V, int idx) -> int {
-    const zd* Wa = Q(lead ? y.o_WR : y.o_WL);
-    TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(Wa, Es, n, V, Es, n, Q(y.o_P), Es, n, n), nb, kOne,
-                             kZero));
-    TRY(launch_inner<double>(s, Q(y.o_P), Es, Q(y.o_P), Es, n, ln.d_scal + idx, 8, nb, nullptr));
-    return 0;
-  };
-  // V = G W_lead into dst
-  auto times_w = [&](const zd* G, int lead, zd* dst) -> int {
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(G, Es, n, Q(lead ? y.o_WR : y.o_WL), Es, n, dst, Es, n, n),
-                             nb, kOne, kZero));
-    return 0;
-  };
-  int r;
-  if (c->cfg.precision == NEGF_PREC_MIXED) {
-    // ---- mixed-precision two-pass RGF.  FP64: the inverses gL_x and the last block
-    // column G_xL W_R (T12 = |W_L' G_1L W_R|^2 stays an FP64 product chain).
-    // complex64 (3M MFMA): the forward/back substitution for the first block column
-    // G_x1 W_L and the site blocks of both columns -- they feed densities, spin and
-    // the injected modes only, at complex64 accuracy.
-    typedef cx<float> zf;
-    zf* F0 = (zf*)Q(y.o_f32);
-    auto F = [&](long long off) { return F0 + off; };
-    const long long Es32 = 2 * Es;  // complex64 stride between energies
-    auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
-    const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
-    auto gbf = [&](const zf* A, const zf* B, zf* C) {
-      GemmBatch g = {};
-      g.A = A; g.sA = Es32; g.lda = n;
-      g.B = B; g.sB = Es32; g.ldb = n;
-      g.C = C; g.sC = Es32; g.ldc = n;
-      g.m = n; g.n = n; g.k = n;
-      g.m3 = 1;
-      return g;
-    };
-    TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, nn, nb));
-    TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, nn, nb));
-    // site blocks of V32 = G32 W_lead for one lead at slice x
-    auto site32 = [&](int lead, int x, const zf* V32) -> int {
-      TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, site_out(lead, x), y.site_per_e,
-                                   y.site_q, c->sp, nb, nullptr));
+      TRY(launch_assemble<double>(s, no, g ? mode : 0, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy,
+                                  c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr, hv(x)));
       return 0;
     };
-    auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
+    // Y = X G (conj=0) or X' G (conj=1)
+    auto xmul = [&](int conj, zd* Y, const zd* G) -> int {
+      if (bd) {
+        TRY(launch_bd_mul<double>(s, no, conj, Y, Es, G, Es, c->X, n, nb, nullptr));
+      } else {
+        TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gbr(c->X, 0, n, G, Es, n, Y, Es, n, n), nb,
+                                 kOne, kZero));
+      }
+      return 0;
+    };
+    // The column chains run on V = G W (W the lead's factor, Gamma = W W'): both
+    // recursions act on the left, so they carry a right factor along, and
+    //   Gn site blocks:  (G Gamma G')_ss = (V V')_ss,
+    //   trace terms:     Re tr(Gam_a G Gam_b G') = |W_a' V_b|_F^2.
+    // Gn/Gp site blocks of V = G W_lead (lead 0 = left, 1 = right) at slice x
+    auto site_out = [&](int lead, int x) { return ln.d_site + ((long long)lead * L + x) * ns; };
+    auto siteblocks = [&](int lead, int x, const zd* V) -> int {
+      TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, site_out(lead, x), y.site_per_e, y.site_q,
+                                    c->sp, nb, nullptr));
+      return 0;
+    };
+    // Bond-current terms of slice x for one lead (Calculate_Observables.F90:111-168)
+    // from V_x = G_x. W and V_{x+1} (null at the last slice: no x bond there).
+    const int npy = c->cfg.npy, npz = c->cfg.npz;
+    auto bonds = [&](int lead, int x, const zd* V, const zd* Vn) -> int {
       if (!c->want_currents) return 0;
-      TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
-                                   site_out(lead, x), y.site_per_e, y.site_q, nb, nullptr));
+      TRY(launch_bondblocks<double>(s, no, V, Es, Vn, Es, hx(x), c->X, n, npy, npz, site_out(lead, x),
+                                    y.site_per_e, y.site_q, nb, nullptr));
       return 0;
     };
-    // scal[idx] = |W_lead' V32|_F^2
-    auto trace32 = [&](int lead, const zf* V32, int idx) -> int {
-      GemmBatch g = gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P));
-      TRY(gemm_batched<float>(s, OP_C, OP_N, g, nb, kOnef, kZerof));
-      TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, ln.d_scal + idx, 8, nb, nullptr));
+    // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b
+    auto trace_term = [&](int lead, const zd* V, int idx) -> int {
+      const zd* Wa = Q(lead ? y.o_WR : y.o_WL);
+      TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(Wa, Es, n, V, Es, n, Q(y.o_P), Es, n, n), nb, kOne,
+                               kZero));
+      TRY(launch_inner<double>(s, Q(y.o_P), Es, Q(y.o_P), Es, n, ln.d_scal + idx, 8, nb, nullptr));
       return 0;
     };
-    // forward: gL_x in FP64, z_x = gL_x X' z_{x-1} in complex64 (z_1 = gL_1 W_L)
+    // V = G W_lead into dst
+    auto times_w = [&](const zd* G, int lead, zd* dst) -> int {
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(G, Es, n, Q(lead ? y.o_WR : y.o_WL), Es, n, dst, Es, n, n),
+                               nb, kOne, kZero));
+      return 0;
+    };
+    int r;
+    if (c->cfg.precision != NEGF_PREC_FP64) {
+      // ---- mixed-precision two-pass RGF.  FP64: the inverses gL_x and the last block
+      // column G_xL W_R (T12 = |W_L' G_1L W_R|^2 stays an FP64 product chain).
+      // complex64 (3M MFMA): the forward/back substitution for the first block column
+      // G_x1 W_L and the site blocks of both columns -- they feed densities, spin and
+      // the injected modes only, at complex64 accuracy.
+      typedef cx<float> zf;
+      zf* F0 = (zf*)Q(y.o_f32);
+      auto F = [&](long long off) { return F0 + off; };
+      const long long Es32 = 2 * Es;  // complex64 stride between energies
+      auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
+      const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
+      auto gbf = [&](const zf* A, const zf* B, zf* C) {
+        GemmBatch g = {};
+        g.A = A; g.sA = Es32; g.lda = n;
+        g.B = B; g.sB = Es32; g.ldb = n;
+        g.C = C; g.sC = Es32; g.ldc = n;
+        g.m = n; g.n = n; g.k = n;
+        g.m3 = 1;
+        return g;
+      };
+      TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, nn, nb));
+      TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, nn, nb));
+      // site blocks of V32 = G32 W_lead for one lead at slice x
+      auto site32 = [&](int lead, int x, const zf* V32) -> int {
+        TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, site_out(lead, x), y.site_per_e,
+                                     y.site_q, c->sp, nb, nullptr));
+        return 0;
+      };
+      auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
+        if (!c->want_currents) return 0;
+        TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
+                                     site_out(lead, x), y.site_per_e, y.site_q, nb, nullptr));
+        return 0;
+      };
+      // scal[idx] = |W_lead' V32|_F^2
+      auto trace32 = [&](int lead, const zf* V32, int idx) -> int {
+        GemmBatch g = gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P));
+        TRY(gemm_batched<float>(s, OP_C, OP_N, g, nb, kOnef, kZerof));
+        TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, ln.d_scal + idx, 8, nb, nullptr));
+        return 0;
+      };
+      // The block inverse gL_x = inv(M_x) of the forward pass.  Mixed form: FP64
+      // Gauss-Jordan.  Refined form: Gauss-Jordan in complex64 on the FP32 matrix
+      // cores, then FP64 Newton steps X <- X + X R with R = I - M X (the residual and
+      // the correction are FP64 products) until max|R| <= kRefineDone, at most
+      // kRefineSteps of them; a block whose complex64 inverse hit a zero pivot, whose
+      // first residual is too large for Newton to contract fast (> kRefineStart) or
+      // whose last one is above kRefineAccept is inverted again in FP64.  M_x stays
+      // in o_M throughout (the complex64 elimination works on a copy).
+      const bool refined = c->cfg.precision == NEGF_PREC_REFINED;
+      GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
+      w32.prow = Q(y.o_prow);  // kGjNBO x n complex64 fit in the complex128 pivot-row panel
+      w32.sProw = Es32;
+      w32.m3 = 1;
+      std::vector<double> rmax(nb);
+      std::vector<int> info32((size_t)y.Is * nb);
+      auto invert_block = [&](int x) -> int {
+        if (!refined) {
+          TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
+          return 0;
+        }
+        zf* M32 = F(y.f_G);  // both free during the forward pass
+        zf* X32 = F(y.f_P);
+        TRY(to_c64(s, M32, Es32, Q(y.o_M), Es, nn, nb));
+        TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nb));
+        TRY(gj_invert<float>(s, M32, Es32, n, X32, Es32, n, n, nb, w32, nullptr));
+        TRY((launch_convert<double, float>(s, slot(x), Es, X32, Es32, nn, nb, nullptr)));
+        TRY(hipMemcpyAsync(info32.data(), ln.Ic, sizeof(int) * info32.size(), hipMemcpyDeviceToHost, s));
+        std::vector<int> act(nb), esc;
+        for (int b = 0; b < nb; ++b) act[b] = b;
+        int* A = ln.d_active;
+        long long steps = 0;
+        for (int step = 0; !act.empty(); ++step) {
+          const int na = (int)act.size();
+          TRY(hipMemcpyAsync(A, act.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+          // R = I - M X
+          TRY(launch_eye<double>(s, Q(y.o_Y), Es, n, na, A));
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(Q(y.o_M), Es, n, slot(x), Es, n, Q(y.o_Y), Es, n, n, A),
+                                   na, kMone, kOne));
+          TRY(launch_maxabs<double>(s, Q(y.o_Y), Es, nn, ln.d_mc, na, A));
+          TRY(hipMemcpyAsync(rmax.data(), ln.d_mc, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+          TRY(hipStreamSynchronize(s));
+          std::vector<int> keep;
+          for (int b : act) {
+            const double rm = rmax[b];
+            const bool bad = !(rm == rm) || (step == 0 && (info32[(size_t)b * y.Is + 2 * n] != 0 ||
+                                                           rm > kRefineStart));
+            if (bad || (step == kRefineSteps && rm > kRefineAccept)) esc.push_back(b);
+            else if (rm > kRefineDone && step < kRefineSteps) keep.push_back(b);
+          }
+          act.swap(keep);
+          if (act.empty()) break;
+          // X <- X + X R for the blocks still refining
+          const int nr = (int)act.size();
+          TRY(hipMemcpyAsync(A, act.data(), sizeof(int) * nr, hipMemcpyHostToDevice, s));
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, Q(y.o_P), Es, n, n, A),
+                                   nr, kOne, kZero));
+          TRY(launch_add<double>(s, slot(x), Es, slot(x), Es, Q(y.o_P), Es, nn, nr, A));
+          steps += nr;
+        }
+        if (!esc.empty()) {  // per-block escalation to the FP64 inverse
+          TRY(hipMemcpyAsync(A, esc.data(), sizeof(int) * esc.size(), hipMemcpyHostToDevice, s));
+          TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, (int)esc.size(), w, A));
+        }
+        c->refine_blocks += nb;
+        c->refine_steps += steps;
+        c->refine_escalated += (long long)esc.size();
+        return 0;
+      };
+      // forward: gL_x in FP64, z_x = gL_x X' z_{x-1} in complex64 (z_1 = gL_1 W_L)
+      for (int x = 0; x < L; ++x) {
+        if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
+        if ((r = invert_block(x))) return r;
+        if (x == 0) {
+          TRY(to_c64(s, F(y.f_g), Es32, slot(0), Es, nn, nb));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0)), nb, kOnef, kZerof));
+        } else {
+          TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
+          TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kZerof));
+        }
+      }
+      // backward: G_xL W_R in FP64; G_x1 W_L = z_x + gL_x X G_{x+1,1} W_L in complex64
+      // (in place of z_x)
+      zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
+      if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
+      zd* colA = bufs[1];
+      zd* colB = nullptr;
+      int pb = 0;
+      for (int x = L - 1; x >= 0; --x) {
+        if (x < L - 1) {
+          colB = bufs[pb];
+          if ((r = xmul(0, Q(y.o_Y), colA))) return r;
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                                   kOne, kZero));
+          TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
+          TRY(launch_bd_mul<float>(s, no, 0, F(y.f_Y), Es32, zs(x + 1), Es32, c->X32, n, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kOnef));
+        }
+        const zd* GxL = (x < L - 1) ? colB : colA;
+        if (x == 0 && (r = trace_term(0, GxL, 0))) return r;  // T12 = |W_L' G_1L W_R|^2 in FP64
+        TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, nn, nb));
+        if ((r = site32(1, x, F(y.f_G)))) return r;
+        if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
+        if (x == L - 1 && (r = trace32(1, F(y.f_G), 3))) return r;  // Modec2, right-lead term
+        if ((r = site32(0, x, zs(x)))) return r;
+        if ((r = bonds32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
+        if (x == L - 1 && (r = trace32(1, zs(x), 2))) return r;  // Modec2, left-lead term
+        if (x == 0 && (r = trace32(0, zs(x), 1))) return r;      // Modec1, left-lead term
+        if (x < L - 1) {
+          colA = colB;
+          pb ^= 1;
+        }
+      }
+      return 0;
+    }
+    if (!(c->cfg.flags & NEGF_FLAG_RGF_THREE_PASS)) {
+      // ---- two-pass RGF.  Forward: gL_x (Recursive_Greens.F90:16-34) and the forward
+      // substitution z_x = gL_x X' z_{x-1}, z_1 = gL_1, of A Y = E_1 (block column 1).
+      auto zs = [&](int x) { return Q(y.o_z + (long long)x * nn); };
+      for (int x = 0; x < L; ++x) {
+        if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
+        TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
+        if (x == 0) {
+          if ((r = times_w(slot(0), 0, zs(0)))) return r;  // z_1 W_L = gL_1 W_L
+        } else {
+          if ((r = xmul(1, Q(y.o_Y), zs(x - 1)))) return r;
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
+                                   kOne, kZero));
+        }
+      }
+      // Backward: the last column G_xL = gL_x X G_{x+1,L}, G_LL = gL_L (:53-63), and the
+      // back substitution G_x1 = z_x + gL_x X G_{x+1,1}, G_L1 = z_L, in place of z_x.
+      // Slice x then has both columns: the Gn/Gp site blocks of both leads, T12 at x = 1
+      // (Calculate_Observables.F90:86-100) and the Modec terms at the ends (:48-53).
+      zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
+      if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
+      zd* colA = bufs[1];  // G_{x+1,L} W_R
+      zd* colB = nullptr;  // G_{x,L} W_R
+      int pb = 0;
+      for (int x = L - 1; x >= 0; --x) {
+        if (x < L - 1) {
+          colB = bufs[pb];
+          if ((r = xmul(0, Q(y.o_Y), colA))) return r;
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                                   kOne, kZero));
+          if ((r = xmul(0, Q(y.o_P), zs(x + 1)))) return r;
+          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_P), Es, n, zs(x), Es, n, n), nb,
+                                   kOne, kOne));
+        }
+        const zd* GxL = (x < L - 1) ? colB : colA;
+        if ((r = siteblocks(1, x, GxL))) return r;
+        if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
+        if (x == L - 1 && (r = trace_term(1, GxL, 3))) return r;  // Modec2, right-lead term
+        if (x == 0 && (r = trace_term(0, GxL, 0))) return r;      // T12
+        if ((r = siteblocks(0, x, zs(x)))) return r;
+        if ((r = bonds(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
+        if (x == L - 1 && (r = trace_term(1, zs(x), 2))) return r;  // Modec2, left-lead term
+        if (x == 0 && (r = trace_term(0, zs(x), 1))) return r;      // Modec1, left-lead term
+        if (x < L - 1) {
+          colA = colB;
+          pb ^= 1;
+        }
+      }
+      return 0;
+    }
+    // ---- three-pass RGF, pass 1: gL
     for (int x = 0; x < L; ++x) {
       if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
       TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
-      if (x == 0) {
-        TRY(to_c64(s, F(y.f_g), Es32, slot(0), Es, nn, nb));
-        TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0)), nb, kOnef, kZerof));
-      } else {
-        TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
-        TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, nb, nullptr));
-        TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kZerof));
-      }
     }
-    // backward: G_xL W_R in FP64; G_x1 W_L = z_x + gL_x X G_{x+1,1} W_L in complex64
-    // (in place of z_x)
-    zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
-    if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
-    zd* colA = bufs[1];
-    zd* colB = nullptr;
-    int pb = 0;
-    for (int x = L - 1; x >= 0; --x) {
-      if (x < L - 1) {
-        colB = bufs[pb];
-        if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                                 kOne, kZero));
-        TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
-        TRY(launch_bd_mul<float>(s, no, 0, F(y.f_Y), Es32, zs(x + 1), Es32, c->X32, n, nb, nullptr));
-        TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kOnef));
-      }
-      const zd* GxL = (x < L - 1) ? colB : colA;
-      if (x == 0 && (r = trace_term(0, GxL, 0))) return r;  // T12 = |W_L' G_1L W_R|^2 in FP64
-      TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, nn, nb));
-      if ((r = site32(1, x, F(y.f_G)))) return r;
-      if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
-      if (x == L - 1 && (r = trace32(1, F(y.f_G), 3))) return r;  // Modec2, right-lead term
-      if ((r = site32(0, x, zs(x)))) return r;
-      if ((r = bonds32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
-      if (x == L - 1 && (r = trace32(1, zs(x), 2))) return r;  // Modec2, left-lead term
-      if (x == 0 && (r = trace32(0, zs(x), 1))) return r;      // Modec1, left-lead term
-      if (x < L - 1) {
-        colA = colB;
-        pb ^= 1;
-      }
-    }
-    return collect(c, ln, nb, iters, sst, res);
-  }
-  if (!(c->cfg.flags & NEGF_FLAG_RGF_THREE_PASS)) {
-    // ---- two-pass RGF.  Forward: gL_x (Recursive_Greens.F90:16-34) and the forward
-    // substitution z_x = gL_x X' z_{x-1}, z_1 = gL_1, of A Y = E_1 (block column 1).
-    auto zs = [&](int x) { return Q(y.o_z + (long long)x * nn); };
-    for (int x = 0; x < L; ++x) {
-      if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
+    // ---- pass 2: last column and gR
+    zd* colA = Q(y.o_colA);
+    zd* colB = Q(y.o_colB);
+    if ((r = times_w(slot(L - 1), 1, colA))) return r;  // G_LL W_R
+    if ((r = siteblocks(1, L - 1, colA))) return r;
+    if ((r = bonds(1, L - 1, colA, nullptr))) return r;
+    if ((r = trace_term(1, colA, 3))) return r;          // Modec2, right-lead term
+    if (L == 1 && (r = trace_term(0, colA, 0))) return r; // T12
+    if ((r = assemble(L - 1, nullptr, 2))) return r;
+    TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(L - 1), Es, n, n, nb, w, nullptr));
+    for (int x = L - 2; x >= 0; --x) {
+      if ((r = xmul(0, Q(y.o_Y), colA))) return r;
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                               kOne, kZero));
+      if ((r = siteblocks(1, x, colB))) return r;
+      if ((r = bonds(1, x, colB, colA))) return r;
+      if (x == 0 && (r = trace_term(0, colB, 0))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
+      if ((r = assemble(x, slot(x + 1), 2))) return r;
       TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
-      if (x == 0) {
-        if ((r = times_w(slot(0), 0, zs(0)))) return r;  // z_1 W_L = gL_1 W_L
-      } else {
-        if ((r = xmul(1, Q(y.o_Y), zs(x - 1)))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
-                                 kOne, kZero));
-      }
+      std::swap(colA, colB);
     }
-    // Backward: the last column G_xL = gL_x X G_{x+1,L}, G_LL = gL_L (:53-63), and the
-    // back substitution G_x1 = z_x + gL_x X G_{x+1,1}, G_L1 = z_L, in place of z_x.
-    // Slice x then has both columns: the Gn/Gp site blocks of both leads, T12 at x = 1
-    // (Calculate_Observables.F90:86-100) and the Modec terms at the ends (:48-53).
-    zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
-    if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
-    zd* colA = bufs[1];  // G_{x+1,L} W_R
-    zd* colB = nullptr;  // G_{x,L} W_R
-    int pb = 0;
-    for (int x = L - 1; x >= 0; --x) {
-      if (x < L - 1) {
-        colB = bufs[pb];
-        if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                                 kOne, kZero));
-        if ((r = xmul(0, Q(y.o_P), zs(x + 1)))) return r;
-        TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_P), Es, n, zs(x), Es, n, n), nb,
-                                 kOne, kOne));
-      }
-      const zd* GxL = (x < L - 1) ? colB : colA;
-      if ((r = siteblocks(1, x, GxL))) return r;
-      if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
-      if (x == L - 1 && (r = trace_term(1, GxL, 3))) return r;  // Modec2, right-lead term
-      if (x == 0 && (r = trace_term(0, GxL, 0))) return r;      // T12
-      if ((r = siteblocks(0, x, zs(x)))) return r;
-      if ((r = bonds(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
-      if (x == L - 1 && (r = trace_term(1, zs(x), 2))) return r;  // Modec2, left-lead term
-      if (x == 0 && (r = trace_term(0, zs(x), 1))) return r;      // Modec1, left-lead term
-      if (x < L - 1) {
-        colA = colB;
-        pb ^= 1;
-      }
+    // ---- pass 3: first column
+    colA = Q(y.o_colA);
+    colB = Q(y.o_colB);
+    if ((r = times_w(slot(0), 0, colA))) return r;  // G_11 W_L
+    if ((r = siteblocks(0, 0, colA))) return r;
+    if ((r = trace_term(0, colA, 1))) return r;            // Modec1, left-lead term
+    if (L == 1 && (r = trace_term(1, colA, 2))) return r;  // Modec2, left-lead term
+    for (int x = 1; x < L; ++x) {
+      if ((r = xmul(1, Q(y.o_Y), colA))) return r;
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
+                               kOne, kZero));
+      if ((r = siteblocks(0, x, colB))) return r;
+      if ((r = bonds(0, x - 1, colA, colB))) return r;
+      if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;
+      std::swap(colA, colB);
     }
-    return collect(c, ln, nb, iters, sst, res);
+    if ((r = bonds(0, L - 1, colA, nullptr))) return r;
+    return 0;
+  };
+  int rr;
+  const bool graph = (c->cfg.flags & NEGF_FLAG_GRAPH) && c->cfg.precision != NEGF_PREC_REFINED;
+  if (graph) {
+    const GraphKey key = {c->uid, c->generation, nb, c->want_currents, ln.E};
+    int slot_g = -1;
+    for (int i = 0; i < Lane::kGraphSlots; ++i)
+      if (ln.gexec[i] && ln.gkey[i] == key) slot_g = i;
+    if (slot_g < 0) {
+      slot_g = 0;  // an empty slot, else the least recently used one
+      for (int i = 0; i < Lane::kGraphSlots; ++i)
+        if (!ln.gexec[i] || ln.guse[i] < ln.guse[slot_g]) {
+          slot_g = i;
+          if (!ln.gexec[i]) break;
+        }
+      if (ln.gexec[slot_g]) {
+        TRY(hipGraphExecDestroy(ln.gexec[slot_g]));
+        ln.gexec[slot_g] = nullptr;
+      }
+      TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      timer_capture(true);  // sampled launch timing stays out of the graph
+      rr = rgf();
+      timer_capture(false);
+      hipGraph_t gr = nullptr;
+      const hipError_t ge = hipStreamEndCapture(s, &gr);
+      if (rr) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rr;
+      }
+      TRY(ge);
+      const hipError_t gi = hipGraphInstantiate(&ln.gexec[slot_g], gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      TRY(gi);
+      ln.gkey[slot_g] = key;
+      ++c->graphs_built;
+    }
+    ln.guse[slot_g] = ++ln.gclock;
+    TRY(hipGraphLaunch(ln.gexec[slot_g], s));
+    ++c->graph_replays;
+  } else if ((rr = rgf())) {
+    return rr;
   }
-  // ---- three-pass RGF, pass 1: gL
-  for (int x = 0; x < L; ++x) {
-    if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
-    TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
-  }
-  // ---- pass 2: last column and gR
-  zd* colA = Q(y.o_colA);
-  zd* colB = Q(y.o_colB);
-  if ((r = times_w(slot(L - 1), 1, colA))) return r;  // G_LL W_R
-  if ((r = siteblocks(1, L - 1, colA))) return r;
-  if ((r = bonds(1, L - 1, colA, nullptr))) return r;
-  if ((r = trace_term(1, colA, 3))) return r;          // Modec2, right-lead term
-  if (L == 1 && (r = trace_term(0, colA, 0))) return r; // T12
-  if ((r = assemble(L - 1, nullptr, 2))) return r;
-  TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(L - 1), Es, n, n, nb, w, nullptr));
-  for (int x = L - 2; x >= 0; --x) {
-    if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                             kOne, kZero));
-    if ((r = siteblocks(1, x, colB))) return r;
-    if ((r = bonds(1, x, colB, colA))) return r;
-    if (x == 0 && (r = trace_term(0, colB, 0))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
-    if ((r = assemble(x, slot(x + 1), 2))) return r;
-    TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
-    std::swap(colA, colB);
-  }
-  // ---- pass 3: first column
-  colA = Q(y.o_colA);
-  colB = Q(y.o_colB);
-  if ((r = times_w(slot(0), 0, colA))) return r;  // G_11 W_L
-  if ((r = siteblocks(0, 0, colA))) return r;
-  if ((r = trace_term(0, colA, 1))) return r;            // Modec1, left-lead term
-  if (L == 1 && (r = trace_term(1, colA, 2))) return r;  // Modec2, left-lead term
-  for (int x = 1; x < L; ++x) {
-    if ((r = xmul(1, Q(y.o_Y), colA))) return r;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                             kOne, kZero));
-    if ((r = siteblocks(0, x, colB))) return r;
-    if ((r = bonds(0, x - 1, colA, colB))) return r;
-    if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;
-    std::swap(colA, colB);
-  }
-  if ((r = bonds(0, L - 1, colA, nullptr))) return r;
   return collect(c, ln, nb, iters, sst, res);
 }
 
@@ -1720,10 +1933,34 @@ extern "C" int negf_profile_end(double* flops, double* ms, long long* launches) 
   return 0;
 }
 
+extern "C" int negf_set_gemm_tiles(int complex64, int count, const int* n_max, const int* tile) {
+  if (set_gemm_tiles(complex64 ? 1 : 0, count, n_max, tile)) {
+    negf_set_error("negf_set_gemm_tiles: rules must be at most 16, with increasing n_max and "
+                   "tile 0 (64x64) or 1 (32x32)");
+    return NEGF_ERR_ARG;
+  }
+  return 0;
+}
+
 // The batched complex product kernel alone at one shape: C_b = A_b B_b for `batch`
 // n x n matrices (complex64 or complex128, 3M or 4-product form), `reps` launches
 // on a stream of its own after one untimed launch.  For hardware-counter runs of
 // the kernel that bounds the sweep; the operands are a constant byte pattern.
+extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
+                               double* ms_per_launch);
+
+extern "C" int negf_bench_gemm_tile(int device, int complex64, int n, int batch, int reps, int m3,
+                                    int tile, double* ms_per_launch) {
+  if (tile != 0 && tile != 1) {
+    negf_set_error("negf_bench_gemm_tile: tile must be 0 (64x64) or 1 (32x32)");
+    return NEGF_ERR_ARG;
+  }
+  force_gemm_tile(tile);
+  const int rc = negf_bench_gemm(device, complex64, n, batch, reps, m3, ms_per_launch);
+  force_gemm_tile(-1);
+  return rc;
+}
+
 extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int reps, int m3,
                                double* ms_per_launch) {
   if (n < 1 || n > 8192 || batch < 1 || reps < 1 || !ms_per_launch) {

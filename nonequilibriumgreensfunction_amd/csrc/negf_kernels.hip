@@ -21,6 +21,8 @@
 // result does not depend on what else is in its batch.
 #include "negf_device.h"
 
+#include <algorithm>
+
 namespace negf {
 
 // ---------------------------------------------------------------------------
@@ -815,6 +817,22 @@ hipError_t launch_bd_mul(hipStream_t s, int norb, int conj, cx<T>* Y, long long 
   NEGF_BD(1, 0) NEGF_BD(1, 1) NEGF_BD(2, 0) NEGF_BD(2, 1) NEGF_BD(4, 0) NEGF_BD(4, 1)
 #undef NEGF_BD
   return hipErrorInvalidValue;
+}
+
+// Zero `words` 32-bit words (a kernel rather than hipMemsetAsync, so that the
+// clear can sit inside a captured HIP graph).
+__global__ void k_zero32(unsigned* p, long long words) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+hipError_t launch_zero(hipStream_t s, void* p, size_t bytes) {
+  const long long words = (long long)(bytes / 4);
+  if (words <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>(1024, (words + 255) / 256);
+  hipLaunchKernelGGL(k_zero32, dim3(blocks), dim3(256), 0, s, (unsigned*)p, words);
+  return hipGetLastError();
 }
 
 template <typename T>

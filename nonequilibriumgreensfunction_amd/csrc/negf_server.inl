@@ -70,7 +70,7 @@ size_t resident_bytes(int n, int L, int nsites, int precision, bool compact) {
     b += 3 * sizeof(zd) * nn + sizeof(double) * (size_t)L * nsites;  // H0, two end slices, V
   else
     b += sizeof(zd) * nn * (size_t)L;  // Htotd
-  if (precision == NEGF_PREC_MIXED) b += sizeof(cx<float>) * nn;
+  if (precision != NEGF_PREC_FP64) b += sizeof(cx<float>) * nn;
   return b;
 }
 
@@ -232,12 +232,13 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
     negf_set_error("server lanes must be 1..%d (0 = %d)", kMaxLanes, kMaxLanes);
     return NEGF_ERR_ARG;
   }
-  if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED) {
+  if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED &&
+      cfg->precision != NEGF_PREC_REFINED) {
     negf_set_error("unknown precision form %d", cfg->precision);
     return NEGF_ERR_ARG;
   }
-  if (cfg->precision == NEGF_PREC_MIXED && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
-    negf_set_error("the mixed-precision form runs on the two-pass RGF only");
+  if (cfg->precision != NEGF_PREC_FP64 && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    negf_set_error("the mixed-precision forms run on the two-pass RGF only");
     return NEGF_ERR_ARG;
   }
   // A request's contact cache would live outside the footprint admission counts,
@@ -299,7 +300,7 @@ extern "C" int negf_server_submit(negf_server* s, const negf_request* req, long 
   SrvRequest* r = new SrvRequest();
   r->req = *req;
   r->lay.init(n, req->npx, req->npy * req->npz, !(s->cfg.flags & NEGF_FLAG_RGF_THREE_PASS),
-              s->cfg.precision == NEGF_PREC_MIXED);
+              s->cfg.precision != NEGF_PREC_FP64);
   r->footprint = resident_bytes(n, req->npx, req->npy * req->npz, s->cfg.precision, !req->Htotd);
   if (lane_bytes_per_energy(r->lay) > s->pool_bytes || r->footprint > s->budget) {
     negf_set_error("request (n=%d, Npx=%d) needs %.1f GB per energy and %.1f GB resident; lane pool "
@@ -364,6 +365,7 @@ extern "C" int negf_server_destroy(negf_server* s) {
     if (t.joinable()) t.join();
   for (int li = 0; li < s->nl; ++li) {
     hipStreamSynchronize(s->lanes[li].st);
+    s->lanes[li].drop_graphs();
     hipFree(s->pools[li]);
     hipStreamDestroy(s->lanes[li].st);
   }

@@ -297,9 +297,44 @@ static std::mutex g_timer_mu;
 
 // Reserves an event pair for one launch; the handles are copied out under the lock
 // (the event array may grow while another host thread launches).
+// Tuning table: per element type, rules (n_max, tile) in increasing n_max; the
+// first rule with n <= n_max picks the tile, none gives the 64x64 tile.  Set once
+// by negf_set_gemm_tiles (before solves start; not synchronised with launches).
+constexpr int kTile64 = 0, kTile32 = 1, kMaxTileRules = 16;
+static int g_rules[2][kMaxTileRules][2];
+static int g_nrules[2] = {0, 0};
+static thread_local int t_force_tile = -1;  // negf_bench_gemm_tile: one tile for this thread
+
+int gemm_tile(int fp32, int n) {
+  if (t_force_tile >= 0) return t_force_tile;
+  for (int i = 0; i < g_nrules[fp32]; ++i)
+    if (n <= g_rules[fp32][i][0]) return g_rules[fp32][i][1];
+  return kTile64;
+}
+
+int set_gemm_tiles(int fp32, int count, const int* n_max, const int* tile) {
+  if (fp32 < 0 || fp32 > 1 || count < 0 || count > kMaxTileRules || (count && (!n_max || !tile)))
+    return -1;
+  for (int i = 0; i < count; ++i) {
+    if (tile[i] != kTile64 && tile[i] != kTile32) return -1;
+    if (i && n_max[i] <= n_max[i - 1]) return -1;
+  }
+  for (int i = 0; i < count; ++i) {
+    g_rules[fp32][i][0] = n_max[i];
+    g_rules[fp32][i][1] = tile[i];
+  }
+  g_nrules[fp32] = count;
+  return 0;
+}
+
+void force_gemm_tile(int tile) { t_force_tile = tile; }
+
+static thread_local bool t_capturing = false;
+void timer_capture(bool on) { t_capturing = on; }
+
 bool timer_slot(double flops, int kind, hipEvent_t* e0, hipEvent_t* e1) {
   GemmTimer& t = gemm_timer();
-  if (!t.on) return false;
+  if (!t.on || t_capturing) return false;
   std::lock_guard<std::mutex> lock(g_timer_mu);
   if (t.seen[kind]++ % (unsigned long long)t.every != 0) return false;
   if (t.used == t.cap) {
@@ -360,10 +395,18 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
     const hipError_t er = hipEventRecord(ev0, s);
     if (er != hipSuccess) return er;
   }
-  // 64x64 workgroup tiles: 128x64 and 64x128 measured within 3% of it for these
-  // shapes, and their FP64 variants do not fit two workgroups per CU without spills
-  const hipError_t e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 2, 2, 1>(s, g, batch, alpha, beta)
-                            : launch_cfg<T, OPA, OPB, EPI, 2, 2, 0>(s, g, batch, alpha, beta);
+  // Workgroup tile from the tuning table (gemm_tile): 64x64 by default -- 128x64 and
+  // 64x128 measured within 3% of it at blocks of 1024 and more, and their FP64
+  // variants do not fit two workgroups per CU without spills -- or 32x32, four
+  // times the workgroups for small blocks.
+  const int tile = gemm_tile(sizeof(T) == sizeof(float), g.m > g.n ? g.m : g.n);
+  hipError_t e;
+  if (tile == kTile32)
+    e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 1, 1, 1>(s, g, batch, alpha, beta)
+             : launch_cfg<T, OPA, OPB, EPI, 1, 1, 0>(s, g, batch, alpha, beta);
+  else
+    e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 2, 2, 1>(s, g, batch, alpha, beta)
+             : launch_cfg<T, OPA, OPB, EPI, 2, 2, 0>(s, g, batch, alpha, beta);
   if (timed && e == hipSuccess) return hipEventRecord(ev1, s);
   return e;
 }
