@@ -52,7 +52,7 @@ hipError_t launch_zero(hipStream_t, void*, size_t);
 template <typename T>
 hipError_t launch_bondblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              const double2*, const double2*, int, int, int, double*, long long,
-                             long long, int, const int*);
+                             long long, int, int, const int*);
 template <typename T>
 hipError_t launch_inner(hipStream_t, const cx<T>*, long long, const cx<T>*, long long, int,
                         double*, long long, int, const int*);
@@ -116,8 +116,9 @@ static const double kH = 6.626e-34;   // Constants.F90:6
 // the max-entry norm) or the last one exceeds kRefineAccept.
 constexpr double kRefineDone = 1e-12, kRefineStart = 0.25, kRefineAccept = 1e-10;
 constexpr int kRefineSteps = 2;
-constexpr int kSiteQ = 56;      // site-output planes per (lead, slice, site)
-constexpr int kSiteQBasic = 8;  // planes without the bond-current terms
+constexpr int kSiteQ = 59;        // site-output planes per (lead, slice, site)
+constexpr int kSiteQBasic = 8;    // planes without the bond-current terms
+constexpr int kSiteQCurrent = 11; // planes up to the pair-summed bond-current terms
 struct Layout {
   long long nn;
   // per-energy region (complex elements)
@@ -132,8 +133,8 @@ struct Layout {
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
   // Site outputs per energy: kSiteQ planes, each (lead, slice, site), at
   // ((q * 2 + lead) * L + x) * nsites + s.  Planes: 0 Re tr of the site block,
-  // 1..3 its spin traces, 4..7 its diagonal per orbital, 8..55 the bond-current
-  // terms per orbital pair (16 each for x, y, z).
+  // 1..3 its spin traces, 4..7 its diagonal per orbital, 8..10 the bond-current
+  // terms (x, y, z), 11..58 the same per orbital pair (16 each for x, y, z).
   long long site_per_e;  // doubles of site output per energy
   long long site_q;      // doubles per plane (2 L nsites)
   // zslots: L more blocks per energy for the forward substitution chain of the
@@ -264,6 +265,7 @@ struct negf_ctx {
   bool blockdiag = true;
   bool ends_equal = false;  // H slice 1 == H slice L bitwise: one decimation serves both leads
   bool want_currents = false;  // the caller's accumulators take bond currents (set per flush)
+  bool want_pairs = false;     // ... and orbital-resolved ones
   int cap_budget = 0;                 // energies per lane the HBM budget allowed at the last allocation
   unsigned long long uid = 0;         // unique per context (set by negf_init)
   unsigned long long generation = 0;  // bumped by every Hamiltonian upload
@@ -1001,7 +1003,8 @@ static int collect(negf_ctx* c, Lane& ln, int nb, const std::vector<int>& iters,
   res.assign(nb, EnergyResult());
   std::vector<double> scal(8 * nb);
   // the planes this flush fills: the bond-current terms only when they were formed
-  const size_t used = (size_t)(c->want_currents ? kSiteQ : kSiteQBasic) * y.site_q;
+  const size_t used =
+      (size_t)(c->want_pairs ? kSiteQ : c->want_currents ? kSiteQCurrent : kSiteQBasic) * y.site_q;
   std::vector<double> site((size_t)y.site_per_e * nb);
   std::vector<int> ints((size_t)y.Is * nb);
   TRY(hipMemcpyAsync(scal.data(), ln.d_scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s));
@@ -1190,7 +1193,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     auto bonds = [&](int lead, int x, const zd* V, const zd* Vn) -> int {
       if (!c->want_currents) return 0;
       TRY(launch_bondblocks<double>(s, no, V, Es, Vn, Es, hx(x), c->X, n, npy, npz, site_out(lead, x),
-                                    y.site_per_e, y.site_q, nb, nullptr));
+                                    y.site_per_e, y.site_q, c->want_pairs ? 1 : 0, nb, nullptr));
       return 0;
     };
     // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b
@@ -1240,7 +1243,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
         if (!c->want_currents) return 0;
         TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
-                                     site_out(lead, x), y.site_per_e, y.site_q, nb, nullptr));
+                                     site_out(lead, x), y.site_per_e, y.site_q, c->want_pairs ? 1 : 0,
+                                     nb, nullptr));
         return 0;
       };
       // scal[idx] = |W_lead' V32|_F^2
@@ -1511,6 +1515,7 @@ static bool wants_currents(const negf_accum* a) {
   return a && (a->Jx || a->Jy || a->Jz || a->ERJx || a->ERJy || a->ERJz || a->ORJx || a->ORJy ||
                a->ORJz);
 }
+static bool wants_pairs(const negf_accum* a) { return a && (a->ORJx || a->ORJy || a->ORJz); }
 
 static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eCtr, double deltaE,
                        const double* mu, double muT, negf_accum* acc) {
@@ -1520,18 +1525,13 @@ static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eC
   const double f2 = (energy <= muT - mu[1]) ? 1.0 : 0.0;
   const double w = deltaE / (2.0 * kPi);
   const double T = e.scal[0];
-  const size_t q = (size_t)L * ns;  // one (lead, plane) block
-  // plane p of lead l
+  const size_t q = (size_t)L * ns;  // one (lead, plane) block, and one site array
+  // plane p of lead l, indexed by k = x * ns + site
   auto pl = [&](int p, int l) { return e.site.data() + ((size_t)p * 2 + l) * q; };
   const double* sl = pl(0, 0);
   const double* sr = pl(0, 1);
-  const bool cur = wants_currents(acc) && e.site.size() >= (size_t)kSiteQ * 2 * q;
-  const double qq_hbar = kQ * kQ / (kH / (2.0 * kPi));  // q^2/hbar (Constants.F90:4-6)
-  const double wl = 2.0 * f1 - 1.0, wr = 2.0 * f2 - 1.0;
-  double* Jd[3] = {acc->Jx, acc->Jy, acc->Jz};
-  double* ERJd[3] = {acc->ERJx, acc->ERJy, acc->ERJz};
-  double* ORJd[3] = {acc->ORJx, acc->ORJy, acc->ORJz};
   double ed = 0.0, hd = 0.0;
+  double* sg[3] = {acc->SigXExp, acc->SigYExp, acc->SigZExp};
   for (int x = 0; x < L; ++x)
     for (int sidx = 0; sidx < ns; ++sidx) {
       const size_t k = (size_t)x * ns + sidx;
@@ -1542,7 +1542,6 @@ static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eC
       const size_t o = (size_t)x + (size_t)L * sidx;  // (x, y, z) with sidx = y + Npy*z
       if (acc->RhoN) acc->RhoN[o] += trN * w;
       if (acc->RhoP) acc->RhoP[o] += trP * w;
-      double* sg[3] = {acc->SigXExp, acc->SigYExp, acc->SigZExp};
       for (int a = 0; a < 3; ++a)
         if (sg[a]) sg[a][o] += (f1 * pl(1 + a, 0)[k] + f2 * pl(1 + a, 1)[k]) * w;
       // Calculate_Gnp.F90:125-130: energy-resolved Re(Gnd - Gpd), orbital-resolved Re Gnd
@@ -1550,21 +1549,39 @@ static void accumulate(negf_ctx* c, const EnergyResult& e, double energy, int eC
       if (acc->ORRhon)
         for (int r = 0; r < no; ++r)
           acc->ORRhon[o + q * r] += (f1 * pl(4 + r, 0)[k] + f2 * pl(4 + r, 1)[k]) * w;
-      if (!cur) continue;
-      // Calculate_Observables.F90:111-168 (accumulated in REAL(8) here)
-      for (int d = 0; d < 3; ++d) {
-        if (!Jd[d] && !ERJd[d] && !ORJd[d]) continue;
-        double sum = 0.0;
-        for (int p = 0; p < no * no; ++p) {
-          const int plane = 8 + 16 * d + p;
-          const double v = qq_hbar * (wl * pl(plane, 0)[k] + wr * pl(plane, 1)[k]);
-          sum += v;
-          if (ORJd[d]) ORJd[d][o + q * p] += v * w;
-        }
-        if (Jd[d]) Jd[d][o] += sum * w;
-        if (ERJd[d] && eCtr >= 0) ERJd[d][o + q * (size_t)eCtr] += sum;
-      }
     }
+  // Calculate_Observables.F90:111-168 (accumulated in REAL(8)): per lead
+  // Gn - Gp = (2f - 1) A, with the q^2/hbar factor; plane by plane, in site order
+  const double qq_hbar = kQ * kQ / (kH / (2.0 * kPi));  // q^2/hbar (Constants.F90:4-6)
+  const double wl = qq_hbar * (2.0 * f1 - 1.0), wr = qq_hbar * (2.0 * f2 - 1.0);
+  double* Jd[3] = {acc->Jx, acc->Jy, acc->Jz};
+  double* ERJd[3] = {acc->ERJx, acc->ERJy, acc->ERJz};
+  double* ORJd[3] = {acc->ORJx, acc->ORJy, acc->ORJz};
+  const size_t have = e.site.size() / (2 * q);  // planes present
+  for (int d = 0; d < 3; ++d) {
+    if ((Jd[d] || (ERJd[d] && eCtr >= 0)) && have >= (size_t)kSiteQCurrent) {
+      const double* a = pl(8 + d, 0);
+      const double* b = pl(8 + d, 1);
+      for (int x = 0; x < L; ++x)
+        for (int sidx = 0; sidx < ns; ++sidx) {
+          const size_t k = (size_t)x * ns + sidx, o = (size_t)x + (size_t)L * sidx;
+          const double v = wl * a[k] + wr * b[k];
+          if (Jd[d]) Jd[d][o] += v * w;
+          if (ERJd[d] && eCtr >= 0) ERJd[d][o + q * (size_t)eCtr] += v;
+        }
+    }
+    if (ORJd[d] && have >= (size_t)kSiteQ)
+      for (int p = 0; p < no * no; ++p) {
+        const double* a = pl(11 + 16 * d + p, 0);
+        const double* b = pl(11 + 16 * d + p, 1);
+        double* out = ORJd[d] + q * p;
+        for (int x = 0; x < L; ++x)
+          for (int sidx = 0; sidx < ns; ++sidx) {
+            const size_t k = (size_t)x * ns + sidx;
+            out[(size_t)x + (size_t)L * sidx] += (wl * a[k] + wr * b[k]) * w;
+          }
+      }
+  }
   if (eCtr >= 0) {
     if (acc->T12) acc->T12[eCtr] += T;
     if (acc->Modec1) acc->Modec1[eCtr] += e.scal[1] + T;
@@ -1608,6 +1625,7 @@ extern "C" int negf_flush(negf_ctx* c, const double* mu, double muT, negf_accum*
   const int total = (int)c->qE.size();
   if (total == 0) return 0;
   c->want_currents = wants_currents(acc);
+  c->want_pairs = wants_pairs(acc);
   if ((rc = ensure_arena(c, total))) {
     c->qE.clear(); c->qdE.clear(); c->qIdx.clear();
     return rc;

@@ -197,16 +197,16 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 //   x: b the same orbitals of slice x+1 (Vn, when given), H_ab = HopX(a, b'),
 //      H_ba = conj(HopX(a, b')) as the reference writes it;
 // with M = A_ba (row b, column a):  out = Re i (H_ab M - conj(M) H_ba) = -Im(...),
-// per orbital pair at out_b + (8 + 16 d + NORB ii + jj) * qstride + s, d = 0 x,
-// 1 y, 2 z (0 where the bond does not exist).  The host sums the pairs for the
-// bond currents and keeps them for the orbital-resolved ones.
+// summed over the orbital pairs at out_b + (8 + d) * qstride + s, d = 0 x, 1 y, 2 z,
+// and, when `pairs`, per orbital pair at out_b + (11 + 16 d + NORB ii + jj) * qstride + s
+// (the orbital-resolved currents); 0 where the bond does not exist.
 // Reduction as in k_siteblocks: 16 rows per workgroup, the k sum in 16 chunks
 // added in chunk order, so results do not depend on the batch.
 template <typename T, int NORB>
 __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV, const cx<T>* Vn,
                                                     long long sVn, const double2* H, const double2* X,
                                                     int n, int npy, int npz, double* out,
-                                                    long long sOut, long long qstride,
+                                                    long long sOut, long long qstride, int pairs,
                                                     const int* active) {
   constexpr int RW = 16, KQ = 16, NQ = 3 * NORB;
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
@@ -264,7 +264,8 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
       const bool has[3] = {hx, sy < npy - 1, sz < npz - 1};
       const int sa0 = s * NORB;
       double* o = out + (long long)b * sOut + s;
-      for (int d = 0; d < 3; ++d)
+      for (int d = 0; d < 3; ++d) {
+        double sum = 0.0;
         for (int ii = 0; ii < NORB; ++ii)
           for (int jj = 0; jj < NORB; ++jj) {
             double val = 0.0;
@@ -288,8 +289,11 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
               const double im = (hab.x * mi + hab.y * mr) - (mr * hba.y - mi * hba.x);
               val = -im;
             }
-            o[(8 + 16 * d + NORB * ii + jj) * qstride] = val;
+            sum += val;
+            if (pairs) o[(11 + 16 * d + NORB * ii + jj) * qstride] = val;
           }
+        o[(8 + d) * qstride] = sum;
+      }
     }
   }
 }
@@ -838,14 +842,14 @@ hipError_t launch_zero(hipStream_t s, void* p, size_t bytes) {
 template <typename T>
 hipError_t launch_bondblocks(hipStream_t s, int norb, const cx<T>* V, long long sV, const cx<T>* Vn,
                              long long sVn, const double2* H, const double2* X, int n, int npy,
-                             int npz, double* out, long long sOut, long long qstride, int batch,
-                             const int* active) {
+                             int npz, double* out, long long sOut, long long qstride, int pairs,
+                             int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
   dim3 grid((n + 15) / 16, 1, batch);
 #define NEGF_BB(NO)                                                                                \
   if (norb == NO) {                                                                                \
     hipLaunchKernelGGL((k_bondblocks<T, NO>), grid, dim3(256), 0, s, V, sV, Vn, sVn, H, X, n, npy, npz, \
-                       out, sOut, qstride, active);                                                \
+                       out, sOut, qstride, pairs, active);                                         \
     return hipGetLastError();                                                                      \
   }
   NEGF_BB(1) NEGF_BB(2) NEGF_BB(4)
@@ -957,7 +961,8 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
                                            long long, const SpinMats&, int, const int*);           \
   template hipError_t launch_bondblocks<T>(hipStream_t, int, const cx<T>*, long long, const cx<T>*, \
                                            long long, const double2*, const double2*, int, int,    \
-                                           int, double*, long long, long long, int, const int*);   \
+                                           int, double*, long long, long long, int, int,           \
+                                           const int*);                                            \
   template hipError_t launch_inner<T>(hipStream_t, const cx<T>*, long long, const cx<T>*,          \
                                       long long, int, double*, long long, int, const int*);        \
   template hipError_t launch_eye<T>(hipStream_t, cx<T>*, long long, int, int, const int*);         \

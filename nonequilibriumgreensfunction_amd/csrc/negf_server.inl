@@ -137,6 +137,7 @@ int admit(negf_server* s, SrvRequest* r) {
   int rc = negf_init(&r->ctx, &cfg);
   if (rc) return rc;
   r->ctx->want_currents = wants_currents(&r->req.acc);
+  r->ctx->want_pairs = wants_pairs(&r->req.acc);
   if (!r->req.Htotd)
     return negf_set_hamiltonian_compact(r->ctx, r->req.H0, r->req.HopX, r->req.gM, r->req.onsite);
   return negf_set_hamiltonian(r->ctx, r->req.Htotd, r->req.HopX, r->req.gM);
