@@ -266,6 +266,20 @@ int negf_zgemm(char transa, char transb, int n, const double* alpha, const doubl
 int negf_gemm_batch(int complex64, int m3, int n, int batch, const void* A, const void* B, void* C);
 int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG, int* info);
 
+/* Allocation-free form of negf_invert_batch.  No counterpart in the reference; for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, hosts that forbid hipMalloc.  G, invG (batch contiguous n x n
+ * matrices each) and info (batch ints) are DEVICE pointers; G is overwritten (the
+ * elimination runs in place) and must not alias invG; the work is queued on
+ * `stream` (a hipStream_t, NULL for the default stream) and the call returns
+ * without synchronising.  The Gauss-Jordan workspace (per matrix a 128 x n panel
+ * and 2n+1 ints) lives in a static 24 MiB device array, so batch * n is limited to
+ * about 12000 for complex128 and 23000 for complex64 (larger requests fail with
+ * NEGF_ERR_MEMORY); the call makes no hipMalloc/hipFree.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller. */
+int negf_invert_batch_noalloc(int complex64, int m3, int n, int batch, void* G, void* invG,
+                              int* info, void* stream);
 
 /* --- Serving and batching of whole transport solves (no single counterpart in
  * the reference: it runs one wire per process, NEGF_Main.F90:21-38) ----------

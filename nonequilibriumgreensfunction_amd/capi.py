@@ -38,7 +38,7 @@ EXPORTS = (
     "negf_server_create", "negf_server_submit", "negf_server_poll",
     "negf_server_wait", "negf_server_destroy", "negf_bench_gemm", "negf_gemm_batch",
     "negf_invert_batch", "negf_refine_stats", "negf_graph_stats", "negf_bench_gemm_tile",
-    "negf_set_gemm_tiles", "negf_set_lane_table",
+    "negf_set_gemm_tiles", "negf_set_lane_table", "negf_invert_batch_noalloc",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -130,6 +130,7 @@ def lib():
         L.negf_bench_gemm_tile.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
         L.negf_set_gemm_tiles.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
         L.negf_set_lane_table.argtypes = [ctypes.c_int, _ip, _ip]
+        L.negf_invert_batch_noalloc.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 4
         L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 10
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6

@@ -1891,6 +1891,97 @@ static int invert_batch_host(int m3, int n, int batch, const void* G, void* invG
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free batched inverse (negf_invert_batch_noalloc).  Its Gauss-Jordan
+// workspace -- per matrix the kGjNBO x n pivot-row panel and the pivot bookkeeping
+// (2n + 1 ints) -- lives in `scratch`, a file-scope device array of kScratchBytes
+// that every GPU the library runs on holds until the process ends.  In host code
+// the array's name is not a device address: its address on each device is taken
+// once, with that device current, by hipGetSymbolAddress and kept per device; only
+// that address is ever passed to kernels.  Every call on a GPU shares that GPU's
+// array, so calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = 24u << 20;  // 24 MiB
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static char* g_scratch_ptr[kMaxScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+static char* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_ptr[dev] = (char*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+static size_t noalloc_bytes(int complex64, int n, int batch) {
+  const size_t el = complex64 ? sizeof(cx<float>) : sizeof(cx<double>);
+  const size_t prow = ((size_t)kGjNBO * n * el + 255) & ~(size_t)255;
+  const size_t ints = (sizeof(int) * (2 * (size_t)n + 1) + 255) & ~(size_t)255;
+  return (prow + ints) * (size_t)batch;
+}
+
+template <typename T>
+static int invert_noalloc(char* base, int m3, int n, int batch, void* G, void* invG, int* info,
+                          hipStream_t st) {
+  typedef cx<T> V;
+  const long long nn = (long long)n * n;
+  const size_t prow_b = ((size_t)kGjNBO * n * sizeof(V) + 255) & ~(size_t)255;
+  const size_t ints_b = (sizeof(int) * (2 * (size_t)n + 1) + 255) & ~(size_t)255;
+  GjWork w;
+  w.prow = base;                                     // batch panels, then batch int blocks
+  w.sProw = (long long)(prow_b / sizeof(V));
+  int* ints = (int*)(base + prow_b * (size_t)batch);
+  w.pivstep = ints;
+  w.pivrow = ints + n;
+  w.info = ints + 2 * n;
+  w.sInt = (long long)(ints_b / sizeof(int));
+  w.m3 = m3 ? 1 : 0;
+  // the info words accumulate a first failure: they start at zero on every call
+  TRY(launch_zero(st, ints, ints_b * (size_t)batch));
+  TRY(gj_invert<T>(st, (V*)G, nn, n, (V*)invG, nn, n, n, batch, w, nullptr));
+  if (info)
+    TRY(hipMemcpy2DAsync(info, sizeof(int), w.info, ints_b, sizeof(int), batch, hipMemcpyDeviceToDevice,
+                         st));
+  return 0;
+}
+
+extern "C" int negf_invert_batch_noalloc(int complex64, int m3, int n, int batch, void* G, void* invG,
+                                         int* info, void* stream) {
+  if (n <= 0 || n > 3072 || batch <= 0 || !G || !invG || G == invG) {
+    negf_set_error("negf_invert_batch_noalloc: bad arguments (n=%d must be 1..3072, batch=%d >= 1, "
+                   "device arrays G and invG distinct and non-null)", n, batch);
+    return NEGF_ERR_ARG;
+  }
+  const size_t need = noalloc_bytes(complex64, n, batch);
+  if (need > kScratchBytes) {
+    negf_set_error("negf_invert_batch_noalloc: n=%d x batch=%d needs %zu bytes of workspace, above "
+                   "the %zu-byte static scratch (batch * n <= %d in complex128, %d in complex64)",
+                   n, batch, need, kScratchBytes, (int)(kScratchBytes / (kGjNBO * 16 + 16)),
+                   (int)(kScratchBytes / (kGjNBO * 8 + 16)));
+    return NEGF_ERR_MEMORY;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || negf_device_count() <= 0) {
+    negf_set_error("no HIP device available: the NEGF library has no CPU fallback");
+    return NEGF_ERR_NO_DEVICE;
+  }
+  char* base = scratch_on(dev);
+  if (!base) {
+    negf_set_error("negf_invert_batch_noalloc: no scratch address on device %d", dev);
+    return NEGF_ERR_HIP;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  timer_capture(true);  // sampled launch timing would create events: none here
+  const int rc = complex64 ? invert_noalloc<float>(base, m3, n, batch, G, invG, info, st)
+                           : invert_noalloc<double>(base, m3, n, batch, G, invG, info, st);
+  timer_capture(false);
+  return rc;
+}
+
 static int batch_args_ok(int n, int batch, const void* a, const void* b, const void* c) {
   if (n <= 0 || n > 3072 || batch <= 0 || !a || !b || !c) {
     negf_set_error("bad arguments (n=%d must be 1..3072, batch=%d >= 1, non-null arrays)", n, batch);
