@@ -48,6 +48,11 @@ from nonequilibriumgreensfunction_amd.hamiltonian import build_wire, energy_grid
 
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32_MFMA = 157.3
+# Truncation of the contact broadening in the mixed forms' density chains: at 1e-6
+# of tr(Gamma) the measured change of the outputs at block 2304 stays below the
+# complex64 chain's own (RhoN 3.0e-6 vs 3.1e-6 of its scale against the plain form,
+# T12 unchanged), for 7% more energy points per second; 1e-5 gives 16% at 1.5e-5.
+GAMMA_TRUNC_MIXED = 1e-6
 
 CONFIGS = {
     # small, CPU-runnable: 64 energies of a clean 6x6 wire
@@ -175,7 +180,7 @@ def copy_calibration(mib):
 
 
 def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmup, profile_every,
-             label, keep=1):
+             label, keep=1, gamma_trunc=0.0):
     """Warm up, then time `steps` steps of one form; returns timings, the
     accumulators of the last `keep` timed steps (by step index) and the launch
     profile."""
@@ -183,7 +188,7 @@ def run_form(w, cfg, precision, flags, per, grid, dE, step_indices, steps, warmu
     prec = PRECISIONS[precision]
     ctx = capi.Context(L, cfg["npy"], cfg["npz"], 4, eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX,
                        max_batch=per, flags=flags, precision=prec, hbm_budget_gb=cfg["budget_gb"],
-                       lanes=LANES[0])
+                       lanes=LANES[0], gamma_trunc=gamma_trunc)
     ctx.set_hamiltonian(w)
 
     def run_step(k, acc):
@@ -282,7 +287,7 @@ def server_workload(name, args):
     # lane pools that hold one energy of the largest request (block 2304, Npx 300:
     # ~42 GB in the mixed forms)
     srv = capi.Server(eta=ETA, sr_tol=SR_TOL, sr_max_iter=SR_MAX, precision=prec, flags=args.flags,
-                      lanes=args.lanes, lane_pool_gb=args.lane_pool_gb)
+                      lanes=args.lanes, lane_pool_gb=args.lane_pool_gb, gamma_trunc=args.gamma_trunc)
     ids = [None] * len(reqs)
     capi.profile_begin(args.profile_every)
     t0 = time.perf_counter()
@@ -320,6 +325,7 @@ def server_workload(name, args):
         "dtype": DTYPES[args.precision],
         "data": "synthetic (seeded 4-band TI wires, host-built Hamiltonians)",
         "config": {"workload": desc, "requests": len(reqs), "energy_points": npts,
+                   "gamma_trunc": args.gamma_trunc,
                    "precision": args.precision},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "note": "per request, submission to completion (queueing included)"},
@@ -401,6 +407,10 @@ def main():
     ap.add_argument("--lane-pool-gb", type=float, default=45.0,
                     help="server workloads: working pool per lane (GB)")
     ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
+    ap.add_argument("--gamma-trunc", type=float, default=None,
+                    help="negf_config.gamma_trunc of the measured form (default: %g in the mixed "
+                         "forms, 0 in the plain one; the plain FP64 compare leg always uses 0)"
+                         % GAMMA_TRUNC_MIXED)
     ap.add_argument("--profile-every", type=int, default=32,
                     help="time one launch in this many of each kind with HIP events")
     ap.add_argument("--gemm-only", action="store_true",
@@ -425,6 +435,8 @@ def main():
         sys.exit(2)
 
     LANES[0] = args.lanes
+    if args.gamma_trunc is None:
+        args.gamma_trunc = GAMMA_TRUNC_MIXED if args.precision != "fp64" else 0.0
     if args.config in SERVER_WORKLOADS:
         print(json.dumps(server_workload(args.config, args)), flush=True)
         return
@@ -457,7 +469,8 @@ def main():
     w = build_wire(L, cfg["npy"], cfg["npz"], disorder=cfg["W"], seed=cfg["seed"])
     ncmp = max(1, min(args.compare_steps, args.steps))
     run = run_form(w, cfg, args.precision, args.flags, per, grid, dE, step_indices, args.steps,
-                   args.warmup, args.profile_every, args.precision, keep=ncmp)
+                   args.warmup, args.profile_every, args.precision, keep=ncmp,
+                   gamma_trunc=args.gamma_trunc)
     npts = per * args.steps
     elapsed = run["elapsed"]
     value = npts / elapsed
@@ -508,11 +521,15 @@ def main():
         "config": {"workload": (f"{args.config}: Npx={L}, Npy=Npz={cfg['npy']} (block {n}), "
                                 f"{cfg['ne']} energies in [{cfg['emin']},{cfg['emax']}], "
                                 f"Anderson W={cfg['W']} seed {cfg['seed']}, eta={ETA}, "
-                                f"SR tol {SR_TOL} max {SR_MAX}; {args.precision} form; timed: "
+                                f"SR tol {SR_TOL} max {SR_MAX}; {args.precision} form"
+                                + (f" with the contact broadening truncated to trace {args.gamma_trunc:g} "
+                                   "(densities and spin; T12 on the whole factors)"
+                                   if args.gamma_trunc > 0 else "") + "; timed: "
                                 f"{len(timed_idx)} of the {cfg['ne']} grid energies, {per} per step "
                                 f"(strided slices of the grid, shifted each step)"),
                    "energies_per_step": per, "energies_timed": int(len(timed_idx)),
-                   "precision": args.precision, "flags": args.flags, "lanes": args.lanes or 4},
+                   "precision": args.precision, "flags": args.flags, "lanes": args.lanes or 4,
+                   "gamma_trunc": args.gamma_trunc},
         "latency_ms": {"p50": 1e3 * float(np.percentile(run["times"], 50)),
                        "p99": 1e3 * float(np.percentile(run["times"], 99)),
                        "note": ("a step's energies complete together: per-energy solve latency = "

@@ -69,6 +69,15 @@ typedef struct {
   int flags;           /* NEGF_FLAG_* (0 = default algorithms) */
   int lanes;           /* concurrent device lanes of negf_flush, 1..4 (0 = the tuning
                           table's value for this block size, else 4) */
+  double gamma_trunc;  /* error-controlled truncation of the contact broadening in the
+                          fused solve (no counterpart in the reference, which always uses
+                          the whole Gamma): the factor Gamma = W W' of each lead stops once
+                          the trace it leaves out is at most gamma_trunc * tr(Gamma), and
+                          the column chains, site blocks and traces run on those columns
+                          only.  0 = keep every column (the whole Gamma up to 1e-15 of its
+                          largest pivot).  The outputs move by about gamma_trunc relative
+                          (measured: T12 ~0.02x, densities ~0.2x); see the precision
+                          forms' documentation for the values used with them. */
 } negf_config;
 
 /* Algorithm selection (results agree to rounding either way).
@@ -307,6 +316,7 @@ typedef struct {
   double hbm_budget_gb; /* device memory for resident Hamiltonians + lane pools; 0 = 85% of free */
   double lane_pool_gb;  /* working pool per lane; 0 = half the budget split over the lanes */
   int lanes;            /* worker lanes (HIP streams), 1..4; 0 = 4 */
+  double gamma_trunc;   /* as negf_config.gamma_trunc, for every request */
 } negf_server_config;
 
 typedef struct {

@@ -51,7 +51,7 @@ class NegfConfig(ctypes.Structure):
                 ("sr_max_iter", ctypes.c_int), ("sr_extra_eta", ctypes.c_int),
                 ("precision", ctypes.c_int), ("device", ctypes.c_int),
                 ("hbm_budget_gb", ctypes.c_double), ("max_batch", ctypes.c_int),
-                ("flags", ctypes.c_int), ("lanes", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("lanes", ctypes.c_int), ("gamma_trunc", ctypes.c_double)]
 
 FLAG_RGF_THREE_PASS = 1
 FLAG_SR_SEPARATE_LEADS = 2
@@ -71,7 +71,8 @@ class NegfServerConfig(ctypes.Structure):
     _fields_ = [("eta", ctypes.c_double), ("sr_tol", ctypes.c_double), ("sr_max_iter", ctypes.c_int),
                 ("sr_extra_eta", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
                 ("flags", ctypes.c_int), ("hbm_budget_gb", ctypes.c_double),
-                ("lane_pool_gb", ctypes.c_double), ("lanes", ctypes.c_int)]
+                ("lane_pool_gb", ctypes.c_double), ("lanes", ctypes.c_int),
+                ("gamma_trunc", ctypes.c_double)]
 
 
 class NegfRequest(ctypes.Structure):
@@ -313,11 +314,11 @@ class Context:
 
     def __init__(self, npx, npy, npz, norb=4, eta=1e-6, sr_tol=1e-10, sr_max_iter=200,
                  sr_extra_eta=True, precision=NEGF_PREC_FP64, device=0, hbm_budget_gb=0.0,
-                 max_batch=0, flags=0, lanes=0):
+                 max_batch=0, flags=0, lanes=0, gamma_trunc=0.0):
         self.dims = (npx, npy, npz, norb)
         self.n = npy * npz * norb
         cfg = NegfConfig(npx, npy, npz, norb, eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)),
-                         precision, device, hbm_budget_gb, max_batch, flags, lanes)
+                         precision, device, hbm_budget_gb, max_batch, flags, lanes, gamma_trunc)
         h = ctypes.c_void_p()
         _check(lib().negf_init(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h
@@ -473,9 +474,9 @@ class Server:
 
     def __init__(self, eta=1e-6, sr_tol=1e-10, sr_max_iter=200, sr_extra_eta=True,
                  precision=NEGF_PREC_FP64, device=0, flags=0, hbm_budget_gb=0.0, lane_pool_gb=0.0,
-                 lanes=0):
+                 lanes=0, gamma_trunc=0.0):
         cfg = NegfServerConfig(eta, sr_tol, sr_max_iter, int(bool(sr_extra_eta)), precision, device,
-                               flags, hbm_budget_gb, lane_pool_gb, lanes)
+                               flags, hbm_budget_gb, lane_pool_gb, lanes, gamma_trunc)
         h = ctypes.c_void_p()
         _check(lib().negf_server_create(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h

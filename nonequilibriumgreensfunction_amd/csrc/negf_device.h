@@ -74,14 +74,27 @@ struct GemmBatch {
 // Pivoted Cholesky factors Gam = W W' of both leads' broadening for a batch of
 // energies (negf_kernels.hip): lead l's matrices at gam[l] + b*sG (overwritten
 // with what the factor leaves out), factors at w[l] + b*sW (complex elements),
-// state at st[l] + b*sSt (bytes, 12n + 40 needed).
+// state at st[l] + b*sSt (pchol_state_bytes(n) bytes).  Left in the state: the
+// number of nonzero columns (int at st + pchol_rank_off(n)) and, at + 4, the
+// truncation rank for tau: the fewest leading columns whose product leaves out a
+// part of trace at most tau * tr(Gam) (the rank itself when tau <= 0).
 struct PcholArgs {
   double2* gam[2];
   double2* w[2];
   char* st[2];
   long long sG, sW, sSt;
   int n;
+  double tau;
 };
+// State: d[n] doubles, used[n] ints, then (8-byte aligned) three pivot keys, the
+// stop threshold, tr(Gam), the rank, and two slots of per-workgroup partial sums
+// of the remaining diagonal (at most 16 workgroups of 256 rows: n <= 4095).
+constexpr int kPcholMaxBlocks = 16;
+__host__ __device__ inline long long pchol_key_off(int n) { return ((12LL * n) + 7) & ~7LL; }
+__host__ __device__ inline long long pchol_rank_off(int n) { return pchol_key_off(n) + 40; }
+__host__ __device__ inline long long pchol_state_bytes(int n) {
+  return pchol_key_off(n) + 48 + 2 * kPcholMaxBlocks * 8;
+}
 hipError_t launch_pchol(hipStream_t s, const PcholArgs& a, int batch);
 
 // Spin matrices gMx, gMy, gMz (norb <= 4) as kernel arguments.

@@ -44,18 +44,21 @@ hipError_t launch_assemble(hipStream_t, int, int, cx<T>*, long long, const cx<T>
                            const double*);
 template <typename T>
 hipError_t launch_bd_mul(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, long long,
-                         const cx<T>*, int, int, const int*);
+                         const cx<T>*, int, int, int, const int*);
 template <typename T>
 hipError_t launch_siteblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
-                             int, double*, long long, long long, const SpinMats&, int, const int*);
+                             int, double*, long long, long long, const SpinMats&, const int*,
+                             long long, int, const int*);
 hipError_t launch_zero(hipStream_t, void*, size_t);
 template <typename T>
 hipError_t launch_bondblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              const double2*, const double2*, int, int, int, double*, long long,
-                             long long, int, int, const int*);
+                             long long, int, const int*, long long, int, const int*);
 template <typename T>
-hipError_t launch_inner(hipStream_t, const cx<T>*, long long, const cx<T>*, long long, int,
+hipError_t launch_inner(hipStream_t, const cx<T>*, long long, const cx<T>*, long long, int, int, int,
                         double*, long long, int, const int*);
+template <typename T>
+hipError_t launch_trunc_cols(hipStream_t, cx<T>*, long long, int, int, const int*, long long, int);
 template <typename T>
 hipError_t launch_eye(hipStream_t, cx<T>*, long long, int, int, const int*);
 template <typename T>
@@ -160,7 +163,7 @@ struct Layout {
     o_GamR = o; o += nn;
     o_WL = o; o += nn;
     o_WR = o; o += nn;
-    pst_bytes = ((12LL * n + 7) & ~7LL) + 32;  // d[n], used[n], 3 pivot keys, threshold
+    pst_bytes = pchol_state_bytes(n);
     o_pst = o; o += (2 * pst_bytes + 15) / 16;
     o_prow = o; o += (long long)kGjNBO * n;
     Es = (o + 31) & ~31LL;
@@ -191,10 +194,13 @@ struct GraphKey {
   unsigned long long ctx;  // the context's unique id (never reused, unlike its address)
   unsigned long long gen;
   int nb;
-  bool currents;
+  bool currents, pairs;
   const void* arena;
+  int cols[4];  // column counts of the factor products (launch shapes)
   bool operator==(const GraphKey& o) const {
-    return ctx == o.ctx && gen == o.gen && nb == o.nb && currents == o.currents && arena == o.arena;
+    return ctx == o.ctx && gen == o.gen && nb == o.nb && currents == o.currents && pairs == o.pairs &&
+           arena == o.arena && cols[0] == o.cols[0] && cols[1] == o.cols[1] && cols[2] == o.cols[2] &&
+           cols[3] == o.cols[3];
   }
 };
 static std::atomic<unsigned long long> g_ctx_uid{1};
@@ -373,6 +379,10 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
     return NEGF_ERR_ARG;
   }
   *out = nullptr;
+  if (!(cfg->gamma_trunc >= 0.0 && cfg->gamma_trunc < 1.0)) {
+    negf_set_error("gamma_trunc must be in [0, 1), got %g", cfg->gamma_trunc);
+    return NEGF_ERR_ARG;
+  }
   if (cfg->npx < 1 || cfg->npy < 1 || cfg->npz < 1 ||
       !(cfg->norb == 1 || cfg->norb == 2 || cfg->norb == 4)) {
     negf_set_error("bad dimensions npx=%d npy=%d npz=%d norb=%d (norb must be 1, 2 or 4)",
@@ -685,6 +695,11 @@ static GemmBatch gbr(const zd* A, long long sA, int lda, const zd* B, long long 
                      long long sC, int ldc, int n, const int* active = nullptr) {
   GemmBatch g = gb(A, sA, lda, B, sB, ldb, C, sC, ldc, n, active);
   g.m3 = 1;
+  return g;
+}
+static GemmBatch shape(GemmBatch g, int m, int ncols) {
+  g.m = m;
+  g.n = ncols;
   return g;
 }
 static const zd kOne = {1.0, 0.0}, kZero = {0.0, 0.0}, kMone = {-1.0, 0.0};
@@ -1122,19 +1137,44 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   // once into a HIP graph per (lane, context, Hamiltonian, batch size) and replayed
   // for later batches of the same shape; the graph reads the energies, Sigma and
   // the Hamiltonian from device memory, so a replay solves the new batch.
+  // Gamma = i (Sigma - Sigma') with DOSc, and the factors Gamma_L = W_L W_L',
+  // Gamma_R = W_R W_R' (pivoted Cholesky), ahead of the RGF launches: the factors'
+  // column counts set those launches' shapes.
+  TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
+  TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
+  PcholArgs pa = {};
+  pa.gam[0] = Q(y.o_GamL); pa.gam[1] = Q(y.o_GamR);
+  pa.w[0] = Q(y.o_WL); pa.w[1] = Q(y.o_WR);
+  pa.st[0] = (char*)Q(y.o_pst); pa.st[1] = pa.st[0] + y.pst_bytes;
+  pa.sG = pa.sW = Es;
+  pa.sSt = Es * (long long)sizeof(zd);
+  pa.n = n;
+  pa.tau = c->cfg.gamma_trunc;
+  TRY(launch_pchol(s, pa, nb));
+  // Column counts, each the batch's largest rounded up to 32 (capped at n): cLf, cR
+  // the nonzero columns of W_L, W_R; cL, cRt their truncation ranks.  The T12 chain
+  // (last block column, V = G W_R) and the left factor of T12 take whole factors;
+  // the first-column chain runs on W_L's leading truncation-rank columns, and the
+  // site and bond blocks of both leads sum over each energy's own truncation rank
+  // (read by the kernels from the factorization's state).  A product's columns past
+  // an energy's own count are zero, so every energy's results are the same bits
+  // whatever the rest of the batch is.
+  const long long rk_off = pchol_rank_off(n);
+  std::vector<int> rk(4 * (size_t)nb);
+  for (int l = 0; l < 2; ++l)
+    TRY(hipMemcpy2DAsync(rk.data() + (size_t)2 * nb * l, 2 * sizeof(int), pa.st[l] + rk_off, pa.sSt,
+                         2 * sizeof(int), nb, hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  int mx[4] = {0, 0, 0, 0};  // rank L, truncation rank L, rank R, truncation rank R
+  for (int l = 0; l < 2; ++l)
+    for (int b = 0; b < nb; ++b)
+      for (int u = 0; u < 2; ++u) mx[2 * l + u] = std::max(mx[2 * l + u], rk[(size_t)2 * nb * l + 2 * b + u]);
+  auto r32 = [&](int v) { return std::min(n, std::max(32, (v + 31) / 32 * 32)); };
+  const int cLf = r32(mx[0]), cL = r32(mx[1]), cR = r32(mx[2]), cRt = r32(mx[3]);
+  const int* kcL = (const int*)(pa.st[0] + rk_off + sizeof(int));  // per-energy truncation ranks
+  const int* kcR = (const int*)(pa.st[1] + rk_off + sizeof(int));
+  const long long skc = pa.sSt / (long long)sizeof(int);
   auto rgf = [&]() -> int {
-    TRY(launch_gamma_dos<double>(s, Q(y.o_SigL), Es, Q(y.o_GamL), Es, n, nb, ln.d_scal + 4, 8));
-    TRY(launch_gamma_dos<double>(s, Q(y.o_SigR), Es, Q(y.o_GamR), Es, n, nb, ln.d_scal + 5, 8));
-    {  // Gamma_L = W_L W_L', Gamma_R = W_R W_R' (pivoted Cholesky)
-      PcholArgs pa = {};
-      pa.gam[0] = Q(y.o_GamL); pa.gam[1] = Q(y.o_GamR);
-      pa.w[0] = Q(y.o_WL); pa.w[1] = Q(y.o_WR);
-      pa.st[0] = (char*)Q(y.o_pst); pa.st[1] = pa.st[0] + y.pst_bytes;
-      pa.sG = pa.sW = Es;
-      pa.sSt = Es * (long long)sizeof(zd);
-      pa.n = n;
-      TRY(launch_pchol(s, pa, nb));
-    }
 
     GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
     w.m3 = 1;
@@ -1166,14 +1206,20 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
                                   c->cfg.eta, S1, Es, S2, Es, g, Es, c->X, n, nb, nullptr, hv(x)));
       return 0;
     };
-    // Y = X G (conj=0) or X' G (conj=1)
-    auto xmul = [&](int conj, zd* Y, const zd* G) -> int {
+    // Y = X G (conj=0) or X' G (conj=1), the first `cols` columns
+    auto xmul = [&](int conj, zd* Y, const zd* G, int cols) -> int {
       if (bd) {
-        TRY(launch_bd_mul<double>(s, no, conj, Y, Es, G, Es, c->X, n, nb, nullptr));
+        TRY(launch_bd_mul<double>(s, no, conj, Y, Es, G, Es, c->X, n, cols, nb, nullptr));
       } else {
-        TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gbr(c->X, 0, n, G, Es, n, Y, Es, n, n), nb,
-                                 kOne, kZero));
+        TRY(gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N,
+                                 shape(gbr(c->X, 0, n, G, Es, n, Y, Es, n, n), n, cols), nb, kOne, kZero));
       }
+      return 0;
+    };
+    // C = A B on the first `cols` columns of B and C
+    auto mulc = [&](const zd* A, const zd* B, zd* C, int cols, zd beta) -> int {
+      TRY(gemm_batched<double>(s, OP_N, OP_N, shape(gbr(A, Es, n, B, Es, n, C, Es, n, n), n, cols), nb,
+                               kOne, beta));
       return 0;
     };
     // The column chains run on V = G W (W the lead's factor, Gamma = W W'): both
@@ -1184,7 +1230,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     auto site_out = [&](int lead, int x) { return ln.d_site + ((long long)lead * L + x) * ns; };
     auto siteblocks = [&](int lead, int x, const zd* V) -> int {
       TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, site_out(lead, x), y.site_per_e, y.site_q,
-                                    c->sp, nb, nullptr));
+                                    c->sp, lead ? kcR : kcL, skc, nb, nullptr));
       return 0;
     };
     // Bond-current terms of slice x for one lead (Calculate_Observables.F90:111-168)
@@ -1193,21 +1239,25 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     auto bonds = [&](int lead, int x, const zd* V, const zd* Vn) -> int {
       if (!c->want_currents) return 0;
       TRY(launch_bondblocks<double>(s, no, V, Es, Vn, Es, hx(x), c->X, n, npy, npz, site_out(lead, x),
-                                    y.site_per_e, y.site_q, c->want_pairs ? 1 : 0, nb, nullptr));
+                                    y.site_per_e, y.site_q, c->want_pairs ? 1 : 0, lead ? kcR : kcL, skc,
+                                    nb, nullptr));
       return 0;
     };
-    // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b
-    auto trace_term = [&](int lead, const zd* V, int idx) -> int {
+    // scal[idx] = |W_lead' V|_F^2 = Re tr(Gam_lead G Gam_b G') for V = G W_b, on
+    // the first m columns of W_lead and ncols of V
+    auto trace_term = [&](int lead, const zd* V, int idx, int m, int ncols) -> int {
       const zd* Wa = Q(lead ? y.o_WR : y.o_WL);
-      TRY(gemm_batched<double>(s, OP_C, OP_N, gbr(Wa, Es, n, V, Es, n, Q(y.o_P), Es, n, n), nb, kOne,
-                               kZero));
-      TRY(launch_inner<double>(s, Q(y.o_P), Es, Q(y.o_P), Es, n, ln.d_scal + idx, 8, nb, nullptr));
+      GemmBatch g = shape(gbr(Wa, Es, n, V, Es, n, Q(y.o_P), Es, n, n), m, ncols);
+      TRY(gemm_batched<double>(s, OP_C, OP_N, g, nb, kOne, kZero));
+      TRY(launch_inner<double>(s, Q(y.o_P), Es, Q(y.o_P), Es, n, m, ncols, ln.d_scal + idx, 8, nb,
+                               nullptr));
       return 0;
     };
-    // V = G W_lead into dst
-    auto times_w = [&](const zd* G, int lead, zd* dst) -> int {
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(G, Es, n, Q(lead ? y.o_WR : y.o_WL), Es, n, dst, Es, n, n),
-                               nb, kOne, kZero));
+    // V = G W_lead (first cols columns) into dst; on W_L's truncation rank, the
+    // columns past each energy's own rank are cleared
+    auto times_w = [&](const zd* G, int lead, zd* dst, int cols) -> int {
+      if (int rc = mulc(G, Q(lead ? y.o_WR : y.o_WL), dst, cols, kZero)) return rc;
+      if (lead == 0) TRY(launch_trunc_cols<double>(s, dst, Es, n, cols, kcL, skc, nb));
       return 0;
     };
     int r;
@@ -1223,35 +1273,40 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       const long long Es32 = 2 * Es;  // complex64 stride between energies
       auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
       const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
-      auto gbf = [&](const zf* A, const zf* B, zf* C) {
+      // C (m x ncols) = A B, A n columns
+      auto gbf = [&](const zf* A, const zf* B, zf* C, int m, int ncols) {
         GemmBatch g = {};
         g.A = A; g.sA = Es32; g.lda = n;
         g.B = B; g.sB = Es32; g.ldb = n;
         g.C = C; g.sC = Es32; g.ldc = n;
-        g.m = n; g.n = n; g.k = n;
+        g.m = m; g.n = ncols; g.k = n;
         g.m3 = 1;
         return g;
       };
-      TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, nn, nb));
-      TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, nn, nb));
+      // complex64 factors: W_L on its truncation rank (cleared past each energy's
+      // own), W_R whole (the Modec2 traces)
+      TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, (long long)n * cL, nb));
+      TRY(launch_trunc_cols<float>(s, F(y.f_WL), Es32, n, cL, kcL, skc, nb));
+      TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, (long long)n * cR, nb));
       // site blocks of V32 = G32 W_lead for one lead at slice x
       auto site32 = [&](int lead, int x, const zf* V32) -> int {
         TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, site_out(lead, x), y.site_per_e,
-                                     y.site_q, c->sp, nb, nullptr));
+                                     y.site_q, c->sp, lead ? kcR : kcL, skc, nb, nullptr));
         return 0;
       };
       auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
         if (!c->want_currents) return 0;
         TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
                                      site_out(lead, x), y.site_per_e, y.site_q, c->want_pairs ? 1 : 0,
-                                     nb, nullptr));
+                                     lead ? kcR : kcL, skc, nb, nullptr));
         return 0;
       };
-      // scal[idx] = |W_lead' V32|_F^2
-      auto trace32 = [&](int lead, const zf* V32, int idx) -> int {
-        GemmBatch g = gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P));
+      // scal[idx] = |W_lead' V32|_F^2 (first m columns of W_lead, ncols of V32)
+      auto trace32 = [&](int lead, const zf* V32, int idx, int m, int ncols) -> int {
+        GemmBatch g = gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P), m, ncols);
         TRY(gemm_batched<float>(s, OP_C, OP_N, g, nb, kOnef, kZerof));
-        TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, ln.d_scal + idx, 8, nb, nullptr));
+        TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, m, ncols, ln.d_scal + idx, 8, nb,
+                                nullptr));
         return 0;
       };
       // The block inverse gL_x = inv(M_x) of the forward pass.  Mixed form: FP64
@@ -1328,40 +1383,41 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         if ((r = invert_block(x))) return r;
         if (x == 0) {
           TRY(to_c64(s, F(y.f_g), Es32, slot(0), Es, nn, nb));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0)), nb, kOnef, kZerof));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0), n, cL), nb, kOnef, kZerof));
         } else {
           TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
-          TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, nb, nullptr));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kZerof));
+          TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, cL, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x), n, cL), nb, kOnef, kZerof));
         }
       }
       // backward: G_xL W_R in FP64; G_x1 W_L = z_x + gL_x X G_{x+1,1} W_L in complex64
       // (in place of z_x)
       zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
-      if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
+      // G_LL W_R (whole W_R: the T12 chain)
+      if ((r = mulc(slot(L - 1), Q(y.o_WR), bufs[1], cR, kZero))) return r;
       zd* colA = bufs[1];
       zd* colB = nullptr;
       int pb = 0;
       for (int x = L - 1; x >= 0; --x) {
         if (x < L - 1) {
           colB = bufs[pb];
-          if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                                   kOne, kZero));
+          if ((r = xmul(0, Q(y.o_Y), colA, cR))) return r;
+          if ((r = mulc(slot(x), Q(y.o_Y), colB, cR, kZero))) return r;
           TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
-          TRY(launch_bd_mul<float>(s, no, 0, F(y.f_Y), Es32, zs(x + 1), Es32, c->X32, n, nb, nullptr));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x)), nb, kOnef, kOnef));
+          TRY(launch_bd_mul<float>(s, no, 0, F(y.f_Y), Es32, zs(x + 1), Es32, c->X32, n, cL, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x), n, cL), nb, kOnef, kOnef));
         }
         const zd* GxL = (x < L - 1) ? colB : colA;
-        if (x == 0 && (r = trace_term(0, GxL, 0))) return r;  // T12 = |W_L' G_1L W_R|^2 in FP64
-        TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, nn, nb));
+        // T12 = |W_L' G_1L W_R|^2 in FP64, whole factors
+        if (x == 0 && (r = trace_term(0, GxL, 0, cLf, cR))) return r;
+        TRY(to_c64(s, F(y.f_G), Es32, GxL, Es, (long long)n * (x == L - 1 ? cR : cRt), nb));
         if ((r = site32(1, x, F(y.f_G)))) return r;
         if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
-        if (x == L - 1 && (r = trace32(1, F(y.f_G), 3))) return r;  // Modec2, right-lead term
+        if (x == L - 1 && (r = trace32(1, F(y.f_G), 3, cR, cR))) return r;  // Modec2, right-lead term
         if ((r = site32(0, x, zs(x)))) return r;
         if ((r = bonds32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
-        if (x == L - 1 && (r = trace32(1, zs(x), 2))) return r;  // Modec2, left-lead term
-        if (x == 0 && (r = trace32(0, zs(x), 1))) return r;      // Modec1, left-lead term
+        if (x == L - 1 && (r = trace32(1, zs(x), 2, cR, cL))) return r;  // Modec2, left-lead term
+        if (x == 0 && (r = trace32(0, zs(x), 1, cL, cL))) return r;      // Modec1, left-lead term
         if (x < L - 1) {
           colA = colB;
           pb ^= 1;
@@ -1377,11 +1433,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         if ((r = assemble(x, x > 0 ? slot(x - 1) : nullptr, 1))) return r;
         TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
         if (x == 0) {
-          if ((r = times_w(slot(0), 0, zs(0)))) return r;  // z_1 W_L = gL_1 W_L
+          if ((r = times_w(slot(0), 0, zs(0), cL))) return r;  // z_1 W_L = gL_1 W_L
         } else {
-          if ((r = xmul(1, Q(y.o_Y), zs(x - 1)))) return r;
-          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, zs(x), Es, n, n), nb,
-                                   kOne, kZero));
+          if ((r = xmul(1, Q(y.o_Y), zs(x - 1), cL))) return r;
+          if ((r = mulc(slot(x), Q(y.o_Y), zs(x), cL, kZero))) return r;
         }
       }
       // Backward: the last column G_xL = gL_x X G_{x+1,L}, G_LL = gL_L (:53-63), and the
@@ -1389,29 +1444,27 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       // Slice x then has both columns: the Gn/Gp site blocks of both leads, T12 at x = 1
       // (Calculate_Observables.F90:86-100) and the Modec terms at the ends (:48-53).
       zd* bufs[2] = {Q(y.o_colA), Q(y.o_colB)};
-      if ((r = times_w(slot(L - 1), 1, bufs[1]))) return r;  // G_LL W_R
+      if ((r = times_w(slot(L - 1), 1, bufs[1], cR))) return r;  // G_LL W_R
       zd* colA = bufs[1];  // G_{x+1,L} W_R
       zd* colB = nullptr;  // G_{x,L} W_R
       int pb = 0;
       for (int x = L - 1; x >= 0; --x) {
         if (x < L - 1) {
           colB = bufs[pb];
-          if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                                   kOne, kZero));
-          if ((r = xmul(0, Q(y.o_P), zs(x + 1)))) return r;
-          TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_P), Es, n, zs(x), Es, n, n), nb,
-                                   kOne, kOne));
+          if ((r = xmul(0, Q(y.o_Y), colA, cR))) return r;
+          if ((r = mulc(slot(x), Q(y.o_Y), colB, cR, kZero))) return r;
+          if ((r = xmul(0, Q(y.o_P), zs(x + 1), cL))) return r;
+          if ((r = mulc(slot(x), Q(y.o_P), zs(x), cL, kOne))) return r;
         }
         const zd* GxL = (x < L - 1) ? colB : colA;
         if ((r = siteblocks(1, x, GxL))) return r;
         if ((r = bonds(1, x, GxL, x < L - 1 ? colA : nullptr))) return r;
-        if (x == L - 1 && (r = trace_term(1, GxL, 3))) return r;  // Modec2, right-lead term
-        if (x == 0 && (r = trace_term(0, GxL, 0))) return r;      // T12
+        if (x == L - 1 && (r = trace_term(1, GxL, 3, cR, cR))) return r;  // Modec2, right-lead term
+        if (x == 0 && (r = trace_term(0, GxL, 0, cLf, cR))) return r;     // T12
         if ((r = siteblocks(0, x, zs(x)))) return r;
         if ((r = bonds(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
-        if (x == L - 1 && (r = trace_term(1, zs(x), 2))) return r;  // Modec2, left-lead term
-        if (x == 0 && (r = trace_term(0, zs(x), 1))) return r;      // Modec1, left-lead term
+        if (x == L - 1 && (r = trace_term(1, zs(x), 2, cR, cL))) return r;  // Modec2, left-lead term
+        if (x == 0 && (r = trace_term(0, zs(x), 1, cLf, cL))) return r;     // Modec1, left-lead term
         if (x < L - 1) {
           colA = colB;
           pb ^= 1;
@@ -1427,20 +1480,19 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     // ---- pass 2: last column and gR
     zd* colA = Q(y.o_colA);
     zd* colB = Q(y.o_colB);
-    if ((r = times_w(slot(L - 1), 1, colA))) return r;  // G_LL W_R
+    if ((r = times_w(slot(L - 1), 1, colA, cR))) return r;  // G_LL W_R
     if ((r = siteblocks(1, L - 1, colA))) return r;
     if ((r = bonds(1, L - 1, colA, nullptr))) return r;
-    if ((r = trace_term(1, colA, 3))) return r;          // Modec2, right-lead term
-    if (L == 1 && (r = trace_term(0, colA, 0))) return r; // T12
+    if ((r = trace_term(1, colA, 3, cR, cR))) return r;           // Modec2, right-lead term
+    if (L == 1 && (r = trace_term(0, colA, 0, cLf, cR))) return r; // T12
     if ((r = assemble(L - 1, nullptr, 2))) return r;
     TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(L - 1), Es, n, n, nb, w, nullptr));
     for (int x = L - 2; x >= 0; --x) {
-      if ((r = xmul(0, Q(y.o_Y), colA))) return r;
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                               kOne, kZero));
+      if ((r = xmul(0, Q(y.o_Y), colA, cR))) return r;
+      if ((r = mulc(slot(x), Q(y.o_Y), colB, cR, kZero))) return r;
       if ((r = siteblocks(1, x, colB))) return r;
       if ((r = bonds(1, x, colB, colA))) return r;
-      if (x == 0 && (r = trace_term(0, colB, 0))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
+      if (x == 0 && (r = trace_term(0, colB, 0, cLf, cR))) return r;  // T12 = Re tr(GamL G1L GamR G1L')
       if ((r = assemble(x, slot(x + 1), 2))) return r;
       TRY(gj_invert<double>(s, Q(y.o_M), Es, n, slot(x), Es, n, n, nb, w, nullptr));
       std::swap(colA, colB);
@@ -1448,17 +1500,16 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     // ---- pass 3: first column
     colA = Q(y.o_colA);
     colB = Q(y.o_colB);
-    if ((r = times_w(slot(0), 0, colA))) return r;  // G_11 W_L
+    if ((r = times_w(slot(0), 0, colA, cL))) return r;  // G_11 W_L
     if ((r = siteblocks(0, 0, colA))) return r;
-    if ((r = trace_term(0, colA, 1))) return r;            // Modec1, left-lead term
-    if (L == 1 && (r = trace_term(1, colA, 2))) return r;  // Modec2, left-lead term
+    if ((r = trace_term(0, colA, 1, cLf, cL))) return r;            // Modec1, left-lead term
+    if (L == 1 && (r = trace_term(1, colA, 2, cR, cL))) return r;   // Modec2, left-lead term
     for (int x = 1; x < L; ++x) {
-      if ((r = xmul(1, Q(y.o_Y), colA))) return r;
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbr(slot(x), Es, n, Q(y.o_Y), Es, n, colB, Es, n, n), nb,
-                               kOne, kZero));
+      if ((r = xmul(1, Q(y.o_Y), colA, cL))) return r;
+      if ((r = mulc(slot(x), Q(y.o_Y), colB, cL, kZero))) return r;
       if ((r = siteblocks(0, x, colB))) return r;
       if ((r = bonds(0, x - 1, colA, colB))) return r;
-      if (x == L - 1 && (r = trace_term(1, colB, 2))) return r;
+      if (x == L - 1 && (r = trace_term(1, colB, 2, cR, cL))) return r;
       std::swap(colA, colB);
     }
     if ((r = bonds(0, L - 1, colA, nullptr))) return r;
@@ -1467,7 +1518,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   int rr;
   const bool graph = (c->cfg.flags & NEGF_FLAG_GRAPH) && c->cfg.precision != NEGF_PREC_REFINED;
   if (graph) {
-    const GraphKey key = {c->uid, c->generation, nb, c->want_currents, ln.E};
+    const GraphKey key = {c->uid, c->generation, nb, c->want_currents, c->want_pairs, ln.E,
+                          {cLf, cL, cR, cRt}};
     int slot_g = -1;
     for (int i = 0; i < Lane::kGraphSlots; ++i)
       if (ln.gexec[i] && ln.gkey[i] == key) slot_g = i;
@@ -2212,7 +2264,7 @@ extern "C" int negf_retarded(negf_ctx* c, const double* InvGRd, double* GRd_h, d
   auto blkp = [&](zd* base, int x) { return base + (size_t)x * nn; };
   // Y = X G (conj 0) or X' G (conj 1)
   auto xmul = [&](int conj, zd* Y, const zd* G) -> hipError_t {
-    if (bd) return launch_bd_mul<double>(s, no, conj, Y, 0, G, 0, c->X, n, 1, nullptr);
+    if (bd) return launch_bd_mul<double>(s, no, conj, Y, 0, G, 0, c->X, n, n, 1, nullptr);
     return gemm_batched<double>(s, conj ? OP_C : OP_N, OP_N, gb(c->X, 0, n, G, 0, n, Y, 0, n, n), 1,
                                 kOne, kZero);
   };

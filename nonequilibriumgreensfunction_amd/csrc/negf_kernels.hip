@@ -103,7 +103,8 @@ __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, c
   Y[(long long)b * sY + (long long)col * n + a] = v;
 }
 
-// Site-diagonal blocks D_s = (B G')[site s].
+// Site-diagonal blocks D_s = (B G')[site s], over the first kc[b * skc] columns of
+// B and G (all n when kc is null).
 // Writes, per site s of slice x:  out[0] = Re tr D,  out[1+a] = Re tr(D sigma_a),
 // out[4+r] = Re D_rr (r < NORB).  out layout: out_b + q*qstride + s.
 // A workgroup takes 16 rows (16 / NORB sites) and splits the sum over k into 16
@@ -115,9 +116,11 @@ template <typename T, int NORB>
 __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long sB,
                                                     const cx<T>* G, long long sG, int n,
                                                     double* out, long long sOut, long long qstride,
-                                                    SpinMats sp, const int* active) {
+                                                    SpinMats sp, const int* kc, long long skc,
+                                                    const int* active) {
   constexpr int RW = 16, KQ = 16;
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int kcols = kc ? kc[(long long)b * skc] : n;  // columns of B and G taken
   const cx<T>* Bb = Bm + (long long)b * sB;
   const cx<T>* Gb = G + (long long)b * sG;
   const int row0 = blockIdx.x * RW;
@@ -128,8 +131,9 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 #pragma unroll
   for (int q = 0; q < NORB; ++q) accr[q] = acci[q] = 0.0;
   if (a < n) {
+    // the chunks are fixed by n, so the sum order does not depend on kcols
     const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
-    for (int k = kbeg; k < kend; ++k) {
+    for (int k = kbeg; k < (kend < kcols ? kend : kcols); ++k) {
       const cx<T> bv = Bb[(long long)k * n + a];
 #pragma unroll
       for (int q = 0; q < NORB; ++q) {
@@ -200,16 +204,18 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 // summed over the orbital pairs at out_b + (8 + d) * qstride + s, d = 0 x, 1 y, 2 z,
 // and, when `pairs`, per orbital pair at out_b + (11 + 16 d + NORB ii + jj) * qstride + s
 // (the orbital-resolved currents); 0 where the bond does not exist.
-// Reduction as in k_siteblocks: 16 rows per workgroup, the k sum in 16 chunks
-// added in chunk order, so results do not depend on the batch.
+// Reduction as in k_siteblocks (over the first kc[b * skc] columns): 16 rows per
+// workgroup, the k sum in 16 chunks added in chunk order, so results do not depend
+// on the batch.
 template <typename T, int NORB>
 __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV, const cx<T>* Vn,
                                                     long long sVn, const double2* H, const double2* X,
                                                     int n, int npy, int npz, double* out,
                                                     long long sOut, long long qstride, int pairs,
-                                                    const int* active) {
+                                                    const int* kc, long long skc, const int* active) {
   constexpr int RW = 16, KQ = 16, NQ = 3 * NORB;
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int kcols = kc ? kc[(long long)b * skc] : n;  // columns of V and Vn taken
   const cx<T>* Vb = V + (long long)b * sV;
   const cx<T>* Vnb = Vn ? Vn + (long long)b * sVn : nullptr;
   const int NN = NORB * npy;
@@ -225,7 +231,7 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
   for (int q = 0; q < NQ; ++q) accr[q] = acci[q] = 0.0;
   if (a < n) {
     const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
-    for (int k = kbeg; k < kend; ++k) {
+    for (int k = kbeg; k < (kend < kcols ? kend : kcols); ++k) {
       const long long col = (long long)k * n;
       const cx<T> va = Vb[col + a];
 #pragma unroll
@@ -298,17 +304,22 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
   }
 }
 
-// out[b*sOut] = Re sum_{ij} P_ij conj(Q_ij) over an n x n block (one workgroup).
+// out[b*sOut] = Re sum_{ij} P_ij conj(Q_ij) over the leading m x ncols part of
+// blocks with leading dimension ld (one workgroup).  Element (i, j) goes to thread
+// (j ld + i) mod 1024 whatever m and ncols are, so entries that are zero leave the
+// sum's bits unchanged when m or ncols grow.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_inner(const cx<T>* P, long long sP, const cx<T>* Q,
-                                                long long sQ, long long nn, double* out,
+                                                long long sQ, int ld, int m, int ncols, double* out,
                                                 long long sOut, const int* active) {
   const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
   const cx<T>* Pb = P + (long long)b * sP;
   const cx<T>* Qb = Q + (long long)b * sQ;
   double acc = 0.0;
-  for (long long i = threadIdx.x; i < nn; i += 1024) {
-    const cx<T> p = Pb[i], q = Qb[i];
+  const long long end = (long long)ld * ncols;
+  for (long long e = threadIdx.x; e < end; e += 1024) {
+    if ((int)(e % ld) >= m) continue;
+    const cx<T> p = Pb[e], q = Qb[e];
     acc += (double)p.x * q.x + (double)p.y * q.y;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
@@ -456,24 +467,65 @@ __global__ void k_gamma(const cx<T>* Sig, long long sS, cx<T>* Gam, long long sG
 // (one MFMA product, in place), so a column sums only over its own panel.
 // Once the largest remaining d_p is at most kPcholTol * max_i Gam_ii the remaining
 // columns are zero (the dropped part is positive semidefinite with trace below
-// n * kPcholTol * max Gam_ii).  blockIdx.y selects the lead, blockIdx.z the matrix.
-// Per-matrix state (bytes from st + z * sSt): d[n] doubles, used[n] ints, then three
-// 64-bit pivot keys (key of column j in slot j % 3) and the stop threshold.
+// n * kPcholTol * max Gam_ii).  Before each column the remaining trace
+// sum_{i unused} d_i -- the trace of the part the columns so far leave out, which
+// is positive semidefinite -- is formed (per workgroup, then over the workgroups
+// in a fixed order, so the same bits on every run); the first column count at
+// which it is at most a.tau * tr(Gam) is kept as the truncation rank.
+// blockIdx.y selects the lead, blockIdx.z the matrix.
+// Per-matrix state (bytes from st + z * sSt, pchol_state_bytes): d[n] doubles,
+// used[n] ints, then three 64-bit pivot keys (key of column j in slot j % 3), the
+// stop threshold, tr(Gam), the number of nonzero columns, the truncation rank
+// (-1 until reached), and the remaining-trace partial sums of the last two columns
+// (slot j % 2 holds those before column j).
 constexpr double kPcholTol = 1e-15;
 constexpr int kPcholNB = 32;
-
-__device__ inline long long pchol_key_off(int n) { return ((12LL * n) + 7) & ~7LL; }
 
 __device__ inline unsigned long long pchol_key(double d, int i) {
   const double c = d > 0.0 ? d : 0.0;  // rounding below zero counts as zero; NaN too
   return ((unsigned long long)__double_as_longlong(c) & ~0xfffull) | (unsigned long long)(4095 - i);
 }
 
+// Sum over a 256-thread workgroup, the same bits on every run; valid in thread 0.
+__device__ inline double block_sum256(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __shared__ double red[4];
+  __syncthreads();  // red may still be read by an earlier call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+struct PcholState {
+  double* d;
+  int* used;
+  unsigned long long* key;
+  double* thr;
+  double* tr0;
+  int* rank;
+  int* rtau;
+  double* part;  // [2][kPcholMaxBlocks]
+};
+__device__ inline PcholState pchol_state(char* st, int n) {
+  PcholState p;
+  p.d = (double*)st;
+  p.used = (int*)(st + 8LL * n);
+  p.key = (unsigned long long*)(st + pchol_key_off(n));
+  p.thr = (double*)(p.key + 3);
+  p.tr0 = p.thr + 1;
+  p.rank = (int*)(st + pchol_rank_off(n));
+  p.rtau = p.rank + 1;
+  p.part = (double*)(st + pchol_rank_off(n) + 8);
+  return p;
+}
+
 __global__ void k_pchol_clear(PcholArgs a) {
-  char* st = a.st[blockIdx.y] + (long long)blockIdx.z * a.sSt;
-  unsigned long long* key = (unsigned long long*)(st + pchol_key_off(a.n));
-  if (threadIdx.x < 3) key[threadIdx.x] = 0ull;
-  if (threadIdx.x == 3) *(double*)(key + 3) = 0.0;
+  const PcholState q = pchol_state(a.st[blockIdx.y] + (long long)blockIdx.z * a.sSt, a.n);
+  if (threadIdx.x < 3) q.key[threadIdx.x] = 0ull;
+  if (threadIdx.x == 3) *q.thr = 0.0;
+  if (threadIdx.x == 4) *q.tr0 = 0.0;
+  if (threadIdx.x == 5) *q.rank = 0;
+  if (threadIdx.x == 6) *q.rtau = -1;
 }
 
 __device__ inline unsigned long long wave_umax64(unsigned long long x) {
@@ -487,49 +539,58 @@ __device__ inline unsigned long long wave_umax64(unsigned long long x) {
 __global__ __launch_bounds__(256) void k_pchol_init(PcholArgs a) {
   const int lead = blockIdx.y;
   const double2* G = a.gam[lead] + (long long)blockIdx.z * a.sG;
-  char* st = a.st[lead] + (long long)blockIdx.z * a.sSt;
-  double* d = (double*)st;
-  int* used = (int*)(st + 8LL * a.n);
-  unsigned long long* key = (unsigned long long*)(st + pchol_key_off(a.n));
+  const PcholState q = pchol_state(a.st[lead] + (long long)blockIdx.z * a.sSt, a.n);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long k = 0ull;
+  double di = 0.0;
   if (i < a.n) {
-    const double di = G[(long long)i * a.n + i].x;
-    d[i] = di;
-    used[i] = 0;
+    di = G[(long long)i * a.n + i].x;
+    q.d[i] = di;
+    q.used[i] = 0;
     k = pchol_key(di, i);
   }
+  const double part = block_sum256(di > 0.0 ? di : 0.0);
+  if (threadIdx.x == 0) q.part[blockIdx.x] = part;  // slot 0: before column 0
   k = wave_umax64(k);
-  if ((threadIdx.x & 63) == 0 && k) atomicMax(&key[0], k);
+  if ((threadIdx.x & 63) == 0 && k) atomicMax(&q.key[0], k);
 }
 
 __global__ __launch_bounds__(256) void k_pchol_col(PcholArgs a, int j, int j0) {
   const int lead = blockIdx.y, n = a.n;
   const double2* G = a.gam[lead] + (long long)blockIdx.z * a.sG;
   double2* W = a.w[lead] + (long long)blockIdx.z * a.sW;
-  char* st = a.st[lead] + (long long)blockIdx.z * a.sSt;
-  double* d = (double*)st;
-  int* used = (int*)(st + 8LL * n);
-  unsigned long long* key = (unsigned long long*)(st + pchol_key_off(n));
-  double* thr = (double*)(key + 3);
-  const unsigned long long kj = key[j % 3];
+  const PcholState q = pchol_state(a.st[lead] + (long long)blockIdx.z * a.sSt, n);
+  const unsigned long long kj = q.key[j % 3];
   // slot (j + 2) % 3 held the key of column j - 1, read by the previous launch; it
   // takes column j + 2's key, written by the next launch
-  if (blockIdx.x == 0 && threadIdx.x == 0) key[(j + 2) % 3] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.key[(j + 2) % 3] = 0ull;
+  // the trace left out before this column, added over the workgroups in order
+  const double* pin = q.part + (j & 1) * kPcholMaxBlocks;
+  double rem = 0.0;
+  for (int w = 0; w < (int)gridDim.x; ++w) rem += pin[w];
+  const double tr0 = j == 0 ? rem : *q.tr0;
   const int p = 4095 - (int)(kj & 0xfffull);
-  const double dp = kj ? d[p] : 0.0;
-  const double t = j == 0 ? dp * kPcholTol : *thr;
-  if (j == 0 && blockIdx.x == 0 && threadIdx.x == 0) *thr = t;
+  const double dp = kj ? q.d[p] : 0.0;
+  const double t = j == 0 ? dp * kPcholTol : *q.thr;
+  if (j == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    *q.thr = t;
+    *q.tr0 = tr0;
+  }
   const bool stop = kj == 0ull || !(dp > t);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (!stop) *q.rank = j + 1;
+    if (*q.rtau < 0 && a.tau > 0.0 && rem <= a.tau * tr0) *q.rtau = j;
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long k = 0ull;
+  double left = 0.0;  // this row's remaining diagonal after the column (unused rows)
   if (i < n) {
     double2* Wj = W + (long long)j * n;
-    if (stop || (used[i] && i != p)) {
+    if (stop || (q.used[i] && i != p)) {
       Wj[i] = make_double2(0.0, 0.0);
     } else if (i == p) {
       Wj[i] = make_double2(sqrt(dp), 0.0);
-      used[i] = 1;
+      q.used[i] = 1;
     } else {
       double2 acc = G[(long long)p * n + i];  // Gam minus the earlier panels' part
       for (int l = j0; l < j; ++l) {
@@ -541,13 +602,25 @@ __global__ __launch_bounds__(256) void k_pchol_col(PcholArgs a, int j, int j0) {
       const double s = sqrt(dp);
       const double2 w = make_double2(acc.x / s, acc.y / s);
       Wj[i] = w;
-      const double di = d[i] - (w.x * w.x + w.y * w.y);
-      d[i] = di;
+      const double di = q.d[i] - (w.x * w.x + w.y * w.y);
+      q.d[i] = di;
       k = pchol_key(di, i);
+      left = di > 0.0 ? di : 0.0;
     }
   }
+  const double part = block_sum256(left);
+  if (threadIdx.x == 0) q.part[((j + 1) & 1) * kPcholMaxBlocks + blockIdx.x] = part;
   k = wave_umax64(k);
-  if ((threadIdx.x & 63) == 0 && k) atomicMax(&key[(j + 1) % 3], k);
+  if ((threadIdx.x & 63) == 0 && k) atomicMax(&q.key[(j + 1) % 3], k);
+}
+
+// truncation rank = the rank where tau was never reached (or tau <= 0)
+__global__ void k_pchol_fin(PcholArgs a) {
+  const PcholState q = pchol_state(a.st[blockIdx.y] + (long long)blockIdx.z * a.sSt, a.n);
+  if (threadIdx.x == 0) {
+    const int r = *q.rank, t = *q.rtau;
+    *q.rtau = (t < 0 || t > r) ? r : t;
+  }
 }
 
 hipError_t launch_pchol(hipStream_t s, const PcholArgs& a, int batch) {
@@ -571,6 +644,7 @@ hipError_t launch_pchol(hipStream_t s, const PcholArgs& a, int batch) {
       if (e != hipSuccess) return e;
     }
   }
+  hipLaunchKernelGGL(k_pchol_fin, dim3(1, 2, batch), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -809,10 +883,10 @@ hipError_t launch_assemble(hipStream_t s, int norb, int mode, cx<T>* M, long lon
 
 template <typename T>
 hipError_t launch_bd_mul(hipStream_t s, int norb, int conj, cx<T>* Y, long long sY,
-                         const cx<T>* G, long long sG, const cx<T>* X, int n, int batch,
+                         const cx<T>* G, long long sG, const cx<T>* X, int n, int cols, int batch,
                          const int* active) {
-  if (batch <= 0) return hipSuccess;
-  dim3 grid((n + 255) / 256, n, batch);
+  if (batch <= 0 || cols <= 0) return hipSuccess;
+  dim3 grid((n + 255) / 256, cols, batch);  // the first `cols` columns of G
 #define NEGF_BD(NO, CJ)                                                                       \
   if (norb == NO && conj == CJ) {                                                             \
     hipLaunchKernelGGL((k_bd_mul<T, NO, CJ>), grid, dim3(256), 0, s, Y, sY, G, sG, X, n, active); \
@@ -843,13 +917,13 @@ template <typename T>
 hipError_t launch_bondblocks(hipStream_t s, int norb, const cx<T>* V, long long sV, const cx<T>* Vn,
                              long long sVn, const double2* H, const double2* X, int n, int npy,
                              int npz, double* out, long long sOut, long long qstride, int pairs,
-                             int batch, const int* active) {
+                             const int* kc, long long skc, int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
   dim3 grid((n + 15) / 16, 1, batch);
 #define NEGF_BB(NO)                                                                                \
   if (norb == NO) {                                                                                \
     hipLaunchKernelGGL((k_bondblocks<T, NO>), grid, dim3(256), 0, s, V, sV, Vn, sVn, H, X, n, npy, npz, \
-                       out, sOut, qstride, pairs, active);                                         \
+                       out, sOut, qstride, pairs, kc, skc, active);                                \
     return hipGetLastError();                                                                      \
   }
   NEGF_BB(1) NEGF_BB(2) NEGF_BB(4)
@@ -860,13 +934,14 @@ hipError_t launch_bondblocks(hipStream_t s, int norb, const cx<T>* V, long long 
 template <typename T>
 hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long sB,
                              const cx<T>* G, long long sG, int n, double* out, long long sOut,
-                             long long qstride, const SpinMats& sp, int batch, const int* active) {
+                             long long qstride, const SpinMats& sp, const int* kc, long long skc,
+                             int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
   dim3 grid((n + 15) / 16, 1, batch);
 #define NEGF_SB(NO)                                                                            \
   if (norb == NO) {                                                                            \
     hipLaunchKernelGGL((k_siteblocks<T, NO>), grid, dim3(256), 0, s, Bm, sB, G, sG, n, out, sOut, \
-                       qstride, sp, active);                                                   \
+                       qstride, sp, kc, skc, active);                                          \
     return hipGetLastError();                                                                  \
   }
   NEGF_SB(1) NEGF_SB(2) NEGF_SB(4)
@@ -876,10 +951,30 @@ hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long
 
 template <typename T>
 hipError_t launch_inner(hipStream_t s, const cx<T>* P, long long sP, const cx<T>* Q, long long sQ,
-                        int n, double* out, long long sOut, int batch, const int* active) {
+                        int ld, int m, int ncols, double* out, long long sOut, int batch,
+                        const int* active) {
   if (batch <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_inner<T>), dim3(batch), dim3(1024), 0, s, P, sP, Q, sQ, (long long)n * n,
-                     out, sOut, active);
+  hipLaunchKernelGGL((k_inner<T>), dim3(batch), dim3(1024), 0, s, P, sP, Q, sQ, ld, m, ncols, out,
+                     sOut, active);
+  return hipGetLastError();
+}
+
+// Y_b(:, kc_b : cols) = 0 (kc_b = kc[b * skc]): a factor product truncated per matrix.
+template <typename T>
+__global__ void k_trunc_cols(cx<T>* Y, long long sY, int n, int cols, const int* kc, long long skc) {
+  const int b = blockIdx.z;
+  const int col = blockIdx.y;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n || col >= cols || col < kc[(long long)b * skc]) return;
+  Y[(long long)b * sY + (long long)col * n + a] = mk<T>(0, 0);
+}
+
+template <typename T>
+hipError_t launch_trunc_cols(hipStream_t s, cx<T>* Y, long long sY, int n, int cols, const int* kc,
+                             long long skc, int batch) {
+  if (batch <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_trunc_cols<T>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, Y, sY, n,
+                     cols, kc, skc);
   return hipGetLastError();
 }
 
@@ -955,16 +1050,20 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
                                          long long, const cx<T>*, int, int, const int*,            \
                                          const double*);                                           \
   template hipError_t launch_bd_mul<T>(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,     \
-                                       long long, const cx<T>*, int, int, const int*);             \
+                                       long long, const cx<T>*, int, int, int, const int*);        \
   template hipError_t launch_siteblocks<T>(hipStream_t, int, const cx<T>*, long long,              \
                                            const cx<T>*, long long, int, double*, long long,       \
-                                           long long, const SpinMats&, int, const int*);           \
+                                           long long, const SpinMats&, const int*, long long, int,  \
+                                           const int*);                                            \
   template hipError_t launch_bondblocks<T>(hipStream_t, int, const cx<T>*, long long, const cx<T>*, \
                                            long long, const double2*, const double2*, int, int,    \
-                                           int, double*, long long, long long, int, int,           \
-                                           const int*);                                            \
+                                           int, double*, long long, long long, int, const int*,    \
+                                           long long, int, const int*);                            \
   template hipError_t launch_inner<T>(hipStream_t, const cx<T>*, long long, const cx<T>*,          \
-                                      long long, int, double*, long long, int, const int*);        \
+                                      long long, int, int, int, double*, long long, int,           \
+                                      const int*);                                                 \
+  template hipError_t launch_trunc_cols<T>(hipStream_t, cx<T>*, long long, int, int, const int*,   \
+                                           long long, int);                                        \
   template hipError_t launch_eye<T>(hipStream_t, cx<T>*, long long, int, int, const int*);         \
   template hipError_t launch_add<T>(hipStream_t, cx<T>*, long long, const cx<T>*, long long,       \
                                     const cx<T>*, long long, long long, int, const int*);          \

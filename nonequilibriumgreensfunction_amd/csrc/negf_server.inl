@@ -134,6 +134,7 @@ int admit(negf_server* s, SrvRequest* r) {
   cfg.device = s->cfg.device;
   cfg.flags = s->cfg.flags;
   cfg.lanes = 1;  // the request context only carries state: the server's lanes run its energies
+  cfg.gamma_trunc = s->cfg.gamma_trunc;
   int rc = negf_init(&r->ctx, &cfg);
   if (rc) return rc;
   r->ctx->want_currents = wants_currents(&r->req.acc);
@@ -236,6 +237,10 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
   if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED &&
       cfg->precision != NEGF_PREC_REFINED) {
     negf_set_error("unknown precision form %d", cfg->precision);
+    return NEGF_ERR_ARG;
+  }
+  if (!(cfg->gamma_trunc >= 0.0 && cfg->gamma_trunc < 1.0)) {
+    negf_set_error("gamma_trunc must be in [0, 1)");
     return NEGF_ERR_ARG;
   }
   if (cfg->precision != NEGF_PREC_FP64 && (cfg->flags & NEGF_FLAG_RGF_THREE_PASS)) {

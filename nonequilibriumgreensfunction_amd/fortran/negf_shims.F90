@@ -38,6 +38,7 @@ module negf_shim_ctl
   integer(c_int), save :: negf_shim_precision = NEGF_PREC_FP64
   integer(c_int), save :: negf_shim_device = 0
   integer(c_int), save :: negf_shim_flags = 0      ! NEGF_FLAG_* algorithm selection
+  real(c_double), save :: negf_shim_gamma_trunc = 0d0  ! negf_config%gamma_trunc (0: whole Gamma)
   logical, save :: negf_shim_staged = .false.
 contains
   subroutine negf_shim_fail(where, rc)
@@ -69,6 +70,7 @@ subroutine NEGF_Prepare_Device
     cfg%max_batch = 0
     cfg%flags = negf_shim_flags
     cfg%lanes = 0
+    cfg%gamma_trunc = negf_shim_gamma_trunc
     rc = negf_init(negf_shim_ctx, cfg)
     if (rc /= 0) call negf_shim_fail("negf_init", rc)
     negf_shim_h_ite = -huge(1)

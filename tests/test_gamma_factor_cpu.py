@@ -10,7 +10,7 @@ identities the fused solve uses with V = G W:
 
 The factorization restated here is the device's rule: implicit pivoting (W keeps
 the original row order), the pivot is the unused row of largest remaining
-diagonal with ties to the lower row, and the columns after the remaining diagonal
+diagonal with ties (within 2^-40 relative) to the lower row, and the columns after the remaining diagonal
 falls to 1e-15 of the largest are zero.  Checked on contact Gammas of the oracle
 at band and gap energies: W W' reproduces Gamma to 1e-13, rows already pivoted
 are zero in later columns, and the identities hold against the direct products."""
@@ -23,16 +23,28 @@ from nonequilibriumgreensfunction_amd.hamiltonian import GMX, GMY, GMZ, build_wi
 TOL = 1e-15
 
 
-def pchol(gam):
+def pchol(gam, tau=0.0, want_rank=False):
+    """W, pivot order (and, with want_rank, the truncation rank for tau: the first
+    column count whose remaining trace sum_{unused} max(d_i, 0) is at most
+    tau * tr(Gamma); the rank itself when tau <= 0 or never reached)."""
     n = gam.shape[0]
     W = np.zeros((n, n), complex)
     d = gam.diagonal().real.copy()
     used = np.zeros(n, bool)
     order = []
     thr = None
+    tr0 = np.maximum(d, 0.0).sum()
+    rtau = None
     for j in range(n):
-        cand = np.where(used, -np.inf, np.maximum(d, 0.0))
-        p = int(np.argmax(cand))  # first maximum: the lower row
+        if rtau is None and tau > 0 and np.maximum(d[~used], 0.0).sum() <= tau * tr0:
+            rtau = j
+        # the device's pivot key: the remaining diagonal's bits with the low 12 bits
+        # replaced by 4095 - row, so values within 2^-40 relative tie and the lower
+        # row wins
+        key = (np.where(d > 0.0, d, 0.0).view(np.uint64) & np.uint64(~0xfff & 0xFFFFFFFFFFFFFFFF)) \
+            | (np.uint64(4095) - np.arange(n, dtype=np.uint64))
+        key[used] = 0
+        p = int(np.argmax(key))
         if used[p]:
             break
         if thr is None:
@@ -47,6 +59,8 @@ def pchol(gam):
         used[p] = True
         order.append(p)
         d[rows] -= np.abs(W[rows, j]) ** 2
+    if want_rank:
+        return W, order, len(order) if rtau is None else min(rtau, len(order))
     return W, order
 
 
@@ -110,3 +124,18 @@ def test_a_slip_in_the_factor_is_caught(case):
     for j in range(n):  # no pivoting and no Schur update of the column: wrong
         W[j:, j] = gl[j:, j] / np.sqrt(gl[j, j].real)
     assert np.abs(W @ W.conj().T - gl).max() > 1e-6 * np.abs(gl).max()
+
+
+def test_truncation_rank_leaves_out_at_most_tau(case):
+    """The truncation rank r for tau: the first r columns leave out a positive
+    semidefinite part of trace <= tau tr(Gamma), and r - 1 columns would not."""
+    for gam in case[1:3]:
+        for tau in (1e-2, 1e-4, 1e-6):
+            W, order, r = pchol(gam, tau, want_rank=True)
+            rest = gam - W[:, :r] @ W[:, :r].conj().T
+            tr = np.trace(gam).real
+            assert np.trace(rest).real <= tau * tr * (1 + 1e-9) + 1e-13 * tr
+            assert np.linalg.eigvalsh(0.5 * (rest + rest.conj().T)).min() >= -1e-12 * tr
+            if r > 0:
+                prev = gam - W[:, :r - 1] @ W[:, :r - 1].conj().T
+                assert np.trace(prev).real > tau * tr
