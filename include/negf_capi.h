@@ -282,7 +282,7 @@ int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, vo
  * elimination runs in place) and must not alias invG; the work is queued on
  * `stream` (a hipStream_t, NULL for the default stream) and the call returns
  * without synchronising.  The Gauss-Jordan workspace (per matrix a 128 x n panel
- * and 2n+1 ints) lives in a static 24 MiB device array, so batch * n is limited to
+ * and 2n+97 ints) lives in a static 24 MiB device array, so batch * n is limited to
  * about 12000 for complex128 and 23000 for complex64 (larger requests fail with
  * NEGF_ERR_MEMORY); the call makes no hipMalloc/hipFree.  Every call on one GPU
  * shares that array: calls that overlap on one GPU (other streams, other host

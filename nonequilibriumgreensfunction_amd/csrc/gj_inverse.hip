@@ -7,22 +7,37 @@
 // which maps onto the MFMA GEMM kernel.
 //
 // Pivoting is implicit: rows are never swapped.  Column k is eliminated with
-// pivot row p_k (largest |Re|+|Im| among rows not yet used, as izamax), and
+// pivot row p_k (largest |Re|+|Im| among the candidate rows not yet used, as
+// izamax; see the blocking below), and
 // pivstep[p_k] = k, pivrow[k] = p_k are recorded.  In place, column k then holds
 // column p_k of the accumulated row transform S, and after all n steps
 //     inv(A)[a][b] = S[pivrow[a]][pivstep[b]],
 // which the final unpermute kernel applies while copying to the output.
 //
-// Blocking.  Columns are processed in outer blocks of kNBO and inner panels of
-// kNBI.  A panel kernel (one 512-thread workgroup per matrix, panel rows held in
-// registers) factors kNBI columns and copies its pivot rows' values in the other
-// columns of the outer block into Prow; a rank-kNBI update (vector kernel) then
-// brings the rest of the outer block up to date.  After the whole outer block, a gather kernel copies
-// the block's pivot rows in every column outside it into Prow, and a rank-kNBO
-// update (the GEMM kernel's Gauss-Jordan epilogue) is applied to those columns:
+// Blocking.  Columns are processed in outer blocks of kNBO and panels of kNBI.
+// The pivots of a panel are chosen by tournament (two-level) partial pivoting,
+// so that no step needs one workgroup to see the whole column:
+//   candidates -- every 256-row chunk runs partial pivoting (LU steps) on its own
+//                 unused rows of the panel and proposes up to kNBI rows (one
+//                 workgroup per chunk and matrix);
+//   final      -- one small workgroup per matrix runs partial pivoting on the
+//                 proposed rows' current values, which fixes the panel's kNBI
+//                 pivot rows p_m in step order, runs the panel's column-by-column
+//                 Gauss-Jordan steps on those rows alone (each step's pivot-row
+//                 values are what every other row's update needs) and copies the
+//                 pivot rows' values in the outer block's other columns into Prow;
+//   update     -- every row takes the panel's steps with those values (the same
+//                 operations, in the same order, as column-by-column elimination
+//                 of the whole panel) and then updates the outer block's other
+//                 columns, A[i][j] <- [i not a pivot of the panel] A[i][j] + sum_m S[i][m] Prow[m][j].
+// With the pivots chosen among all rows (n <= 256: one chunk) this is exactly
+// column-by-column partial-pivoting elimination, bit for bit.  After the
+// whole outer block, a gather kernel copies the block's pivot rows in every column
+// outside it into Prow, and a rank-kNBO update (the GEMM kernel's Gauss-Jordan
+// epilogue) is applied to those columns:
 //     A[i][j] <- [p(i) not in block] A[i][j] + sum_m S[i][m] Prow[m][j].
-// Two launches per panel: under concurrent streams each dependent launch costs
-// tens of microseconds of dispatch latency, more than the panel's own work.
+// Every kernel of a panel step is many small workgroups, so under concurrent
+// streams they find room beside other work on the compute units.
 #include "negf_device.h"
 
 namespace negf {
@@ -31,6 +46,7 @@ template <typename T>
 hipError_t gemm_gj_update(hipStream_t s, const GemmBatch& g, int batch);
 
 constexpr int kNBI = 8;    // panel width
+constexpr int kCandSlots = 2;  // candidates per lane of the pivot-selection wave
 constexpr int kNBO = kGjNBO;  // outer block width (two GEMM tiles; the update skips its columns)
 
 __global__ void k_gj_init(int* pivstep, int* pivrow, int* info, long long sInt, int n,
@@ -71,169 +87,290 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long x) {
   return x;
 }
 
-// Two barriers per pivot column: (1) after each wave has published its best key,
-// every thread reduces the NT/64 keys itself -- no broadcast barrier -- and the
-// thread holding the pivot row publishes the row scaled by 1/pivot; (2) after
-// that, every thread eliminates.  The LDS scratch alternates between two buffers
-// by column parity, so column jj+1 never overwrites what a slower wave still
-// reads for column jj.  512 threads (two waves per SIMD) hold R rows each.
-template <typename T, int R, int NT>
-__global__ __launch_bounds__(NT) void k_gj_panel(cx<T>* S, long long sS, int n, int ld,
-                                                             int c0, int nb, int k0, int kb, GjWork w,
-                                                             const int* active) {
+// Loads whose address is always valid (a clamped row or column) and whose value is
+// selected afterwards: unconditional loads are issued back to back.
+template <typename V>
+__device__ inline V load_sel(const V* p, bool keep) {
+  const V t = *p;
+  V z;
+  z.x = keep ? t.x : 0;
+  z.y = keep ? t.y : 0;
+  return z;
+}
+
+// Pivot key of an element of row i (see above).
+template <typename V>
+__device__ inline unsigned long long pivot_key(V x, int i) {
+  double m = (double)(fabs(x.x) + fabs(x.y));
+  if (m != m) m = (double)INFINITY;
+  return ((unsigned long long)__double_as_longlong(m) & ~0xfffull) | (unsigned long long)(4095 - i);
+}
+
+// Candidate pivots of one 256-row chunk (one row per thread): partial pivoting on
+// the chunk's unused rows of the panel [c0, c0 + nb), the later columns updated as
+// the Gauss-Jordan steps update them (v[j] -= v[m] * pivot row[j] / pivot).  Two barriers
+// per column; the LDS scratch alternates by column parity.  Writes the chosen
+// rows (global indices, -1 once the chunk has no unused row left) to the
+// candidate slots of the chunk.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gj_cand(const cx<T>* S, long long sS, int n, int ld, int c0,
+                                                 int nb, GjWork w, const int* active) {
   typedef cx<T> V;
-  constexpr int NW = NT / 64;
-  const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
-  V* A = S + (long long)b * sS;
-  int* pivstep = w.pivstep + (long long)b * w.sInt;
-  int* pivrow = w.pivrow + (long long)b * w.sInt;
-  int* info = w.info + (long long)b * w.sInt;
-
-  __shared__ unsigned long long red_k[2][NW];
-  __shared__ V psc[2][kNBI + 1];  // pivot row scaled by 1/pivot; [kNBI] = 1/pivot
-
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  const V* A = S + (long long)b * sS;
+  const int* pivstep = w.pivstep + (long long)b * w.sInt;
+  int* cand = w.info + (long long)b * w.sInt + 1 + blockIdx.x * kNBI;
   const int tid = threadIdx.x, lane = tid & 63;
-  V v[R][kNBI];
-  bool used[R];
-  int pp[kNBI];  // pivot row of each column (the same in every thread), -1 if none
-#pragma unroll
-  for (int j = 0; j < kNBI; ++j) pp[j] = -1;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int i = tid + r * NT;
-    used[r] = true;
-#pragma unroll
-    for (int j = 0; j < kNBI; ++j) v[r][j] = mk<T>(0, 0);
-    if (i < n) {
-      used[r] = pivstep[i] < n;
-#pragma unroll
-      for (int j = 0; j < kNBI; ++j)
-        if (j < nb) v[r][j] = A[(long long)(c0 + j) * ld + i];
-    }
-  }
-
-#pragma unroll
+  const int i = blockIdx.x * 256 + tid;
+  __shared__ unsigned long long red[2][4];
+  __shared__ V prs[2][kNBI];
+  V v[kNBI];
+  bool avail = i < n && pivstep[i] >= n;
+  if (tid >= nb && tid < kNBI) cand[tid] = -1;  // slots past a short last panel
+#pragma clang loop unroll(full)
+  for (int j = 0; j < kNBI; ++j)
+    v[j] = load_sel(A + (long long)(c0 + (j < nb ? j : 0)) * ld + (i < n ? i : 0), i < n && j < nb);
+  // predicated rather than broken off: full unrolling keeps v[] in registers
+  bool more = true;  // the chunk still has unused rows (the same in every thread)
+#pragma clang loop unroll(full)
   for (int jj = 0; jj < kNBI; ++jj) {
-    if (jj < nb) {
+    if (jj < nb && more) {
       const int par = jj & 1;
-      unsigned long long best = 0;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int i = tid + r * NT;
-        if (i < n && !used[r]) {
-          double m = (double)(fabs(v[r][jj].x) + fabs(v[r][jj].y));
-          if (m != m) m = (double)INFINITY;
-          best = umax64(best, ((unsigned long long)__double_as_longlong(m) & ~0xfffull) |
-                                  (unsigned long long)(4095 - i));
+      unsigned long long k = 0;
+      if (avail) k = pivot_key(v[jj], i);
+      k = wave_max_u64(k);
+      if (lane == 0) red[par][tid >> 6] = k;
+      __syncthreads();
+      const unsigned long long g = umax64(umax64(red[par][0], red[par][1]), umax64(red[par][2], red[par][3]));
+      if (g == 0) {
+        if (tid < kNBI && tid >= jj) cand[tid] = -1;
+        more = false;
+      } else {
+        const int q = 4095 - (int)(g & 0xfffull);
+        if (tid == 0) cand[jj] = q;
+        if (i == q) {  // the pivot row scaled by 1 / pivot
+          const V inv = cinv(v[jj]);
+#pragma clang loop unroll(full)
+          for (int j = 0; j < kNBI; ++j) prs[par][j] = cmul(v[j], inv);
+          avail = false;
         }
-      }
-      best = wave_max_u64(best);
-      if (lane == 0) red_k[par][tid >> 6] = best;
-      __syncthreads();  // (1)
-      unsigned long long g = red_k[par][0];
-#pragma unroll
-      for (int q = 1; q < NW; ++q) g = umax64(g, red_k[par][q]);
-      if (g == 0) continue;  // no unused row left (the same in every thread)
-      const int p = 4095 - (int)(g & 0xfffull);
-      pp[jj] = p;
-      if (tid == 0) {
-        if ((g & ~0xfffull) == 0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
-        pivstep[p] = c0 + jj;
-        pivrow[c0 + jj] = p;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (tid + r * NT == p) {
-          const V inv = cinv(v[r][jj]);
-#pragma unroll
-          for (int j = 0; j < kNBI; ++j) psc[par][j] = cmul(v[r][j], inv);
-          psc[par][kNBI] = inv;
+        __syncthreads();
+        if (avail && (g & ~0xfffull) != 0) {  // the later columns, as the Gauss-Jordan step does them
+          const V f = v[jj];
+#pragma clang loop unroll(full)
+          for (int j = 0; j < kNBI; ++j)
+            if (j > jj) v[j] = csub(v[j], cmul(f, prs[par][j]));
         }
-      __syncthreads();  // (2)
-      const V inv = psc[par][kNBI];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int i = tid + r * NT;
-        if (i >= n) continue;
-        if (i == p) {
-#pragma unroll
-          for (int j = 0; j < kNBI; ++j) v[r][j] = psc[par][j];
-          v[r][jj] = inv;
-          used[r] = true;
-        } else {
-          const V f = v[r][jj];
-#pragma unroll
-          for (int j = 0; j < kNBI; ++j) v[r][j] = csub(v[r][j], cmul(f, psc[par][j]));
-          v[r][jj] = cneg(cmul(f, inv));
-        }
-      }
-    }
-  }
-  // write the factored panel back
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int i = tid + r * NT;
-    if (i < n) {
-#pragma unroll
-      for (int j = 0; j < kNBI; ++j)
-        if (j < nb) A[(long long)(c0 + j) * ld + i] = v[r][j];
-    }
-  }
-  // The pivot rows' values in the other columns of the outer block [k0, k0 + kb)
-  // -> Prow rows c0 - k0 + m, for the rank-nb update of the block that follows.
-  // Those columns are up to date (the previous update has completed) and this
-  // kernel does not write them.
-  V* prow = (V*)w.prow + (long long)b * w.sProw;
-#pragma unroll
-  for (int m = 0; m < kNBI; ++m) {
-    const int p = pp[m];
-    if (m < nb && p >= 0) {
-      for (int jc = tid; jc < kb; jc += NT) {
-        const int j = k0 + jc;
-        if (j < c0 || j >= c0 + nb) prow[(long long)(c0 - k0 + m) * n + j] = A[(long long)j * ld + p];
       }
     }
   }
 }
 
-// Rank-nb update of the outer block after panel [c0, c0 + nb):
-//     A[i][j] <- [p(i) not in panel] A[i][j] + sum_m A[i][c0 + m] Prow[c0 - k0 + m][j]
-// for every column j of [k0, k0 + kb) outside the panel.  It is memory-bound
-// (nb = 8), so it runs on the vector units: one thread per row, 16 columns per
-// workgroup, the panel row in registers and the 8 x 16 pivot-row block in LDS.
+// The panel's pivots (one wave per matrix, kCandSlots candidates per lane, at most
+// 12 x kNBI): partial pivoting, as above, on the candidates' current values, ties
+// to the lower row; records pivstep/pivrow and INFO.  Then the column-by-column
+// Gauss-Jordan steps of the panel restricted to its pivot rows: step m takes pivot
+// row p_m as the earlier steps left it, v, and records
+//     T[m][j] = v[j] / v[m] (j != m),  T[m][m] = 1 / v[m]
+// -- the pivot row's values after its own step, the same for every row's update --
+// in the panel's own columns of Prow, Prow[c0 - k0 + m][c0 + j] = T[m][j].  The
+// pivot rows' values in the outer block's other columns go to Prow[c0 - k0 + m][j].
+// (A one-wave workgroup: its barriers only order the LDS traffic.)
 template <typename T>
-__global__ __launch_bounds__(256) void k_gj_update_block(cx<T>* S, long long sS, int n, int ld, int c0,
-                                                         int nb, int k0, int kb, GjWork w,
-                                                         const int* active) {
+__global__ __launch_bounds__(64) void k_gj_pivots(const cx<T>* S, long long sS, int n, int ld, int c0,
+                                                  int nb, int k0, int kb, int nchunks, GjWork w,
+                                                  const int* active) {
   typedef cx<T> V;
-  constexpr int CW = 16;  // columns per workgroup
-  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
+  const V* A = S + (long long)b * sS;
+  int* pivstep = w.pivstep + (long long)b * w.sInt;
+  int* pivrow = w.pivrow + (long long)b * w.sInt;
+  int* info = w.info + (long long)b * w.sInt;
+  const int* cand = info + 1;
+  V* prow = (V*)w.prow + (long long)b * w.sProw;
+  const int tid = threadIdx.x;
+  __shared__ V prs[2][kNBI];
+  __shared__ int piv[kNBI];
+  __shared__ V Tm[kNBI][kNBI];
+  const int ncand = nchunks * kNBI;
+  int rr[kCandSlots];
+  bool av[kCandSlots];
+  V v[kCandSlots][kNBI];
+#pragma clang loop unroll(full)
+  for (int q = 0; q < kCandSlots; ++q) {
+    const int idx = tid + 64 * q;
+    rr[q] = idx < ncand ? cand[idx] : -1;
+    av[q] = rr[q] >= 0;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < kNBI; ++j)
+      v[q][j] = load_sel(A + (long long)(c0 + (j < nb ? j : 0)) * ld + (av[q] ? rr[q] : 0), av[q] && j < nb);
+  }
+  // compile-time trip counts keep v[][] and u[] in registers
+#pragma clang loop unroll(full)
+  for (int jj = 0; jj < kNBI; ++jj) {
+    if (jj < nb) {
+      const int par = jj & 1;
+      unsigned long long k = 0;
+#pragma clang loop unroll(full)
+      for (int q = 0; q < kCandSlots; ++q)
+        if (av[q]) k = umax64(k, pivot_key(v[q][jj], rr[q]));
+      const unsigned long long g = wave_max_u64(k);
+      // g == 0 cannot happen: every chunk proposes min(nb, its unused rows) rows, the
+      // panel has nb <= unused rows, and every candidate's key is positive (row bits);
+      // row 0 keeps a broken call in bounds
+      const int p = g ? 4095 - (int)(g & 0xfffull) : 0;
+      if (tid == 0) {
+        piv[jj] = p;
+        if ((g & ~0xfffull) == 0 && *info == 0) *info = c0 + jj + 1;  // LAPACK-style INFO
+      }
+#pragma clang loop unroll(full)
+      for (int q = 0; q < kCandSlots; ++q)
+        if (av[q] && rr[q] == p) {
+          const V inv = cinv(v[q][jj]);
+#pragma clang loop unroll(full)
+          for (int j = 0; j < kNBI; ++j) prs[par][j] = cmul(v[q][j], inv);
+          av[q] = false;
+        }
+      __syncthreads();
+      if ((g & ~0xfffull) != 0) {
+#pragma clang loop unroll(full)
+        for (int q = 0; q < kCandSlots; ++q)
+          if (av[q]) {
+            const V f = v[q][jj];
+#pragma clang loop unroll(full)
+            for (int j = 0; j < kNBI; ++j)
+              if (j > jj) v[q][j] = csub(v[q][j], cmul(f, prs[par][j]));
+          }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < nb) {
+    pivstep[piv[tid]] = c0 + tid;
+    pivrow[c0 + tid] = piv[tid];
+  }
+  // the pivot rows' values in the outer block's other columns (a column per lane
+  // and pass; all loads of a column before its stores)
+  for (int jc = tid; jc < kb; jc += 64) {
+    const int j = k0 + jc;
+    if (j >= c0 && j < c0 + nb) continue;
+    V t[kNBI];
+#pragma clang loop unroll(full)
+    for (int m = 0; m < kNBI; ++m) t[m] = load_sel(A + (long long)j * ld + piv[m < nb ? m : 0], m < nb);
+#pragma clang loop unroll(full)
+    for (int m = 0; m < kNBI; ++m)
+      if (m < nb) prow[(long long)(c0 - k0 + m) * n + j] = t[m];
+  }
+  // Gauss-Jordan steps on the pivot rows: lane l < nb follows row p_l
+  V u[kNBI];
+#pragma clang loop unroll(full)
+  for (int j = 0; j < kNBI; ++j)
+    u[j] = load_sel(A + (long long)(c0 + (j < nb ? j : 0)) * ld + piv[tid < nb ? tid : 0], tid < nb && j < nb);
+#pragma clang loop unroll(full)
+  for (int m = 0; m < kNBI; ++m) {
+    if (m < nb) {
+      if (tid == m) {
+        const V inv = cinv(u[m]);
+#pragma clang loop unroll(full)
+        for (int j = 0; j < kNBI; ++j) Tm[m][j] = cmul(u[j], inv);
+        Tm[m][m] = inv;
+      }
+      __syncthreads();
+      if (tid > m && tid < nb) {  // later pivot rows take step m like any other row
+        const V f = u[m];
+#pragma clang loop unroll(full)
+        for (int j = 0; j < kNBI; ++j) u[j] = csub(u[j], cmul(f, Tm[m][j]));
+        u[m] = cneg(cmul(f, Tm[m][m]));
+      }
+    }
+  }
+  for (int e = tid; e < nb * nb; e += 64) {
+    const int m = e / nb, j = e % nb;
+    prow[(long long)(c0 - k0 + m) * n + c0 + j] = Tm[m][j];
+  }
+}
+
+// The panel's Gauss-Jordan steps on every row (32 rows x 8 column groups per
+// workgroup), with the pivot rows' step values T from Prow: for m = 0..nb-1, row
+// p_m becomes T[m]; every other row takes f = v[m], v[j] -= f T[m][j], v[m] = -f T[m][m].
+// The result S[i][0..nb) goes in place of the panel, then the outer block's other
+// columns are updated,
+//     A[i][j] <- [i not a pivot of the panel] A[i][j] + sum_m S[i][m] Prow[c0 - k0 + m][j].
+// A workgroup owns whole rows, so it reads the panel's old values before it writes
+// the new ones; the pivot rows' old values come from Prow.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gj_apply(cx<T>* S, long long sS, int n, int ld, int c0, int nb,
+                                                  int k0, int kb, GjWork w, const int* active) {
+  typedef cx<T> V;
+  constexpr int RW = 32, CG = 8;
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
   V* A = S + (long long)b * sS;
   const int* pivstep = w.pivstep + (long long)b * w.sInt;
   const V* prow = (const V*)w.prow + (long long)b * w.sProw;
-  const int jc0 = blockIdx.y * CW;
-  __shared__ V P[kNBI][CW];
-  if (threadIdx.x < kNBI * CW) {
-    const int m = threadIdx.x / CW, u = threadIdx.x % CW, jc = jc0 + u;
-    P[m][u] = (m < nb && jc < kb) ? prow[(long long)(c0 - k0 + m) * n + k0 + jc] : mk<T>(0, 0);
+  __shared__ V Ps[kNBI][kGjNBO];
+  for (int e = threadIdx.x; e < kNBI * kb; e += 256) {
+    const int m = e / kb, jc = e % kb;
+    Ps[m][jc] = m < nb ? prow[(long long)(c0 - k0 + m) * n + k0 + jc] : mk<T>(0, 0);
   }
   __syncthreads();
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  V sv[kNBI];
-#pragma unroll
-  for (int m = 0; m < kNBI; ++m) sv[m] = m < nb ? A[(long long)(c0 + m) * ld + i] : mk<T>(0, 0);
-  const int ps = pivstep[i];
-  const bool keep = !(ps >= c0 && ps < c0 + nb);
-#pragma unroll 4
-  for (int u = 0; u < CW; ++u) {
-    const int jc = jc0 + u, j = k0 + jc;
-    if (jc >= kb) break;
-    if (j >= c0 && j < c0 + nb) continue;
-    V acc = keep ? A[(long long)j * ld + i] : mk<T>(0, 0);
-#pragma unroll
-    for (int m = 0; m < kNBI; ++m) acc = cadd(acc, cmul(sv[m], P[m][u]));
-    A[(long long)j * ld + i] = acc;
+  const int i = blockIdx.x * RW + (threadIdx.x % RW), g = threadIdx.x / RW;
+  const bool live = i < n;
+  const int pc = c0 - k0;  // the panel's first column within the block
+  V v[kNBI];
+  bool keep = true;
+  if (live) {
+    const int ps = pivstep[i];
+    keep = !(ps >= c0 && ps < c0 + nb);
+#pragma clang loop unroll(full)
+    for (int j = 0; j < kNBI; ++j) v[j] = load_sel(A + (long long)(c0 + (j < nb ? j : 0)) * ld + i, j < nb);
+#pragma clang loop unroll(full)
+    for (int m = 0; m < kNBI; ++m) {
+      if (m < nb) {
+        if (ps == c0 + m) {
+#pragma clang loop unroll(full)
+          for (int j = 0; j < kNBI; ++j) v[j] = Ps[m][pc + j];
+        } else {
+          const V f = v[m];
+#pragma clang loop unroll(full)
+          for (int j = 0; j < kNBI; ++j) v[j] = csub(v[j], cmul(f, Ps[m][pc + j]));
+          v[m] = cneg(cmul(f, Ps[m][pc + m]));
+        }
+      }
+    }
+  }
+  __syncthreads();  // every read of the panel's old values is done
+  if (live) {
+    if (g == 0) {
+#pragma clang loop unroll(full)
+      for (int m = 0; m < kNBI; ++m)
+        if (m < nb) A[(long long)(c0 + m) * ld + i] = v[m];
+    }
+    // columns g, g + CG, ... in batches of U: every load of a batch is issued
+    // before its stores
+    constexpr int U = 16;
+    for (int jb = g; jb < kb; jb += CG * U) {
+      V acc[U];
+#pragma clang loop unroll(full)
+      for (int u = 0; u < U; ++u) {
+        const int jc = jb + u * CG, j = k0 + jc;
+        const bool on = jc < kb && (j < c0 || j >= c0 + nb) && keep;
+        acc[u] = load_sel(A + (long long)(jc < kb ? j : k0) * ld + i, on);
+      }
+#pragma clang loop unroll(full)
+      for (int u = 0; u < U; ++u) {
+        const int jc = jb + u * CG;
+        if (jc < kb) {
+#pragma clang loop unroll(full)
+          for (int m = 0; m < kNBI; ++m)
+            if (m < nb) acc[u] = cadd(acc[u], cmul(v[m], Ps[m][jc]));
+        }
+      }
+#pragma clang loop unroll(full)
+      for (int u = 0; u < U; ++u) {
+        const int jc = jb + u * CG, j = k0 + jc;
+        if (jc < kb && (j < c0 || j >= c0 + nb)) A[(long long)j * ld + i] = acc[u];
+      }
+    }
   }
 }
 
@@ -255,7 +392,7 @@ __global__ __launch_bounds__(64) void k_gj_gather(const cx<T>* S, long long sS, 
   const int m0 = blockIdx.y * 8;
   if (j >= n || (j >= k0 && j < k0 + kb)) return;
   V t[8];
-#pragma unroll
+#pragma clang loop unroll(full)
   for (int u = 0; u < 8; ++u) {
     t[u] = mk<T>(0, 0);
     if (m0 + u < kb) {
@@ -263,7 +400,7 @@ __global__ __launch_bounds__(64) void k_gj_gather(const cx<T>* S, long long sS, 
       t[u] = A[(long long)j * ld + (p < n ? p : 0)];
     }
   }
-#pragma unroll
+#pragma clang loop unroll(full)
   for (int u = 0; u < 8; ++u)
     if (m0 + u < kb) prow[(long long)(m0 + u) * n + j] = t[u];
 }
@@ -297,35 +434,24 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
   if (n > 3072) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_gj_init, dim3((n + 255) / 256, batch), dim3(256), 0, s, w.pivstep, w.pivrow,
                      w.info, w.sInt, n, active);
-  // panel rows live in registers: 512 threads x R rows, R = ceil(n / 512) <= 6
-  constexpr int NT = 512;
-  const int R = (n + NT - 1) / NT;
+  const int nchunks = (n + 255) / 256;  // <= kGjCandInts / kNBI at n <= 3072
   for (int k0 = 0; k0 < n; k0 += kNBO) {
     const int kb = (n - k0 < kNBO) ? n - k0 : kNBO;
     for (int c0 = k0; c0 < k0 + kb; c0 += kNBI) {
       const int nb = (k0 + kb - c0 < kNBI) ? k0 + kb - c0 : kNBI;
-#define NEGF_PANEL(RR)                                                                             \
-  hipLaunchKernelGGL((k_gj_panel<T, RR, NT>), dim3(batch), dim3(NT), 0, s, S, sS, n, ld, c0, nb, k0, \
-                     kb, w, active)
-      // sampled timing: the panel's columns read and written, and its pivot rows'
-      // values in the outer block copied out (nb x kb elements read and written)
+      // sampled timing of the panel step: the outer block read and written once,
+      // and the panel's columns read once more for the candidates
       hipEvent_t ev0, ev1;
-      const double bytes = 2.0 * sizeof(cx<T>) * nb * ((double)n + kb) * batch;
+      const double bytes = sizeof(cx<T>) * (double)n * (2.0 * kb + nb) * batch;
       const bool timed = timer_slot(bytes, kTimerPanel64 + (sizeof(T) == sizeof(float) ? 1 : 0), &ev0, &ev1);
       if (timed && hipEventRecord(ev0, s) != hipSuccess) return hipGetLastError();
-      switch (R) {
-        case 1: NEGF_PANEL(1); break;
-        case 2: NEGF_PANEL(2); break;
-        case 3: NEGF_PANEL(3); break;
-        case 4: NEGF_PANEL(4); break;
-        case 5: NEGF_PANEL(5); break;
-        default: NEGF_PANEL(6); break;
-      }
-#undef NEGF_PANEL
+      hipLaunchKernelGGL((k_gj_cand<T>), dim3(nchunks, batch), dim3(256), 0, s, S, sS, n, ld, c0, nb, w,
+                         active);
+      hipLaunchKernelGGL((k_gj_pivots<T>), dim3(batch), dim3(64), 0, s, S, sS, n, ld, c0, nb, k0, kb,
+                         nchunks, w, active);
+      hipLaunchKernelGGL((k_gj_apply<T>), dim3((n + 31) / 32, batch), dim3(256), 0, s, S, sS, n, ld, c0,
+                         nb, k0, kb, w, active);
       if (timed && hipEventRecord(ev1, s) != hipSuccess) return hipGetLastError();
-      if (kb > nb)  // bring the rest of the outer block up to date (rank nb)
-        hipLaunchKernelGGL((k_gj_update_block<T>), dim3((n + 255) / 256, (kb + 15) / 16, batch), dim3(256),
-                           0, s, S, sS, n, ld, c0, nb, k0, kb, w, active);
     }
     if (n > kb) {  // rank-kb update of every column outside the outer block
       hipLaunchKernelGGL((k_gj_gather<T>), dim3((n + 63) / 64, (kb + 7) / 8, batch), dim3(64), 0, s, S,

@@ -69,6 +69,12 @@ struct GemmBatch {
   int skip_lo, skip_hi;
   int m3;  // 1: 3-product complex MFMA form (normwise-accurate, 3/4 of the work);
            // 0: 4-product form (each of Re, Im accurate relative to its own terms)
+  // Optional per-matrix column count (plain products with beta 0 or 1): matrix b's
+  // B is zero past column ncol[b * sNcol], so whole tiles past it are not
+  // computed -- C is cleared there (beta 0) or left as it is (beta 1).
+  const int* ncol = nullptr;
+  long long sNcol = 0;
+  float work = 1.0f;  // fraction of the m x n x k products really computed (timing only)
 };
 
 // Pivoted Cholesky factors Gam = W W' of both leads' broadening for a batch of
@@ -121,6 +127,10 @@ template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
 constexpr int kGjNBO = 128;  // Gauss-Jordan outer block width (columns per trailing update)
+// Gauss-Jordan int workspace per matrix: pivstep[n], pivrow[n], info, then the
+// panel's pivot candidates (8 per 256-row chunk, n <= 3072).
+constexpr int kGjCandInts = 96;
+inline long long gj_ints(int n) { return 2LL * n + 1 + kGjCandInts; }
 
 // Opt-in timing of the MFMA GEMM launches (the dominant kernels) and of the
 // Gauss-Jordan panel launches: when enabled, one launch in `every` of each kind is

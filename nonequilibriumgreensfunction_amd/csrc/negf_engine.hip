@@ -181,7 +181,7 @@ struct Layout {
     c_W4 = o; o += nn;
     c_prow = o; o += (long long)kGjNBO * n;
     Cs = (o + 31) & ~31LL;
-    Is = ((2LL * n + 1) + 63) & ~63LL;
+    Is = (gj_ints(n) + 63) & ~63LL;
     site_q = 2LL * L * nsites;
     site_per_e = kSiteQ * site_q;
   }
@@ -1174,6 +1174,10 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   const int* kcL = (const int*)(pa.st[0] + rk_off + sizeof(int));  // per-energy truncation ranks
   const int* kcR = (const int*)(pa.st[1] + rk_off + sizeof(int));
   const long long skc = pa.sSt / (long long)sizeof(int);
+  // the first-column chain's products skip the tiles past each energy's own rank
+  double wl = 0.0;
+  for (int b = 0; b < nb; ++b) wl += std::min(rk[2 * b + 1], cL);
+  const float fracL = (float)(wl / ((double)nb * cL));
   auto rgf = [&]() -> int {
 
     GjWork w = gjw(Q(y.o_prow), Es, ln.Ie, y.Is, n);
@@ -1216,10 +1220,16 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       }
       return 0;
     };
-    // C = A B on the first `cols` columns of B and C
-    auto mulc = [&](const zd* A, const zd* B, zd* C, int cols, zd beta) -> int {
-      TRY(gemm_batched<double>(s, OP_N, OP_N, shape(gbr(A, Es, n, B, Es, n, C, Es, n, n), n, cols), nb,
-                               kOne, beta));
+    // C = A B on the first `cols` columns of B and C (with kc: on each energy's own
+    // count of them, B being zero past it)
+    auto mulc = [&](const zd* A, const zd* B, zd* C, int cols, zd beta, const int* kc = nullptr) -> int {
+      GemmBatch g = shape(gbr(A, Es, n, B, Es, n, C, Es, n, n), n, cols);
+      if (kc) {
+        g.ncol = kc;
+        g.sNcol = skc;
+        g.work = fracL;
+      }
+      TRY(gemm_batched<double>(s, OP_N, OP_N, g, nb, kOne, beta));
       return 0;
     };
     // The column chains run on V = G W (W the lead's factor, Gamma = W W'): both
@@ -1256,7 +1266,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     // V = G W_lead (first cols columns) into dst; on W_L's truncation rank, the
     // columns past each energy's own rank are cleared
     auto times_w = [&](const zd* G, int lead, zd* dst, int cols) -> int {
-      if (int rc = mulc(G, Q(lead ? y.o_WR : y.o_WL), dst, cols, kZero)) return rc;
+      if (int rc = mulc(G, Q(lead ? y.o_WR : y.o_WL), dst, cols, kZero, lead ? nullptr : kcL)) return rc;
       if (lead == 0) TRY(launch_trunc_cols<double>(s, dst, Es, n, cols, kcL, skc, nb));
       return 0;
     };
@@ -1273,14 +1283,20 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       const long long Es32 = 2 * Es;  // complex64 stride between energies
       auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
       const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
-      // C (m x ncols) = A B, A n columns
-      auto gbf = [&](const zf* A, const zf* B, zf* C, int m, int ncols) {
+      // C (m x ncols) = A B, A n columns; with kc, B is zero past each energy's own
+      // column count
+      auto gbf = [&](const zf* A, const zf* B, zf* C, int m, int ncols, const int* kc = nullptr) {
         GemmBatch g = {};
         g.A = A; g.sA = Es32; g.lda = n;
         g.B = B; g.sB = Es32; g.ldb = n;
         g.C = C; g.sC = Es32; g.ldc = n;
         g.m = m; g.n = ncols; g.k = n;
         g.m3 = 1;
+        if (kc) {
+          g.ncol = kc;
+          g.sNcol = skc;
+          g.work = fracL;
+        }
         return g;
       };
       // complex64 factors: W_L on its truncation rank (cleared past each energy's
@@ -1383,11 +1399,13 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         if ((r = invert_block(x))) return r;
         if (x == 0) {
           TRY(to_c64(s, F(y.f_g), Es32, slot(0), Es, nn, nb));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0), n, cL), nb, kOnef, kZerof));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_WL), zs(0), n, cL, kcL), nb, kOnef,
+                                  kZerof));
         } else {
           TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
           TRY(launch_bd_mul<float>(s, no, 1, F(y.f_Y), Es32, zs(x - 1), Es32, c->X32, n, cL, nb, nullptr));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x), n, cL), nb, kOnef, kZerof));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x), n, cL, kcL), nb, kOnef,
+                                  kZerof));
         }
       }
       // backward: G_xL W_R in FP64; G_x1 W_L = z_x + gL_x X G_{x+1,1} W_L in complex64
@@ -1405,7 +1423,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
           if ((r = mulc(slot(x), Q(y.o_Y), colB, cR, kZero))) return r;
           TRY(to_c64(s, F(y.f_g), Es32, slot(x), Es, nn, nb));
           TRY(launch_bd_mul<float>(s, no, 0, F(y.f_Y), Es32, zs(x + 1), Es32, c->X32, n, cL, nb, nullptr));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x), n, cL), nb, kOnef, kOnef));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(F(y.f_g), F(y.f_Y), zs(x), n, cL, kcL), nb, kOnef,
+                                  kOnef));
         }
         const zd* GxL = (x < L - 1) ? colB : colA;
         // T12 = |W_L' G_1L W_R|^2 in FP64, whole factors
@@ -1436,7 +1455,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
           if ((r = times_w(slot(0), 0, zs(0), cL))) return r;  // z_1 W_L = gL_1 W_L
         } else {
           if ((r = xmul(1, Q(y.o_Y), zs(x - 1), cL))) return r;
-          if ((r = mulc(slot(x), Q(y.o_Y), zs(x), cL, kZero))) return r;
+          if ((r = mulc(slot(x), Q(y.o_Y), zs(x), cL, kZero, kcL))) return r;
         }
       }
       // Backward: the last column G_xL = gL_x X G_{x+1,L}, G_LL = gL_L (:53-63), and the
@@ -1454,7 +1473,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
           if ((r = xmul(0, Q(y.o_Y), colA, cR))) return r;
           if ((r = mulc(slot(x), Q(y.o_Y), colB, cR, kZero))) return r;
           if ((r = xmul(0, Q(y.o_P), zs(x + 1), cL))) return r;
-          if ((r = mulc(slot(x), Q(y.o_P), zs(x), cL, kOne))) return r;
+          if ((r = mulc(slot(x), Q(y.o_P), zs(x), cL, kOne, kcL))) return r;
         }
         const zd* GxL = (x < L - 1) ? colB : colA;
         if ((r = siteblocks(1, x, GxL))) return r;
@@ -1506,7 +1525,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     if (L == 1 && (r = trace_term(1, colA, 2, cR, cL))) return r;   // Modec2, left-lead term
     for (int x = 1; x < L; ++x) {
       if ((r = xmul(1, Q(y.o_Y), colA, cL))) return r;
-      if ((r = mulc(slot(x), Q(y.o_Y), colB, cL, kZero))) return r;
+      if ((r = mulc(slot(x), Q(y.o_Y), colB, cL, kZero, kcL))) return r;
       if ((r = siteblocks(0, x, colB))) return r;
       if ((r = bonds(0, x - 1, colA, colB))) return r;
       if (x == L - 1 && (r = trace_term(1, colB, 2, cR, cL))) return r;
@@ -1839,9 +1858,9 @@ extern "C" int negf_invert_matrix(int n, const double* G, double* invG, int* inf
   HIPCHK(hipMalloc(&dA, sizeof(zd) * nn));
   HIPCHK(hipMalloc(&dO, sizeof(zd) * nn));
   HIPCHK(hipMalloc(&prow, sizeof(zd) * kGjNBO * n));
-  HIPCHK(hipMalloc(&ints, sizeof(int) * (2 * n + 1)));
+  HIPCHK(hipMalloc(&ints, sizeof(int) * gj_ints(n)));
   HIPCHK(hipMemcpy(dA, G, sizeof(zd) * nn, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(ints, 0, sizeof(int) * (2 * n + 1)));
+  HIPCHK(hipMemset(ints, 0, sizeof(int) * gj_ints(n)));
   GjWork w = gjw(prow, 0, ints, 0, n);
   HIPCHK(gj_invert<double>(nullptr, dA, 0, n, dO, 0, n, n, 1, w, nullptr));
   int inf = 0;
@@ -1916,7 +1935,7 @@ static int invert_batch_host(int m3, int n, int batch, const void* G, void* invG
   typedef cx<T> V;
   const long long nn = (long long)n * n;
   const size_t bytes = sizeof(V) * (size_t)nn * batch;
-  const long long sInt = 2LL * n + 1;
+  const long long sInt = gj_ints(n);
   V *dA = nullptr, *dO = nullptr, *prow = nullptr;
   int* ints = nullptr;
   HIPCHK(hipMalloc(&dA, bytes));
@@ -1946,7 +1965,7 @@ static int invert_batch_host(int m3, int n, int batch, const void* G, void* invG
 // ---------------------------------------------------------------------------
 // Allocation-free batched inverse (negf_invert_batch_noalloc).  Its Gauss-Jordan
 // workspace -- per matrix the kGjNBO x n pivot-row panel and the pivot bookkeeping
-// (2n + 1 ints) -- lives in `scratch`, a file-scope device array of kScratchBytes
+// (gj_ints(n) ints) -- lives in `scratch`, a file-scope device array of kScratchBytes
 // that every GPU the library runs on holds until the process ends.  In host code
 // the array's name is not a device address: its address on each device is taken
 // once, with that device current, by hipGetSymbolAddress and kept per device; only
@@ -1972,7 +1991,7 @@ static char* scratch_on(int dev) {
 static size_t noalloc_bytes(int complex64, int n, int batch) {
   const size_t el = complex64 ? sizeof(cx<float>) : sizeof(cx<double>);
   const size_t prow = ((size_t)kGjNBO * n * el + 255) & ~(size_t)255;
-  const size_t ints = (sizeof(int) * (2 * (size_t)n + 1) + 255) & ~(size_t)255;
+  const size_t ints = (sizeof(int) * (size_t)gj_ints(n) + 255) & ~(size_t)255;
   return (prow + ints) * (size_t)batch;
 }
 
@@ -1982,7 +2001,7 @@ static int invert_noalloc(char* base, int m3, int n, int batch, void* G, void* i
   typedef cx<T> V;
   const long long nn = (long long)n * n;
   const size_t prow_b = ((size_t)kGjNBO * n * sizeof(V) + 255) & ~(size_t)255;
-  const size_t ints_b = (sizeof(int) * (2 * (size_t)n + 1) + 255) & ~(size_t)255;
+  const size_t ints_b = (sizeof(int) * (size_t)gj_ints(n) + 255) & ~(size_t)255;
   GjWork w;
   w.prow = base;                                     // batch panels, then batch int blocks
   w.sProw = (long long)(prow_b / sizeof(V));
@@ -2256,9 +2275,9 @@ extern "C" int negf_retarded(negf_ctx* c, const double* InvGRd, double* GRd_h, d
   HIPCHK(db.alloc(&M, blk));
   HIPCHK(db.alloc(&T1, blk * 2));
   HIPCHK(db.alloc(&prow, sizeof(zd) * kGjNBO * n));
-  HIPCHK(db.alloc(&ints, sizeof(int) * (2 * n + 1)));
+  HIPCHK(db.alloc(&ints, sizeof(int) * gj_ints(n)));
   HIPCHK(hipMemcpyAsync(A, InvGRd, blk * L, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemsetAsync(ints, 0, sizeof(int) * (2 * n + 1), s));
+  HIPCHK(hipMemsetAsync(ints, 0, sizeof(int) * gj_ints(n), s));
   GjWork w = gjw(prow, 0, ints, 0, n);
   const bool bd = c->blockdiag;
   auto blkp = [&](zd* base, int x) { return base + (size_t)x * nn; };

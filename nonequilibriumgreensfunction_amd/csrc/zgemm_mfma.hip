@@ -107,6 +107,18 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(GemmBatch g, int tiles_m, 
   const int n0 = (tile / tiles_m) * BN;
   if (m0 >= g.m || n0 >= g.n) return;
   if (EPI == EPI_GJ && n0 >= g.skip_lo && n0 + BN <= g.skip_hi) return;
+  if (EPI != EPI_GJ && g.ncol && n0 >= g.ncol[(long long)b * g.sNcol]) {
+    // the tile lies past this matrix's columns, where B is zero
+    const bool b0 = beta.x == (T)0 && beta.y == (T)0, b1 = beta.x == (T)1 && beta.y == (T)0;
+    if (b0) {
+      cx<T>* C0 = (cx<T>*)g.C + (long long)b * g.sC;
+      for (int e = threadIdx.x; e < BM * BN; e += kThreads) {
+        const int row = m0 + e % BM, col = n0 + e / BM;
+        if (row < g.m && col < g.n) C0[(long long)col * g.ldc + row] = mk<T>(0, 0);
+      }
+    }
+    if (b0 || b1) return;
+  }
 
   const cx<T>* A = (const cx<T>*)g.A + (long long)b * g.sA;
   const cx<T>* B = (const cx<T>*)g.B + (long long)b * g.sB;
@@ -390,7 +402,7 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
   const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
-  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch, kind, &ev0, &ev1);
+  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch * g.work, kind, &ev0, &ev1);
   if (timed) {
     const hipError_t er = hipEventRecord(ev0, s);
     if (er != hipSuccess) return er;
