@@ -106,7 +106,9 @@ typedef struct {
  * column chains, site and bond blocks, traces) into a HIP graph once per lane,
  * context, Hamiltonian upload and batch size, and replay it for later batches of
  * the same shape (plain and mixed forms; the refined form decides per block on
- * the host).  Results are bitwise those of the launches issued one by one. */
+ * the host).  Results are bitwise those of the launches issued one by one.
+ * Batches of more than about 20000 launches (e.g. blocks of 2304 over 200 slices)
+ * run without a graph. */
 #define NEGF_FLAG_GRAPH 16
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every

@@ -1535,7 +1535,16 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     return 0;
   };
   int rr;
-  const bool graph = (c->cfg.flags & NEGF_FLAG_GRAPH) && c->cfg.precision != NEGF_PREC_REFINED;
+  // Graphs are captured only up to kMaxGraphLaunches launches (estimated below: per
+  // slice the block inverse -- three launches per 8-column panel, two per 128-column
+  // block -- and ~16 more); a capture of ~2e5 launches (block 2304, 200 slices)
+  // ended the host process with a segmentation fault inside the capture, so longer
+  // batches run eagerly.
+  constexpr long long kMaxGraphLaunches = 20000;
+  const long long est_launches =
+      (long long)L * (3LL * ((n + 7) / 8) + 2LL * ((n + 127) / 128) + 16);
+  const bool graph = (c->cfg.flags & NEGF_FLAG_GRAPH) && c->cfg.precision != NEGF_PREC_REFINED &&
+                     est_launches <= kMaxGraphLaunches;
   if (graph) {
     const GraphKey key = {c->uid, c->generation, nb, c->want_currents, c->want_pairs, ln.E,
                           {cLf, cL, cR, cRt}};
