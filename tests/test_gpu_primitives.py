@@ -29,8 +29,8 @@ def test_zgemm_matches_numpy(rng, n, ops):
     assert np.all(np.abs(got - ref) <= bound + 1e-300)
 
 
-# sizes cover every panel shape of the inverse (512 threads x ceil(n/512) rows, 1..6)
-# at both ends, and partial outer blocks
+# sizes cover one to eleven 256-row chunks of the inverse's tournament pivot
+# selection, with full and partial last chunks, and partial outer blocks
 @pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 63, 64, 65, 100, 144, 256, 300, 512, 513, 1024, 1025,
                                1600, 2048, 2100, 2304, 2561, 2800])
 def test_invert_matches_lapack(rng, n):
