@@ -88,9 +88,10 @@ SERVER_WORKLOADS = {
 }
 ETA, SR_TOL, SR_MAX = 1e-6, 1e-10, 200
 PRECISIONS = {"fp64": capi.NEGF_PREC_FP64, "mixed": capi.NEGF_PREC_MIXED,
-              "refined": capi.NEGF_PREC_REFINED}
+              "refined": capi.NEGF_PREC_REFINED, "lu32": capi.NEGF_PREC_LU32}
 DTYPES = {"fp64": "f64", "mixed": "f64+c64 (mixed)",
-          "refined": "f64+c64 (complex64 block inverses with FP64 refinement)"}
+          "refined": "f64+c64 (complex64 block inverses with FP64 refinement)",
+          "lu32": "c64+f64 (complex64 block LU, FP64-refined transmission solve)"}
 MU, MUT = (-0.1, 0.0), 0.0
 LANES = [0]  # set from --lanes
 OUTPUTS = ("T12", "Modec1", "Modec2", "DOSc1", "DOSc2", "EDensity", "HDensity", "RhoN", "RhoP",
@@ -574,6 +575,10 @@ def main():
     if args.precision == "refined":
         b, st, e = run["refine"]
         result["refine"] = {"block_inverses": b, "fp64_newton_steps": st, "escalated_to_fp64": e,
+                            "note": "counts since the context was created (warmup steps included)"}
+    if args.precision == "lu32":
+        b, st, e = run["refine"]
+        result["refine"] = {"energies": b, "fp64_refinement_sweeps": st, "escalated_to_fp64": e,
                             "note": "counts since the context was created (warmup steps included)"}
     if mp_ctx is not None:
         t0 = time.perf_counter()

@@ -50,6 +50,18 @@ extern "C" {
                                X <- X + X (I - M X) until max|I - M X| <= 1e-12 (at most two);
                                a block whose complex64 inverse is singular or whose residual does
                                not contract is inverted again in FP64 (escalation) */
+#define NEGF_PREC_LU32 3 /* block LU in complex64: every RGF block inverse (Gauss-Jordan on
+                            FP32 MFMA), both column chains and the site blocks in complex64;
+                            FP64 contacts.  T12 = |W_L' G_1L W_R|^2 (Gamma_R = W_R W_R') from
+                            the leading columns of W_R (at most 64) solved to FP64 accuracy by
+                            iterative refinement of the block-tridiagonal solve A Y = E_L W_R
+                            (FP64 residuals, corrections through the complex64 block LU, two
+                            sweeps), plus the remaining columns' complex64 contribution.  An
+                            energy whose complex64 inverse is singular, whose last correction
+                            is above 1e-9 of the solution, or whose complex64 part of T12 is
+                            above 1e-3 of max(1, T12) is solved again in the plain FP64 form
+                            (escalation).  Needs a site-block-diagonal HopX (otherwise every
+                            energy takes the plain FP64 form) and the two-pass RGF. */
 
 typedef struct negf_ctx negf_ctx;
 
@@ -170,7 +182,8 @@ int negf_contact_cache_stats(negf_ctx* ctx, int* entries, long long* hits);
 /* NEGF_PREC_REFINED, no counterpart in the reference (whose Invert_Matrix is one
  * FP64 zgetrf + zgetri, MKL_Routines_Cluster.F90:19,23): block inverses taken since
  * negf_init, FP64 Newton steps applied to them, and blocks escalated to the FP64
- * inverse. */
+ * inverse.  NEGF_PREC_LU32: energies solved, FP64 refinement sweeps (summed over
+ * energies) and energies escalated to the plain FP64 form. */
 int negf_refine_stats(negf_ctx* ctx, long long* blocks, long long* steps, long long* escalated);
 /* NEGF_FLAG_GRAPH, no counterpart in the reference: graphs captured and graph
  * replays (batches launched from a graph) since negf_init. */

@@ -27,6 +27,7 @@ NEGF_STATUS_SR_NAN = 1 << 21
 NEGF_PREC_FP64 = 0
 NEGF_PREC_MIXED = 1
 NEGF_PREC_REFINED = 2
+NEGF_PREC_LU32 = 3
 
 # Every entry point include/negf_capi.h declares (checked by the CPU test suite).
 EXPORTS = (

@@ -79,6 +79,14 @@ hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long l
 template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
                           const int*);
+template <typename T>
+hipError_t launch_clear_lead_cols(hipStream_t, cx<T>*, long long, int, int, const int*, long long, int);
+hipError_t launch_lu32_resid(hipStream_t, int, cx<float>*, long long, const double2*, long long,
+                             const double2*, const double2*, const double2*, long long, const double2*,
+                             long long, const double*, const double2*, const cx<float>*,
+                             const cx<float>*, long long, const double*, double, int, int, int);
+hipError_t launch_add_c64(hipStream_t, double2*, long long, const cx<float>*, long long, long long, int);
+hipError_t launch_min_cols(hipStream_t, int*, const int*, long long, int, int);
 // complex128 -> complex64 copies of nn elements for `batch` strided matrices
 static inline hipError_t to_c64(hipStream_t s, cx<float>* Y, long long sY, const cx<double>* X,
                                 long long sX, long long nn, int batch) {
@@ -119,6 +127,18 @@ static const double kH = 6.626e-34;   // Constants.F90:6
 // the max-entry norm) or the last one exceeds kRefineAccept.
 constexpr double kRefineDone = 1e-12, kRefineStart = 0.25, kRefineAccept = 1e-10;
 constexpr int kRefineSteps = 2;
+// LU32 form: T12's leading factor columns solved to FP64 accuracy (at most
+// kLu32Cols of them) by kLu32Sweeps refinement sweeps; an energy is solved again in
+// the plain FP64 form when the last sweep's correction exceeds kLu32Accept of the
+// solution (max-entry norms), or when the complex64 part of T12 (the factor's other
+// columns) exceeds kLu32Tail of max(1, T12): complex64 holds that part to about
+// 1e-6 relative, so the bound keeps its error near 1e-9 of T12's scale.
+constexpr int kLu32Cols = 64, kLu32Sweeps = 2;
+constexpr double kLu32Accept = 1e-9, kLu32Tail = 1e-3;
+// Arena layout kind of a precision form (Layout::init).
+static int layout_form(int precision) {
+  return precision == NEGF_PREC_FP64 ? 0 : precision == NEGF_PREC_LU32 ? 2 : 1;
+}
 constexpr int kSiteQ = 59;        // site-output planes per (lead, slice, site)
 constexpr int kSiteQBasic = 8;    // planes without the bond-current terms
 constexpr int kSiteQCurrent = 11; // planes up to the pair-summed bond-current terms
@@ -129,8 +149,13 @@ struct Layout {
   // o_pst: the factorization's state, pst_bytes per lead
   long long Es, o_slots, o_z, o_M, o_colA, o_colB, o_Y, o_P, o_SigL, o_SigR, o_GamL, o_GamR,
       o_WL, o_WR, o_pst, pst_bytes, o_prow;
-  // mixed-precision form: complex64 region (offsets in complex64 elements from o_f32)
+  // mixed-precision forms: complex64 region (offsets in complex64 elements from o_f32)
   long long o_f32, f_z, f_g, f_G, f_Y, f_P, f_WL, f_WR;
+  // NEGF_PREC_LU32: the complex64 block inverses (f_gs, L blocks), the last-column
+  // chain's two buffers (f_S0, f_S1), the refinement's complex64 solves (f_y32) and
+  // its FP64 iterate (o_yr), each L slices of n x kLu32Cols
+  long long f_gs, f_S0, f_S1, f_y32, o_yr;
+  int form;  // 0 plain FP64, 1 mixed / refined, 2 LU32
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
@@ -142,16 +167,25 @@ struct Layout {
   long long site_q;      // doubles per plane (2 L nsites)
   // zslots: L more blocks per energy for the forward substitution chain of the
   // two-pass RGF (first block column from the left-connected inverses)
-  // mixed: the first-column chain runs in complex64
-  void init(int n, int L, int nsites, bool zslots, bool mixed) {
+  // form 1 (mixed, refined): the first-column chain runs in complex64
+  // form 2 (LU32): no FP64 slots; the block inverses and both chains in complex64
+  void init(int n, int L, int nsites, bool zslots, int form_) {
+    form = form_;
+    const bool mixed = form != 0;
     nn = (long long)n * n;
+    const long long ry = (long long)L * n * kLu32Cols;  // one refinement array
     long long o = 0;
-    o_slots = o; o += (long long)L * nn;
+    o_slots = o; if (form != 2) o += (long long)L * nn;
     o_z = o; if (zslots && !mixed) o += (long long)L * nn;
     o_f32 = o;
-    f_z = 0; f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_P = f_Y + nn;
+    f_gs = 0;
+    f_z = form == 2 ? (long long)L * nn : 0;
+    f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_P = f_Y + nn;
     f_WL = f_P + nn; f_WR = f_WL + nn;
-    if (mixed) o += (f_WR + nn + 1) / 2;  // two complex64 per complex128 slot
+    f_S0 = f_WR + nn; f_S1 = f_S0 + nn; f_y32 = f_S1 + nn;
+    if (form == 1) o += (f_WR + nn + 1) / 2;  // two complex64 per complex128 slot
+    if (form == 2) o += (f_y32 + ry + 1) / 2;
+    o_yr = o; if (form == 2) o += ry;
     o_M = o; o += nn;
     o_colA = o; o += nn;
     o_colB = o; o += nn;
@@ -218,6 +252,7 @@ struct Lane {
   double* d_scal = nullptr;    // cap * 8
   double* d_site = nullptr;    // cap * site_per_e
   double* d_mc = nullptr;      // 2cap
+  double* d_ref = nullptr;     // 4cap: LU32 refinement norms (per sweep, and the solution's)
   int* d_active = nullptr;     // 4cap: Sancho-Rubio active lists
   // contact-cache entries written by this lane's current batch: (energy, slot,
   // iters L/R, status L/R), published once the batch's stream has drained
@@ -335,11 +370,11 @@ static void free_arena(negf_ctx* c) {
   for (Lane& l : c->lanes) {
     l.drop_graphs();
     hipFree(l.E); hipFree(l.C); hipFree(l.Ie); hipFree(l.Ic);
-    hipFree(l.d_energy); hipFree(l.d_scal); hipFree(l.d_site); hipFree(l.d_mc);
+    hipFree(l.d_energy); hipFree(l.d_scal); hipFree(l.d_site); hipFree(l.d_mc); hipFree(l.d_ref);
     hipFree(l.d_active);
     l.E = l.C = nullptr;
     l.Ie = l.Ic = nullptr;
-    l.d_energy = l.d_scal = l.d_site = l.d_mc = nullptr;
+    l.d_energy = l.d_scal = l.d_site = l.d_mc = l.d_ref = nullptr;
     l.d_active = nullptr;
     l.cap = 0;
   }
@@ -395,7 +430,7 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
     return NEGF_ERR_ARG;
   }
   if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED &&
-      cfg->precision != NEGF_PREC_REFINED) {
+      cfg->precision != NEGF_PREC_REFINED && cfg->precision != NEGF_PREC_LU32) {
     negf_set_error("unknown precision form %d", cfg->precision);
     return NEGF_ERR_ARG;
   }
@@ -412,8 +447,7 @@ extern "C" int negf_init(negf_ctx** out, const negf_config* cfg) {
   c->n = n;
   c->L = cfg->npx;
   c->nsites = cfg->npy * cfg->npz;
-  c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS),
-              cfg->precision != NEGF_PREC_FP64);
+  c->lay.init(n, c->L, c->nsites, !(cfg->flags & NEGF_FLAG_RGF_THREE_PASS), layout_form(cfg->precision));
   // Four lanes by default: at n >= 1024 the inverse chains are latency-bound
   // (panel kernels of one workgroup per matrix), and four concurrent chains keep
   // the matrix cores fed with the other lanes' GEMMs (measured: 11.8 -> 10.1 s per
@@ -646,7 +680,7 @@ static int ensure_arena(negf_ctx* c, int want) {
   if (ready) return 0;
   const Layout& y = c->lay;
   size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(3 * y.Is) +
-                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2) + 2 * sizeof(int);
+                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2 + 4) + 2 * sizeof(int);
   free_arena(c);
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
@@ -669,6 +703,7 @@ static int ensure_arena(negf_ctx* c, int want) {
     HIPCHK(hipMalloc(&l.d_scal, sizeof(double) * 8 * cap));
     HIPCHK(hipMalloc(&l.d_site, sizeof(double) * (size_t)y.site_per_e * cap));
     HIPCHK(hipMalloc(&l.d_mc, sizeof(double) * 2 * cap));
+    HIPCHK(hipMalloc(&l.d_ref, sizeof(double) * 4 * cap));
     HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 4 * cap));
     l.cap = cap;
   }
@@ -1053,9 +1088,27 @@ static int collect(negf_ctx* c, Lane& ln, int nb, const std::vector<int>& iters,
 }
 
 // Fused solve of nb energies (nb <= cap) resident in the arena.
-static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std::vector<EnergyResult>& res) {
+// plain_ovr: solve in the plain FP64 form with the three-pass RGF, whose arena
+// (L + 11 blocks per energy) fits inside any form's; the LU32 form's escalated
+// energies take this path (its contacts keep the context's product form).
+static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std::vector<EnergyResult>& res,
+                     bool plain_ovr = false) {
   const int n = c->n, L = c->L, no = c->cfg.norb, ns = c->nsites;
-  const Layout& y = c->lay;
+  if (c->cfg.precision == NEGF_PREC_LU32 && !c->blockdiag && !plain_ovr) {
+    // the refinement's residual uses a site-block-diagonal HopX: a dense one takes
+    // the plain FP64 form for every energy
+    c->refine_escalated += nb;
+    return run_batch(c, ln, nb, energies, res, true);
+  }
+  Layout y3;
+  if (plain_ovr) y3.init(n, L, ns, false, 0);
+  const Layout& y = plain_ovr ? y3 : c->lay;
+  if (y.Es > c->lay.Es) {
+    negf_set_error("internal: plain-form arena larger than the context's");
+    return NEGF_ERR_STATE;
+  }
+  const int prec = plain_ovr ? NEGF_PREC_FP64 : c->cfg.precision;
+  const bool three_pass = plain_ovr || (c->cfg.flags & NEGF_FLAG_RGF_THREE_PASS);
   const long long Es = y.Es, nn = y.nn;
   hipStream_t s = ln.st;
   zd* E0 = ln.E;
@@ -1270,8 +1323,161 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       if (lead == 0) TRY(launch_trunc_cols<double>(s, dst, Es, n, cols, kcL, skc, nb));
       return 0;
     };
+    // ---- LU32 form: the block LU of the RGF in complex64.  Forward: M_x assembled
+    // in FP64 from the complex64 g_{x-1}, inverted in complex64 (Gauss-Jordan on FP32
+    // MFMA) into g_x, with the first-column chain's forward substitution.  Backward:
+    // the last-column chain S_x = G_xL W_R (all of W_R's columns) and the first
+    // column's back substitution, both in complex64, with the site and bond blocks
+    // and the Modec traces of both leads.  T12 = |W_L' G_1L W_R|^2 splits over W_R's
+    // columns: the leading rb = min(rank, kLu32Cols) of them are solved to FP64
+    // accuracy by iterative refinement of the block-tridiagonal system
+    //   A Y = E_L W_R(:, 0:rb),   Y_x = G_xL W_R(:, 0:rb),
+    // starting from the complex64 chain: each sweep forms the residual R = B - A Y in
+    // FP64 (H_x Y_x on FP64 MFMA, the couplings and z in one kernel), solves
+    // A D = R with the complex64 factors (forward z_x = g_x (R_x + X' z_{x-1}), back
+    // D_x = z_x + g_x X D_{x+1}) and adds D to Y in FP64.  The other columns keep the
+    // chain's complex64 values (scal[6], added on the host after the checks).
+    auto rgf_lu32 = [&]() -> int {
+      typedef cx<float> zf;
+      zf* F0 = (zf*)Q(y.o_f32);
+      auto F = [&](long long off) { return F0 + off; };
+      const long long Es32 = 2 * Es;
+      const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
+      auto gs = [&](int x) { return F(y.f_gs + (long long)x * nn); };
+      auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
+      const int RB = std::min(cR, kLu32Cols);  // refined columns of this batch
+      const long long ry = (long long)n * RB;  // one slice of the refinement arrays
+      auto y32 = [&](int x) { return F(y.f_y32 + (long long)x * ry); };
+      auto Yr = [&](int x) { return Q(y.o_yr + (long long)x * ry); };
+      zf* M32 = F(y.f_G);
+      zf* T32 = F(y.f_Y);
+      int* rb = ln.d_active;  // per energy: min(rank of W_R, kLu32Cols)
+      auto gbf = [&](const zf* A, const zf* B, zf* C, int m, int ncols, const int* kc = nullptr) {
+        GemmBatch g = {};
+        g.A = A; g.sA = Es32; g.lda = n;
+        g.B = B; g.sB = Es32; g.ldb = n;
+        g.C = C; g.sC = Es32; g.ldc = n;
+        g.m = m; g.n = ncols; g.k = n;
+        g.m3 = 1;
+        if (kc) {
+          g.ncol = kc;
+          g.sNcol = skc;
+          g.work = fracL;
+        }
+        return g;
+      };
+      auto site32 = [&](int lead, int x, const zf* V32) -> int {
+        TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, site_out(lead, x), y.site_per_e,
+                                     y.site_q, c->sp, lead ? kcR : kcL, skc, nb, nullptr));
+        return 0;
+      };
+      auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
+        if (!c->want_currents) return 0;
+        TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
+                                     site_out(lead, x), y.site_per_e, y.site_q, c->want_pairs ? 1 : 0,
+                                     lead ? kcR : kcL, skc, nb, nullptr));
+        return 0;
+      };
+      auto trace32 = [&](int lead, const zf* V32, int idx, int m, int ncols) -> int {
+        TRY(gemm_batched<float>(s, OP_C, OP_N, gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P), m, ncols), nb,
+                                kOnef, kZerof));
+        TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, m, ncols, ln.d_scal + idx, 8, nb,
+                                nullptr));
+        return 0;
+      };
+      // complex64 factors: W_L on its truncation rank (cleared past each energy's
+      // own), W_R whole
+      TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, (long long)n * cL, nb));
+      TRY(launch_trunc_cols<float>(s, F(y.f_WL), Es32, n, cL, kcL, skc, nb));
+      TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, (long long)n * cR, nb));
+      TRY(launch_min_cols(s, rb, (const int*)(pa.st[1] + rk_off), skc, kLu32Cols, nb));
+      GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
+      w32.prow = Q(y.o_prow);
+      w32.sProw = Es32;
+      w32.m3 = 1;
+      TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nb));  // the first zero pivot stays
+      int r;
+      // forward: g_x = inv(M_x) and z_x = g_x X' z_{x-1} (z_1 = g_1 W_L), complex64
+      for (int x = 0; x < L; ++x) {
+        if (x > 0) TRY((launch_convert<double, float>(s, Q(y.o_P), Es, gs(x - 1), Es32, nn, nb, nullptr)));
+        if ((r = assemble(x, x > 0 ? Q(y.o_P) : nullptr, 1))) return r;
+        TRY(to_c64(s, M32, Es32, Q(y.o_M), Es, nn, nb));
+        TRY(gj_invert<float>(s, M32, Es32, n, gs(x), Es32, n, n, nb, w32, nullptr));
+        if (x == 0) {
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(0), F(y.f_WL), zs(0), n, cL, kcL), nb, kOnef, kZerof));
+        } else {
+          TRY(launch_bd_mul<float>(s, no, 1, T32, Es32, zs(x - 1), Es32, c->X32, n, cL, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, zs(x), n, cL, kcL), nb, kOnef, kZerof));
+        }
+      }
+      // backward: S_x = g_x X S_{x+1} (S_L = g_L W_R), z_x += g_x X z_{x+1}, complex64;
+      // S_x's leading RB columns seed the refinement iterate Y_x
+      zf* Sb[2] = {F(y.f_S0), F(y.f_S1)};
+      TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(L - 1), F(y.f_WR), Sb[1], n, cR), nb, kOnef, kZerof));
+      zf* Scur = Sb[1];
+      int pb = 0;
+      for (int x = L - 1; x >= 0; --x) {
+        zf* Sx = Scur;
+        if (x < L - 1) {
+          Sx = Sb[pb];
+          TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, Scur, Es32, c->X32, n, cR, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, Sx, n, cR), nb, kOnef, kZerof));
+          TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, zs(x + 1), Es32, c->X32, n, cL, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, zs(x), n, cL, kcL), nb, kOnef, kOnef));
+        }
+        TRY((launch_convert<double, float>(s, Yr(x), Es, Sx, Es32, ry, nb, nullptr)));
+        if ((r = site32(1, x, Sx))) return r;
+        if ((r = bonds32(1, x, Sx, x < L - 1 ? Scur : nullptr))) return r;
+        if (x == L - 1 && (r = trace32(1, Sx, 3, cR, cR))) return r;  // Modec2, right-lead term
+        if ((r = site32(0, x, zs(x)))) return r;
+        if ((r = bonds32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
+        if (x == L - 1 && (r = trace32(1, zs(x), 2, cR, cL))) return r;  // Modec2, left-lead term
+        if (x == 0 && (r = trace32(0, zs(x), 1, cL, cL))) return r;      // Modec1, left-lead term
+        if (x < L - 1) {
+          Scur = Sx;
+          pb ^= 1;
+        }
+      }
+      // scal[6]: T12 over W_R's columns past each energy's rb, from complex64 S_1
+      TRY(launch_clear_lead_cols<float>(s, Scur, Es32, n, cR, rb, 1, nb));
+      TRY((launch_convert<double, float>(s, Q(y.o_colA), Es, Scur, Es32, (long long)n * cR, nb, nullptr)));
+      if ((r = trace_term(0, Q(y.o_colA), 6, cLf, cR))) return r;
+      // refinement sweeps on Y (FP64); d_ref[sweep * nb + b] = max|D|, then max|Y|
+      for (int sw = 0; sw < kLu32Sweeps; ++sw) {
+        for (int x = 0; x < L; ++x) {
+          GemmBatch g = shape(gbr(hx(x), 0, n, Yr(x), Es, n, Q(y.o_P), Es, n, n), n, RB);
+          TRY(gemm_batched<double>(s, OP_N, OP_N, g, nb, kOne, kZero));
+          if (x == 0)
+            TRY(gemm_batched<double>(s, OP_N, OP_N,
+                                     shape(gbr(Q(y.o_SigL), Es, n, Yr(0), Es, n, Q(y.o_P), Es, n, n), n, RB),
+                                     nb, kOne, kOne));
+          if (x == L - 1)
+            TRY(gemm_batched<double>(s, OP_N, OP_N,
+                                     shape(gbr(Q(y.o_SigR), Es, n, Yr(x), Es, n, Q(y.o_P), Es, n, n), n, RB),
+                                     nb, kOne, kOne));
+          TRY(launch_lu32_resid(s, no, T32, Es32, x == L - 1 ? Q(y.o_WR) : nullptr, Es,
+                                x > 0 ? Yr(x - 1) : nullptr, Yr(x), x < L - 1 ? Yr(x + 1) : nullptr, Es,
+                                Q(y.o_P), Es, hv(x), c->X, c->X32, x > 0 ? y32(x - 1) : nullptr, Es32,
+                                ln.d_energy, c->cfg.eta, n, RB, nb));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, y32(x), n, RB), nb, kOnef, kZerof));
+        }
+        for (int x = L - 2; x >= 0; --x) {
+          TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, y32(x + 1), Es32, c->X32, n, RB, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, y32(x), n, RB), nb, kOnef, kOnef));
+        }
+        TRY(launch_maxabs<float>(s, y32(0), Es32, (long long)L * ry, ln.d_ref + (size_t)sw * nb, nb, nullptr));
+        TRY(launch_add_c64(s, Yr(0), Es, y32(0), Es32, (long long)L * ry, nb));
+      }
+      TRY(launch_maxabs<double>(s, Yr(0), Es, (long long)L * ry, ln.d_ref + (size_t)kLu32Sweeps * nb, nb,
+                                nullptr));
+      // scal[0]: T12 over the refined columns (those past each energy's rb cleared)
+      TRY(launch_trunc_cols<double>(s, Yr(0), Es, n, RB, rb, 1, nb));
+      if ((r = trace_term(0, Yr(0), 0, cLf, RB))) return r;
+      return 0;
+    };
     int r;
-    if (c->cfg.precision != NEGF_PREC_FP64) {
+    if (prec == NEGF_PREC_LU32) return rgf_lu32();
+    if (prec != NEGF_PREC_FP64) {
       // ---- mixed-precision two-pass RGF.  FP64: the inverses gL_x and the last block
       // column G_xL W_R (T12 = |W_L' G_1L W_R|^2 stays an FP64 product chain).
       // complex64 (3M MFMA): the forward/back substitution for the first block column
@@ -1333,7 +1539,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       // first residual is too large for Newton to contract fast (> kRefineStart) or
       // whose last one is above kRefineAccept is inverted again in FP64.  M_x stays
       // in o_M throughout (the complex64 elimination works on a copy).
-      const bool refined = c->cfg.precision == NEGF_PREC_REFINED;
+      const bool refined = prec == NEGF_PREC_REFINED;
       GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
       w32.prow = Q(y.o_prow);  // kGjNBO x n complex64 fit in the complex128 pivot-row panel
       w32.sProw = Es32;
@@ -1444,7 +1650,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       }
       return 0;
     }
-    if (!(c->cfg.flags & NEGF_FLAG_RGF_THREE_PASS)) {
+    if (!three_pass) {
       // ---- two-pass RGF.  Forward: gL_x (Recursive_Greens.F90:16-34) and the forward
       // substitution z_x = gL_x X' z_{x-1}, z_1 = gL_1, of A Y = E_1 (block column 1).
       auto zs = [&](int x) { return Q(y.o_z + (long long)x * nn); };
@@ -1543,7 +1749,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   constexpr long long kMaxGraphLaunches = 20000;
   const long long est_launches =
       (long long)L * (3LL * ((n + 7) / 8) + 2LL * ((n + 127) / 128) + 16);
-  const bool graph = (c->cfg.flags & NEGF_FLAG_GRAPH) && c->cfg.precision != NEGF_PREC_REFINED &&
+  const bool graph = (c->cfg.flags & NEGF_FLAG_GRAPH) && prec != NEGF_PREC_REFINED && !plain_ovr &&
                      est_launches <= kMaxGraphLaunches;
   if (graph) {
     const GraphKey key = {c->uid, c->generation, nb, c->want_currents, c->want_pairs, ln.E,
@@ -1585,7 +1791,41 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   } else if ((rr = rgf())) {
     return rr;
   }
-  return collect(c, ln, nb, iters, sst, res);
+  if ((rr = collect(c, ln, nb, iters, sst, res))) return rr;
+  if (prec != NEGF_PREC_LU32) return 0;
+  // LU32: accept an energy's T12 (refined part + complex64 part) when its complex64
+  // block inverses had no zero pivot, its last refinement correction is at most
+  // kLu32Accept of the solution and the complex64 part is at most kLu32Tail of
+  // max(1, T12); otherwise solve it again in the plain FP64 form.
+  std::vector<double> nrm((size_t)(kLu32Sweeps + 1) * nb);
+  std::vector<int> ic((size_t)y.Is * nb);
+  TRY(hipMemcpyAsync(nrm.data(), ln.d_ref, sizeof(double) * nrm.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipMemcpyAsync(ic.data(), ln.Ic, sizeof(int) * ic.size(), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  std::vector<int> esc;
+  std::vector<double> eesc;
+  for (int b = 0; b < nb; ++b) {
+    const double tb = res[b].scal[0], ts = res[b].scal[6];
+    const double dlast = nrm[(size_t)(kLu32Sweeps - 1) * nb + b], ymax = nrm[(size_t)kLu32Sweeps * nb + b];
+    const bool ok = ic[(size_t)b * y.Is + 2 * n] == 0 && dlast <= kLu32Accept * ymax &&
+                    ts <= kLu32Tail * std::max(1.0, std::fabs(tb)) && std::isfinite(tb);
+    if (ok) {
+      res[b].scal[0] = tb + ts;
+      res[b].scal[6] = 0.0;
+    } else {
+      esc.push_back(b);
+      eesc.push_back(energies[b]);
+    }
+  }
+  c->refine_blocks += nb;
+  c->refine_steps += (long long)kLu32Sweeps * nb;
+  c->refine_escalated += (long long)esc.size();
+  if (!esc.empty()) {
+    std::vector<EnergyResult> r2;
+    if ((rr = run_batch(c, ln, (int)esc.size(), eesc.data(), r2, true))) return rr;
+    for (size_t k = 0; k < esc.size(); ++k) res[esc[k]] = r2[k];
+  }
+  return 0;
 }
 
 // Accumulate one energy's results in the reference's output layout.

@@ -676,6 +676,116 @@ __global__ void k_convert(cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX
 }
 
 // ---------------------------------------------------------------------------
+// Iterative refinement of the block-tridiagonal solve A Y = B (A = zI - H - Sigma
+// with the slice couplings -X, -X'; Recursive_Greens.F90:6) against complex64
+// block-LU factors.  One row of slices at a time, element (i, j) of energy b:
+//   R = B_x - A Y |_x = B_x - z Y_x + P + X Y_{x+1} + X' Y_{x-1}
+// in FP64, with P = (H_x + Sigma_x) Y_x from the FP64 GEMM (plus V_x Y_x for a
+// compact Hamiltonian), and written in complex64 together with the forward
+// substitution's coupling to the previous slice:
+//   T = c64(R) + X32' y_{x-1}.
+// X is site-block-diagonal (NORB x NORB blocks).  Null B, Ym, Yp, ym, V are zero.
+template <int NORB>
+__global__ void k_lu32_resid(cx<float>* T, long long sT, const double2* B, long long sB,
+                             const double2* Ym, const double2* Y0, const double2* Yp, long long sY,
+                             const double2* P, long long sP, const double* V, const double2* X,
+                             const cx<float>* X32, const cx<float>* ym, long long sy,
+                             const double* energy, double eta, int n) {
+  const int b = blockIdx.z, j = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long e = (long long)j * n + i;
+  const int base = (i / NORB) * NORB;
+  const double2 y0 = Y0[(long long)b * sY + e];
+  const double2 z = mk<double>(energy[b], eta);
+  double2 r = cadd(P[(long long)b * sP + e], cneg(cmul(z, y0)));
+  if (B) r = cadd(r, B[(long long)b * sB + e]);
+  if (V) {
+    const double v = V[i / NORB];
+    r.x += v * y0.x;
+    r.y += v * y0.y;
+  }
+  const long long col = (long long)b * sY + (long long)j * n + base;
+#pragma unroll
+  for (int k = 0; k < NORB; ++k) {
+    if (Yp) r = cadd(r, cmul(X[(long long)(base + k) * n + i], Yp[col + k]));
+    if (Ym) r = cadd(r, cmul(cconj(X[(long long)i * n + base + k]), Ym[col + k]));
+  }
+  cx<float> t = mk<float>((float)r.x, (float)r.y);
+  if (ym) {
+    const long long c32 = (long long)b * sy + (long long)j * n + base;
+#pragma unroll
+    for (int k = 0; k < NORB; ++k) t = cadd(t, cmul(cconj(X32[(long long)i * n + base + k]), ym[c32 + k]));
+  }
+  T[(long long)b * sT + e] = t;
+}
+
+template <int NORB>
+static hipError_t lu32_resid_n(hipStream_t s, cx<float>* T, long long sT, const double2* B,
+                               long long sB, const double2* Ym, const double2* Y0, const double2* Yp,
+                               long long sY, const double2* P, long long sP, const double* V,
+                               const double2* X, const cx<float>* X32, const cx<float>* ym,
+                               long long sy, const double* energy, double eta, int n, int cols,
+                               int batch) {
+  hipLaunchKernelGGL((k_lu32_resid<NORB>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, T, sT,
+                     B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_lu32_resid(hipStream_t s, int norb, cx<float>* T, long long sT, const double2* B,
+                             long long sB, const double2* Ym, const double2* Y0, const double2* Yp,
+                             long long sY, const double2* P, long long sP, const double* V,
+                             const double2* X, const cx<float>* X32, const cx<float>* ym,
+                             long long sy, const double* energy, double eta, int n, int cols,
+                             int batch) {
+  if (batch <= 0 || cols <= 0) return hipSuccess;
+  if (norb == 1)
+    return lu32_resid_n<1>(s, T, sT, B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n,
+                           cols, batch);
+  if (norb == 2)
+    return lu32_resid_n<2>(s, T, sT, B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n,
+                           cols, batch);
+  if (norb == 4)
+    return lu32_resid_n<4>(s, T, sT, B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n,
+                           cols, batch);
+  return hipErrorInvalidValue;
+}
+
+// Y += D (FP64 accumulate of a complex64 correction), nn elements per matrix.
+__global__ void k_add_c64(double2* Y, long long sY, const cx<float>* D, long long sD, long long nn) {
+  const int b = blockIdx.y;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cx<float> d = D[(long long)b * sD + i];
+    double2& y = Y[(long long)b * sY + i];
+    y.x += (double)d.x;
+    y.y += (double)d.y;
+  }
+}
+
+hipError_t launch_add_c64(hipStream_t s, double2* Y, long long sY, const cx<float>* D, long long sD,
+                          long long nn, int batch) {
+  if (batch <= 0 || nn <= 0) return hipSuccess;
+  long long blocks = (nn + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_add_c64, dim3((unsigned)blocks, batch), dim3(256), 0, s, Y, sY, D, sD, nn);
+  return hipGetLastError();
+}
+
+// out[b] = min(in[b * sIn], cap): per-matrix column counts capped (device-side, so
+// the launch sequence that uses them needs no host read-back).
+__global__ void k_min_cols(int* out, const int* in, long long sIn, int cap, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) out[b] = min(in[(long long)b * sIn], cap);
+}
+
+hipError_t launch_min_cols(hipStream_t s, int* out, const int* in, long long sIn, int cap, int batch) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_min_cols, dim3((batch + 63) / 64), dim3(64), 0, s, out, in, sIn, cap, batch);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Stage-wise observables on materialised Gn/Gp blocks (Calculate_Observables).
 
 // out[b*n + i] = Re M_b[i][i]
@@ -959,13 +1069,17 @@ hipError_t launch_inner(hipStream_t s, const cx<T>* P, long long sP, const cx<T>
   return hipGetLastError();
 }
 
-// Y_b(:, kc_b : cols) = 0 (kc_b = kc[b * skc]): a factor product truncated per matrix.
+// Y_b(:, kc_b : cols) = 0 (kc_b = kc[b * skc]): a factor product truncated per
+// matrix; with `lead`, the leading columns Y_b(:, 0 : kc_b) are cleared instead.
 template <typename T>
-__global__ void k_trunc_cols(cx<T>* Y, long long sY, int n, int cols, const int* kc, long long skc) {
+__global__ void k_trunc_cols(cx<T>* Y, long long sY, int n, int cols, const int* kc, long long skc,
+                             int lead) {
   const int b = blockIdx.z;
   const int col = blockIdx.y;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n || col >= cols || col < kc[(long long)b * skc]) return;
+  if (a >= n || col >= cols) return;
+  const int k = kc[(long long)b * skc];
+  if (lead ? col >= k : col < k) return;
   Y[(long long)b * sY + (long long)col * n + a] = mk<T>(0, 0);
 }
 
@@ -974,7 +1088,16 @@ hipError_t launch_trunc_cols(hipStream_t s, cx<T>* Y, long long sY, int n, int c
                              long long skc, int batch) {
   if (batch <= 0 || cols <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_trunc_cols<T>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, Y, sY, n,
-                     cols, kc, skc);
+                     cols, kc, skc, 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_clear_lead_cols(hipStream_t s, cx<T>* Y, long long sY, int n, int cols,
+                                  const int* kc, long long skc, int batch) {
+  if (batch <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_trunc_cols<T>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, Y, sY, n,
+                     cols, kc, skc, 1);
   return hipGetLastError();
 }
 
@@ -1064,6 +1187,8 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
                                       const int*);                                                 \
   template hipError_t launch_trunc_cols<T>(hipStream_t, cx<T>*, long long, int, int, const int*,   \
                                            long long, int);                                        \
+  template hipError_t launch_clear_lead_cols<T>(hipStream_t, cx<T>*, long long, int, int,          \
+                                                const int*, long long, int);                       \
   template hipError_t launch_eye<T>(hipStream_t, cx<T>*, long long, int, int, const int*);         \
   template hipError_t launch_add<T>(hipStream_t, cx<T>*, long long, const cx<T>*, long long,       \
                                     const cx<T>*, long long, long long, int, const int*);          \

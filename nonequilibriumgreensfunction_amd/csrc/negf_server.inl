@@ -35,7 +35,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 size_t lane_bytes_per_energy(const Layout& y) {
   return align256(sizeof(zd) * (size_t)y.Es) + align256(sizeof(zd) * 2 * (size_t)y.Cs) +
          align256(sizeof(int) * (size_t)y.Is) + align256(sizeof(int) * 2 * (size_t)y.Is) +
-         align256(sizeof(double) * (size_t)y.site_per_e) + 3 * 256 + align256(sizeof(double) * 8) +
+         align256(sizeof(double) * (size_t)y.site_per_e) + 4 * 256 + align256(sizeof(double) * 8) +
          align256(sizeof(int) * 4);
 }
 
@@ -56,6 +56,7 @@ void bind_lane(Lane& ln, const Layout& y, int nb, char* pool) {
   ln.d_energy = (double*)take(sizeof(double) * nb);
   ln.d_scal = (double*)take(sizeof(double) * 8 * nb);
   ln.d_mc = (double*)take(sizeof(double) * 2 * nb);
+  ln.d_ref = (double*)take(sizeof(double) * 4 * nb);
   ln.d_active = (int*)take(sizeof(int) * 4 * nb);
   ln.cap = nb;
 }
@@ -235,7 +236,7 @@ extern "C" int negf_server_create(negf_server** out, const negf_server_config* c
     return NEGF_ERR_ARG;
   }
   if (cfg->precision != NEGF_PREC_FP64 && cfg->precision != NEGF_PREC_MIXED &&
-      cfg->precision != NEGF_PREC_REFINED) {
+      cfg->precision != NEGF_PREC_REFINED && cfg->precision != NEGF_PREC_LU32) {
     negf_set_error("unknown precision form %d", cfg->precision);
     return NEGF_ERR_ARG;
   }
@@ -306,7 +307,7 @@ extern "C" int negf_server_submit(negf_server* s, const negf_request* req, long 
   SrvRequest* r = new SrvRequest();
   r->req = *req;
   r->lay.init(n, req->npx, req->npy * req->npz, !(s->cfg.flags & NEGF_FLAG_RGF_THREE_PASS),
-              s->cfg.precision != NEGF_PREC_FP64);
+              layout_form(s->cfg.precision));
   r->footprint = resident_bytes(n, req->npx, req->npy * req->npz, s->cfg.precision, !req->Htotd);
   if (lane_bytes_per_energy(r->lay) > s->pool_bytes || r->footprint > s->budget) {
     negf_set_error("request (n=%d, Npx=%d) needs %.1f GB per energy and %.1f GB resident; lane pool "
