@@ -7,12 +7,14 @@ Default workload (the configuration the energy-points/s target is stated on):
 the 4-band TI wire with Npx=200, Npy=Npz=24 (block n=2304), 1024 energies in
 [-0.3, 0.3], Anderson disorder W=0.3 seed 1, eta=1e-6, Sancho-Rubio tol 1e-10
 / max 200; each step takes a strided slice of that grid so every step samples
-the whole range.  The timed form is the mixed-precision form (FP64 contacts,
-inverses and the T12 chain; complex64 first-column substitution and site-block
-products), the fastest form meeting the <= 1e-8 relative transmission error;
-the plain FP64 form is timed on the same energies right after, and the mixed
-form's error against it is reported for every output (the plain form itself is
-pinned to the complex128 CPU oracle by the test suite).
+the whole range.  The timed form is the complex64 block-LU form (lu32: FP64
+contacts; block inverses, both column chains and site blocks in complex64; T12
+from an FP64-refined solve of the block-tridiagonal system on the leading
+columns of the right lead's broadening factor), the fastest form meeting the
+<= 1e-8 relative transmission error; the plain FP64 form is timed on the same
+energies right after, and the timed form's error against it is reported for
+every output (the plain form itself is pinned to the complex128 CPU oracle by
+the test suite).
 
 Prints one JSON line.  `roofline` is for the dominant kernel -- the MFMA kernel
 kind with the largest summed duration among the sampled launches (the FP64 3M
@@ -63,10 +65,12 @@ CONFIGS = {
     # plain FP64 mid-size: block 1024, Npx=100, W=0.5
     "c2": dict(npx=100, npy=16, npz=16, ne=512, emin=-0.4, emax=0.4, W=0.5, seed=1, per_step=64,
                budget_gb=0.0),
-    # precision target size: block 2304, Npx=200, W=0.3.  245 GB of energies in flight
-    # (the library caps it at 90% of free HBM) holds two energies per lane on four lanes
-    "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=8,
-               budget_gb=245.0),
+    # precision target size: block 2304, Npx=200, W=0.3.  262 GB of energies in flight
+    # (the library caps it at 90% of free HBM, ~290 GB once Htotd is resident) holds
+    # three energies per lane on four lanes in the lu32 form (20.8 GB each); the plain
+    # FP64 form (36.8 GB each) fits one per lane
+    "c4": dict(npx=200, npy=24, npz=24, ne=1024, emin=-0.3, emax=0.3, W=0.3, seed=1, per_step=12,
+               budget_gb=262.0),
 }
 # Many-wire workloads run through the serving / batching layer (negf_server):
 #  c3 -- a seeded fraction of the batching configuration (256 disorder realisations x
@@ -389,17 +393,17 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS) + sorted(SERVER_WORKLOADS))
-    ap.add_argument("--precision", default="mixed", choices=tuple(PRECISIONS))
+    ap.add_argument("--precision", default="lu32", choices=tuple(PRECISIONS))
     ap.add_argument("--lanes", type=int, default=0, help="concurrent device lanes (0 = library default)")
     ap.add_argument("--per-step", type=int, default=0, help="energies per step (default per config)")
     ap.add_argument("--budget-gb", type=float, default=-1.0,
                     help="HBM for energies in flight (default per config; 0 = library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
-                    help="skip the plain-FP64 comparison leg of the mixed form")
+                    help="skip the plain-FP64 comparison leg of a reduced-precision form")
     ap.add_argument("--compare-steps", type=int, default=3,
                     help="timed steps of the plain-FP64 comparison leg (after the same warmup, "
-                         "capped at one step, on the energies of the last timed mixed steps)")
+                         "capped at one step, on the energies of the last timed steps)")
     ap.add_argument("--cpu-ranks", type=int, default=0,
                     help="CPU baseline ranks (0 = one per host core, at most 16)")
     ap.add_argument("--cpu-baseline-mode", default="auto", choices=("auto", "full", "pieces"))
@@ -413,7 +417,7 @@ def main():
                     help="server workloads: working pool per lane (GB)")
     ap.add_argument("--flags", type=int, default=0, help="NEGF_FLAG_* algorithm selection")
     ap.add_argument("--gamma-trunc", type=float, default=None,
-                    help="negf_config.gamma_trunc of the measured form (default: %g in the mixed "
+                    help="negf_config.gamma_trunc of the measured form (default: %g in the reduced-precision "
                          "forms, 0 in the plain one; the plain FP64 compare leg always uses 0)"
                          % GAMMA_TRUNC_MIXED)
     ap.add_argument("--profile-every", type=int, default=32,
@@ -483,7 +487,7 @@ def main():
     compare = None
     if args.precision != "fp64" and not args.no_compare:
         # the plain FP64 form, warmed up the same way (one step at most), on the
-        # energies of the last `ncmp` timed mixed steps
+        # energies of the last `ncmp` timed steps
         last = sorted(run["accs"])
         base = run_form(w, cfg, "fp64", args.flags, per, grid, dE, lambda k: step_indices(last[k - 1]),
                         ncmp, min(args.warmup, 1), args.profile_every, "fp64", keep=ncmp)
@@ -498,8 +502,8 @@ def main():
             "speedup_vs_plain_fp64": value / plain_rate,
             "error_vs_plain_fp64": err,
             "note": (f"plain FP64 form after {min(args.warmup, 1)} warmup step, {ncmp} timed steps on "
-                     "the energies of the last timed mixed steps; errors: max over those steps of "
-                     "|mixed - fp64| / the output's natural scale; the plain FP64 form agrees with "
+                     "the energies of the last timed steps; errors: max over those steps of "
+                     "|timed form - fp64| / the output's natural scale; the plain FP64 form agrees with "
                      "the complex128 CPU oracle to 1e-10 in the test suite"),
         }
 

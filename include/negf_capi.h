@@ -57,11 +57,12 @@ extern "C" {
                             iterative refinement of the block-tridiagonal solve A Y = E_L W_R
                             (FP64 residuals, corrections through the complex64 block LU, two
                             sweeps), plus the remaining columns' complex64 contribution.  An
-                            energy whose complex64 inverse is singular, whose last correction
-                            is above 1e-9 of the solution, or whose complex64 part of T12 is
-                            above 1e-3 of max(1, T12) is solved again in the plain FP64 form
-                            (escalation).  Needs a site-block-diagonal HopX (otherwise every
-                            energy takes the plain FP64 form) and the two-pass RGF. */
+                            energy whose complex64 inverse is singular, whose refinement does not
+                            contract to an estimated error of 1e-10 of the solution, or whose
+                            complex64 part of T12 is above 1e-3 of max(1, T12) is solved again
+                            in the plain FP64 form (escalation).  Needs a site-block-diagonal
+                            HopX (otherwise every energy takes the plain FP64 form) and the
+                            two-pass RGF. */
 
 typedef struct negf_ctx negf_ctx;
 

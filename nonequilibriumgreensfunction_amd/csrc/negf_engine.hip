@@ -44,19 +44,24 @@ hipError_t launch_assemble(hipStream_t, int, int, cx<T>*, long long, const cx<T>
                            const double*);
 template <typename T>
 hipError_t launch_bd_mul(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, long long,
-                         const cx<T>*, int, int, int, const int*);
+                         const cx<T>*, int, int, int, const int*, int accum = 0);
+// Site and bond blocks over the columns [kfirst, min(kc_b, klast)) of the factor
+// products, written or (accum) added; traces written or (accum) added.
+constexpr int kAllCols = 1 << 30;
 template <typename T>
 hipError_t launch_siteblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              int, double*, long long, long long, const SpinMats&, const int*,
-                             long long, int, const int*);
+                             long long, int, const int*, int kfirst = 0, int klast = kAllCols,
+                             int accum = 0);
 hipError_t launch_zero(hipStream_t, void*, size_t);
 template <typename T>
 hipError_t launch_bondblocks(hipStream_t, int, const cx<T>*, long long, const cx<T>*, long long,
                              const double2*, const double2*, int, int, int, double*, long long,
-                             long long, int, const int*, long long, int, const int*);
+                             long long, int, const int*, long long, int, const int*, int kfirst = 0,
+                             int klast = kAllCols, int accum = 0);
 template <typename T>
 hipError_t launch_inner(hipStream_t, const cx<T>*, long long, const cx<T>*, long long, int, int, int,
-                        double*, long long, int, const int*);
+                        double*, long long, int, const int*, int accum = 0);
 template <typename T>
 hipError_t launch_trunc_cols(hipStream_t, cx<T>*, long long, int, int, const int*, long long, int);
 template <typename T>
@@ -79,14 +84,11 @@ hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long l
 template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
                           const int*);
-template <typename T>
-hipError_t launch_clear_lead_cols(hipStream_t, cx<T>*, long long, int, int, const int*, long long, int);
 hipError_t launch_lu32_resid(hipStream_t, int, cx<float>*, long long, const double2*, long long,
-                             const double2*, const double2*, const double2*, long long, const double2*,
-                             long long, const double*, const double2*, const cx<float>*,
-                             const cx<float>*, long long, const double*, double, int, int, int);
+                             const double2*, long long, long long, const double*, long long,
+                             const double2*, const double2*, const double2*, long long, const double*,
+                             double, int, int, int, int, int);
 hipError_t launch_add_c64(hipStream_t, double2*, long long, const cx<float>*, long long, long long, int);
-hipError_t launch_min_cols(hipStream_t, int*, const int*, long long, int, int);
 // complex128 -> complex64 copies of nn elements for `batch` strided matrices
 static inline hipError_t to_c64(hipStream_t s, cx<float>* Y, long long sY, const cx<double>* X,
                                 long long sX, long long nn, int batch) {
@@ -128,13 +130,16 @@ static const double kH = 6.626e-34;   // Constants.F90:6
 constexpr double kRefineDone = 1e-12, kRefineStart = 0.25, kRefineAccept = 1e-10;
 constexpr int kRefineSteps = 2;
 // LU32 form: T12's leading factor columns solved to FP64 accuracy (at most
-// kLu32Cols of them) by kLu32Sweeps refinement sweeps; an energy is solved again in
-// the plain FP64 form when the last sweep's correction exceeds kLu32Accept of the
-// solution (max-entry norms), or when the complex64 part of T12 (the factor's other
-// columns) exceeds kLu32Tail of max(1, T12): complex64 holds that part to about
-// 1e-6 relative, so the bound keeps its error near 1e-9 of T12's scale.
+// kLu32Cols of them) by kLu32Sweeps refinement sweeps.  With d0, d1 the last two
+// corrections and y the solution (max-entry norms), the error left after the last
+// sweep is about d1 * (d1 / d0) when the sweeps contract (d1 <= kLu32Contract d0).
+// An energy is accepted when that estimate is at most kLu32Accept * y, or d1 is at
+// rounding level (kLu32Floor * y), and the complex64 part of T12 (the factor's
+// other columns) is at most kLu32Tail of max(1, T12): complex64 holds that part to
+// about 1e-6 relative, so its error stays near 1e-9 of T12's scale.  Otherwise it
+// is solved again in the plain FP64 form.
 constexpr int kLu32Cols = 64, kLu32Sweeps = 2;
-constexpr double kLu32Accept = 1e-9, kLu32Tail = 1e-3;
+constexpr double kLu32Contract = 0.1, kLu32Accept = 1e-10, kLu32Floor = 1e-13, kLu32Tail = 1e-3;
 // Arena layout kind of a precision form (Layout::init).
 static int layout_form(int precision) {
   return precision == NEGF_PREC_FP64 ? 0 : precision == NEGF_PREC_LU32 ? 2 : 1;
@@ -152,9 +157,11 @@ struct Layout {
   // mixed-precision forms: complex64 region (offsets in complex64 elements from o_f32)
   long long o_f32, f_z, f_g, f_G, f_Y, f_P, f_WL, f_WR;
   // NEGF_PREC_LU32: the complex64 block inverses (f_gs, L blocks), the last-column
-  // chain's two buffers (f_S0, f_S1), the refinement's complex64 solves (f_y32) and
-  // its FP64 iterate (o_yr), each L slices of n x kLu32Cols
-  long long f_gs, f_S0, f_S1, f_y32, o_yr;
+  // chain's two buffers (f_S0, f_S1) and the refinement's FP64 iterate (o_vr: L
+  // slices of n x 2 kLu32Cols, the last and the first block column side by side).
+  // The refinement's complex64 residuals and solves and its FP64 products reuse the
+  // first-column chain's region f_z (dead by then), sized to hold them.
+  long long f_gs, f_S0, f_S1, o_vr;
   int form;  // 0 plain FP64, 1 mixed / refined, 2 LU32
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
@@ -173,19 +180,21 @@ struct Layout {
     form = form_;
     const bool mixed = form != 0;
     nn = (long long)n * n;
-    const long long ry = (long long)L * n * kLu32Cols;  // one refinement array
+    const long long ry = (long long)L * n * 2 * kLu32Cols;  // one refinement array
     long long o = 0;
     o_slots = o; if (form != 2) o += (long long)L * nn;
     o_z = o; if (zslots && !mixed) o += (long long)L * nn;
     o_f32 = o;
     f_gs = 0;
     f_z = form == 2 ? (long long)L * nn : 0;
-    f_g = f_z + (long long)L * nn; f_G = f_g + nn; f_Y = f_G + nn; f_P = f_Y + nn;
+    // LU32: f_z also holds the refinement's two complex64 arrays and FP64 products
+    const long long zsz = form == 2 ? std::max((long long)L * nn, 4 * ry) : (long long)L * nn;
+    f_g = f_z + zsz; f_G = f_g + nn; f_Y = f_G + nn; f_P = f_Y + nn;
     f_WL = f_P + nn; f_WR = f_WL + nn;
-    f_S0 = f_WR + nn; f_S1 = f_S0 + nn; f_y32 = f_S1 + nn;
+    f_S0 = f_WR + nn; f_S1 = f_S0 + nn;
     if (form == 1) o += (f_WR + nn + 1) / 2;  // two complex64 per complex128 slot
-    if (form == 2) o += (f_y32 + ry + 1) / 2;
-    o_yr = o; if (form == 2) o += ry;
+    if (form == 2) o += (f_S1 + nn + 1) / 2;
+    o_vr = o; if (form == 2) o += ry;
     o_M = o; o += nn;
     o_colA = o; o += nn;
     o_colB = o; o += nn;
@@ -252,7 +261,7 @@ struct Lane {
   double* d_scal = nullptr;    // cap * 8
   double* d_site = nullptr;    // cap * site_per_e
   double* d_mc = nullptr;      // 2cap
-  double* d_ref = nullptr;     // 4cap: LU32 refinement norms (per sweep, and the solution's)
+  double* d_ref = nullptr;     // 8cap: LU32 refinement norms (per system: per sweep, and the solution's)
   int* d_active = nullptr;     // 4cap: Sancho-Rubio active lists
   // contact-cache entries written by this lane's current batch: (energy, slot,
   // iters L/R, status L/R), published once the batch's stream has drained
@@ -680,7 +689,7 @@ static int ensure_arena(negf_ctx* c, int want) {
   if (ready) return 0;
   const Layout& y = c->lay;
   size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(3 * y.Is) +
-                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2 + 4) + 2 * sizeof(int);
+                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2 + 8) + 2 * sizeof(int);
   free_arena(c);
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
@@ -703,7 +712,7 @@ static int ensure_arena(negf_ctx* c, int want) {
     HIPCHK(hipMalloc(&l.d_scal, sizeof(double) * 8 * cap));
     HIPCHK(hipMalloc(&l.d_site, sizeof(double) * (size_t)y.site_per_e * cap));
     HIPCHK(hipMalloc(&l.d_mc, sizeof(double) * 2 * cap));
-    HIPCHK(hipMalloc(&l.d_ref, sizeof(double) * 4 * cap));
+    HIPCHK(hipMalloc(&l.d_ref, sizeof(double) * 8 * cap));
     HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 4 * cap));
     l.cap = cap;
   }
@@ -1327,31 +1336,36 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     // in FP64 from the complex64 g_{x-1}, inverted in complex64 (Gauss-Jordan on FP32
     // MFMA) into g_x, with the first-column chain's forward substitution.  Backward:
     // the last-column chain S_x = G_xL W_R (all of W_R's columns) and the first
-    // column's back substitution, both in complex64, with the site and bond blocks
-    // and the Modec traces of both leads.  T12 = |W_L' G_1L W_R|^2 splits over W_R's
-    // columns: the leading rb = min(rank, kLu32Cols) of them are solved to FP64
-    // accuracy by iterative refinement of the block-tridiagonal system
-    //   A Y = E_L W_R(:, 0:rb),   Y_x = G_xL W_R(:, 0:rb),
-    // starting from the complex64 chain: each sweep forms the residual R = B - A Y in
-    // FP64 (H_x Y_x on FP64 MFMA, the couplings and z in one kernel), solves
-    // A D = R with the complex64 factors (forward z_x = g_x (R_x + X' z_{x-1}), back
-    // D_x = z_x + g_x X D_{x+1}) and adds D to Y in FP64.  The other columns keep the
-    // chain's complex64 values (scal[6], added on the host after the checks).
+    // column's back substitution Z_x = G_x1 W_Lt (W_L on its truncation rank), both in
+    // complex64.  The leading kLu32Cols columns of both are then solved to FP64
+    // accuracy by iterative refinement of the block-tridiagonal systems
+    //   A Y = E_L W_R(:, 0:RB),   A Z = E_1 W_Lt(:, 0:RL),
+    // starting from the complex64 chains: each sweep forms the residual R = B - A V in
+    // FP64 (H_x V_x on FP64 MFMA, the couplings and z in one kernel), solves A D = R
+    // with the complex64 factors (forward u_x = g_x (R_x + X' u_{x-1}), back
+    // D_x = u_x + g_x X D_{x+1}) and adds D to V in FP64.  Every output is a sum over
+    // factor columns (site and bond blocks of V V', traces |W' V|^2): columns below
+    // kLu32Cols come from the refined V, the others from the complex64 chains.  T12's
+    // complex64 part is kept apart (scal[6]) for the acceptance check on the host.
     auto rgf_lu32 = [&]() -> int {
       typedef cx<float> zf;
+      constexpr int K = kLu32Cols;
       zf* F0 = (zf*)Q(y.o_f32);
       auto F = [&](long long off) { return F0 + off; };
       const long long Es32 = 2 * Es;
       const zf kOnef = {1.f, 0.f}, kZerof = {0.f, 0.f};
       auto gs = [&](int x) { return F(y.f_gs + (long long)x * nn); };
       auto zs = [&](int x) { return F(y.f_z + (long long)x * nn); };
-      const int RB = std::min(cR, kLu32Cols);  // refined columns of this batch
-      const long long ry = (long long)n * RB;  // one slice of the refinement arrays
-      auto y32 = [&](int x) { return F(y.f_y32 + (long long)x * ry); };
-      auto Yr = [&](int x) { return Q(y.o_yr + (long long)x * ry); };
+      // refined columns: RB of the last block column, then RL of the first, side by
+      // side in one iterate V (slice stride rs) solved together
+      const int RB = std::min(cR, K), RL = std::min(cL, K), R = RB + RL;
+      const long long rs = (long long)n * R;
+      auto Vr = [&](int x) { return Q(y.o_vr + (long long)x * rs); };
+      auto Yr = [&](int x) { return Vr(x); };
+      auto Zr = [&](int x) { return Vr(x) + (long long)n * RB; };
       zf* M32 = F(y.f_G);
       zf* T32 = F(y.f_Y);
-      int* rb = ln.d_active;  // per energy: min(rank of W_R, kLu32Cols)
+      zd* WLt = Q(y.o_colB);  // W_L on its truncation rank, FP64
       auto gbf = [&](const zf* A, const zf* B, zf* C, int m, int ncols, const int* kc = nullptr) {
         GemmBatch g = {};
         g.A = A; g.sA = Es32; g.lda = n;
@@ -1366,38 +1380,59 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         }
         return g;
       };
-      auto site32 = [&](int lead, int x, const zf* V32) -> int {
-        TRY(launch_siteblocks<float>(s, no, V32, Es32, V32, Es32, n, site_out(lead, x), y.site_per_e,
-                                     y.site_q, c->sp, lead ? kcR : kcL, skc, nb, nullptr));
+      // site and bond blocks of one lead over the factor columns [kf, kl), complex64
+      // (written) or FP64 (added)
+      auto blocks32 = [&](int lead, int x, const zf* V, const zf* Vn) -> int {
+        const int* kc = lead ? kcR : kcL;
+        TRY(launch_siteblocks<float>(s, no, V, Es32, V, Es32, n, site_out(lead, x), y.site_per_e, y.site_q,
+                                     c->sp, kc, skc, nb, nullptr, K, kAllCols, 0));
+        if (c->want_currents)
+          TRY(launch_bondblocks<float>(s, no, V, Es32, Vn, Es32, hx(x), c->X, n, npy, npz, site_out(lead, x),
+                                       y.site_per_e, y.site_q, c->want_pairs ? 1 : 0, kc, skc, nb, nullptr,
+                                       K, kAllCols, 0));
         return 0;
       };
-      auto bonds32 = [&](int lead, int x, const zf* V32, const zf* Vn32) -> int {
-        if (!c->want_currents) return 0;
-        TRY(launch_bondblocks<float>(s, no, V32, Es32, Vn32, Es32, hx(x), c->X, n, npy, npz,
-                                     site_out(lead, x), y.site_per_e, y.site_q, c->want_pairs ? 1 : 0,
-                                     lead ? kcR : kcL, skc, nb, nullptr));
+      auto blocks64 = [&](int lead, int x, const zd* V, const zd* Vn) -> int {
+        const int* kc = lead ? kcR : kcL;
+        TRY(launch_siteblocks<double>(s, no, V, Es, V, Es, n, site_out(lead, x), y.site_per_e, y.site_q,
+                                      c->sp, kc, skc, nb, nullptr, 0, K, 1));
+        if (c->want_currents)
+          TRY(launch_bondblocks<double>(s, no, V, Es, Vn, Es, hx(x), c->X, n, npy, npz, site_out(lead, x),
+                                        y.site_per_e, y.site_q, c->want_pairs ? 1 : 0, kc, skc, nb, nullptr,
+                                        0, K, 1));
         return 0;
       };
-      auto trace32 = [&](int lead, const zf* V32, int idx, int m, int ncols) -> int {
-        TRY(gemm_batched<float>(s, OP_C, OP_N, gbf(F(lead ? y.f_WR : y.f_WL), V32, F(y.f_P), m, ncols), nb,
-                                kOnef, kZerof));
+      // scal[idx] += |Wa' V|_F^2 over m rows of Wa and ncols columns of V
+      auto trace32 = [&](const zf* Wa, const zf* V, int idx, int m, int ncols) -> int {
+        if (m <= 0 || ncols <= 0) return 0;
+        TRY(gemm_batched<float>(s, OP_C, OP_N, gbf(Wa, V, F(y.f_P), m, ncols), nb, kOnef, kZerof));
         TRY(launch_inner<float>(s, F(y.f_P), Es32, F(y.f_P), Es32, n, m, ncols, ln.d_scal + idx, 8, nb,
-                                nullptr));
+                                nullptr, 1));
+        return 0;
+      };
+      auto trace64 = [&](const zd* Wa, const zd* V, int idx, int m, int ncols) -> int {
+        if (m <= 0 || ncols <= 0) return 0;
+        GemmBatch g = shape(gbr(Wa, Es, n, V, Es, n, Q(y.o_P), Es, n, n), m, ncols);
+        TRY(gemm_batched<double>(s, OP_C, OP_N, g, nb, kOne, kZero));
+        TRY(launch_inner<double>(s, Q(y.o_P), Es, Q(y.o_P), Es, n, m, ncols, ln.d_scal + idx, 8, nb, nullptr,
+                                 1));
         return 0;
       };
       // complex64 factors: W_L on its truncation rank (cleared past each energy's
-      // own), W_R whole
+      // own), W_R whole; and W_L's truncation in FP64
       TRY(to_c64(s, F(y.f_WL), Es32, Q(y.o_WL), Es, (long long)n * cL, nb));
       TRY(launch_trunc_cols<float>(s, F(y.f_WL), Es32, n, cL, kcL, skc, nb));
       TRY(to_c64(s, F(y.f_WR), Es32, Q(y.o_WR), Es, (long long)n * cR, nb));
-      TRY(launch_min_cols(s, rb, (const int*)(pa.st[1] + rk_off), skc, kLu32Cols, nb));
+      TRY(hipMemcpy2DAsync(WLt, sizeof(zd) * Es, Q(y.o_WL), sizeof(zd) * Es, sizeof(zd) * n * cL, nb,
+                           hipMemcpyDeviceToDevice, s));
+      TRY(launch_trunc_cols<double>(s, WLt, Es, n, cL, kcL, skc, nb));
       GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
       w32.prow = Q(y.o_prow);
       w32.sProw = Es32;
       w32.m3 = 1;
       TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nb));  // the first zero pivot stays
       int r;
-      // forward: g_x = inv(M_x) and z_x = g_x X' z_{x-1} (z_1 = g_1 W_L), complex64
+      // forward: g_x = inv(M_x) and z_x = g_x X' z_{x-1} (z_1 = g_1 W_Lt), complex64
       for (int x = 0; x < L; ++x) {
         if (x > 0) TRY((launch_convert<double, float>(s, Q(y.o_P), Es, gs(x - 1), Es32, nn, nb, nullptr)));
         if ((r = assemble(x, x > 0 ? Q(y.o_P) : nullptr, 1))) return r;
@@ -1411,7 +1446,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         }
       }
       // backward: S_x = g_x X S_{x+1} (S_L = g_L W_R), z_x += g_x X z_{x+1}, complex64;
-      // S_x's leading RB columns seed the refinement iterate Y_x
+      // their leading columns seed the refinement iterates Y_x, Z_x
       zf* Sb[2] = {F(y.f_S0), F(y.f_S1)};
       TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(L - 1), F(y.f_WR), Sb[1], n, cR), nb, kOnef, kZerof));
       zf* Scur = Sb[1];
@@ -1425,54 +1460,82 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
           TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, zs(x + 1), Es32, c->X32, n, cL, nb, nullptr));
           TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, zs(x), n, cL, kcL), nb, kOnef, kOnef));
         }
-        TRY((launch_convert<double, float>(s, Yr(x), Es, Sx, Es32, ry, nb, nullptr)));
-        if ((r = site32(1, x, Sx))) return r;
-        if ((r = bonds32(1, x, Sx, x < L - 1 ? Scur : nullptr))) return r;
-        if (x == L - 1 && (r = trace32(1, Sx, 3, cR, cR))) return r;  // Modec2, right-lead term
-        if ((r = site32(0, x, zs(x)))) return r;
-        if ((r = bonds32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
-        if (x == L - 1 && (r = trace32(1, zs(x), 2, cR, cL))) return r;  // Modec2, left-lead term
-        if (x == 0 && (r = trace32(0, zs(x), 1, cL, cL))) return r;      // Modec1, left-lead term
+        TRY((launch_convert<double, float>(s, Yr(x), Es, Sx, Es32, (long long)n * RB, nb, nullptr)));
+        TRY((launch_convert<double, float>(s, Zr(x), Es, zs(x), Es32, (long long)n * RL, nb, nullptr)));
+        if ((r = blocks32(1, x, Sx, x < L - 1 ? Scur : nullptr))) return r;
+        if ((r = blocks32(0, x, zs(x), x < L - 1 ? zs(x + 1) : nullptr))) return r;
+        // the traces' complex64 columns (those from kLu32Cols on)
+        if (x == L - 1 && (r = trace32(F(y.f_WR), Sx + (long long)K * n, 3, cR, cR - K))) return r;  // Modec2, right
+        if (x == L - 1 && (r = trace32(F(y.f_WR), zs(x) + (long long)K * n, 2, cR, cL - K))) return r;  // Modec2, left
+        if (x == 0 && (r = trace32(F(y.f_WL), zs(0) + (long long)K * n, 1, cL, cL - K))) return r;  // Modec1, left
         if (x < L - 1) {
           Scur = Sx;
           pb ^= 1;
         }
       }
-      // scal[6]: T12 over W_R's columns past each energy's rb, from complex64 S_1
-      TRY(launch_clear_lead_cols<float>(s, Scur, Es32, n, cR, rb, 1, nb));
-      TRY((launch_convert<double, float>(s, Q(y.o_colA), Es, Scur, Es32, (long long)n * cR, nb, nullptr)));
-      if ((r = trace_term(0, Q(y.o_colA), 6, cLf, cR))) return r;
-      // refinement sweeps on Y (FP64); d_ref[sweep * nb + b] = max|D|, then max|Y|
-      for (int sw = 0; sw < kLu32Sweeps; ++sw) {
-        for (int x = 0; x < L; ++x) {
-          GemmBatch g = shape(gbr(hx(x), 0, n, Yr(x), Es, n, Q(y.o_P), Es, n, n), n, RB);
-          TRY(gemm_batched<double>(s, OP_N, OP_N, g, nb, kOne, kZero));
-          if (x == 0)
-            TRY(gemm_batched<double>(s, OP_N, OP_N,
-                                     shape(gbr(Q(y.o_SigL), Es, n, Yr(0), Es, n, Q(y.o_P), Es, n, n), n, RB),
-                                     nb, kOne, kOne));
-          if (x == L - 1)
-            TRY(gemm_batched<double>(s, OP_N, OP_N,
-                                     shape(gbr(Q(y.o_SigR), Es, n, Yr(x), Es, n, Q(y.o_P), Es, n, n), n, RB),
-                                     nb, kOne, kOne));
-          TRY(launch_lu32_resid(s, no, T32, Es32, x == L - 1 ? Q(y.o_WR) : nullptr, Es,
-                                x > 0 ? Yr(x - 1) : nullptr, Yr(x), x < L - 1 ? Yr(x + 1) : nullptr, Es,
-                                Q(y.o_P), Es, hv(x), c->X, c->X32, x > 0 ? y32(x - 1) : nullptr, Es32,
-                                ln.d_energy, c->cfg.eta, n, RB, nb));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, y32(x), n, RB), nb, kOnef, kZerof));
-        }
-        for (int x = L - 2; x >= 0; --x) {
-          TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, y32(x + 1), Es32, c->X32, n, RB, nb, nullptr));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, y32(x), n, RB), nb, kOnef, kOnef));
-        }
-        TRY(launch_maxabs<float>(s, y32(0), Es32, (long long)L * ry, ln.d_ref + (size_t)sw * nb, nb, nullptr));
-        TRY(launch_add_c64(s, Yr(0), Es, y32(0), Es32, (long long)L * ry, nb));
+      // scal[6]: T12 over W_R's columns from RB on, from complex64 S_1
+      if (cR > RB) {
+        TRY((launch_convert<double, float>(s, Q(y.o_colA), Es, Scur, Es32, (long long)n * cR, nb, nullptr)));
+        if ((r = trace64(Q(y.o_WL), Q(y.o_colA) + (long long)RB * n, 6, cLf, cR - RB))) return r;
       }
-      TRY(launch_maxabs<double>(s, Yr(0), Es, (long long)L * ry, ln.d_ref + (size_t)kLu32Sweeps * nb, nb,
-                                nullptr));
-      // scal[0]: T12 over the refined columns (those past each energy's rb cleared)
-      TRY(launch_trunc_cols<double>(s, Yr(0), Es, n, RB, rb, 1, nb));
-      if ((r = trace_term(0, Yr(0), 0, cLf, RB))) return r;
+      // Refinement of A V = [E_L W_R | E_1 W_Lt] on the R columns of V.  Per sweep: the
+      // FP64 products P_x = H_x V_x for every slice (one GEMM per energy, batched over
+      // the slices) and the contact terms, the residual of every slice in complex64
+      // (T), the forward substitution u_x = g_x (T_x + X' u_{x-1}) and the back
+      // substitution u_x += g_x X u_{x+1} (skinny products on 32x32 tiles), then
+      // V += u.  d_ref[sw nb + b] = max|u| of sweep sw, then max|V| (max-entry norms
+      // over the energy's L slices).
+      {
+        auto Ts = [&](int x) { return F(y.f_z + (long long)x * rs); };
+        auto Us = [&](int x) { return F(y.f_z + (long long)L * rs + (long long)x * rs); };
+        zd* P0 = (zd*)F(y.f_z + 2LL * L * rs);  // FP64 products, slice stride rs
+        const zd* Hb = c->compact ? (const zd*)c->H0 : (const zd*)c->H;
+        const long long sH = c->compact ? 0 : nn;
+        for (int sw = 0; sw < kLu32Sweeps; ++sw) {
+          for (int b = 0; b < nb; ++b) {
+            GemmBatch g = shape(gbr(Hb, sH, n, Vr(0) + (long long)b * Es, rs, n, P0 + (long long)b * Es, rs, n,
+                                    n), n, R);
+            TRY(gemm_batched<double>(s, OP_N, OP_N, g, L, kOne, kZero));
+          }
+          TRY(gemm_batched<double>(s, OP_N, OP_N, shape(gbr(Q(y.o_SigL), Es, n, Vr(0), Es, n, P0, Es, n, n), n, R),
+                                   nb, kOne, kOne));
+          TRY(gemm_batched<double>(s, OP_N, OP_N,
+                                   shape(gbr(Q(y.o_SigR), Es, n, Vr(L - 1), Es, n, P0 + (long long)(L - 1) * rs, Es,
+                                             n, n), n, R),
+                                   nb, kOne, kOne));
+          TRY(launch_lu32_resid(s, no, Ts(0), Es32, Vr(0), Es, P0, Es, rs, c->compact ? c->V : nullptr, ns,
+                                c->X, Q(y.o_WR), WLt, Es, ln.d_energy, c->cfg.eta, n, R, RB, L, nb));
+          {
+          struct SkinnyTiles {  // 32x32 tiles for these launches, restored on every exit
+            SkinnyTiles() { force_gemm_tile(1); }
+            ~SkinnyTiles() { force_gemm_tile(-1); }
+          } skinny;
+          for (int x = 0; x < L; ++x) {
+            if (x > 0) {
+              TRY(launch_bd_mul<float>(s, no, 1, Ts(x), Es32, Us(x - 1), Es32, c->X32, n, R, nb, nullptr, 1));
+            }
+            TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), Ts(x), Us(x), n, R), nb, kOnef, kZerof));
+          }
+          for (int x = L - 2; x >= 0; --x) {
+            TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, Us(x + 1), Es32, c->X32, n, R, nb, nullptr));
+            TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, Us(x), n, R), nb, kOnef, kOnef));
+          }
+          }
+          TRY(launch_maxabs<float>(s, Us(0), Es32, (long long)L * rs, ln.d_ref + (size_t)sw * nb, nb, nullptr));
+          TRY(launch_add_c64(s, Vr(0), Es, Us(0), Es32, (long long)L * rs, nb));
+        }
+        TRY(launch_maxabs<double>(s, Vr(0), Es, (long long)L * rs, ln.d_ref + (size_t)kLu32Sweeps * nb, nb,
+                                  nullptr));
+      }
+      // the refined columns' site and bond blocks and traces
+      for (int x = 0; x < L; ++x) {
+        if ((r = blocks64(1, x, Yr(x), x < L - 1 ? Yr(x + 1) : nullptr))) return r;
+        if ((r = blocks64(0, x, Zr(x), x < L - 1 ? Zr(x + 1) : nullptr))) return r;
+      }
+      if ((r = trace64(Q(y.o_WL), Yr(0), 0, cLf, RB))) return r;     // T12
+      if ((r = trace64(WLt, Zr(0), 1, cL, RL))) return r;            // Modec1, left-lead term
+      if ((r = trace64(Q(y.o_WR), Zr(L - 1), 2, cR, RL))) return r;  // Modec2, left-lead term
+      if ((r = trace64(Q(y.o_WR), Yr(L - 1), 3, cR, RB))) return r;  // Modec2, right-lead term
       return 0;
     };
     int r;
@@ -1794,9 +1857,9 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   if ((rr = collect(c, ln, nb, iters, sst, res))) return rr;
   if (prec != NEGF_PREC_LU32) return 0;
   // LU32: accept an energy's T12 (refined part + complex64 part) when its complex64
-  // block inverses had no zero pivot, its last refinement correction is at most
-  // kLu32Accept of the solution and the complex64 part is at most kLu32Tail of
-  // max(1, T12); otherwise solve it again in the plain FP64 form.
+  // block inverses had no zero pivot, its refinement converged (the estimate of the
+  // error left, from the last two corrections) and the complex64 part is small
+  // (kLu32Tail); otherwise solve it again in the plain FP64 form.
   std::vector<double> nrm((size_t)(kLu32Sweeps + 1) * nb);
   std::vector<int> ic((size_t)y.Is * nb);
   TRY(hipMemcpyAsync(nrm.data(), ln.d_ref, sizeof(double) * nrm.size(), hipMemcpyDeviceToHost, s));
@@ -1806,8 +1869,13 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   std::vector<double> eesc;
   for (int b = 0; b < nb; ++b) {
     const double tb = res[b].scal[0], ts = res[b].scal[6];
-    const double dlast = nrm[(size_t)(kLu32Sweeps - 1) * nb + b], ymax = nrm[(size_t)kLu32Sweeps * nb + b];
-    const bool ok = ic[(size_t)b * y.Is + 2 * n] == 0 && dlast <= kLu32Accept * ymax &&
+    auto conv = [&](int base) {  // the last two corrections and the solution of one system
+      const double d0 = nrm[(size_t)(base + kLu32Sweeps - 2) * nb + b];
+      const double d1 = nrm[(size_t)(base + kLu32Sweeps - 1) * nb + b];
+      const double vmax = nrm[(size_t)(base + kLu32Sweeps) * nb + b];
+      return d1 <= kLu32Floor * vmax || (d1 <= kLu32Contract * d0 && d1 * d1 <= kLu32Accept * vmax * d0);
+    };
+    const bool ok = ic[(size_t)b * y.Is + 2 * n] == 0 && conv(0) &&
                     ts <= kLu32Tail * std::max(1.0, std::fabs(tb)) && std::isfinite(tb);
     if (ok) {
       res[b].scal[0] = tb + ts;

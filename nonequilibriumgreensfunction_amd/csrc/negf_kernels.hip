@@ -82,10 +82,10 @@ __global__ void k_assemble(cx<T>* M, long long sM, const cx<T>* H, int add_diag,
   M[(long long)b * sM + e] = v;
 }
 
-// Y_b = X G_b (CONJ = 0) or X' G_b (CONJ = 1), X site-block-diagonal.
+// Y_b = X G_b (CONJ = 0) or X' G_b (CONJ = 1), X site-block-diagonal; accum: Y_b +=.
 template <typename T, int NORB, int CONJ>
 __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, const cx<T>* X,
-                         int n, const int* active) {
+                         int n, const int* active, int accum) {
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
   const int col = blockIdx.y;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,13 +100,14 @@ __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, c
     else
       v = cadd(v, cmul(X[(long long)(a0 + q) * n + a0 + r], Gb[a0 + q]));
   }
-  Y[(long long)b * sY + (long long)col * n + a] = v;
+  cx<T>& y = Y[(long long)b * sY + (long long)col * n + a];
+  y = accum ? cadd(y, v) : v;
 }
 
-// Site-diagonal blocks D_s = (B G')[site s], over the first kc[b * skc] columns of
-// B and G (all n when kc is null).
-// Writes, per site s of slice x:  out[0] = Re tr D,  out[1+a] = Re tr(D sigma_a),
-// out[4+r] = Re D_rr (r < NORB).  out layout: out_b + q*qstride + s.
+// Site-diagonal blocks D_s = (B G')[site s], over the columns [kfirst, min(kc[b * skc],
+// klast)) of B and G (kc null: n).
+// Writes (accum: adds), per site s of slice x:  out[0] = Re tr D,  out[1+a] =
+// Re tr(D sigma_a), out[4+r] = Re D_rr (r < NORB).  out layout: out_b + q*qstride + s.
 // A workgroup takes 16 rows (16 / NORB sites) and splits the sum over k into 16
 // chunks, one per 16-thread row group: with 16 rows per workgroup there are n / 16
 // workgroups per matrix (144 at n = 2304), enough to spread the stream of B and G
@@ -117,10 +118,11 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
                                                     const cx<T>* G, long long sG, int n,
                                                     double* out, long long sOut, long long qstride,
                                                     SpinMats sp, const int* kc, long long skc,
-                                                    const int* active) {
+                                                    const int* active, int kfirst, int klast,
+                                                    int accum) {
   constexpr int RW = 16, KQ = 16;
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
-  const int kcols = kc ? kc[(long long)b * skc] : n;  // columns of B and G taken
+  const int kcols = min(kc ? kc[(long long)b * skc] : n, klast);  // columns of B and G taken
   const cx<T>* Bb = Bm + (long long)b * sB;
   const cx<T>* Gb = G + (long long)b * sG;
   const int row0 = blockIdx.x * RW;
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
   if (a < n) {
     // the chunks are fixed by n, so the sum order does not depend on kcols
     const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
-    for (int k = kbeg; k < (kend < kcols ? kend : kcols); ++k) {
+    for (int k = max(kbeg, kfirst); k < (kend < kcols ? kend : kcols); ++k) {
       const cx<T> bv = Bb[(long long)k * n + a];
 #pragma unroll
       for (int q = 0; q < NORB; ++q) {
@@ -171,13 +173,14 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
           di[r][q] = xi;
         }
       double* o = out + (long long)b * sOut + s;
+      auto put = [&](long long off, double v) { o[off] = accum ? o[off] + v : v; };
       double trace = 0.0;
 #pragma unroll
       for (int r = 0; r < NORB; ++r) {
         trace += dr[r][r];
-        o[(4 + r) * qstride] = dr[r][r];  // per orbital (the orbital-resolved density)
+        put((4 + r) * qstride, dr[r][r]);  // per orbital (the orbital-resolved density)
       }
-      o[0] = trace;
+      put(0, trace);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         double acc = 0.0;
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 #pragma unroll
           for (int q = 0; q < NORB; ++q)
             acc += dr[r][q] * sp.re[m][q][r] - di[r][q] * sp.im[m][q][r];
-        o[(m + 1) * qstride] = acc;
+        put((m + 1) * qstride, acc);
       }
     }
   }
@@ -204,18 +207,19 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 // summed over the orbital pairs at out_b + (8 + d) * qstride + s, d = 0 x, 1 y, 2 z,
 // and, when `pairs`, per orbital pair at out_b + (11 + 16 d + NORB ii + jj) * qstride + s
 // (the orbital-resolved currents); 0 where the bond does not exist.
-// Reduction as in k_siteblocks (over the first kc[b * skc] columns): 16 rows per
-// workgroup, the k sum in 16 chunks added in chunk order, so results do not depend
-// on the batch.
+// Reduction as in k_siteblocks (over the columns [kfirst, min(kc[b * skc], klast)),
+// written or, with accum, added): 16 rows per workgroup, the k sum in 16 chunks
+// added in chunk order, so results do not depend on the batch.
 template <typename T, int NORB>
 __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV, const cx<T>* Vn,
                                                     long long sVn, const double2* H, const double2* X,
                                                     int n, int npy, int npz, double* out,
                                                     long long sOut, long long qstride, int pairs,
-                                                    const int* kc, long long skc, const int* active) {
+                                                    const int* kc, long long skc, const int* active,
+                                                    int kfirst, int klast, int accum) {
   constexpr int RW = 16, KQ = 16, NQ = 3 * NORB;
   const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
-  const int kcols = kc ? kc[(long long)b * skc] : n;  // columns of V and Vn taken
+  const int kcols = min(kc ? kc[(long long)b * skc] : n, klast);  // columns of V and Vn taken
   const cx<T>* Vb = V + (long long)b * sV;
   const cx<T>* Vnb = Vn ? Vn + (long long)b * sVn : nullptr;
   const int NN = NORB * npy;
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
   for (int q = 0; q < NQ; ++q) accr[q] = acci[q] = 0.0;
   if (a < n) {
     const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
-    for (int k = kbeg; k < (kend < kcols ? kend : kcols); ++k) {
+    for (int k = max(kbeg, kfirst); k < (kend < kcols ? kend : kcols); ++k) {
       const long long col = (long long)k * n;
       const cx<T> va = Vb[col + a];
 #pragma unroll
@@ -296,22 +300,26 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
               val = -im;
             }
             sum += val;
-            if (pairs) o[(11 + 16 * d + NORB * ii + jj) * qstride] = val;
+            if (pairs) {
+              double& op = o[(11 + 16 * d + NORB * ii + jj) * qstride];
+              op = accum ? op + val : val;
+            }
           }
-        o[(8 + d) * qstride] = sum;
+        double& os = o[(8 + d) * qstride];
+        os = accum ? os + sum : sum;
       }
     }
   }
 }
 
-// out[b*sOut] = Re sum_{ij} P_ij conj(Q_ij) over the leading m x ncols part of
-// blocks with leading dimension ld (one workgroup).  Element (i, j) goes to thread
+// out[b*sOut] = Re sum_{ij} P_ij conj(Q_ij) (accum: +=) over the leading m x ncols
+// part of blocks with leading dimension ld (one workgroup).  Element (i, j) goes to thread
 // (j ld + i) mod 1024 whatever m and ncols are, so entries that are zero leave the
 // sum's bits unchanged when m or ncols grow.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_inner(const cx<T>* P, long long sP, const cx<T>* Q,
                                                 long long sQ, int ld, int m, int ncols, double* out,
-                                                long long sOut, const int* active) {
+                                                long long sOut, const int* active, int accum) {
   const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
   const cx<T>* Pb = P + (long long)b * sP;
   const cx<T>* Qb = Q + (long long)b * sQ;
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(1024) void k_inner(const cx<T>* P, long long sP, co
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < 16; ++w) t += red[w];
-    out[(long long)b * sOut] = t;
+    out[(long long)b * sOut] = accum ? out[(long long)b * sOut] + t : t;
   }
 }
 
@@ -676,78 +684,70 @@ __global__ void k_convert(cx<Td>* Y, long long sY, const cx<Ts>* X, long long sX
 }
 
 // ---------------------------------------------------------------------------
-// Iterative refinement of the block-tridiagonal solve A Y = B (A = zI - H - Sigma
+// Iterative refinement of the block-tridiagonal solve A V = B (A = zI - H - Sigma
 // with the slice couplings -X, -X'; Recursive_Greens.F90:6) against complex64
-// block-LU factors.  One row of slices at a time, element (i, j) of energy b:
-//   R = B_x - A Y |_x = B_x - z Y_x + P + X Y_{x+1} + X' Y_{x-1}
-// in FP64, with P = (H_x + Sigma_x) Y_x from the FP64 GEMM (plus V_x Y_x for a
-// compact Hamiltonian), and written in complex64 together with the forward
-// substitution's coupling to the previous slice:
-//   T = c64(R) + X32' y_{x-1}.
-// X is site-block-diagonal (NORB x NORB blocks).  Null B, Ym, Yp, ym, V are zero.
+// block-LU factors: the residual of every slice of an R-column iterate in one
+// launch.  Element (i, j) of slice x of energy b:
+//   T = c64(B_x - z V_x + P_x + X V_{x+1} + X' V_{x-1})
+// in FP64, with P_x = (H_x + Sigma_x) V_x from the FP64 GEMM (plus U_x V_x for a
+// compact Hamiltonian's site potentials U_x).  B holds the two block columns' right
+// sides: Wr (columns j < RB) on the last slice, Wl (columns RB.. as Wl's 0..) on
+// the first.  V, P, T share the slice stride sS and have energy strides sV, sP, sT.
+// X is site-block-diagonal (NORB x NORB blocks).
 template <int NORB>
-__global__ void k_lu32_resid(cx<float>* T, long long sT, const double2* B, long long sB,
-                             const double2* Ym, const double2* Y0, const double2* Yp, long long sY,
-                             const double2* P, long long sP, const double* V, const double2* X,
-                             const cx<float>* X32, const cx<float>* ym, long long sy,
-                             const double* energy, double eta, int n) {
-  const int b = blockIdx.z, j = blockIdx.y;
+__global__ void k_lu32_resid(cx<float>* T, long long sT, const double2* V, long long sV,
+                             const double2* P, long long sP, long long sS, const double* U,
+                             long long sU, const double2* X, const double2* Wr, const double2* Wl,
+                             long long sW, const double* energy, double eta, int n, int RB, int L) {
+  const int b = blockIdx.z / L, x = blockIdx.z - b * L, j = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long long e = (long long)j * n + i;
   const int base = (i / NORB) * NORB;
-  const double2 y0 = Y0[(long long)b * sY + e];
+  const double2* Vx = V + (long long)b * sV + (long long)x * sS;
+  const double2 v0 = Vx[e];
   const double2 z = mk<double>(energy[b], eta);
-  double2 r = cadd(P[(long long)b * sP + e], cneg(cmul(z, y0)));
-  if (B) r = cadd(r, B[(long long)b * sB + e]);
-  if (V) {
-    const double v = V[i / NORB];
-    r.x += v * y0.x;
-    r.y += v * y0.y;
+  double2 r = cadd(P[(long long)b * sP + (long long)x * sS + e], cneg(cmul(z, v0)));
+  if (U) {
+    const double u = U[(long long)x * sU + i / NORB];
+    r.x += u * v0.x;
+    r.y += u * v0.y;
   }
-  const long long col = (long long)b * sY + (long long)j * n + base;
+  if (x == L - 1 && j < RB) r = cadd(r, Wr[(long long)b * sW + e]);
+  if (x == 0 && j >= RB) r = cadd(r, Wl[(long long)b * sW + (long long)(j - RB) * n + i]);
+  const long long col = (long long)j * n + base;
 #pragma unroll
   for (int k = 0; k < NORB; ++k) {
-    if (Yp) r = cadd(r, cmul(X[(long long)(base + k) * n + i], Yp[col + k]));
-    if (Ym) r = cadd(r, cmul(cconj(X[(long long)i * n + base + k]), Ym[col + k]));
+    if (x < L - 1) r = cadd(r, cmul(X[(long long)(base + k) * n + i], Vx[sS + col + k]));
+    if (x > 0) r = cadd(r, cmul(cconj(X[(long long)i * n + base + k]), Vx[-sS + col + k]));
   }
-  cx<float> t = mk<float>((float)r.x, (float)r.y);
-  if (ym) {
-    const long long c32 = (long long)b * sy + (long long)j * n + base;
-#pragma unroll
-    for (int k = 0; k < NORB; ++k) t = cadd(t, cmul(cconj(X32[(long long)i * n + base + k]), ym[c32 + k]));
-  }
-  T[(long long)b * sT + e] = t;
+  T[(long long)b * sT + (long long)x * sS + e] = mk<float>((float)r.x, (float)r.y);
 }
 
 template <int NORB>
-static hipError_t lu32_resid_n(hipStream_t s, cx<float>* T, long long sT, const double2* B,
-                               long long sB, const double2* Ym, const double2* Y0, const double2* Yp,
-                               long long sY, const double2* P, long long sP, const double* V,
-                               const double2* X, const cx<float>* X32, const cx<float>* ym,
-                               long long sy, const double* energy, double eta, int n, int cols,
-                               int batch) {
-  hipLaunchKernelGGL((k_lu32_resid<NORB>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, T, sT,
-                     B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n);
+static hipError_t lu32_resid_n(hipStream_t s, cx<float>* T, long long sT, const double2* V, long long sV,
+                               const double2* P, long long sP, long long sS, const double* U,
+                               long long sU, const double2* X, const double2* Wr, const double2* Wl,
+                               long long sW, const double* energy, double eta, int n, int R, int RB,
+                               int L, int batch) {
+  hipLaunchKernelGGL((k_lu32_resid<NORB>), dim3((n + 255) / 256, R, batch * L), dim3(256), 0, s, T, sT, V,
+                     sV, P, sP, sS, U, sU, X, Wr, Wl, sW, energy, eta, n, RB, L);
   return hipGetLastError();
 }
 
-hipError_t launch_lu32_resid(hipStream_t s, int norb, cx<float>* T, long long sT, const double2* B,
-                             long long sB, const double2* Ym, const double2* Y0, const double2* Yp,
-                             long long sY, const double2* P, long long sP, const double* V,
-                             const double2* X, const cx<float>* X32, const cx<float>* ym,
-                             long long sy, const double* energy, double eta, int n, int cols,
+hipError_t launch_lu32_resid(hipStream_t s, int norb, cx<float>* T, long long sT, const double2* V,
+                             long long sV, const double2* P, long long sP, long long sS, const double* U,
+                             long long sU, const double2* X, const double2* Wr, const double2* Wl,
+                             long long sW, const double* energy, double eta, int n, int R, int RB, int L,
                              int batch) {
-  if (batch <= 0 || cols <= 0) return hipSuccess;
+  if (batch <= 0 || R <= 0) return hipSuccess;
+  if ((long long)batch * L > 65535) return hipErrorInvalidValue;
   if (norb == 1)
-    return lu32_resid_n<1>(s, T, sT, B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n,
-                           cols, batch);
+    return lu32_resid_n<1>(s, T, sT, V, sV, P, sP, sS, U, sU, X, Wr, Wl, sW, energy, eta, n, R, RB, L, batch);
   if (norb == 2)
-    return lu32_resid_n<2>(s, T, sT, B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n,
-                           cols, batch);
+    return lu32_resid_n<2>(s, T, sT, V, sV, P, sP, sS, U, sU, X, Wr, Wl, sW, energy, eta, n, R, RB, L, batch);
   if (norb == 4)
-    return lu32_resid_n<4>(s, T, sT, B, sB, Ym, Y0, Yp, sY, P, sP, V, X, X32, ym, sy, energy, eta, n,
-                           cols, batch);
+    return lu32_resid_n<4>(s, T, sT, V, sV, P, sP, sS, U, sU, X, Wr, Wl, sW, energy, eta, n, R, RB, L, batch);
   return hipErrorInvalidValue;
 }
 
@@ -769,19 +769,6 @@ hipError_t launch_add_c64(hipStream_t s, double2* Y, long long sY, const cx<floa
   long long blocks = (nn + 255) / 256;
   if (blocks > 2048) blocks = 2048;
   hipLaunchKernelGGL(k_add_c64, dim3((unsigned)blocks, batch), dim3(256), 0, s, Y, sY, D, sD, nn);
-  return hipGetLastError();
-}
-
-// out[b] = min(in[b * sIn], cap): per-matrix column counts capped (device-side, so
-// the launch sequence that uses them needs no host read-back).
-__global__ void k_min_cols(int* out, const int* in, long long sIn, int cap, int batch) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < batch) out[b] = min(in[(long long)b * sIn], cap);
-}
-
-hipError_t launch_min_cols(hipStream_t s, int* out, const int* in, long long sIn, int cap, int batch) {
-  if (batch <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_min_cols, dim3((batch + 63) / 64), dim3(64), 0, s, out, in, sIn, cap, batch);
   return hipGetLastError();
 }
 
@@ -994,12 +981,12 @@ hipError_t launch_assemble(hipStream_t s, int norb, int mode, cx<T>* M, long lon
 template <typename T>
 hipError_t launch_bd_mul(hipStream_t s, int norb, int conj, cx<T>* Y, long long sY,
                          const cx<T>* G, long long sG, const cx<T>* X, int n, int cols, int batch,
-                         const int* active) {
+                         const int* active, int accum) {
   if (batch <= 0 || cols <= 0) return hipSuccess;
   dim3 grid((n + 255) / 256, cols, batch);  // the first `cols` columns of G
 #define NEGF_BD(NO, CJ)                                                                       \
   if (norb == NO && conj == CJ) {                                                             \
-    hipLaunchKernelGGL((k_bd_mul<T, NO, CJ>), grid, dim3(256), 0, s, Y, sY, G, sG, X, n, active); \
+    hipLaunchKernelGGL((k_bd_mul<T, NO, CJ>), grid, dim3(256), 0, s, Y, sY, G, sG, X, n, active, accum); \
     return hipGetLastError();                                                                 \
   }
   NEGF_BD(1, 0) NEGF_BD(1, 1) NEGF_BD(2, 0) NEGF_BD(2, 1) NEGF_BD(4, 0) NEGF_BD(4, 1)
@@ -1027,13 +1014,14 @@ template <typename T>
 hipError_t launch_bondblocks(hipStream_t s, int norb, const cx<T>* V, long long sV, const cx<T>* Vn,
                              long long sVn, const double2* H, const double2* X, int n, int npy,
                              int npz, double* out, long long sOut, long long qstride, int pairs,
-                             const int* kc, long long skc, int batch, const int* active) {
+                             const int* kc, long long skc, int batch, const int* active, int kfirst,
+                             int klast, int accum) {
   if (batch <= 0) return hipSuccess;
   dim3 grid((n + 15) / 16, 1, batch);
 #define NEGF_BB(NO)                                                                                \
   if (norb == NO) {                                                                                \
     hipLaunchKernelGGL((k_bondblocks<T, NO>), grid, dim3(256), 0, s, V, sV, Vn, sVn, H, X, n, npy, npz, \
-                       out, sOut, qstride, pairs, kc, skc, active);                                \
+                       out, sOut, qstride, pairs, kc, skc, active, kfirst, klast, accum);          \
     return hipGetLastError();                                                                      \
   }
   NEGF_BB(1) NEGF_BB(2) NEGF_BB(4)
@@ -1045,13 +1033,13 @@ template <typename T>
 hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long sB,
                              const cx<T>* G, long long sG, int n, double* out, long long sOut,
                              long long qstride, const SpinMats& sp, const int* kc, long long skc,
-                             int batch, const int* active) {
+                             int batch, const int* active, int kfirst, int klast, int accum) {
   if (batch <= 0) return hipSuccess;
   dim3 grid((n + 15) / 16, 1, batch);
 #define NEGF_SB(NO)                                                                            \
   if (norb == NO) {                                                                            \
     hipLaunchKernelGGL((k_siteblocks<T, NO>), grid, dim3(256), 0, s, Bm, sB, G, sG, n, out, sOut, \
-                       qstride, sp, kc, skc, active);                                          \
+                       qstride, sp, kc, skc, active, kfirst, klast, accum);                    \
     return hipGetLastError();                                                                  \
   }
   NEGF_SB(1) NEGF_SB(2) NEGF_SB(4)
@@ -1062,24 +1050,20 @@ hipError_t launch_siteblocks(hipStream_t s, int norb, const cx<T>* Bm, long long
 template <typename T>
 hipError_t launch_inner(hipStream_t s, const cx<T>* P, long long sP, const cx<T>* Q, long long sQ,
                         int ld, int m, int ncols, double* out, long long sOut, int batch,
-                        const int* active) {
+                        const int* active, int accum) {
   if (batch <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_inner<T>), dim3(batch), dim3(1024), 0, s, P, sP, Q, sQ, ld, m, ncols, out,
-                     sOut, active);
+                     sOut, active, accum);
   return hipGetLastError();
 }
 
-// Y_b(:, kc_b : cols) = 0 (kc_b = kc[b * skc]): a factor product truncated per
-// matrix; with `lead`, the leading columns Y_b(:, 0 : kc_b) are cleared instead.
+// Y_b(:, kc_b : cols) = 0 (kc_b = kc[b * skc]): a factor product truncated per matrix.
 template <typename T>
-__global__ void k_trunc_cols(cx<T>* Y, long long sY, int n, int cols, const int* kc, long long skc,
-                             int lead) {
+__global__ void k_trunc_cols(cx<T>* Y, long long sY, int n, int cols, const int* kc, long long skc) {
   const int b = blockIdx.z;
   const int col = blockIdx.y;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n || col >= cols) return;
-  const int k = kc[(long long)b * skc];
-  if (lead ? col >= k : col < k) return;
+  if (a >= n || col >= cols || col < kc[(long long)b * skc]) return;
   Y[(long long)b * sY + (long long)col * n + a] = mk<T>(0, 0);
 }
 
@@ -1088,16 +1072,7 @@ hipError_t launch_trunc_cols(hipStream_t s, cx<T>* Y, long long sY, int n, int c
                              long long skc, int batch) {
   if (batch <= 0 || cols <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_trunc_cols<T>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, Y, sY, n,
-                     cols, kc, skc, 0);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_clear_lead_cols(hipStream_t s, cx<T>* Y, long long sY, int n, int cols,
-                                  const int* kc, long long skc, int batch) {
-  if (batch <= 0 || cols <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_trunc_cols<T>), dim3((n + 255) / 256, cols, batch), dim3(256), 0, s, Y, sY, n,
-                     cols, kc, skc, 1);
+                     cols, kc, skc);
   return hipGetLastError();
 }
 
@@ -1173,22 +1148,20 @@ hipError_t launch_convert(hipStream_t s, cx<Td>* Y, long long sY, const cx<Ts>* 
                                          long long, const cx<T>*, int, int, const int*,            \
                                          const double*);                                           \
   template hipError_t launch_bd_mul<T>(hipStream_t, int, int, cx<T>*, long long, const cx<T>*,     \
-                                       long long, const cx<T>*, int, int, int, const int*);        \
+                                       long long, const cx<T>*, int, int, int, const int*, int);   \
   template hipError_t launch_siteblocks<T>(hipStream_t, int, const cx<T>*, long long,              \
                                            const cx<T>*, long long, int, double*, long long,       \
                                            long long, const SpinMats&, const int*, long long, int,  \
-                                           const int*);                                            \
+                                           const int*, int, int, int);                             \
   template hipError_t launch_bondblocks<T>(hipStream_t, int, const cx<T>*, long long, const cx<T>*, \
                                            long long, const double2*, const double2*, int, int,    \
                                            int, double*, long long, long long, int, const int*,    \
-                                           long long, int, const int*);                            \
+                                           long long, int, const int*, int, int, int);             \
   template hipError_t launch_inner<T>(hipStream_t, const cx<T>*, long long, const cx<T>*,          \
                                       long long, int, int, int, double*, long long, int,           \
-                                      const int*);                                                 \
+                                      const int*, int);                                            \
   template hipError_t launch_trunc_cols<T>(hipStream_t, cx<T>*, long long, int, int, const int*,   \
                                            long long, int);                                        \
-  template hipError_t launch_clear_lead_cols<T>(hipStream_t, cx<T>*, long long, int, int,          \
-                                                const int*, long long, int);                       \
   template hipError_t launch_eye<T>(hipStream_t, cx<T>*, long long, int, int, const int*);         \
   template hipError_t launch_add<T>(hipStream_t, cx<T>*, long long, const cx<T>*, long long,       \
                                     const cx<T>*, long long, long long, int, const int*);          \

@@ -35,7 +35,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 size_t lane_bytes_per_energy(const Layout& y) {
   return align256(sizeof(zd) * (size_t)y.Es) + align256(sizeof(zd) * 2 * (size_t)y.Cs) +
          align256(sizeof(int) * (size_t)y.Is) + align256(sizeof(int) * 2 * (size_t)y.Is) +
-         align256(sizeof(double) * (size_t)y.site_per_e) + 4 * 256 + align256(sizeof(double) * 8) +
+         align256(sizeof(double) * (size_t)y.site_per_e) + 3 * 256 + 2 * align256(sizeof(double) * 8) +
          align256(sizeof(int) * 4);
 }
 
@@ -56,7 +56,7 @@ void bind_lane(Lane& ln, const Layout& y, int nb, char* pool) {
   ln.d_energy = (double*)take(sizeof(double) * nb);
   ln.d_scal = (double*)take(sizeof(double) * 8 * nb);
   ln.d_mc = (double*)take(sizeof(double) * 2 * nb);
-  ln.d_ref = (double*)take(sizeof(double) * 4 * nb);
+  ln.d_ref = (double*)take(sizeof(double) * 8 * nb);
   ln.d_active = (int*)take(sizeof(int) * 4 * nb);
   ln.cap = nb;
 }
