@@ -51,18 +51,21 @@ extern "C" {
                                a block whose complex64 inverse is singular or whose residual does
                                not contract is inverted again in FP64 (escalation) */
 #define NEGF_PREC_LU32 3 /* block LU in complex64: every RGF block inverse (Gauss-Jordan on
-                            FP32 MFMA), both column chains and the site blocks in complex64;
-                            FP64 contacts.  T12 = |W_L' G_1L W_R|^2 (Gamma_R = W_R W_R') from
-                            the leading columns of W_R (at most 64) solved to FP64 accuracy by
-                            iterative refinement of the block-tridiagonal solve A Y = E_L W_R
+                            FP32 MFMA) and both column chains in complex64; FP64 contacts.
+                            The leading 64 columns of both block columns' factor products
+                            (G_xL W_R, G_x1 W_L with Gamma = W W') are solved to FP64
+                            accuracy by iterative refinement of the block-tridiagonal system
                             (FP64 residuals, corrections through the complex64 block LU, two
-                            sweeps), plus the remaining columns' complex64 contribution.  An
-                            energy whose complex64 inverse is singular, whose refinement does not
-                            contract to an estimated error of 1e-10 of the solution, or whose
-                            complex64 part of T12 is above 1e-3 of max(1, T12) is solved again
-                            in the plain FP64 form (escalation).  Needs a site-block-diagonal
-                            HopX (otherwise every energy takes the plain FP64 form) and the
-                            two-pass RGF. */
+                            sweeps); every output sums those refined columns with the other
+                            columns' complex64 values.  T12's part beyond W_R's leading
+                            columns comes from a complex64 first-row chain and a Gaussian
+                            sketch (32 columns) of both factors' tails.  An energy whose
+                            complex64 inverse is singular, whose refinement does not contract
+                            to an estimated error of 1e-10 of the solution, or whose T12 parts
+                            beyond the refined columns exceed 1e-3 (right tail) or 1e-9 (both
+                            tails) of max(1, T12) is solved again in the plain FP64 form
+                            (escalation).  Needs a site-block-diagonal HopX (otherwise every
+                            energy takes the plain FP64 form) and the two-pass RGF. */
 
 typedef struct negf_ctx negf_ctx;
 

@@ -140,6 +140,14 @@ constexpr int kRefineSteps = 2;
 // is solved again in the plain FP64 form.
 constexpr int kLu32Cols = 64, kLu32Sweeps = 2;
 constexpr double kLu32Contract = 0.1, kLu32Accept = 1e-10, kLu32Floor = 1e-13, kLu32Tail = 1e-3;
+// T12's part from the tails of both factors (columns past kLu32Cols of W_L and of
+// W_R) is estimated from kLu32Sketch Gaussian combinations of W_R's tail columns
+// (relative standard error ~ kLu32Sketch^-1/2); the estimate must stay below
+// kLu32TailTail of max(1, T12), which bounds its error near 1e-10 of that scale.
+// (Measured on a five-slice block-2304 wire, the worst case for evanescent tails:
+// 1e-11 to 1.5e-10 of the scale.)
+constexpr int kLu32Sketch = 32;
+constexpr double kLu32TailTail = 1e-9;
 // Arena layout kind of a precision form (Layout::init).
 static int layout_form(int precision) {
   return precision == NEGF_PREC_FP64 ? 0 : precision == NEGF_PREC_LU32 ? 2 : 1;
@@ -157,11 +165,12 @@ struct Layout {
   // mixed-precision forms: complex64 region (offsets in complex64 elements from o_f32)
   long long o_f32, f_z, f_g, f_G, f_Y, f_P, f_WL, f_WR;
   // NEGF_PREC_LU32: the complex64 block inverses (f_gs, L blocks), the last-column
-  // chain's two buffers (f_S0, f_S1) and the refinement's FP64 iterate (o_vr: L
+  // chain's two buffers (f_S0, f_S1), the first-row chain's two (f_R0, f_R1: n x
+  // kLu32Cols), W_L whole in complex64 (f_g) and the refinement's FP64 iterate (o_vr: L
   // slices of n x 2 kLu32Cols, the last and the first block column side by side).
   // The refinement's complex64 residuals and solves and its FP64 products reuse the
   // first-column chain's region f_z (dead by then), sized to hold them.
-  long long f_gs, f_S0, f_S1, o_vr;
+  long long f_gs, f_S0, f_S1, f_R0, f_R1, o_vr;
   int form;  // 0 plain FP64, 1 mixed / refined, 2 LU32
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow;
@@ -192,8 +201,9 @@ struct Layout {
     f_g = f_z + zsz; f_G = f_g + nn; f_Y = f_G + nn; f_P = f_Y + nn;
     f_WL = f_P + nn; f_WR = f_WL + nn;
     f_S0 = f_WR + nn; f_S1 = f_S0 + nn;
+    f_R0 = f_S1 + nn; f_R1 = f_R0 + (long long)n * kLu32Cols;
     if (form == 1) o += (f_WR + nn + 1) / 2;  // two complex64 per complex128 slot
-    if (form == 2) o += (f_S1 + nn + 1) / 2;
+    if (form == 2) o += (f_R1 + (long long)n * kLu32Cols + 1) / 2;
     o_vr = o; if (form == 2) o += ry;
     o_M = o; o += nn;
     o_colA = o; o += nn;
@@ -334,6 +344,7 @@ struct negf_ctx {
   zd* betaR = nullptr;   // -HopX
   zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
   cx<float>* X32 = nullptr;  // HopX in complex64 (mixed-precision form)
+  cx<float>* omega = nullptr;  // LU32: the sketch matrix (n x kLu32Sketch)
   SpinMats sp;
   Layout lay;
   Lane lanes[kMaxLanes];
@@ -483,6 +494,7 @@ extern "C" int negf_finalize(negf_ctx* c) {
   free_arena(c);
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
   hipFree(c->X32);
+  hipFree(c->omega);
   hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
   hipFree(c->cc.buf);
   for (Lane& l : c->lanes)
@@ -536,6 +548,27 @@ static int upload_couplings(negf_ctx* c, const double* HopX, const double* gM) {
     if (!c->X32) HIPCHK(hipMalloc(&c->X32, sizeof(cx<float>) * nn));
     HIPCHK(to_c64(c->st, c->X32, 0, c->X, 0, nn, 1));
     HIPCHK(hipStreamSynchronize(c->st));
+  }
+  if (c->cfg.precision == NEGF_PREC_LU32 && !c->omega) {
+    // the LU32 form's sketch of W_R's tail columns: n x kLu32Sketch complex
+    // Gaussian entries of unit variance (Box-Muller on a SplitMix64 stream, fixed
+    // seed), so the same wire gives the same bits on every run
+    const size_t cnt = (size_t)n * kLu32Sketch;
+    std::vector<cx<float>> om(cnt);
+    unsigned long long st = 0x4c5533325345544ull;
+    auto next = [&]() {
+      unsigned long long z = (st += 0x9e3779b97f4a7c15ull);
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      z ^= z >> 31;
+      return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);  // (0, 1)
+    };
+    for (size_t k = 0; k < cnt; ++k) {
+      const double r = std::sqrt(-std::log(next())), t = 2.0 * kPi * next();
+      om[k] = mk<float>((float)(r * std::cos(t)), (float)(r * std::sin(t)));  // E|w|^2 = 1
+    }
+    HIPCHK(hipMalloc(&c->omega, sizeof(cx<float>) * cnt));
+    HIPCHK(hipMemcpy(c->omega, om.data(), sizeof(cx<float>) * cnt, hipMemcpyHostToDevice));
   }
   return 0;
 }
@@ -1345,8 +1378,14 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     // with the complex64 factors (forward u_x = g_x (R_x + X' u_{x-1}), back
     // D_x = u_x + g_x X D_{x+1}) and adds D to V in FP64.  Every output is a sum over
     // factor columns (site and bond blocks of V V', traces |W' V|^2): columns below
-    // kLu32Cols come from the refined V, the others from the complex64 chains.  T12's
-    // complex64 part is kept apart (scal[6]) for the acceptance check on the host.
+    // kLu32Cols come from the refined V, the others from the complex64 chains.  T12 =
+    // |W_L' G_1L W_R|^2 over W_R's columns past kLu32Cols (its tail, ~1e-5 of its trace
+    // at band energies) splits once more, over W_L's columns: with W_L's leading ones
+    // from the first-row chain (W_Lb' G_1L, complex64, formed in the forward pass), and
+    // with W_L's tail from a Gaussian sketch of W_R's tail carried through the last-
+    // column chain.  So the last-column chain runs on a few dozen columns instead of
+    // all of W_R.  The two complex64 parts go to scal[6] and scal[7] for the
+    // acceptance check on the host.
     auto rgf_lu32 = [&]() -> int {
       typedef cx<float> zf;
       constexpr int K = kLu32Cols;
@@ -1432,7 +1471,19 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       w32.m3 = 1;
       TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nb));  // the first zero pivot stays
       int r;
-      // forward: g_x = inv(M_x) and z_x = g_x X' z_{x-1} (z_1 = g_1 W_Lt), complex64
+      // W_L whole in complex64 (its leading columns start the first-row chain, its
+      // tail columns take the tail-tail estimate)
+      TRY(to_c64(s, F(y.f_g), Es32, Q(y.o_WL), Es, (long long)n * cLf, nb));
+      // T12 beyond W_R's refined columns: when the site blocks need (nearly) all of
+      // W_R's columns anyway (a wide truncation rank), the last-column chain carries all
+      // of them and that part is exact in complex64; otherwise the chain runs narrow and
+      // the part comes from the first-row chain and the sketch ("tail")
+      const bool tail = cR > K && std::max(cRt, RB) + kLu32Sketch <= n;
+      const int RLb = std::min(cLf, K);           // W_L's leading columns (first-row chain)
+      zf* Rb[2] = {F(y.f_R0), F(y.f_R1)};         // first-row chain, as its adjoint (n x RLb)
+      // forward: g_x = inv(M_x), z_x = g_x X' z_{x-1} (z_1 = g_1 W_Lt) and, when W_R has
+      // a tail, the first-row chain (W_Lb' G_1x)' = g_x' X' (...)_{x-1} (start g_1' W_Lb),
+      // complex64; after slice L it holds (W_Lb' G_1L)'
       for (int x = 0; x < L; ++x) {
         if (x > 0) TRY((launch_convert<double, float>(s, Q(y.o_P), Es, gs(x - 1), Es32, nn, nb, nullptr)));
         if ((r = assemble(x, x > 0 ? Q(y.o_P) : nullptr, 1))) return r;
@@ -1444,10 +1495,26 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
           TRY(launch_bd_mul<float>(s, no, 1, T32, Es32, zs(x - 1), Es32, c->X32, n, cL, nb, nullptr));
           TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, zs(x), n, cL, kcL), nb, kOnef, kZerof));
         }
+        if (tail) {
+          if (x == 0) {
+            TRY(gemm_batched<float>(s, OP_C, OP_N, gbf(gs(0), F(y.f_g), Rb[0], n, RLb), nb, kOnef, kZerof));
+          } else {
+            TRY(launch_bd_mul<float>(s, no, 1, T32, Es32, Rb[(x - 1) & 1], Es32, c->X32, n, RLb, nb, nullptr));
+            TRY(gemm_batched<float>(s, OP_C, OP_N, gbf(gs(x), T32, Rb[x & 1], n, RLb), nb, kOnef, kZerof));
+          }
+        }
       }
-      // backward: S_x = g_x X S_{x+1} (S_L = g_L W_R), z_x += g_x X z_{x+1}, complex64;
-      // their leading columns seed the refinement iterates Y_x, Z_x
+      // T12's right-tail, left-lead part: |(W_Lb' G_1L) W_Rs|^2 (W_Rs = W_R's columns
+      // from K on), complex64, into scal[6]
+      if (tail && (r = trace32(F(y.f_WR) + (long long)K * n, Rb[(L - 1) & 1], 6, cR - K, RLb))) return r;
+      // backward: S_x = g_x X S_{x+1}, z_x += g_x X z_{x+1}, complex64.  S starts as
+      // g_L W_R (all of W_R's columns: the right-lead Modec term and the site blocks of
+      // slice L), then continues on its first wD columns (the refined ones and those the
+      // site blocks take) followed by the sketch g_L W_Rs Omega (kLu32Sketch columns).
+      // The leading columns of S and z seed the refinement iterates Y_x, Z_x.
       zf* Sb[2] = {F(y.f_S0), F(y.f_S1)};
+      const int wD = tail ? std::max(cRt, RB) : cR;  // S's exact columns after slice L
+      const int wS = wD + (tail ? kLu32Sketch : 0);  // ... and with the sketch (<= n)
       TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(L - 1), F(y.f_WR), Sb[1], n, cR), nb, kOnef, kZerof));
       zf* Scur = Sb[1];
       int pb = 0;
@@ -1455,8 +1522,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         zf* Sx = Scur;
         if (x < L - 1) {
           Sx = Sb[pb];
-          TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, Scur, Es32, c->X32, n, cR, nb, nullptr));
-          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, Sx, n, cR), nb, kOnef, kZerof));
+          TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, Scur, Es32, c->X32, n, wS, nb, nullptr));
+          TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, Sx, n, wS), nb, kOnef, kZerof));
           TRY(launch_bd_mul<float>(s, no, 0, T32, Es32, zs(x + 1), Es32, c->X32, n, cL, nb, nullptr));
           TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(x), T32, zs(x), n, cL, kcL), nb, kOnef, kOnef));
         }
@@ -1468,16 +1535,26 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         if (x == L - 1 && (r = trace32(F(y.f_WR), Sx + (long long)K * n, 3, cR, cR - K))) return r;  // Modec2, right
         if (x == L - 1 && (r = trace32(F(y.f_WR), zs(x) + (long long)K * n, 2, cR, cL - K))) return r;  // Modec2, left
         if (x == 0 && (r = trace32(F(y.f_WL), zs(0) + (long long)K * n, 1, cL, cL - K))) return r;  // Modec1, left
-        if (x < L - 1) {
-          Scur = Sx;
+        if (x == L - 1 && tail) {
+          // narrow S: its first wD columns, then the sketch g_L W_Rs Omega, in the other buffer
+          GemmBatch g = gbf(Sx + (long long)K * n, c->omega, Sb[0] + (long long)wD * n, n, kLu32Sketch);
+          g.sB = 0;  // one Omega for every energy; its rows past cR - K are not read
+          g.k = cR - K;
+          TRY(gemm_batched<float>(s, OP_N, OP_N, g, nb, kOnef, kZerof));
+          TRY(hipMemcpy2DAsync(Sb[0], sizeof(zf) * Es32, Sx, sizeof(zf) * Es32, sizeof(zf) * n * wD, nb,
+                               hipMemcpyDeviceToDevice, s));
+          Sx = Sb[0];
+          pb = 1;
+        } else if (x < L - 1) {
           pb ^= 1;
         }
+        Scur = Sx;
       }
-      // scal[6]: T12 over W_R's columns from RB on, from complex64 S_1
-      if (cR > RB) {
-        TRY((launch_convert<double, float>(s, Q(y.o_colA), Es, Scur, Es32, (long long)n * cR, nb, nullptr)));
-        if ((r = trace64(Q(y.o_WL), Q(y.o_colA) + (long long)RB * n, 6, cLf, cR - RB))) return r;
-      }
+      // scal[7]: |W_Ls' G_1L W_Rs Omega|^2, kLu32Sketch times the tail-tail estimate;
+      // or, with the chain on all of W_R, scal[6] = |W_L' G_1L W_Rs|^2 in complex64
+      if (tail && (r = trace32(F(y.f_g) + (long long)K * n, Scur + (long long)wD * n, 7, cLf - K, kLu32Sketch)))
+        return r;
+      if (!tail && cR > K && (r = trace32(F(y.f_g), Scur + (long long)K * n, 6, cLf, cR - K))) return r;
       // Refinement of A V = [E_L W_R | E_1 W_Lt] on the R columns of V.  Per sweep: the
       // FP64 products P_x = H_x V_x for every slice (one GEMM per energy, batched over
       // the slices) and the contact terms, the residual of every slice in complex64
@@ -1856,10 +1933,12 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   }
   if ((rr = collect(c, ln, nb, iters, sst, res))) return rr;
   if (prec != NEGF_PREC_LU32) return 0;
-  // LU32: accept an energy's T12 (refined part + complex64 part) when its complex64
-  // block inverses had no zero pivot, its refinement converged (the estimate of the
-  // error left, from the last two corrections) and the complex64 part is small
-  // (kLu32Tail); otherwise solve it again in the plain FP64 form.
+  // LU32: T12 = |W_L' G_1L W_R|^2 summed over three column blocks: W_R's refined
+  // columns (FP64 accurate), W_R's tail with W_L's leading columns (complex64), and
+  // both tails (the sketch estimate).  Accept an energy when its complex64 block
+  // inverses had no zero pivot, its refinement converged (the estimate of the error
+  // left, from the last two corrections) and the two complex64 parts are small
+  // (kLu32Tail, kLu32TailTail); otherwise solve it again in the plain FP64 form.
   std::vector<double> nrm((size_t)(kLu32Sweeps + 1) * nb);
   std::vector<int> ic((size_t)y.Is * nb);
   TRY(hipMemcpyAsync(nrm.data(), ln.d_ref, sizeof(double) * nrm.size(), hipMemcpyDeviceToHost, s));
@@ -1868,18 +1947,22 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
   std::vector<int> esc;
   std::vector<double> eesc;
   for (int b = 0; b < nb; ++b) {
-    const double tb = res[b].scal[0], ts = res[b].scal[6];
+    const double tb = res[b].scal[0];
+    // T12's parts: refined columns, right tail with the left lead's leading columns,
+    // and the tail-tail estimate
+    const double tbs = res[b].scal[6], tss = res[b].scal[7] / kLu32Sketch;
     auto conv = [&](int base) {  // the last two corrections and the solution of one system
       const double d0 = nrm[(size_t)(base + kLu32Sweeps - 2) * nb + b];
       const double d1 = nrm[(size_t)(base + kLu32Sweeps - 1) * nb + b];
       const double vmax = nrm[(size_t)(base + kLu32Sweeps) * nb + b];
       return d1 <= kLu32Floor * vmax || (d1 <= kLu32Contract * d0 && d1 * d1 <= kLu32Accept * vmax * d0);
     };
-    const bool ok = ic[(size_t)b * y.Is + 2 * n] == 0 && conv(0) &&
-                    ts <= kLu32Tail * std::max(1.0, std::fabs(tb)) && std::isfinite(tb);
+    const double scale = std::max(1.0, std::fabs(tb));
+    const bool ok = ic[(size_t)b * y.Is + 2 * n] == 0 && conv(0) && tbs <= kLu32Tail * scale &&
+                    tss <= kLu32TailTail * scale && std::isfinite(tb + tbs + tss);
     if (ok) {
-      res[b].scal[0] = tb + ts;
-      res[b].scal[6] = 0.0;
+      res[b].scal[0] = tb + tbs + tss;
+      res[b].scal[6] = res[b].scal[7] = 0.0;
     } else {
       esc.push_back(b);
       eesc.push_back(energies[b]);

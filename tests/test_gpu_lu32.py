@@ -116,6 +116,27 @@ def test_lu32_bench_size_matches_oracle(name, tag, dims, kw):
     assert esc == 0
 
 
+@pytest.mark.parametrize("tag", ["c2", "c4x"])
+def test_lu32_truncated_broadening_keeps_t12(tag):
+    """With the broadening truncated for the densities (gamma_trunc = 1e-5, the bench
+    setting), the last-column chain runs narrow and T12's part past W_R's refined
+    columns comes from the first-row chain and the tail sketch: T12 still matches the
+    oracle (whole Gamma) within 1e-8 of its scale, and no energy is escalated."""
+    name = "c2" if tag == "c2" else "c4"
+    (t, dims, kw, _, _), = [c for c in MB.CASES[name] if c[0] == tag]
+    print(f"lu32 truncated broadening, fixture {tag}", flush=True)
+    w = build_wire(*dims, **kw)
+    energies, dE, digest, ref, noise = MB.load(name, tag)
+    assert digest == MB.wire_digest(w)
+    acc, it, (ne, sweeps, esc) = _run(w, [float(e) for e in energies], dE, gamma_trunc=1e-5)
+    got = _collect(acc)
+    sc = P.scales(ref)
+    err = np.abs(got["T12"] - ref["T12"]) / sc["T12"]
+    print(f"{tag}: T12 rel err {err.max():.2e}, escalated {esc}", flush=True)
+    assert np.all(err <= T_RTOL), err
+    assert esc == 0
+
+
 def test_lu32_compact_equals_dense():
     # The complex64 chains see the same assembled slices either way; the FP64
     # residual adds the site potentials of a compact slice apart from the template
