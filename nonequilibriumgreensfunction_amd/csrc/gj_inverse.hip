@@ -474,6 +474,178 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Block Gauss-Jordan with pivoting inside 128-column diagonal blocks only
+// (gj_invert_local).  For each outer block K = [k0, k0 + kb):
+//   diag   -- one 1024-thread workgroup per matrix inverts D = S[K, K] by
+//             Gauss-Jordan with partial pivoting among the block's rows (the block
+//             held in registers, 16 entries per thread; column m and the pivot row
+//             pass through LDS) and writes Dinv (kb x kb, column-major);
+//   row    -- R = Dinv S[K, :] (a GEMM; stored transposed, n x kb);
+//   cols   -- A'[i] = e_{i-k0} for i in K, -S[i, K] otherwise; S[:, K] = 0; R[:, K] = Dinv;
+//   update -- S = keep(S) + A' R with the GEMM's Gauss-Jordan epilogue, keep() zeroing
+//             the rows of K (pivstep = identity):
+//               rows in K:      R          (the block row; Dinv in the columns of K),
+//               rows not in K:  S[i, :] - S[i, K] R  (-S[i, K] Dinv in the columns of K,
+//                               which cols cleared first).
+// The result is the inverse, in place and unpermuted.  Without row interchanges between
+// blocks its accuracy rests on the diagonal blocks being well conditioned: it is
+// for callers that check their results (the complex64 block LU whose solves are
+// refined in FP64, which falls back to the partially pivoted FP64 inverse when
+// the refinement does not contract).  4 launches per 128 columns instead of 50.
+constexpr int kLocalRows = 128;  // the diagonal block (kGjNBO)
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS, int ld, int k0, int kb,
+                                                   cx<T>* Dinv, long long sD, int* info, long long sInt,
+                                                   const int* active) {
+  typedef cx<T> V;
+  const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
+  const V* A = S + (long long)b * sS;
+  V* Di = Dinv + (long long)b * sD;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int r = tid & (kLocalRows - 1), cg = tid >> 7;  // row; columns cg + 8 q
+  __shared__ V colv[2][kLocalRows];
+  __shared__ unsigned long long key[2][kLocalRows];
+  __shared__ V rowv[kLocalRows];
+  __shared__ int pstep[kLocalRows], prow[kLocalRows];
+  __shared__ int pivs;
+  V v[16];
+#pragma clang loop unroll(full)
+  for (int q = 0; q < 16; ++q) {
+    const int c = cg + 8 * q;
+    v[q] = load_sel(A + (long long)(k0 + (c < kb ? c : 0)) * ld + k0 + (r < kb ? r : 0), r < kb && c < kb);
+  }
+  bool used = r >= kb;
+  for (int m = 0; m < kb; ++m) {
+    const int par = m & 1;
+    if (cg == (m & 7)) {  // the owners of column m publish it
+      V x = v[0];
+#pragma clang loop unroll(full)
+      for (int q = 1; q < 16; ++q)
+        if (q == (m >> 3)) x = v[q];
+      colv[par][r] = x;
+      key[par][r] = used ? 0ull : pivot_key(x, r);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const unsigned long long g = wave_max_u64(umax64(key[par][lane], key[par][lane + 64]));
+      if (lane == 0) {
+        int p = 4095 - (int)(g & 0xfffull);
+        if ((g & ~0xfffull) == 0) {  // no nonzero candidate: singular block
+          if (info[(long long)b * sInt] == 0) info[(long long)b * sInt] = k0 + m + 1;
+          if (g == 0) {  // every row used up cannot happen (kb steps, kb rows); keep p valid
+            p = 0;
+          }
+        }
+        pivs = p < kb ? p : 0;
+        pstep[pivs] = m;
+        prow[m] = pivs;
+      }
+    }
+    __syncthreads();
+    const int p = pivs;
+    if (r == p) {  // the pivot row, scaled: T[m][c] = v[c] / a (c != m), T[m][m] = 1 / a
+      const V inva = cinv(colv[par][p]);
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 16; ++q) {
+        const int c = cg + 8 * q;
+        const V t = (c == m) ? inva : cmul(v[q], inva);
+        v[q] = t;
+        if (c < kb) rowv[c] = t;
+      }
+      used = true;
+    }
+    __syncthreads();
+    if (r != p && r < kb) {  // every other row takes the step
+      const V f = colv[par][r];
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 16; ++q) {
+        const int c = cg + 8 * q;
+        if (c < kb) v[q] = (c == m) ? cneg(cmul(f, rowv[m])) : csub(v[q], cmul(f, rowv[c]));
+      }
+    }
+  }
+  __syncthreads();
+  // inv(D)[a][b'] = S[prow[a]][pstep... ]: entry (r, c) goes to (pstep[r], prow[c])
+  if (r < kb) {
+    const int a = pstep[r];
+#pragma clang loop unroll(full)
+    for (int q = 0; q < 16; ++q) {
+      const int c = cg + 8 * q;
+      if (c < kb) Di[(long long)prow[c] * kb + a] = v[q];
+    }
+  }
+}
+
+// A'[i][m] = (i in K) ? delta(i - k0, m) : -S[i][k0 + m];  S[i][k0 + m] = 0;  and for the
+// rows of K, R[m][i] = Dinv[m][i - k0] (R stored transposed: R^T (n x kb), ld n).
+template <typename T>
+__global__ void k_gjl_cols(cx<T>* S, long long sS, int ld, int n, int k0, int kb, cx<T>* Ap, long long sA,
+                           cx<T>* Rt, long long sR, const cx<T>* Dinv, long long sD, const int* active) {
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int m = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cx<T>* Sc = S + (long long)b * sS + (long long)(k0 + m) * ld;
+  const bool inK = i >= k0 && i < k0 + kb;
+  Ap[(long long)b * sA + (long long)m * n + i] = inK ? mk<T>(i - k0 == m ? 1 : 0, 0) : cneg(Sc[i]);
+  Sc[i] = mk<T>(0, 0);
+  if (inK) Rt[(long long)b * sR + (long long)m * n + i] = Dinv[(long long)b * sD + (long long)(i - k0) * kb + m];
+}
+
+// pivstep = identity: the epilogue's row mask for "rows of K" (info is left to the
+// caller, which clears it before a sequence of inversions and reads the first failure)
+__global__ void k_gjl_init(int* pivstep, long long sInt, int n, const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    pivstep[(long long)b * sInt + i] = i;
+}
+
+template <typename T>
+hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n, int batch, const GjWork& w,
+                           const int* active) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (!w.tmp) return hipErrorInvalidValue;
+  // workspaces: A' (n x kb) at w.tmp; R^T (n x kb) then Dinv (kb x kb) in the pivot-row
+  // panel (kGjNBO x n complex128 holds 2 kGjNBO x n of complex64, and both fit)
+  cx<T>* Ap = (cx<T>*)w.tmp;
+  cx<T>* Rt = (cx<T>*)w.prow;
+  cx<T>* Di = Rt + (long long)n * (n < kLocalRows ? n : kLocalRows);
+  hipLaunchKernelGGL(k_gjl_init, dim3((n + 255) / 256, batch), dim3(256), 0, s, w.pivstep, w.sInt, n, active);
+  for (int k0 = 0; k0 < n; k0 += kLocalRows) {
+    const int kb = (n - k0 < kLocalRows) ? n - k0 : kLocalRows;
+    hipLaunchKernelGGL((k_gjl_diag<T>), dim3(batch), dim3(1024), 0, s, S, sS, ld, k0, kb, Di, w.sProw, w.info,
+                       w.sInt, active);
+    GemmBatch g = {};  // R^T = S[K, :]^T Dinv^T
+    g.A = S + k0; g.sA = sS; g.lda = ld;
+    g.B = Di; g.sB = w.sProw; g.ldb = kb;
+    g.C = Rt; g.sC = w.sProw; g.ldc = n;
+    g.m = n; g.n = kb; g.k = kb;
+    g.active = active;
+    g.m3 = w.m3;
+    hipError_t e = gemm_batched<T>(s, OP_T, OP_T, g, batch, mk<T>(1, 0), mk<T>(0, 0));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gjl_cols<T>), dim3((n + 255) / 256, kb, batch), dim3(256), 0, s, S, sS, ld, n, k0, kb,
+                       Ap, w.sTmp, Rt, w.sProw, Di, w.sProw, active);
+    GemmBatch u = {};  // S = keep(S) + A' R
+    u.A = Ap; u.sA = w.sTmp; u.lda = n;
+    u.B = Rt; u.sB = w.sProw; u.ldb = n;
+    u.C = S; u.sC = sS; u.ldc = ld;
+    u.m = n; u.n = n; u.k = kb;
+    u.active = active; u.pivstep = w.pivstep; u.sPiv = w.sInt;
+    u.pv_lo = k0; u.pv_hi = k0 + kb;
+    u.skip_lo = u.skip_hi = 0;
+    u.m3 = w.m3;
+    e = gemm_gj_update<T>(s, u, batch);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+template hipError_t gj_invert_local<float>(hipStream_t, cx<float>*, long long, int, int, int, const GjWork&,
+                                           const int*);
+
 template hipError_t gj_invert<double>(hipStream_t, cx<double>*, long long, int, cx<double>*,
                                       long long, int, int, int, const GjWork&, const int*);
 template hipError_t gj_invert<float>(hipStream_t, cx<float>*, long long, int, cx<float>*,

@@ -118,6 +118,8 @@ struct GjWork {
   long long sProw;  // batch stride of prow (complex elements)
   long long sInt;   // batch stride of pivstep, pivrow and info (ints)
   int m3 = 0;       // complex product form of the trailing updates (GemmBatch::m3)
+  void* tmp = nullptr;  // gj_invert_local: n x 128 complex per matrix, batch stride sTmp
+  long long sTmp = 0;
 };
 
 template <typename T>
@@ -126,6 +128,11 @@ hipError_t gemm_batched(hipStream_t s, int opa, int opb, const GemmBatch& g, int
 template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
+// In-place inverse by block Gauss-Jordan with pivoting inside 128-column diagonal
+// blocks only (gj_inverse.hip; complex64), for callers that check their results.
+template <typename T>
+hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n, int batch, const GjWork& w,
+                           const int* active);
 constexpr int kGjNBO = 128;  // Gauss-Jordan outer block width (columns per trailing update)
 // Gauss-Jordan int workspace per matrix: pivstep[n], pivrow[n], info, then the
 // panel's pivot candidates (8 per 256-row chunk, n <= 3072).

@@ -1402,7 +1402,6 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       auto Vr = [&](int x) { return Q(y.o_vr + (long long)x * rs); };
       auto Yr = [&](int x) { return Vr(x); };
       auto Zr = [&](int x) { return Vr(x) + (long long)n * RB; };
-      zf* M32 = F(y.f_G);
       zf* T32 = F(y.f_Y);
       zd* WLt = Q(y.o_colB);  // W_L on its truncation rank, FP64
       auto gbf = [&](const zf* A, const zf* B, zf* C, int m, int ncols, const int* kc = nullptr) {
@@ -1465,10 +1464,14 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       TRY(hipMemcpy2DAsync(WLt, sizeof(zd) * Es, Q(y.o_WL), sizeof(zd) * Es, sizeof(zd) * n * cL, nb,
                            hipMemcpyDeviceToDevice, s));
       TRY(launch_trunc_cols<double>(s, WLt, Es, n, cL, kcL, skc, nb));
+      // block inverses: complex64 block Gauss-Jordan pivoting inside 128-column
+      // diagonal blocks (the refinement checks the solves built on them)
       GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
       w32.prow = Q(y.o_prow);
       w32.sProw = Es32;
       w32.m3 = 1;
+      w32.tmp = F(y.f_P);
+      w32.sTmp = Es32;
       TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nb));  // the first zero pivot stays
       int r;
       // W_L whole in complex64 (its leading columns start the first-row chain, its
@@ -1487,8 +1490,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       for (int x = 0; x < L; ++x) {
         if (x > 0) TRY((launch_convert<double, float>(s, Q(y.o_P), Es, gs(x - 1), Es32, nn, nb, nullptr)));
         if ((r = assemble(x, x > 0 ? Q(y.o_P) : nullptr, 1))) return r;
-        TRY(to_c64(s, M32, Es32, Q(y.o_M), Es, nn, nb));
-        TRY(gj_invert<float>(s, M32, Es32, n, gs(x), Es32, n, n, nb, w32, nullptr));
+        TRY(to_c64(s, gs(x), Es32, Q(y.o_M), Es, nn, nb));
+        TRY(gj_invert_local<float>(s, gs(x), Es32, n, n, nb, w32, nullptr));
         if (x == 0) {
           TRY(gemm_batched<float>(s, OP_N, OP_N, gbf(gs(0), F(y.f_WL), zs(0), n, cL, kcL), nb, kOnef, kZerof));
         } else {
