@@ -495,6 +495,15 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
 // the refinement does not contract).  4 launches per 128 columns instead of 50.
 constexpr int kLocalRows = 128;  // the diagonal block (kGjNBO)
 
+// The diagonal block's inverse.  Thread (r, g) = (tid % 128, tid / 128) holds row r
+// of D in columns g + 8 q (q < 16).  Per 8-column panel: its columns go to LDS, one
+// wave runs the panel's Gauss-Jordan steps on all the block's rows (partial pivoting
+// among the unused rows, shuffles only), publishing each row's final panel values
+// S[r][.] and the pivot rows; the pivot rows' values in every column go to LDS; then
+// each thread updates its other columns as the Gauss-Jordan steps do,
+//   v[c] <- [r not a pivot of the panel] v[c] + sum_m S[r][m] Prow[m][c],
+// and takes S[r][.] in the panel's own columns.  Three barriers per 8 columns.
+// Finally inv(D)[pstep[r]][prow[c]] = D[r][c] goes to Dinv (kb x kb, column-major).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS, int ld, int k0, int kb,
                                                    cx<T>* Dinv, long long sD, int* info, long long sInt,
@@ -504,76 +513,125 @@ __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS,
   const V* A = S + (long long)b * sS;
   V* Di = Dinv + (long long)b * sD;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int r = tid & (kLocalRows - 1), cg = tid >> 7;  // row; columns cg + 8 q
-  __shared__ V colv[2][kLocalRows];
-  __shared__ unsigned long long key[2][kLocalRows];
-  __shared__ V rowv[kLocalRows];
+  const int r = tid & (kLocalRows - 1), cg = tid >> 7;
+  __shared__ V pan[kNBI][kLocalRows];    // the panel's columns
+  __shared__ V stab[kLocalRows][kNBI];   // each row's final panel values
+  __shared__ V prw[kNBI][kLocalRows];    // the panel's pivot rows, every column
+  __shared__ int pv[kNBI];
   __shared__ int pstep[kLocalRows], prow[kLocalRows];
-  __shared__ int pivs;
   V v[16];
 #pragma clang loop unroll(full)
   for (int q = 0; q < 16; ++q) {
     const int c = cg + 8 * q;
     v[q] = load_sel(A + (long long)(k0 + (c < kb ? c : 0)) * ld + k0 + (r < kb ? r : 0), r < kb && c < kb);
   }
-  bool used = r >= kb;
-  for (int m = 0; m < kb; ++m) {
-    const int par = m & 1;
-    if (cg == (m & 7)) {  // the owners of column m publish it
+  bool used0 = lane >= kb, used1 = lane + 64 >= kb;  // wave 0: rows lane, lane + 64
+  int* inf = info + (long long)b * sInt;
+  for (int P = 0; P * kNBI < kb; ++P) {
+    const int nb = (kb - P * kNBI < kNBI) ? kb - P * kNBI : kNBI;
+    if (cg < nb) {
       V x = v[0];
 #pragma clang loop unroll(full)
       for (int q = 1; q < 16; ++q)
-        if (q == (m >> 3)) x = v[q];
-      colv[par][r] = x;
-      key[par][r] = used ? 0ull : pivot_key(x, r);
+        if (q == P) x = v[q];
+      pan[cg][r] = x;
     }
     __syncthreads();
     if (tid < 64) {
-      const unsigned long long g = wave_max_u64(umax64(key[par][lane], key[par][lane + 64]));
-      if (lane == 0) {
-        int p = 4095 - (int)(g & 0xfffull);
-        if ((g & ~0xfffull) == 0) {  // no nonzero candidate: singular block
-          if (info[(long long)b * sInt] == 0) info[(long long)b * sInt] = k0 + m + 1;
-          if (g == 0) {  // every row used up cannot happen (kb steps, kb rows); keep p valid
-            p = 0;
+      V w0[kNBI], w1[kNBI];
+#pragma clang loop unroll(full)
+      for (int j = 0; j < kNBI; ++j) {
+        w0[j] = j < nb ? pan[j][lane] : mk<T>(0, 0);
+        w1[j] = j < nb ? pan[j][lane + 64] : mk<T>(0, 0);
+      }
+#pragma clang loop unroll(full)
+      for (int j = 0; j < kNBI; ++j) {
+        if (j < nb) {
+          unsigned long long k = 0;
+          if (!used0) k = pivot_key(w0[j], lane);
+          if (!used1) k = umax64(k, pivot_key(w1[j], lane + 64));
+          const unsigned long long g = wave_max_u64(k);
+          int p = 4095 - (int)(g & 0xfffull);
+          if ((g & ~0xfffull) == 0 && lane == 0 && *inf == 0) *inf = k0 + P * kNBI + j + 1;  // zero pivot
+          p = (g != 0 && p < kb) ? p : 0;
+          const int lp = p & 63, hi = p >> 6;
+          V t[kNBI];
+#pragma clang loop unroll(full)
+          for (int c = 0; c < kNBI; ++c) {
+            const V x = hi ? w1[c] : w0[c];
+            t[c].x = __shfl(x.x, lp);
+            t[c].y = __shfl(x.y, lp);
+          }
+          const V inva = cinv(t[j]);
+#pragma clang loop unroll(full)
+          for (int c = 0; c < kNBI; ++c) t[c] = (c == j) ? inva : cmul(t[c], inva);
+          // rows lane and lane + 64: the pivot row becomes t, the others take the step
+          auto step = [&](V(&w)[kNBI], int row) {
+            if (row == p) {
+#pragma clang loop unroll(full)
+              for (int c = 0; c < kNBI; ++c) w[c] = t[c];
+            } else {
+              const V f = w[j];
+#pragma clang loop unroll(full)
+              for (int c = 0; c < kNBI; ++c) w[c] = (c == j) ? cneg(cmul(f, t[j])) : csub(w[c], cmul(f, t[c]));
+            }
+          };
+          step(w0, lane);
+          step(w1, lane + 64);
+          if (p == lane) used0 = true;
+          if (p == lane + 64) used1 = true;
+          if (lane == 0) {
+            pv[j] = p;
+            pstep[p] = P * kNBI + j;
+            prow[P * kNBI + j] = p;
           }
         }
-        pivs = p < kb ? p : 0;
-        pstep[pivs] = m;
-        prow[m] = pivs;
+      }
+#pragma clang loop unroll(full)
+      for (int j = 0; j < kNBI; ++j) {
+        stab[lane][j] = w0[j];
+        stab[lane + 64][j] = w1[j];
       }
     }
     __syncthreads();
-    const int p = pivs;
-    if (r == p) {  // the pivot row, scaled: T[m][c] = v[c] / a (c != m), T[m][m] = 1 / a
-      const V inva = cinv(colv[par][p]);
+    int ip = -1;  // this row's place among the panel's pivots
 #pragma clang loop unroll(full)
-      for (int q = 0; q < 16; ++q) {
-        const int c = cg + 8 * q;
-        const V t = (c == m) ? inva : cmul(v[q], inva);
-        v[q] = t;
-        if (c < kb) rowv[c] = t;
-      }
-      used = true;
+    for (int m = 0; m < kNBI; ++m)
+      if (m < nb && pv[m] == r) ip = m;
+    if (ip >= 0) {
+#pragma clang loop unroll(full)
+      for (int q = 0; q < 16; ++q) prw[ip][cg + 8 * q] = v[q];
     }
     __syncthreads();
-    if (r != p && r < kb) {  // every other row takes the step
-      const V f = colv[par][r];
+    if (r < kb) {
+      V sr[kNBI];
+#pragma clang loop unroll(full)
+      for (int m = 0; m < kNBI; ++m) sr[m] = stab[r][m];
 #pragma clang loop unroll(full)
       for (int q = 0; q < 16; ++q) {
         const int c = cg + 8 * q;
-        if (c < kb) v[q] = (c == m) ? cneg(cmul(f, rowv[m])) : csub(v[q], cmul(f, rowv[c]));
+        if (q == P) {
+          if (cg < nb) v[q] = stab[r][cg];
+        } else {
+          V acc = ip >= 0 ? mk<T>(0, 0) : v[q];
+#pragma clang loop unroll(full)
+          for (int m = 0; m < kNBI; ++m)
+            if (m < nb) acc = cadd(acc, cmul(sr[m], prw[m][c]));
+          v[q] = acc;
+        }
       }
     }
   }
   __syncthreads();
-  // inv(D)[a][b'] = S[prow[a]][pstep... ]: entry (r, c) goes to (pstep[r], prow[c])
   if (r < kb) {
     const int a = pstep[r];
 #pragma clang loop unroll(full)
     for (int q = 0; q < 16; ++q) {
       const int c = cg + 8 * q;
-      if (c < kb) Di[(long long)prow[c] * kb + a] = v[q];
+      if (c < kb) {
+        const int bc = prow[c];
+        if (a >= 0 && a < kb && bc >= 0 && bc < kb) Di[(long long)bc * kb + a] = v[q];
+      }
     }
   }
 }
