@@ -475,8 +475,8 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
 }
 
 // ---------------------------------------------------------------------------
-// Block Gauss-Jordan with pivoting inside 128-column diagonal blocks only
-// (gj_invert_local).  For each outer block K = [k0, k0 + kb):
+// Block Gauss-Jordan with pivoting inside diagonal blocks only (gj_invert_local;
+// 128-column blocks in complex64, 64-column blocks in complex128).  For each outer block K = [k0, k0 + kb):
 //   diag   -- one 1024-thread workgroup per matrix inverts D = S[K, K] by
 //             Gauss-Jordan with partial pivoting among the block's rows (the block
 //             held in registers, 16 entries per thread; column m and the pivot row
@@ -493,63 +493,69 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
 // for callers that check their results (the complex64 block LU whose solves are
 // refined in FP64, which falls back to the partially pivoted FP64 inverse when
 // the refinement does not contract).  4 launches per 128 columns instead of 50.
-constexpr int kLocalRows = 128;  // the diagonal block (kGjNBO)
+// the diagonal block: 128 columns in complex64, 64 in complex128 (registers per thread)
+template <typename T> struct LocalBlock { static constexpr int kb = sizeof(T) == sizeof(float) ? 128 : 64; };
 
-// The diagonal block's inverse.  Thread (r, g) = (tid % 128, tid / 128) holds row r
-// of D in columns g + 8 q (q < 16).  Per 8-column panel: its columns go to LDS, one
-// wave runs the panel's Gauss-Jordan steps on all the block's rows (partial pivoting
-// among the unused rows, shuffles only), publishing each row's final panel values
-// S[r][.] and the pivot rows; the pivot rows' values in every column go to LDS; then
-// each thread updates its other columns as the Gauss-Jordan steps do,
+// The diagonal block's inverse (blocks of KB = 128 columns in complex64, 64 in
+// complex128; KB x 8 threads).  Thread (r, g) = (tid % KB, tid / KB) holds row r of D
+// in columns g + 8 q (q < KB / 8).  Per 8-column panel: its columns go to LDS, one
+// wave runs the panel's Gauss-Jordan steps on all the block's rows (KB / 64 rows per
+// lane; partial pivoting among the unused rows, shuffles only), publishing each
+// row's final panel values S[r][.] and the pivot rows; the pivot rows' values in
+// every column go to LDS; then each thread updates its other columns as the
+// Gauss-Jordan steps do,
 //   v[c] <- [r not a pivot of the panel] v[c] + sum_m S[r][m] Prow[m][c],
 // and takes S[r][.] in the panel's own columns.  Three barriers per 8 columns.
 // Finally inv(D)[pstep[r]][prow[c]] = D[r][c] goes to Dinv (kb x kb, column-major).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS, int ld, int k0, int kb,
-                                                   cx<T>* Dinv, long long sD, int* info, long long sInt,
-                                                   const int* active) {
+template <typename T, int KB>
+__global__ __launch_bounds__(KB * 8) void k_gjl_diag(const cx<T>* S, long long sS, int ld, int k0, int kb,
+                                                     cx<T>* Dinv, long long sD, int* info, long long sInt,
+                                                     const int* active) {
   typedef cx<T> V;
+  constexpr int NQ = KB / 8, HR = KB / 64;
   const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
   const V* A = S + (long long)b * sS;
   V* Di = Dinv + (long long)b * sD;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int r = tid & (kLocalRows - 1), cg = tid >> 7;
-  __shared__ V pan[kNBI][kLocalRows];    // the panel's columns
-  __shared__ V stab[kLocalRows][kNBI];   // each row's final panel values
-  __shared__ V prw[kNBI][kLocalRows];    // the panel's pivot rows, every column
+  const int r = tid % KB, cg = tid / KB;
+  __shared__ V pan[kNBI][KB];          // the panel's columns
+  __shared__ V stab[KB][kNBI + 1];     // each row's final panel values (padded: no bank conflicts)
+  __shared__ V prw[kNBI][KB];          // the panel's pivot rows, every column
   __shared__ int pv[kNBI];
-  __shared__ int pstep[kLocalRows], prow[kLocalRows];
-  V v[16];
+  __shared__ int pstep[KB], prow[KB];
+  V v[NQ];
 #pragma clang loop unroll(full)
-  for (int q = 0; q < 16; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int c = cg + 8 * q;
     v[q] = load_sel(A + (long long)(k0 + (c < kb ? c : 0)) * ld + k0 + (r < kb ? r : 0), r < kb && c < kb);
   }
-  bool used0 = lane >= kb, used1 = lane + 64 >= kb;  // wave 0: rows lane, lane + 64
+  bool used[HR];  // wave 0: rows lane + 64 h
+#pragma clang loop unroll(full)
+  for (int h = 0; h < HR; ++h) used[h] = lane + 64 * h >= kb;
   int* inf = info + (long long)b * sInt;
   for (int P = 0; P * kNBI < kb; ++P) {
     const int nb = (kb - P * kNBI < kNBI) ? kb - P * kNBI : kNBI;
     if (cg < nb) {
       V x = v[0];
 #pragma clang loop unroll(full)
-      for (int q = 1; q < 16; ++q)
+      for (int q = 1; q < NQ; ++q)
         if (q == P) x = v[q];
       pan[cg][r] = x;
     }
     __syncthreads();
     if (tid < 64) {
-      V w0[kNBI], w1[kNBI];
+      V w[HR][kNBI];
 #pragma clang loop unroll(full)
-      for (int j = 0; j < kNBI; ++j) {
-        w0[j] = j < nb ? pan[j][lane] : mk<T>(0, 0);
-        w1[j] = j < nb ? pan[j][lane + 64] : mk<T>(0, 0);
-      }
+      for (int h = 0; h < HR; ++h)
+#pragma clang loop unroll(full)
+        for (int j = 0; j < kNBI; ++j) w[h][j] = j < nb ? pan[j][lane + 64 * h] : mk<T>(0, 0);
 #pragma clang loop unroll(full)
       for (int j = 0; j < kNBI; ++j) {
         if (j < nb) {
           unsigned long long k = 0;
-          if (!used0) k = pivot_key(w0[j], lane);
-          if (!used1) k = umax64(k, pivot_key(w1[j], lane + 64));
+#pragma clang loop unroll(full)
+          for (int h = 0; h < HR; ++h)
+            if (!used[h]) k = umax64(k, pivot_key(w[h][j], lane + 64 * h));
           const unsigned long long g = wave_max_u64(k);
           int p = 4095 - (int)(g & 0xfffull);
           if ((g & ~0xfffull) == 0 && lane == 0 && *inf == 0) *inf = k0 + P * kNBI + j + 1;  // zero pivot
@@ -558,28 +564,30 @@ __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS,
           V t[kNBI];
 #pragma clang loop unroll(full)
           for (int c = 0; c < kNBI; ++c) {
-            const V x = hi ? w1[c] : w0[c];
+            V x = w[0][c];
+#pragma clang loop unroll(full)
+            for (int h = 1; h < HR; ++h)
+              if (hi == h) x = w[h][c];
             t[c].x = __shfl(x.x, lp);
             t[c].y = __shfl(x.y, lp);
           }
           const V inva = cinv(t[j]);
 #pragma clang loop unroll(full)
           for (int c = 0; c < kNBI; ++c) t[c] = (c == j) ? inva : cmul(t[c], inva);
-          // rows lane and lane + 64: the pivot row becomes t, the others take the step
-          auto step = [&](V(&w)[kNBI], int row) {
-            if (row == p) {
+          // the pivot row becomes t, the others take the step
 #pragma clang loop unroll(full)
-              for (int c = 0; c < kNBI; ++c) w[c] = t[c];
+          for (int h = 0; h < HR; ++h) {
+            if (lane + 64 * h == p) {
+#pragma clang loop unroll(full)
+              for (int c = 0; c < kNBI; ++c) w[h][c] = t[c];
+              used[h] = true;
             } else {
-              const V f = w[j];
+              const V f = w[h][j];
 #pragma clang loop unroll(full)
-              for (int c = 0; c < kNBI; ++c) w[c] = (c == j) ? cneg(cmul(f, t[j])) : csub(w[c], cmul(f, t[c]));
+              for (int c = 0; c < kNBI; ++c)
+                w[h][c] = (c == j) ? cneg(cmul(f, t[j])) : csub(w[h][c], cmul(f, t[c]));
             }
-          };
-          step(w0, lane);
-          step(w1, lane + 64);
-          if (p == lane) used0 = true;
-          if (p == lane + 64) used1 = true;
+          }
           if (lane == 0) {
             pv[j] = p;
             pstep[p] = P * kNBI + j;
@@ -588,10 +596,9 @@ __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS,
         }
       }
 #pragma clang loop unroll(full)
-      for (int j = 0; j < kNBI; ++j) {
-        stab[lane][j] = w0[j];
-        stab[lane + 64][j] = w1[j];
-      }
+      for (int h = 0; h < HR; ++h)
+#pragma clang loop unroll(full)
+        for (int j = 0; j < kNBI; ++j) stab[lane + 64 * h][j] = w[h][j];
     }
     __syncthreads();
     int ip = -1;  // this row's place among the panel's pivots
@@ -600,7 +607,7 @@ __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS,
       if (m < nb && pv[m] == r) ip = m;
     if (ip >= 0) {
 #pragma clang loop unroll(full)
-      for (int q = 0; q < 16; ++q) prw[ip][cg + 8 * q] = v[q];
+      for (int q = 0; q < NQ; ++q) prw[ip][cg + 8 * q] = v[q];
     }
     __syncthreads();
     if (r < kb) {
@@ -608,7 +615,7 @@ __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS,
 #pragma clang loop unroll(full)
       for (int m = 0; m < kNBI; ++m) sr[m] = stab[r][m];
 #pragma clang loop unroll(full)
-      for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < NQ; ++q) {
         const int c = cg + 8 * q;
         if (q == P) {
           if (cg < nb) v[q] = stab[r][cg];
@@ -626,7 +633,7 @@ __global__ __launch_bounds__(1024) void k_gjl_diag(const cx<T>* S, long long sS,
   if (r < kb) {
     const int a = pstep[r];
 #pragma clang loop unroll(full)
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int c = cg + 8 * q;
       if (c < kb) {
         const int bc = prow[c];
@@ -667,14 +674,15 @@ hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n,
   if (!w.tmp) return hipErrorInvalidValue;
   // workspaces: A' (n x kb) at w.tmp; R^T (n x kb) then Dinv (kb x kb) in the pivot-row
   // panel (kGjNBO x n complex128 holds 2 kGjNBO x n of complex64, and both fit)
+  constexpr int KB = LocalBlock<T>::kb;
   cx<T>* Ap = (cx<T>*)w.tmp;
   cx<T>* Rt = (cx<T>*)w.prow;
-  cx<T>* Di = Rt + (long long)n * (n < kLocalRows ? n : kLocalRows);
+  cx<T>* Di = Rt + (long long)n * (n < KB ? n : KB);
   hipLaunchKernelGGL(k_gjl_init, dim3((n + 255) / 256, batch), dim3(256), 0, s, w.pivstep, w.sInt, n, active);
-  for (int k0 = 0; k0 < n; k0 += kLocalRows) {
-    const int kb = (n - k0 < kLocalRows) ? n - k0 : kLocalRows;
-    hipLaunchKernelGGL((k_gjl_diag<T>), dim3(batch), dim3(1024), 0, s, S, sS, ld, k0, kb, Di, w.sProw, w.info,
-                       w.sInt, active);
+  for (int k0 = 0; k0 < n; k0 += KB) {
+    const int kb = (n - k0 < KB) ? n - k0 : KB;
+    hipLaunchKernelGGL((k_gjl_diag<T, KB>), dim3(batch), dim3(KB * 8), 0, s, S, sS, ld, k0, kb, Di, w.sProw,
+                       w.info, w.sInt, active);
     GemmBatch g = {};  // R^T = S[K, :]^T Dinv^T
     g.A = S + k0; g.sA = sS; g.lda = ld;
     g.B = Di; g.sB = w.sProw; g.ldb = kb;
@@ -703,6 +711,8 @@ hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n,
 
 template hipError_t gj_invert_local<float>(hipStream_t, cx<float>*, long long, int, int, int, const GjWork&,
                                            const int*);
+template hipError_t gj_invert_local<double>(hipStream_t, cx<double>*, long long, int, int, int, const GjWork&,
+                                            const int*);
 
 template hipError_t gj_invert<double>(hipStream_t, cx<double>*, long long, int, cx<double>*,
                                       long long, int, int, int, const GjWork&, const int*);

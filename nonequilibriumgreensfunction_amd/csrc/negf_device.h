@@ -118,7 +118,7 @@ struct GjWork {
   long long sProw;  // batch stride of prow (complex elements)
   long long sInt;   // batch stride of pivstep, pivrow and info (ints)
   int m3 = 0;       // complex product form of the trailing updates (GemmBatch::m3)
-  void* tmp = nullptr;  // gj_invert_local: n x 128 complex per matrix, batch stride sTmp
+  void* tmp = nullptr;  // gj_invert_local: n x 128 (complex64) or n x 64 (complex128) per matrix
   long long sTmp = 0;
 };
 
@@ -128,8 +128,8 @@ hipError_t gemm_batched(hipStream_t s, int opa, int opb, const GemmBatch& g, int
 template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
-// In-place inverse by block Gauss-Jordan with pivoting inside 128-column diagonal
-// blocks only (gj_inverse.hip; complex64), for callers that check their results.
+// In-place inverse by block Gauss-Jordan with pivoting inside diagonal blocks only
+// (gj_inverse.hip; 128-column blocks in complex64, 64 in complex128).
 template <typename T>
 hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n, int batch, const GjWork& w,
                            const int* active);

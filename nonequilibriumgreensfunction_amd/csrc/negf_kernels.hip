@@ -109,10 +109,10 @@ __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, c
 // Writes (accum: adds), per site s of slice x:  out[0] = Re tr D,  out[1+a] =
 // Re tr(D sigma_a), out[4+r] = Re D_rr (r < NORB).  out layout: out_b + q*qstride + s.
 // A workgroup takes 16 rows (16 / NORB sites) and splits the sum over k into 16
-// chunks, one per 16-thread row group: with 16 rows per workgroup there are n / 16
-// workgroups per matrix (144 at n = 2304), enough to spread the stream of B and G
-// over the whole GPU.  The 16 chunk sums are added in chunk order by one thread per
-// site, so the result does not depend on the batch.
+// chunks of the matrix's own column range, one per 16-thread row group: with 16 rows
+// per workgroup there are n / 16 workgroups per matrix (144 at n = 2304), enough to
+// spread the stream of B and G over the whole GPU.  The 16 chunk sums are added in
+// chunk order by one thread per site, so the result does not depend on the batch.
 template <typename T, int NORB>
 __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long sB,
                                                     const cx<T>* G, long long sG, int n,
@@ -133,9 +133,11 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
 #pragma unroll
   for (int q = 0; q < NORB; ++q) accr[q] = acci[q] = 0.0;
   if (a < n) {
-    // the chunks are fixed by n, so the sum order does not depend on kcols
-    const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
-    for (int k = max(kbeg, kfirst); k < (kend < kcols ? kend : kcols); ++k) {
+    // the columns [kfirst, kcols) in 16 chunks fixed by this matrix's own range, so
+    // the sum order does not depend on the rest of the batch
+    const int k_lo = kfirst, k_n = kcols > kfirst ? kcols - kfirst : 0;
+    const int kbeg = k_lo + (k_n * kq) / KQ, kend = k_lo + (k_n * (kq + 1)) / KQ;
+    for (int k = kbeg; k < kend; ++k) {
       const cx<T> bv = Bb[(long long)k * n + a];
 #pragma unroll
       for (int q = 0; q < NORB; ++q) {
@@ -234,8 +236,9 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
 #pragma unroll
   for (int q = 0; q < NQ; ++q) accr[q] = acci[q] = 0.0;
   if (a < n) {
-    const int kbeg = (int)(((long long)n * kq) / KQ), kend = (int)(((long long)n * (kq + 1)) / KQ);
-    for (int k = max(kbeg, kfirst); k < (kend < kcols ? kend : kcols); ++k) {
+    const int k_lo = kfirst, k_n = kcols > kfirst ? kcols - kfirst : 0;
+    const int kbeg = k_lo + (k_n * kq) / KQ, kend = k_lo + (k_n * (kq + 1)) / KQ;
+    for (int k = kbeg; k < kend; ++k) {
       const long long col = (long long)k * n;
       const cx<T> va = Vb[col + a];
 #pragma unroll
