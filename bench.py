@@ -120,12 +120,13 @@ def cpu_baseline(mp_ctx, cfg, energies, k_mean, ranks, mode):
 
 
 # Committed summaries per (workload, kernel kind) at the shape the bench launches
-# the kernel with: `alone_tflops` (the kernel alone, bench.py --gemm-only) and,
-# where counters were collected, `traffic_bytes_per_launch` (rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading for
-# a copy kernel of known size).
-TRAFFIC_PROFILES = {("c4", "fp32_gemm_3m"): "traffic_c64_gemm3m_n2304_L200.md",
-                    ("c4", "fp64_gemm_3m"): "traffic_fp64_gemm3m_n2304_b2.md"}
+# the kernel with: `alone_tflops` (the kernel alone: bench.py --gemm-only for the
+# FP64 product, --inverse-only for the complex64 trailing update) and, where
+# counters were collected, `traffic_bytes_per_launch` (rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE passes, each scaled by the same counter's reading for a copy
+# kernel of known size).
+TRAFFIC_PROFILES = {("c4", "fp64_gemm_3m"): "traffic_fp64_gemm3m_n2304_b3.md",
+                    ("c4", "fp32_gj_update_3m"): "traffic_c64_gj_update3m_n2304_b3.md"}
 GEMM_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update", "fp64_gemm_3m",
               "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
 KERNEL_NAMES = {
@@ -134,6 +135,7 @@ KERNEL_NAMES = {
     "fp64_gj_update_3m": "k_gemm<double,0,1,1,2,2,0,1> (FP64 Gauss-Jordan trailing update, 3M form)",
     "fp64_gj_update": "k_gemm<double,0,1,1,2,2,*,0> (FP64 Gauss-Jordan trailing update)",
     "fp32_gemm_3m": "k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form)",
+    "fp32_gj_update_3m": "k_gemm<float,0,1,1,2,2,0,1> (complex64 Gauss-Jordan trailing update, 3M form)",
 }
 
 
@@ -172,6 +174,26 @@ def gemm_only(args):
                       "n": n, "batch": batch, "launches": reps + 1, "ms_per_launch": ms,
                       "algorithmic_flops_per_launch": flops, "tflops": flops / ms / 1e9,
                       "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.gemm_c64 else 16)}))
+
+
+def inverse_only(args):
+    """The complex64 Gauss-Jordan inverse alone (negf_invert_batch, 3M form) at the
+    block size of the workload and the batch one lane inverts, on diagonally
+    weighted random matrices: its rank-128 trailing updates (k_gemm<float,0,1,1,...>)
+    are the sweep's, for rocprofv3 --pmc counter runs."""
+    cfg = CONFIGS[args.config]
+    n = cfg["npy"] * cfg["npz"] * 4
+    per = args.per_step or cfg["per_step"]
+    batch = max(1, per // (args.lanes or 4))
+    rng = np.random.default_rng(7)
+    G = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))) / np.sqrt(n)
+    G += 4.0 * np.eye(n)
+    for _ in range(2):
+        t0 = time.perf_counter()
+        _, info = capi.invert_batch(G.astype(np.complex64), complex64=True, m3=True)
+    print(json.dumps({"kernel": "k_gemm<float,0,1,1,2,2,0,1> inside negf_invert_batch", "n": n, "batch": batch,
+                      "outer_blocks": (n + 127) // 128, "seconds_incl_host_copies": time.perf_counter() - t0,
+                      "info": info.tolist()}))
 
 
 def copy_calibration(mib):
@@ -427,6 +449,9 @@ def main():
                          "it with, for rocprofv3 --pmc counter runs")
     ap.add_argument("--gemm-c64", action="store_true",
                     help="with --gemm-only: the complex64 product (default FP64)")
+    ap.add_argument("--inverse-only", action="store_true",
+                    help="run only the complex64 Gauss-Jordan inverse at the block size and batch of one "
+                         "lane, for rocprofv3 --pmc counter runs of its trailing updates")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 8 times (the known-size "
                          "reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -436,6 +461,9 @@ def main():
         return
     if args.gemm_only:
         gemm_only(args)
+        return
+    if args.inverse_only:
+        inverse_only(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: this benchmark runs on exactly one GPU "
@@ -564,13 +592,13 @@ def main():
                               "executed_frac counts the 6 n^3 the 3M form runs on the matrix cores; the "
                               "lanes run concurrently, so durations include other lanes' kernels sharing "
                               "the CUs; traffic: HBM-side bytes per launch from FETCH_SIZE/WRITE_SIZE "
-                              "counter runs of the kernel alone at this shape (bench.py --gemm-only), "
-                              "scaled by a copy-kernel calibration")},
+                              "counter runs of the kernel alone at this shape (bench.py --gemm-only or "
+                              "--inverse-only), scaled by a copy-kernel calibration")},
         "kernels": kernel_table(prof, args.config),
     }
     alone = profile_value(args.config, dom, "alone_tflops") if dom else None
     if alone is not None:
-        # the same kernel and shape with the GPU to itself (bench.py --gemm-only)
+        # the same kernel and shape with the GPU to itself (bench.py --gemm-only or --inverse-only)
         result["roofline"]["alone"] = {"achieved": alone, "frac": alone / peak,
                                        "executed_frac": executed * alone / peak,
                                        "source": "profiles/" + TRAFFIC_PROFILES[(args.config, dom)]}

@@ -143,7 +143,8 @@ inline long long gj_ints(int n) { return 2LL * n + 1 + kGjCandInts; }
 // Gauss-Jordan panel launches: when enabled, one launch in `every` of each kind is
 // bracketed by HIP events on its own stream and its algorithmic work is recorded
 // with its kind.  Kinds 0..7 = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing
-// update), work = 8 m n k flops per complex product; kinds 8, 9 = the FP64 / FP32
+// update), work = 8 m n k flops per complex product (products with a transposed or
+// adjoint operand are not sampled, so each kind is one kernel instance); kinds 8, 9 = the FP64 / FP32
 // Gauss-Jordan panel, work = the bytes its columns move (read and written once).
 constexpr int kTimerKinds = 10;
 constexpr int kTimerPanel64 = 8;

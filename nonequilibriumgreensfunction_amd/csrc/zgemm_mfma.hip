@@ -402,7 +402,11 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
   const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
-  const bool timed = timer_slot(8.0 * g.m * (double)ncols * g.k * batch * g.work, kind, &ev0, &ev1);
+  // a kind's samples are one kernel instance: the plain products (both operands
+  // untransposed) and the Gauss-Jordan updates; the transposed and adjoint products
+  // (small block-row products and traces) are not sampled
+  const bool sampled = EPI == EPI_GJ || (OPA == OP_N && OPB == OP_N);
+  const bool timed = sampled && timer_slot(8.0 * g.m * (double)ncols * g.k * batch * g.work, kind, &ev0, &ev1);
   if (timed) {
     const hipError_t er = hipEventRecord(ev0, s);
     if (er != hipSuccess) return er;

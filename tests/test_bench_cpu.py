@@ -37,10 +37,16 @@ def test_traffic_summary_parses_and_exceeds_algorithmic_bytes():
             continue
         cfg = b.CONFIGS[config]
         n = cfg["npy"] * cfg["npz"] * 4
-        # two energies per lane: A and B read once, C written once
-        alg = 3 * 2 * n * n * (8 if kind.startswith("fp32") else 16)
+        batch = cfg["per_step"] // 4  # energies per lane
+        if "gj_update" in kind:
+            # rank-128 update: both thin operands read once, the updated n - 128
+            # columns of C read and written once
+            alg = batch * 8 * (2 * n * 128 + 2 * n * (n - 128))
+        else:  # A and B read once, C written once
+            alg = 3 * batch * n * n * (8 if kind.startswith("fp32") else 16)
         assert t >= alg, (t, alg)
-    assert b.traffic_from_profiles("c4", "fp32_gemm_3m") is not None
+    assert b.traffic_from_profiles("c4", "fp64_gemm_3m") is not None
+    assert b.traffic_from_profiles("c4", "fp32_gj_update_3m") is not None
     assert b.traffic_from_profiles("c2", "fp64_gemm_3m") is None
 
 
