@@ -10,7 +10,8 @@ module negf_capi
   integer(c_int), parameter :: NEGF_OK = 0, NEGF_ERR_ARG = -1, NEGF_ERR_HIP = -2, &
       NEGF_ERR_NO_DEVICE = -3, NEGF_ERR_MEMORY = -4, NEGF_ERR_STATE = -5
   integer(c_int), parameter :: NEGF_STATUS_SR_NOT_CONVERGED = 1048576, NEGF_STATUS_SR_NAN = 2097152
-  integer(c_int), parameter :: NEGF_PREC_FP64 = 0, NEGF_PREC_MIXED = 1, NEGF_PREC_REFINED = 2
+  integer(c_int), parameter :: NEGF_PREC_FP64 = 0, NEGF_PREC_MIXED = 1, NEGF_PREC_REFINED = 2, &
+                                NEGF_PREC_LU32 = 3
 
   type, bind(C) :: negf_config
     integer(c_int) :: npx, npy, npz, norb

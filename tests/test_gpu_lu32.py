@@ -1,14 +1,17 @@
 """The complex64 block-LU form (NEGF_PREC_LU32) against the complex128 oracle.
 
-Every RGF block inverse, both column chains and the site blocks run in complex64;
-the contacts stay FP64.  T12 = |W_L' G_1L W_R|^2 takes the leading columns of W_R
-(at most 64) from an FP64-refined solve of the block-tridiagonal system and the
-rest from the complex64 chain; an energy that does not pass the form's checks is
-solved again in the plain FP64 form.  Tolerances as for the mixed form: T12 within
+Every RGF block inverse and both column chains run in complex64; the contacts stay
+FP64.  The leading factor columns (at most 64) of both block columns are solved
+again to FP64 accuracy by iterative refinement of the block-tridiagonal system;
+every output sums over factor columns, taking those columns from the refined
+solve and the others from the complex64 chains.  T12 = |W_L' G_1L W_R|^2 adds the
+complex64 parts past W_R's refined columns (the first-row chain and the tail
+sketch, or the exact tail when the chain runs wide); an energy that does not pass
+the form's checks is solved again in the plain FP64 form.  Tolerances: T12 within
 1e-8 of its natural scale (the open-channel count) or the oracle's own spread,
-DOSc within 1e-7, the complex64-chain outputs (densities, spin, modes, currents)
-within 2e-5 of their scales; Sancho-Rubio iteration counts equal to the oracle's.
-The measured errors are printed."""
+DOSc within 1e-7, the other outputs (densities, spin, modes, currents) within 2e-5
+of their scales; Sancho-Rubio iteration counts equal to the oracle's.  The
+measured errors are printed."""
 import os
 import sys
 
@@ -73,7 +76,7 @@ def check(got, ref, noise, label):
     for k in ("Modec1", "Modec2", "EDensity", "HDensity", "RhoN", "RhoP", "SigX", "SigY", "SigZ",
               "ContactCurrent"):
         errs[k] = float(np.abs(np.asarray(got[k]) - np.asarray(ref[k])).max()) / sc[k]
-    print(f"{label}: complex64-path relative errors", {k: f"{v:.1e}" for k, v in errs.items()},
+    print(f"{label}: relative errors", {k: f"{v:.1e}" for k, v in errs.items()},
           flush=True)
     for k, v in errs.items():
         assert v <= C64_RTOL, (k, v)
