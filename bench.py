@@ -177,10 +177,12 @@ def gemm_only(args):
 
 
 def inverse_only(args):
-    """The complex64 Gauss-Jordan inverse alone (negf_invert_batch, 3M form) at the
+    """The complex64 block inverse of the LU32 sweep alone (negf_invert_batch_local:
+    block Gauss-Jordan pivoting inside 128-column diagonal blocks, 3M form) at the
     block size of the workload and the batch one lane inverts, on diagonally
-    weighted random matrices: its rank-128 trailing updates (k_gemm<float,0,1,1,...>)
-    are the sweep's, for rocprofv3 --pmc counter runs."""
+    weighted random matrices: its rank-128 trailing updates (k_gemm<float,0,1,1,...>
+    over all n columns) and diagonal-block inverses are the sweep's launches, for
+    rocprofv3 --pmc counter runs."""
     cfg = CONFIGS[args.config]
     n = cfg["npy"] * cfg["npz"] * 4
     per = args.per_step or cfg["per_step"]
@@ -190,8 +192,8 @@ def inverse_only(args):
     G += 4.0 * np.eye(n)
     for _ in range(2):
         t0 = time.perf_counter()
-        _, info = capi.invert_batch(G.astype(np.complex64), complex64=True, m3=True)
-    print(json.dumps({"kernel": "k_gemm<float,0,1,1,2,2,0,1> inside negf_invert_batch", "n": n, "batch": batch,
+        _, info = capi.invert_batch(G.astype(np.complex64), complex64=True, m3=True, local=True)
+    print(json.dumps({"kernel": "k_gemm<float,0,1,1,2,2,0,1> inside negf_invert_batch_local", "n": n, "batch": batch,
                       "outer_blocks": (n + 127) // 128, "seconds_incl_host_copies": time.perf_counter() - t0,
                       "info": info.tolist()}))
 

@@ -293,6 +293,17 @@ int negf_zgemm(char transa, char transb, int n, const double* alpha, const doubl
  *                      (1-based) or 0. */
 int negf_gemm_batch(int complex64, int m3, int n, int batch, const void* A, const void* B, void* C);
 int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG, int* info);
+/*   negf_invert_batch_local: invG_b = G_b^-1 by block Gauss-Jordan with pivoting
+ *                      inside the diagonal blocks only (128 columns in complex64, 64
+ *                      in complex128), the inverse NEGF_PREC_LU32 takes for its RGF
+ *                      blocks and its decimation.  No counterpart in the reference
+ *                      (Invert_Matrix pivots over all rows): there are no row
+ *                      interchanges between blocks, so it is accurate only when every
+ *                      diagonal block met during the elimination is well conditioned,
+ *                      and is meant for callers that check their results.  info[b] =
+ *                      the first zero pivot inside a diagonal block (1-based) or 0;
+ *                      the inverse is then undefined. */
+int negf_invert_batch_local(int complex64, int m3, int n, int batch, const void* G, void* invG, int* info);
 
 /* Allocation-free form of negf_invert_batch.  No counterpart in the reference; for
  * callers that cannot allocate device memory -- code under HIP graph capture, tight

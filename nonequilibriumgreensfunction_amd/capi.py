@@ -39,7 +39,7 @@ EXPORTS = (
     "negf_server_create", "negf_server_submit", "negf_server_poll",
     "negf_server_wait", "negf_server_destroy", "negf_bench_gemm", "negf_gemm_batch",
     "negf_invert_batch", "negf_refine_stats", "negf_graph_stats", "negf_bench_gemm_tile",
-    "negf_set_gemm_tiles", "negf_set_lane_table", "negf_invert_batch_noalloc",
+    "negf_set_gemm_tiles", "negf_set_lane_table", "negf_invert_batch_noalloc", "negf_invert_batch_local",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -118,6 +118,7 @@ def lib():
         L.negf_profile_begin.argtypes = [ctypes.c_int]
         L.negf_gemm_batch.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3
         L.negf_invert_batch.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [_ip]
+        L.negf_invert_batch_local.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [_ip]
         L.negf_server_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(NegfServerConfig)]
         L.negf_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(NegfRequest),
                                          ctypes.POINTER(ctypes.c_longlong)]
@@ -240,17 +241,19 @@ def gemm_batch(A, B, complex64=False, m3=True):
     return np.swapaxes(c, 1, 2)
 
 
-def invert_batch(G, complex64=False, m3=True):
+def invert_batch(G, complex64=False, m3=True, local=False):
     """(inverses, info) of a stack G of shape (batch, n, n) with the sweep's
-    Gauss-Jordan inverse in the given form (negf_invert_batch)."""
+    Gauss-Jordan inverse in the given form (negf_invert_batch), or with local=True
+    the block Gauss-Jordan with pivoting inside diagonal blocks
+    (negf_invert_batch_local)."""
     dt = np.complex64 if complex64 else np.complex128
     batch, n, _ = G.shape
     g = np.ascontiguousarray(np.swapaxes(G, 1, 2), dtype=dt)
     out = np.zeros_like(g)
     info = np.zeros(batch, np.int32)
-    _check(lib().negf_invert_batch(int(bool(complex64)), int(bool(m3)), n, batch,
-                                   g.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
-                                   info.ctypes.data_as(_ip)))
+    fn = lib().negf_invert_batch_local if local else lib().negf_invert_batch
+    _check(fn(int(bool(complex64)), int(bool(m3)), n, batch, g.ctypes.data_as(ctypes.c_void_p),
+              out.ctypes.data_as(ctypes.c_void_p), info.ctypes.data_as(_ip)))
     return np.swapaxes(out, 1, 2), info
 
 

@@ -493,143 +493,264 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
 // for callers that check their results (the complex64 block LU whose solves are
 // refined in FP64, which falls back to the partially pivoted FP64 inverse when
 // the refinement does not contract).  4 launches per 128 columns instead of 50.
-// the diagonal block: 128 columns in complex64, 64 in complex128 (registers per thread)
-template <typename T> struct LocalBlock { static constexpr int kb = sizeof(T) == sizeof(float) ? 128 : 64; };
+
+// Pivot keys of the diagonal-block inverse: the magnitude |re|+|im| with its low
+// bits replaced by the row's rank (127 - i), so the larger magnitude wins and ties
+// go to the lower row.  complex64: a 32-bit key (magnitudes within 2^-16 relative
+// count as ties; a magnitude below 2^-126 counts as zero); complex128: the 64-bit key
+// of the partially pivoted inverse (pivot_key).  The row bits of a real candidate
+// never make a key zero unless the row is 127 with magnitude zero, which the
+// zero-pivot test sees either way.
+template <typename T> struct LocalKey;
+template <> struct LocalKey<float> {
+  typedef unsigned int K;
+  __device__ static K make(float2 x, int i) {
+    float m = fabsf(x.x) + fabsf(x.y);
+    if (m != m) m = INFINITY;
+    return (__float_as_uint(m) & ~0x7fu) | (unsigned)(127 - i);
+  }
+  __device__ static int row(K k) { return 127 - (int)(k & 0x7fu); }
+  __device__ static bool zero(K k) { return (k >> 7) == 0; }
+};
+template <> struct LocalKey<double> {
+  typedef unsigned long long K;
+  __device__ static K make(double2 x, int i) { return pivot_key(x, i); }
+  __device__ static int row(K k) { return 4095 - (int)(k & 0xfffull); }
+  __device__ static bool zero(K k) { return (k & ~0xfffull) == 0; }
+};
+
+// One 32-bit lane value through DPP; rows outside `RM` keep `old`.
+template <int CTRL, int RM>
+__device__ inline unsigned dpp32(unsigned old, unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ inline unsigned dpp_max(unsigned x) {
+  const unsigned y = dpp32<CTRL, RM>(x, x);
+  return x > y ? x : y;
+}
+template <int CTRL, int RM>
+__device__ inline unsigned long long dpp_max(unsigned long long x) {
+  const unsigned lo = dpp32<CTRL, RM>((unsigned)x, (unsigned)x);
+  const unsigned hi = dpp32<CTRL, RM>((unsigned)(x >> 32), (unsigned)(x >> 32));
+  const unsigned long long y = ((unsigned long long)hi << 32) | lo;
+  return x > y ? x : y;
+}
+__device__ inline unsigned readlane_u(unsigned x, int l) { return (unsigned)__builtin_amdgcn_readlane((int)x, l); }
+__device__ inline unsigned long long readlane_u(unsigned long long x, int l) {
+  return ((unsigned long long)readlane_u((unsigned)(x >> 32), l) << 32) | readlane_u((unsigned)x, l);
+}
+// Max over the wave, returned uniform: within each row of 16 lanes (quad xor 1, quad
+// xor 2, half-row mirror, row mirror), then lane 15 of rows 0 and 2 into rows 1 and
+// 3, lane 31 into rows 2 and 3, so lane 63 holds the maximum; read from there.
+template <typename K>
+__device__ inline K wave_max_uniform(K x) {
+  x = dpp_max<0xb1, 0xf>(x);
+  x = dpp_max<0x4e, 0xf>(x);
+  x = dpp_max<0x141, 0xf>(x);
+  x = dpp_max<0x140, 0xf>(x);
+  x = dpp_max<0x142, 0xa>(x);  // row_bcast:15
+  x = dpp_max<0x143, 0xc>(x);  // row_bcast:31
+  return readlane_u(x, 63);
+}
+template <typename V>
+__device__ inline V readlane_c(V x, int l);
+template <>
+__device__ inline float2 readlane_c(float2 x, int l) {
+  return mk<float>(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.x), l)),
+                   __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x.y), l)));
+}
+template <>
+__device__ inline double2 readlane_c(double2 x, int l) {
+  auto rd = [&](double d) {
+    const long long b = __double_as_longlong(d);
+    const unsigned lo = readlane_u((unsigned)b, l), hi = readlane_u((unsigned)((unsigned long long)b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  };
+  return mk<double>(rd(x.x), rd(x.y));
+}
 
 // The diagonal block's inverse (blocks of KB = 128 columns in complex64, 64 in
 // complex128; KB x 8 threads).  Thread (r, g) = (tid % KB, tid / KB) holds row r of D
-// in columns g + 8 q (q < KB / 8).  Per 8-column panel: its columns go to LDS, one
-// wave runs the panel's Gauss-Jordan steps on all the block's rows (KB / 64 rows per
-// lane; partial pivoting among the unused rows, shuffles only), publishing each
-// row's final panel values S[r][.] and the pivot rows; the pivot rows' values in
-// every column go to LDS; then each thread updates its other columns as the
-// Gauss-Jordan steps do,
-//   v[c] <- [r not a pivot of the panel] v[c] + sum_m S[r][m] Prow[m][c],
-// and takes S[r][.] in the panel's own columns.  Three barriers per 8 columns.
-// Finally inv(D)[pstep[r]][prow[c]] = D[r][c] goes to Dinv (kb x kb, column-major).
+// in columns g + 8 q (q < KB / 8): one column of every 8-column panel.  Per panel P:
+//   factor -- one wave (wave 0) runs the panel's Gauss-Jordan steps on all the
+//             block's rows (KB / 64 rows per lane; partial pivoting among the unused
+//             rows: a DPP max to lane 63, the pivot row read with readlane) and
+//             publishes each row's final panel values S[r][.] and the pivots;
+//   update -- every thread takes the steps in its other columns,
+//               v[c] <- [r not a pivot of the panel] v[c] + sum_m S[r][m] Prow[m][c],
+//             with Prow the pivot rows (staged in LDS), and S[r][.] in the panel's own
+//             columns.
+// Look-ahead: each thread first updates its column of panel P + 1 with panel P and
+// publishes it, so wave 0 factors panel P + 1 while the other waves update the rest
+// of the block with panel P.  S and the pivots are double-buffered; three barriers per
+// panel.  Finally inv(D)[pstep[r]][prow[c]] = D[r][c] goes to Dinv (kb x kb,
+// column-major), and info gets the first zero pivot (1-based) if it has none yet.
 template <typename T, int KB>
 __global__ __launch_bounds__(KB * 8) void k_gjl_diag(const cx<T>* S, long long sS, int ld, int k0, int kb,
                                                      cx<T>* Dinv, long long sD, int* info, long long sInt,
                                                      const int* active) {
   typedef cx<T> V;
+  typedef typename LocalKey<T>::K Key;
   constexpr int NQ = KB / 8, HR = KB / 64;
   const int b = active ? active[blockIdx.x] : (int)blockIdx.x;
   const V* A = S + (long long)b * sS;
   V* Di = Dinv + (long long)b * sD;
   const int tid = threadIdx.x, lane = tid & 63;
   const int r = tid % KB, cg = tid / KB;
-  __shared__ V pan[kNBI][KB];          // the panel's columns
-  __shared__ V stab[KB][kNBI + 1];     // each row's final panel values (padded: no bank conflicts)
-  __shared__ V prw[kNBI][KB];          // the panel's pivot rows, every column
-  __shared__ int pv[kNBI];
+  __shared__ V pan[kNBI][KB];            // the next panel's columns
+  __shared__ V stab[2][KB][kNBI + 1];    // each row's final panel values (padded)
+  __shared__ V prw[kNBI][KB];            // the panel's pivot rows, every column
+  __shared__ int pv[2][kNBI];            // the panel's pivot rows
   __shared__ int pstep[KB], prow[KB];
+  __shared__ V vstash[NQ][64];           // wave 0's own columns while it factors
   V v[NQ];
 #pragma clang loop unroll(full)
   for (int q = 0; q < NQ; ++q) {
     const int c = cg + 8 * q;
     v[q] = load_sel(A + (long long)(k0 + (c < kb ? c : 0)) * ld + k0 + (r < kb ? r : 0), r < kb && c < kb);
   }
+  if (tid < KB) pstep[tid] = -1;
+  const int npan = (kb + kNBI - 1) / kNBI;
+  auto panel_nb = [&](int P) { return (kb - P * kNBI < kNBI) ? kb - P * kNBI : kNBI; };
+  int* inf = info + (long long)b * sInt;
   bool used[HR];  // wave 0: rows lane + 64 h
 #pragma clang loop unroll(full)
   for (int h = 0; h < HR; ++h) used[h] = lane + 64 * h >= kb;
-  int* inf = info + (long long)b * sInt;
-  for (int P = 0; P * kNBI < kb; ++P) {
-    const int nb = (kb - P * kNBI < kNBI) ? kb - P * kNBI : kNBI;
-    if (cg < nb) {
-      V x = v[0];
+  // wave 0: the Gauss-Jordan steps of panel P on the panel's columns (from pan)
+  auto factor = [&](int P, int buf) {
+    const int nb = panel_nb(P);
+    V w[HR][kNBI];
 #pragma clang loop unroll(full)
-      for (int q = 1; q < NQ; ++q)
-        if (q == P) x = v[q];
-      pan[cg][r] = x;
-    }
-    __syncthreads();
-    if (tid < 64) {
-      V w[HR][kNBI];
+    for (int h = 0; h < HR; ++h)
 #pragma clang loop unroll(full)
-      for (int h = 0; h < HR; ++h)
+      for (int j = 0; j < kNBI; ++j) w[h][j] = j < nb ? pan[j][lane + 64 * h] : mk<T>(0, 0);
 #pragma clang loop unroll(full)
-        for (int j = 0; j < kNBI; ++j) w[h][j] = j < nb ? pan[j][lane + 64 * h] : mk<T>(0, 0);
+    for (int j = 0; j < kNBI; ++j) {
+      if (j < nb) {
+        Key k = 0;
 #pragma clang loop unroll(full)
-      for (int j = 0; j < kNBI; ++j) {
-        if (j < nb) {
-          unsigned long long k = 0;
+        for (int h = 0; h < HR; ++h) {
+          const Key kh = used[h] ? (Key)0 : LocalKey<T>::make(w[h][j], lane + 64 * h);
+          k = kh > k ? kh : k;
+        }
+        const Key g = wave_max_uniform(k);
+        if (LocalKey<T>::zero(g) && lane == 0 && *inf == 0) *inf = k0 + P * kNBI + j + 1;  // zero pivot
+        int p = __builtin_amdgcn_readfirstlane(LocalKey<T>::row(g));
+        p = (p >= 0 && p < kb) ? p : 0;
+        const int lp = p & 63, hi = p >> 6;
+        V t[kNBI];
 #pragma clang loop unroll(full)
-          for (int h = 0; h < HR; ++h)
-            if (!used[h]) k = umax64(k, pivot_key(w[h][j], lane + 64 * h));
-          const unsigned long long g = wave_max_u64(k);
-          int p = 4095 - (int)(g & 0xfffull);
-          if ((g & ~0xfffull) == 0 && lane == 0 && *inf == 0) *inf = k0 + P * kNBI + j + 1;  // zero pivot
-          p = (g != 0 && p < kb) ? p : 0;
-          const int lp = p & 63, hi = p >> 6;
-          V t[kNBI];
+        for (int c = 0; c < kNBI; ++c) {
+          V x = w[0][c];
+#pragma clang loop unroll(full)
+          for (int h = 1; h < HR; ++h)
+            if (hi == h) x = w[h][c];
+          t[c] = readlane_c(x, lp);
+        }
+        const V inva = cinv(t[j]);
+#pragma clang loop unroll(full)
+        for (int c = 0; c < kNBI; ++c) t[c] = (c == j) ? inva : cmul(t[c], inva);
+        // the pivot row becomes t, the others take the step (selects, no branches:
+        // every lane runs the same instructions)
+#pragma clang loop unroll(full)
+        for (int h = 0; h < HR; ++h) {
+          const bool pr = lane + 64 * h == p;
+          const V f = w[h][j];
 #pragma clang loop unroll(full)
           for (int c = 0; c < kNBI; ++c) {
-            V x = w[0][c];
-#pragma clang loop unroll(full)
-            for (int h = 1; h < HR; ++h)
-              if (hi == h) x = w[h][c];
-            t[c].x = __shfl(x.x, lp);
-            t[c].y = __shfl(x.y, lp);
+            const V u = (c == j) ? cneg(cmul(f, t[j])) : csub(w[h][c], cmul(f, t[c]));
+            w[h][c].x = pr ? t[c].x : u.x;
+            w[h][c].y = pr ? t[c].y : u.y;
           }
-          const V inva = cinv(t[j]);
-#pragma clang loop unroll(full)
-          for (int c = 0; c < kNBI; ++c) t[c] = (c == j) ? inva : cmul(t[c], inva);
-          // the pivot row becomes t, the others take the step
-#pragma clang loop unroll(full)
-          for (int h = 0; h < HR; ++h) {
-            if (lane + 64 * h == p) {
-#pragma clang loop unroll(full)
-              for (int c = 0; c < kNBI; ++c) w[h][c] = t[c];
-              used[h] = true;
-            } else {
-              const V f = w[h][j];
-#pragma clang loop unroll(full)
-              for (int c = 0; c < kNBI; ++c)
-                w[h][c] = (c == j) ? cneg(cmul(f, t[j])) : csub(w[h][c], cmul(f, t[c]));
-            }
-          }
-          if (lane == 0) {
-            pv[j] = p;
-            pstep[p] = P * kNBI + j;
-            prow[P * kNBI + j] = p;
-          }
+          used[h] = used[h] || pr;
         }
+        // uniform values: every lane writes the same word
+        pv[buf][j] = p;
+        pstep[p] = P * kNBI + j;
+        prow[P * kNBI + j] = p;
       }
-#pragma clang loop unroll(full)
-      for (int h = 0; h < HR; ++h)
-#pragma clang loop unroll(full)
-        for (int j = 0; j < kNBI; ++j) stab[lane + 64 * h][j] = w[h][j];
     }
-    __syncthreads();
-    int ip = -1;  // this row's place among the panel's pivots
+#pragma clang loop unroll(full)
+    for (int h = 0; h < HR; ++h)
+#pragma clang loop unroll(full)
+      for (int j = 0; j < kNBI; ++j) stab[buf][lane + 64 * h][j] = w[h][j];
+  };
+  // this row's place among panel P's pivots (-1: not a pivot)
+  auto place = [&](int P, int buf) {
+    const int nb = panel_nb(P);
+    int ip = -1;
 #pragma clang loop unroll(full)
     for (int m = 0; m < kNBI; ++m)
-      if (m < nb && pv[m] == r) ip = m;
+      if (m < nb && pv[buf][m] == r) ip = m;
+    return ip;
+  };
+  // pivot rows of panel P: their values in every column to prw
+  auto stage_rows = [&](int P, int buf) {
+    const int ip = place(P, buf);
     if (ip >= 0) {
 #pragma clang loop unroll(full)
       for (int q = 0; q < NQ; ++q) prw[ip][cg + 8 * q] = v[q];
     }
-    __syncthreads();
-    if (r < kb) {
-      V sr[kNBI];
+  };
+  // P = -1 only factors panel 0; P >= 0 applies panel P and factors panel P + 1
+  for (int P = -1; P < npan; ++P) {
+    const int buf = P & 1;
+    const bool ahead = P + 1 < npan, have = P >= 0;
+    const int nb = have ? panel_nb(P) : 0;
+    const int ip = have ? place(P, buf) : -1;
+    V sr[kNBI];  // loaded before each use: wave 0 does not hold them through factor()
+    auto load_sr = [&]() {
 #pragma clang loop unroll(full)
-      for (int m = 0; m < kNBI; ++m) sr[m] = stab[r][m];
+      for (int m = 0; m < kNBI; ++m) sr[m] = stab[buf][r][m];
+    };
+    // one column q (not of panel P) takes panel P's steps
+    auto step_col = [&](int q) {
+      V acc = ip >= 0 ? mk<T>(0, 0) : v[q];
+#pragma clang loop unroll(full)
+      for (int m = 0; m < kNBI; ++m)
+        if (m < nb) acc = cadd(acc, cmul(sr[m], prw[m][cg + 8 * q]));
+      v[q] = acc;
+    };
+    if (ahead) {  // look-ahead: panel P + 1's columns first, published for wave 0
+      if (have) {
+        load_sr();
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NQ; ++q)
+          if (q == P + 1) step_col(q);
+      }
+      if (cg < panel_nb(P + 1)) {
+        V x = v[0];
+#pragma clang loop unroll(full)
+        for (int q = 1; q < NQ; ++q)
+          if (q == P + 1) x = v[q];
+        pan[cg][r] = x;
+      }
+    }
+    __syncthreads();
+    if (ahead && tid < 64) {  // wave 0 parks its columns in LDS while it factors
+#pragma clang loop unroll(full)
+      for (int q = 0; q < NQ; ++q) vstash[q][lane] = v[q];
+      factor(P + 1, buf ^ 1);
+#pragma clang loop unroll(full)
+      for (int q = 0; q < NQ; ++q) v[q] = vstash[q][lane];
+    }
+    // (rows past kb hold zeros and stay zero: no row test, so every branch here is uniform)
+    if (have) {
+      load_sr();
 #pragma clang loop unroll(full)
       for (int q = 0; q < NQ; ++q) {
-        const int c = cg + 8 * q;
         if (q == P) {
-          if (cg < nb) v[q] = stab[r][cg];
-        } else {
-          V acc = ip >= 0 ? mk<T>(0, 0) : v[q];
-#pragma clang loop unroll(full)
-          for (int m = 0; m < kNBI; ++m)
-            if (m < nb) acc = cadd(acc, cmul(sr[m], prw[m][c]));
-          v[q] = acc;
+          if (cg < nb) v[q] = stab[buf][r][cg];
+        } else if (q != P + 1 || !ahead) {
+          step_col(q);
         }
       }
     }
+    __syncthreads();
+    if (ahead) stage_rows(P + 1, buf ^ 1);
+    __syncthreads();
   }
-  __syncthreads();
   if (r < kb) {
     const int a = pstep[r];
 #pragma clang loop unroll(full)

@@ -129,7 +129,11 @@ template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
 // In-place inverse by block Gauss-Jordan with pivoting inside diagonal blocks only
-// (gj_inverse.hip; 128-column blocks in complex64, 64 in complex128).
+// (gj_inverse.hip; 128-column blocks in complex64, 64 in complex128: the diagonal
+// block's inverse holds it in registers).  Workspace per matrix: GjWork::prow holds
+// n x kb then kb x kb elements (kb = min(n, LocalBlock<T>::kb); stride sProw),
+// GjWork::tmp n x kb (stride sTmp), pivstep n ints; info is not cleared.
+template <typename T> struct LocalBlock { static constexpr int kb = sizeof(T) == sizeof(float) ? 128 : 64; };
 template <typename T>
 hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n, int batch, const GjWork& w,
                            const int* active);

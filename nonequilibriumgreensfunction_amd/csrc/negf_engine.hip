@@ -2496,6 +2496,64 @@ extern "C" int negf_gemm_batch(int complex64, int m3, int n, int batch, const vo
                    : gemm_batch_host<double>(m3, n, batch, A, B, C);
 }
 
+// Block Gauss-Jordan with pivoting inside the diagonal blocks (gj_invert_local) on
+// copies of the host matrices: the inverse the complex64 block-LU form's RGF and its
+// decimation take.
+template <typename T>
+static int invert_local_host(int m3, int n, int batch, const void* G, void* invG, int* info) {
+  typedef cx<T> V;
+  constexpr int KB = LocalBlock<T>::kb;
+  const long long nn = (long long)n * n;
+  const size_t bytes = sizeof(V) * (size_t)nn * batch;
+  const long long kb = n < KB ? n : KB;
+  const long long sWork = (long long)n * kb + kb * kb;  // R^T (n x kb), then Dinv (kb x kb)
+  const long long sInt = gj_ints(n);
+  V *dA = nullptr, *work = nullptr, *tmp = nullptr;
+  int* ints = nullptr;
+  hipError_t e = hipSuccess;
+  auto step = [&](hipError_t r) { if (e == hipSuccess) e = r; return e == hipSuccess; };
+  if (step(hipMalloc(&dA, bytes)) && step(hipMalloc(&work, sizeof(V) * (size_t)sWork * batch)) &&
+      step(hipMalloc(&tmp, sizeof(V) * (size_t)n * kb * batch)) &&
+      step(hipMalloc(&ints, sizeof(int) * (size_t)sInt * batch)) &&
+      step(hipMemcpy(dA, G, bytes, hipMemcpyHostToDevice)) &&
+      step(hipMemset(ints, 0, sizeof(int) * (size_t)sInt * batch))) {
+    GjWork w;
+    w.prow = work;
+    w.sProw = sWork;
+    w.pivstep = ints;
+    w.pivrow = ints + n;
+    w.info = ints + 2 * n;
+    w.sInt = sInt;
+    w.m3 = m3 ? 1 : 0;
+    w.tmp = tmp;
+    w.sTmp = (long long)n * kb;
+    std::vector<int> h((size_t)sInt * batch);
+    if (step(gj_invert_local<T>(nullptr, dA, nn, n, n, batch, w, nullptr)) &&
+        step(hipMemcpy(invG, dA, bytes, hipMemcpyDeviceToHost)) &&
+        step(hipMemcpy(h.data(), ints, sizeof(int) * h.size(), hipMemcpyDeviceToHost))) {
+      for (int b = 0; b < batch; ++b)
+        if (info) info[b] = h[(size_t)b * sInt + 2 * n];
+    }
+  }
+  (void)hipFree(dA);
+  (void)hipFree(work);
+  (void)hipFree(tmp);
+  (void)hipFree(ints);
+  if (e != hipSuccess) {
+    negf_set_error("negf_invert_batch_local: %s", hipGetErrorString(e));
+    return NEGF_ERR_HIP;
+  }
+  return 0;
+}
+
+extern "C" int negf_invert_batch_local(int complex64, int m3, int n, int batch, const void* G, void* invG,
+                                       int* info) {
+  int rc = batch_args_ok(n, batch, G, invG, G);
+  if (rc) return rc;
+  return complex64 ? invert_local_host<float>(m3, n, batch, G, invG, info)
+                   : invert_local_host<double>(m3, n, batch, G, invG, info);
+}
+
 extern "C" int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG,
                                  int* info) {
   int rc = batch_args_ok(n, batch, G, invG, G);

@@ -123,3 +123,61 @@ def test_invert_forms_at_sweep_sizes(rng, n, batch, m3):
         rel = np.abs(X[b] - ref).max() / np.abs(ref).max()
         assert rel < 1e-12, (b, rel)
         assert np.abs(G[b] @ X[b] - np.eye(n)).max() < 1e-11
+
+
+def _block_shuffled(rng, n, kb, dt):
+    # diagonally weighted (eigenvalues of R / sqrt(n) + 4 I lie in 4 +- 1.5), then the
+    # rows of every kb-row diagonal block shuffled among themselves: the diagonal blocks
+    # stay well conditioned, and the elimination needs row interchanges inside them
+    G = _rand(rng, n) / np.sqrt(n) + 4.0 * np.eye(n)
+    perm = np.concatenate([k0 + rng.permutation(min(kb, n - k0)) for k0 in range(0, n, kb)])
+    return G[perm].astype(dt).astype(np.complex128)
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 100, 128, 129, 300, 1024, 2304])
+@pytest.mark.parametrize("c64", [True, False], ids=["c64", "c128"])
+def test_invert_local_matches_lapack(rng, n, c64):
+    """negf_invert_batch_local (block Gauss-Jordan pivoting inside 128-column
+    diagonal blocks in complex64, 64-column in complex128; the complex64 block-LU
+    form's inverse) against LAPACK: one partial block up to the benchmark's 2304
+    (18 and 36 blocks).  complex128: 1e-12 relative like the partially pivoted
+    inverse; complex64: within 16x LAPACK's own complex64 inverse (cgetrf/cgetri
+    through numpy) on the same matrix, or 16 eps."""
+    kb = 128 if c64 else 64
+    dt = np.complex64 if c64 else np.complex128
+    batch = 2
+    print(f"local inverse c64={c64} n={n} batch={batch}", flush=True)
+    G = np.stack([_block_shuffled(rng, n, kb, dt) for _ in range(batch)])
+    X, info = capi.invert_batch(G, complex64=c64, m3=True, local=True)
+    assert np.all(info == 0), info
+    for b in range(batch):
+        ref = np.linalg.inv(G[b])
+        rel = np.abs(X[b] - ref).max() / np.abs(ref).max()
+        if c64:
+            lap = np.linalg.inv(G[b].astype(np.complex64)).astype(np.complex128)
+            rel_lap = np.abs(lap - ref).max() / np.abs(ref).max()
+            eps = np.finfo(np.float32).eps
+            print(f"  n={n} b={b}: rel err {rel:.2e}, LAPACK complex64 {rel_lap:.2e}", flush=True)
+            assert rel <= max(16 * rel_lap, 16 * eps), (b, rel, rel_lap)
+        else:
+            assert rel < 1e-12, (b, rel)
+            assert np.abs(G[b] @ X[b] - np.eye(n)).max() < 1e-11
+
+
+@pytest.mark.parametrize("c64", [True, False], ids=["c64", "c128"])
+def test_invert_local_reports_zero_pivot_in_a_diagonal_block(rng, c64):
+    # [[0, I], [I, 0]] is its own inverse, but its first diagonal block is zero: the
+    # block Gauss-Jordan, which pivots only inside diagonal blocks, meets a zero pivot
+    # in column 1 (info 1).  The second matrix of the batch is regular: its inverse and
+    # info are unaffected.
+    kb = 128 if c64 else 64
+    n = 2 * kb
+    swap = np.zeros((n, n), complex)
+    swap[:kb, kb:] = np.eye(kb)
+    swap[kb:, :kb] = np.eye(kb)
+    good = _block_shuffled(rng, n, kb, np.complex64 if c64 else np.complex128)
+    X, info = capi.invert_batch(np.stack([swap, good]), complex64=c64, m3=True, local=True)
+    assert info.tolist() == [1, 0], info
+    ref = np.linalg.inv(good)
+    rel = np.abs(X[1] - ref).max() / np.abs(ref).max()
+    assert rel < (1e-5 if c64 else 1e-12), rel
