@@ -39,9 +39,8 @@ def test_traffic_summary_parses_and_exceeds_algorithmic_bytes():
         n = cfg["npy"] * cfg["npz"] * 4
         batch = cfg["per_step"] // 4  # energies per lane
         if "gj_update" in kind:
-            # rank-128 update: both thin operands read once, the updated n - 128
-            # columns of C read and written once
-            alg = batch * 8 * (2 * n * 128 + 2 * n * (n - 128))
+            # rank-128 update: both thin operands read once, C read and written once
+            alg = batch * 8 * (2 * n * 128 + 2 * n * n)
         else:  # A and B read once, C written once
             alg = 3 * batch * n * n * (8 if kind.startswith("fp32") else 16)
         assert t >= alg, (t, alg)
