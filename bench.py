@@ -128,7 +128,7 @@ def cpu_baseline(mp_ctx, cfg, energies, k_mean, ranks, mode):
 TRAFFIC_PROFILES = {("c4", "fp64_gemm_3m"): "traffic_fp64_gemm3m_n2304_b3.md",
                     ("c4", "fp32_gj_update_3m"): "traffic_c64_gj_update3m_n2304_b3.md"}
 GEMM_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update", "fp64_gemm_3m",
-              "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m")
+              "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m", "fp32x3_gemm", "fp32x3_gj_update")
 KERNEL_NAMES = {
     "fp64_gemm_3m": "k_gemm<double,0,0,0,2,2,0,1> (FP64 MFMA complex product, 3M form)",
     "fp64_gemm": "k_gemm<double,0,0,0,2,2,0,0> (FP64 MFMA complex product, 4-product form)",
@@ -136,7 +136,20 @@ KERNEL_NAMES = {
     "fp64_gj_update": "k_gemm<double,0,1,1,2,2,*,0> (FP64 Gauss-Jordan trailing update)",
     "fp32_gemm_3m": "k_gemm<float,0,0,0,2,2,0,1> (complex64 MFMA product, 3M form)",
     "fp32_gj_update_3m": "k_gemm<float,0,1,1,2,2,0,1> (complex64 Gauss-Jordan trailing update, 3M form)",
+    "fp32x3_gemm": "k_gemm_x3<0,0,0> (complex64 product on the bf16 matrix cores, operands split three ways)",
+    "fp32x3_gj_update": "k_gemm_x3<0,1,1> (complex64 Gauss-Jordan trailing update on the bf16 matrix cores)",
 }
+PEAK_BF16_MFMA = 2500.0  # TFLOP/s, MI355X dense bf16 matrix
+
+
+def kind_peak(kind):
+    """(peak TFLOP/s of the matrix cores a kind runs on, executed flops per
+    algorithmic flop): the 3M forms run 6 of 8 m n k; the bf16 split form runs four
+    real products of six bf16 products each, 48 m n k."""
+    if kind and kind.startswith("fp32x3"):
+        return PEAK_BF16_MFMA, 6.0
+    peak = PEAK_FP32_MFMA if kind and kind.startswith("fp32") else PEAK_FP64_MFMA
+    return peak, 0.75 if kind and kind.endswith("_3m") else 1.0
 
 
 def profile_value(config, kind, key):
@@ -168,12 +181,16 @@ def gemm_only(args):
     lanes = args.lanes or 4
     batch = max(1, per // lanes)
     reps = 8
-    ms = capi.bench_gemm(n, batch, complex64=args.gemm_c64, m3=True, reps=reps)
+    m3 = 2 if args.gemm_bf16x3 else True
+    esize = 8 if (args.gemm_c64 or args.gemm_bf16x3) else 16
+    ms = capi.bench_gemm(n, batch, complex64=args.gemm_c64 or args.gemm_bf16x3, m3=m3, reps=reps)
     flops = 8.0 * n ** 3 * batch
-    print(json.dumps({"kernel": "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.gemm_c64 else "double"),
+    kname = ("k_gemm_x3<0,0,0>" if args.gemm_bf16x3 else
+             "k_gemm<%s,0,0,0,2,2,0,1>" % ("float" if args.gemm_c64 else "double"))
+    print(json.dumps({"kernel": kname,
                       "n": n, "batch": batch, "launches": reps + 1, "ms_per_launch": ms,
                       "algorithmic_flops_per_launch": flops, "tflops": flops / ms / 1e9,
-                      "algorithmic_bytes_per_launch": 3 * batch * n * n * (8 if args.gemm_c64 else 16)}))
+                      "algorithmic_bytes_per_launch": 3 * batch * n * n * esize}))
 
 
 def inverse_only(args):
@@ -192,7 +209,8 @@ def inverse_only(args):
     G += 4.0 * np.eye(n)
     for _ in range(2):
         t0 = time.perf_counter()
-        _, info = capi.invert_batch(G.astype(np.complex64), complex64=True, m3=True, local=True)
+        _, info = capi.invert_batch(G.astype(np.complex64), complex64=True, m3=True if args.fp32_mfma else 2,
+                                    local=True)
     print(json.dumps({"kernel": "k_gemm<float,0,1,1,2,2,0,1> inside negf_invert_batch_local", "n": n, "batch": batch,
                       "outer_blocks": (n + 127) // 128, "seconds_incl_host_copies": time.perf_counter() - t0,
                       "info": info.tolist()}))
@@ -346,7 +364,7 @@ def server_workload(name, args):
     prof = capi.profile_end()
     srv.close()
     dominant = dominant_kind(prof)
-    peak = PEAK_FP32_MFMA if dominant and dominant.startswith("fp32") else PEAK_FP64_MFMA
+    peak, _ = kind_peak(dominant)
     g_flops, g_ms, g_launches = prof[dominant] if dominant else (0.0, 0.0, 0)
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     out = {
@@ -451,9 +469,14 @@ def main():
                          "it with, for rocprofv3 --pmc counter runs")
     ap.add_argument("--gemm-c64", action="store_true",
                     help="with --gemm-only: the complex64 product (default FP64)")
+    ap.add_argument("--gemm-bf16x3", action="store_true",
+                    help="with --gemm-only: the complex64 product on the bf16 matrix cores (three-way split)")
     ap.add_argument("--inverse-only", action="store_true",
                     help="run only the complex64 Gauss-Jordan inverse at the block size and batch of one "
                          "lane, for rocprofv3 --pmc counter runs of its trailing updates")
+    ap.add_argument("--fp32-mfma", action="store_true",
+                    help="lu32 form: complex64 products on the FP32 matrix cores (NEGF_FLAG_LU32_FP32_MFMA) "
+                         "instead of the bf16 split form; with --inverse-only, the same for the inverse")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB 8 times (the known-size "
                          "reference for rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -461,6 +484,8 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+    if args.fp32_mfma:
+        args.flags |= capi.FLAG_LU32_FP32_MFMA
     if args.gemm_only:
         gemm_only(args)
         return
@@ -540,10 +565,9 @@ def main():
     iters = run["iters"]
     prof = run["prof"]
     dom = dominant_kind(prof)
-    peak = PEAK_FP32_MFMA if dom and dom.startswith("fp32") else PEAK_FP64_MFMA
+    peak, executed = kind_peak(dom)
     g_flops, g_ms, g_launches = prof[dom] if dom else (0.0, 0.0, 0)
     gemm_tflops = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
-    executed = 0.75 if dom and dom.endswith("_3m") else 1.0
     result = {
         "metric": "energy-points/s on 1 GPU vs plain FP64 form at stated error; p50/p99 solve latency",
         "value": value,
@@ -591,7 +615,9 @@ def main():
                      "note": ("kernel: the MFMA kind with the largest summed sampled duration; achieved "
                               "= algorithmic 8 n^3 flops per complex product x batch / HIP-event duration "
                               "on the launching stream (one launch in --profile-every of each kind); "
-                              "executed_frac counts the 6 n^3 the 3M form runs on the matrix cores; the "
+                              "peak: the matrix cores the kind runs on (FP64, FP32, or bf16 for the "
+                              "split form); executed_frac counts what they run: 6 of 8 n^3 in the 3M "
+                              "form, 48 in the bf16 split form (four real products of six); the "
                               "lanes run concurrently, so durations include other lanes' kernels sharing "
                               "the CUs; traffic: HBM-side bytes per launch from FETCH_SIZE/WRITE_SIZE "
                               "counter runs of the kernel alone at this shape (bench.py --gemm-only or "

@@ -126,6 +126,12 @@ typedef struct {
  * Batches of more than about 20000 launches (e.g. blocks of 2304 over 200 slices)
  * run without a graph. */
 #define NEGF_FLAG_GRAPH 16
+/* NEGF_PREC_LU32 only: take the complex64 column-chain products on the FP32 matrix
+ * cores (3M form) instead of the default bf16 matrix cores with every FP32 operand
+ * split three ways (four real products of six split products each, FP32 accuracy;
+ * 1.3 times the FP32 kernel's throughput at block 2304).  The block inverses' rank-128
+ * updates run on the FP32 matrix cores either way. */
+#define NEGF_FLAG_LU32_FP32_MFMA 32
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
@@ -286,7 +292,9 @@ int negf_zgemm(char transa, char transb, int n, const double* alpha, const doubl
  * Recursive_Greens.F90:18-95, MKL_Routines_Cluster.F90:19,23).  `batch` n x n
  * column-major matrices, contiguous; complex64 != 0: COMPLEX(4) pairs of floats,
  * else COMPLEX(8); m3 != 0: the 3M complex product form (3 real products, normwise
- * accurate), else the 4-product form.
+ * accurate), else the 4-product form; m3 == 2 (complex64 only): the products on the
+ * bf16 matrix cores with every FP32 operand split three ways (FP32 accuracy, each
+ * component as accurate as in the 4-product form), the form NEGF_PREC_LU32 takes.
  *   negf_gemm_batch:   C_b = A_b B_b
  *   negf_invert_batch: invG_b = G_b^-1 (the sweep's Gauss-Jordan inverse; m3 selects
  *                      the form of its trailing updates); info[b] = first zero pivot
@@ -382,12 +390,14 @@ int negf_server_destroy(negf_server* srv);
  * per launch costs host time and stream markers, so a uniform sample keeps the
  * timed run close to an untimed one; 1 = every launch).  end() synchronises the
  * device and returns, per kind, the sampled launches' algorithmic work, their
- * summed duration in ms and their count; each argument is an array of 10:
+ * summed duration in ms and their count; each argument is an array of 12:
  * [0..7] GEMMs indexed by 4 * (3M complex form) + 2 * (FP32) + (Gauss-Jordan
  * trailing update) -- [0] FP64 product, [1] FP64 update, [2] FP32 product,
  * [3] FP32 update, [4..7] the same in the 3M form -- with work = 8 m n k flops per
  * complex product; [8] FP64 and [9] FP32 Gauss-Jordan panels, with work = the
- * bytes the panel's columns and pivot rows move (read and written once). */
+ * bytes the panel's columns and pivot rows move (read and written once);
+ * [10] complex64 product and [11] update on the bf16 matrix cores (split form),
+ * work = 8 m n k flops. */
 int negf_profile_begin(int sample_every);
 int negf_profile_end(double* flops, double* ms, long long* launches);
 

@@ -23,7 +23,7 @@ LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(PKG, "lib", "obj")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["zgemm_mfma.hip", "gj_inverse.hip", "negf_kernels.hip", "negf_engine.hip"]
+HIP_SOURCES = ["zgemm_mfma.hip", "zgemm_x3.hip", "gj_inverse.hip", "negf_kernels.hip", "negf_engine.hip"]
 # Packed-FP32 instructions are disabled for every device compile.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

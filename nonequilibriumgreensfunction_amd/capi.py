@@ -59,6 +59,7 @@ FLAG_SR_SEPARATE_LEADS = 2
 FLAG_SINGLE_LANE = 4
 FLAG_CACHE_CONTACTS = 8
 FLAG_GRAPH = 16
+FLAG_LU32_FP32_MFMA = 32
 
 
 class NegfAccum(ctypes.Structure):
@@ -134,7 +135,7 @@ def lib():
         L.negf_set_gemm_tiles.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
         L.negf_set_lane_table.argtypes = [ctypes.c_int, _ip, _ip]
         L.negf_invert_batch_noalloc.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 4
-        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 10
+        L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 12
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -169,7 +170,7 @@ def profile_begin(sample_every=1):
 
 PROFILE_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update",
                  "fp64_gemm_3m", "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m",
-                 "fp64_gj_panel", "fp32_gj_panel")
+                 "fp64_gj_panel", "fp32_gj_panel", "fp32x3_gemm", "fp32x3_gj_update")
 
 
 def profile_end():
@@ -208,12 +209,18 @@ def _load_tuning(L, path=None):
     return t
 
 
+def _form(m3):
+    """Product form code of the C ABI: 0 4-product, 1 3M, 2 (complex64) the bf16
+    three-way split products."""
+    return 2 if m3 == 2 else int(bool(m3))
+
+
 def bench_gemm_tile(n, batch, tile, complex64=True, m3=True, reps=8, device=0):
     """Mean launch time (ms) of the product kernel with workgroup tile `tile`
     (0: 64x64, 1: 32x32), for the autotuner."""
     ms = ctypes.c_double(0.0)
     _check(lib().negf_bench_gemm_tile(int(device), int(bool(complex64)), int(n), int(batch), int(reps),
-                                      int(bool(m3)), int(tile), ctypes.byref(ms)))
+                                      _form(m3), int(tile), ctypes.byref(ms)))
     return ms.value
 
 
@@ -222,7 +229,7 @@ def bench_gemm(n, batch, complex64=True, m3=True, reps=8, device=0):
     (negf_bench_gemm): the kernel that bounds the sweep, for counter runs."""
     ms = ctypes.c_double(0.0)
     _check(lib().negf_bench_gemm(int(device), int(bool(complex64)), int(n), int(batch), int(reps),
-                                 int(bool(m3)), ctypes.byref(ms)))
+                                 _form(m3), ctypes.byref(ms)))
     return ms.value
 
 
@@ -235,7 +242,7 @@ def gemm_batch(A, B, complex64=False, m3=True):
     a = np.ascontiguousarray(np.swapaxes(A, 1, 2), dtype=dt)
     b = np.ascontiguousarray(np.swapaxes(B, 1, 2), dtype=dt)
     c = np.zeros_like(a)
-    _check(lib().negf_gemm_batch(int(bool(complex64)), int(bool(m3)), n, batch,
+    _check(lib().negf_gemm_batch(int(bool(complex64)), _form(m3), n, batch,
                                  a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
                                  c.ctypes.data_as(ctypes.c_void_p)))
     return np.swapaxes(c, 1, 2)
@@ -252,7 +259,7 @@ def invert_batch(G, complex64=False, m3=True, local=False):
     out = np.zeros_like(g)
     info = np.zeros(batch, np.int32)
     fn = lib().negf_invert_batch_local if local else lib().negf_invert_batch
-    _check(fn(int(bool(complex64)), int(bool(m3)), n, batch, g.ctypes.data_as(ctypes.c_void_p),
+    _check(fn(int(bool(complex64)), _form(m3), n, batch, g.ctypes.data_as(ctypes.c_void_p),
               out.ctypes.data_as(ctypes.c_void_p), info.ctypes.data_as(_ip)))
     return np.swapaxes(out, 1, 2), info
 

@@ -464,7 +464,8 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
       g.active = active; g.pivstep = w.pivstep; g.sPiv = w.sInt;
       g.pv_lo = k0; g.pv_hi = k0 + kb;
       g.skip_lo = k0; g.skip_hi = k0 + kb;
-      g.m3 = w.m3;
+      g.m3 = w.m3 ? 1 : 0;
+      g.x3 = w.m3 == 2;
       hipError_t e = gemm_gj_update<T>(s, g, batch);
       if (e != hipSuccess) return e;
     }
@@ -810,7 +811,8 @@ hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n,
     g.C = Rt; g.sC = w.sProw; g.ldc = n;
     g.m = n; g.n = kb; g.k = kb;
     g.active = active;
-    g.m3 = w.m3;
+    g.m3 = w.m3 ? 1 : 0;
+    g.x3 = w.m3 == 2;
     hipError_t e = gemm_batched<T>(s, OP_T, OP_T, g, batch, mk<T>(1, 0), mk<T>(0, 0));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_gjl_cols<T>), dim3((n + 255) / 256, kb, batch), dim3(256), 0, s, S, sS, ld, n, k0, kb,
@@ -823,7 +825,8 @@ hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n,
     u.active = active; u.pivstep = w.pivstep; u.sPiv = w.sInt;
     u.pv_lo = k0; u.pv_hi = k0 + kb;
     u.skip_lo = u.skip_hi = 0;
-    u.m3 = w.m3;
+    u.m3 = w.m3 ? 1 : 0;
+    u.x3 = w.m3 == 2;
     e = gemm_gj_update<T>(s, u, batch);
     if (e != hipSuccess) return e;
   }

@@ -75,6 +75,9 @@ struct GemmBatch {
   const int* ncol = nullptr;
   long long sNcol = 0;
   float work = 1.0f;  // fraction of the m x n x k products really computed (timing only)
+  // complex64 only, 64x64 tiles: the product on the bf16 matrix cores with each FP32
+  // operand split three ways (zgemm_x3.hip; FP32 accuracy, four real products)
+  int x3 = 0;
 };
 
 // Pivoted Cholesky factors Gam = W W' of both leads' broadening for a batch of
@@ -117,7 +120,8 @@ struct GjWork {
   int* info;        // 1: first zero pivot (1-based) or 0
   long long sProw;  // batch stride of prow (complex elements)
   long long sInt;   // batch stride of pivstep, pivrow and info (ints)
-  int m3 = 0;       // complex product form of the trailing updates (GemmBatch::m3)
+  int m3 = 0;       // complex product form of the trailing updates: 0 4-product, 1 3M,
+                    // 2 (complex64) the bf16 three-way split products (GemmBatch::x3)
   void* tmp = nullptr;  // gj_invert_local: n x 128 (complex64) or n x 64 (complex128) per matrix
   long long sTmp = 0;
 };
@@ -125,6 +129,10 @@ struct GjWork {
 template <typename T>
 hipError_t gemm_batched(hipStream_t s, int opa, int opb, const GemmBatch& g, int batch,
                         cx<T> alpha, cx<T> beta);
+// Complex64 products on the bf16 matrix cores (zgemm_x3.hip): op pairs as
+// gemm_batched, epi EPI_GJ with (OP_N, OP_T) as gemm_gj_update.
+hipError_t gemm_x3(hipStream_t s, int opa, int opb, int epi, const GemmBatch& g, int batch, float2 alpha,
+                   float2 beta);
 template <typename T>
 hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, long long sOut,
                      int ldo, int n, int batch, const GjWork& w, const int* active);
@@ -149,9 +157,11 @@ inline long long gj_ints(int n) { return 2LL * n + 1 + kGjCandInts; }
 // with its kind.  Kinds 0..7 = 4 * (3M form) + 2 * (FP32) + (Gauss-Jordan trailing
 // update), work = 8 m n k flops per complex product (products with a transposed or
 // adjoint operand are not sampled, so each kind is one kernel instance); kinds 8, 9 = the FP64 / FP32
-// Gauss-Jordan panel, work = the bytes its columns move (read and written once).
-constexpr int kTimerKinds = 10;
+// Gauss-Jordan panel, work = the bytes its columns move (read and written once);
+// kinds 10, 11 = the complex64 bf16-split product and trailing update (8 m n k).
+constexpr int kTimerKinds = 12;
 constexpr int kTimerPanel64 = 8;
+constexpr int kTimerX3 = 10;
 struct GemmTimer {
   bool on = false;
   int every = 1;         // time one launch in `every` of each kind (uniform sample)

@@ -1411,6 +1411,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     auto rgf_lu32 = [&]() -> int {
       typedef cx<float> zf;
       constexpr int K = kLu32Cols;
+      // complex64 products on the bf16 matrix cores (three-way split) unless asked otherwise
+      const bool x3 = !(c->cfg.flags & NEGF_FLAG_LU32_FP32_MFMA);
       zf* F0 = (zf*)Q(y.o_f32);
       auto F = [&](long long off) { return F0 + off; };
       const long long Es32 = 2 * Es;
@@ -1433,6 +1435,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         g.C = C; g.sC = Es32; g.ldc = n;
         g.m = m; g.n = ncols; g.k = n;
         g.m3 = 1;
+        g.x3 = x3 ? 1 : 0;
         if (kc) {
           g.ncol = kc;
           g.sNcol = skc;
@@ -1491,6 +1494,8 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
       w32.prow = Q(y.o_prow);
       w32.sProw = Es32;
+      // the trailing updates (rank 128) stay on the FP32 matrix cores: at this short K the
+      // split form is no faster alone and slower beside the other lanes' kernels
       w32.m3 = 1;
       w32.tmp = F(y.f_P);
       w32.sTmp = Es32;
@@ -2349,6 +2354,7 @@ static int gemm_batch_host(int m3, int n, int batch, const void* A, const void* 
   g.lda = g.ldb = g.ldc = n;
   g.m = g.n = g.k = n;
   g.m3 = m3 ? 1 : 0;
+  g.x3 = m3 == 2;
   HIPCHK(gemm_batched<T>(nullptr, OP_N, OP_N, g, batch, mk<T>(1, 0), mk<T>(0, 0)));
   HIPCHK(hipMemcpy(C, dC, bytes, hipMemcpyDeviceToHost));
   hipFree(dA); hipFree(dB); hipFree(dC);
@@ -2376,7 +2382,7 @@ static int invert_batch_host(int m3, int n, int batch, const void* G, void* invG
   w.pivrow = ints + n;
   w.info = ints + 2 * n;
   w.sInt = sInt;
-  w.m3 = m3 ? 1 : 0;
+  w.m3 = m3 == 2 ? 2 : m3 ? 1 : 0;
   HIPCHK(gj_invert<T>(nullptr, dA, nn, n, dO, nn, n, n, batch, w, nullptr));
   HIPCHK(hipMemcpy(invG, dO, bytes, hipMemcpyDeviceToHost));
   std::vector<int> h((size_t)sInt * batch);
@@ -2384,6 +2390,15 @@ static int invert_batch_host(int m3, int n, int batch, const void* G, void* invG
   for (int b = 0; b < batch; ++b)
     if (info) info[b] = h[(size_t)b * sInt + 2 * n];
   hipFree(dA); hipFree(dO); hipFree(prow); hipFree(ints);
+  return 0;
+}
+
+// m3 == 2 (products on the bf16 matrix cores, split three ways) exists for complex64 only.
+static int form_ok(int complex64, int m3) {
+  if (m3 == 2 && !complex64) {
+    negf_set_error("the bf16 split product form (m3 = 2) is for complex64 only");
+    return NEGF_ERR_ARG;
+  }
   return 0;
 }
 
@@ -2435,7 +2450,7 @@ static int invert_noalloc(char* base, int m3, int n, int batch, void* G, void* i
   w.pivrow = ints + n;
   w.info = ints + 2 * n;
   w.sInt = (long long)(ints_b / sizeof(int));
-  w.m3 = m3 ? 1 : 0;
+  w.m3 = m3 == 2 ? 2 : m3 ? 1 : 0;
   // the info words accumulate a first failure: they start at zero on every call
   TRY(launch_zero(st, ints, ints_b * (size_t)batch));
   TRY(gj_invert<T>(st, (V*)G, nn, n, (V*)invG, nn, n, n, batch, w, nullptr));
@@ -2452,6 +2467,7 @@ extern "C" int negf_invert_batch_noalloc(int complex64, int m3, int n, int batch
                    "device arrays G and invG distinct and non-null)", n, batch);
     return NEGF_ERR_ARG;
   }
+  if (int rc = form_ok(complex64, m3)) return rc;
   const size_t need = noalloc_bytes(complex64, n, batch);
   if (need > kScratchBytes) {
     negf_set_error("negf_invert_batch_noalloc: n=%d x batch=%d needs %zu bytes of workspace, above "
@@ -2492,6 +2508,7 @@ extern "C" int negf_gemm_batch(int complex64, int m3, int n, int batch, const vo
                                void* C) {
   int rc = batch_args_ok(n, batch, A, B, C);
   if (rc) return rc;
+  if ((rc = form_ok(complex64, m3))) return rc;
   return complex64 ? gemm_batch_host<float>(m3, n, batch, A, B, C)
                    : gemm_batch_host<double>(m3, n, batch, A, B, C);
 }
@@ -2524,7 +2541,7 @@ static int invert_local_host(int m3, int n, int batch, const void* G, void* invG
     w.pivrow = ints + n;
     w.info = ints + 2 * n;
     w.sInt = sInt;
-    w.m3 = m3 ? 1 : 0;
+    w.m3 = m3 == 2 ? 2 : m3 ? 1 : 0;
     w.tmp = tmp;
     w.sTmp = (long long)n * kb;
     std::vector<int> h((size_t)sInt * batch);
@@ -2550,6 +2567,7 @@ extern "C" int negf_invert_batch_local(int complex64, int m3, int n, int batch, 
                                        int* info) {
   int rc = batch_args_ok(n, batch, G, invG, G);
   if (rc) return rc;
+  if ((rc = form_ok(complex64, m3))) return rc;
   return complex64 ? invert_local_host<float>(m3, n, batch, G, invG, info)
                    : invert_local_host<double>(m3, n, batch, G, invG, info);
 }
@@ -2558,6 +2576,7 @@ extern "C" int negf_invert_batch(int complex64, int m3, int n, int batch, const 
                                  int* info) {
   int rc = batch_args_ok(n, batch, G, invG, G);
   if (rc) return rc;
+  if ((rc = form_ok(complex64, m3))) return rc;
   return complex64 ? invert_batch_host<float>(m3, n, batch, G, invG, info)
                    : invert_batch_host<double>(m3, n, batch, G, invG, info);
 }
@@ -2630,6 +2649,7 @@ extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int 
     negf_set_error("negf_bench_gemm: bad arguments");
     return NEGF_ERR_ARG;
   }
+  if (int rf = form_ok(complex64, m3)) return rf;
   int rc = require_device(device);
   if (rc) return rc;
   const size_t bytes = (complex64 ? 8 : 16) * (size_t)n * n * batch;
@@ -2648,6 +2668,7 @@ extern "C" int negf_bench_gemm(int device, int complex64, int n, int batch, int 
     g.lda = g.ldb = g.ldc = n;
     g.m = g.n = g.k = n;
     g.m3 = m3 ? 1 : 0;
+    g.x3 = m3 == 2;
     auto launch = [&]() {
       return complex64 ? gemm_batched<float>(s, OP_N, OP_N, g, batch, mk<float>(1, 0), mk<float>(0, 0))
                        : gemm_batched<double>(s, OP_N, OP_N, g, batch, mk<double>(1, 0), mk<double>(0, 0));

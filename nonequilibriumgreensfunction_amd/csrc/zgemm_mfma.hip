@@ -401,7 +401,15 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
   if (batch <= 0 || g.m <= 0 || g.n <= 0) return hipSuccess;
   const int ncols = (EPI == EPI_GJ) ? g.n - (g.skip_hi - g.skip_lo) : g.n;
   hipEvent_t ev0, ev1;
-  const int kind = (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
+  // Workgroup tile from the tuning table (gemm_tile): 64x64 by default -- 128x64 and
+  // 64x128 measured within 3% of it at blocks of 1024 and more, and their FP64
+  // variants do not fit two workgroups per CU without spills -- or 32x32, four
+  // times the workgroups for small blocks.  Complex64 products that ask for the bf16
+  // split form take it with 64x64 tiles (zgemm_x3.hip).
+  const int tile = gemm_tile(sizeof(T) == sizeof(float), g.m > g.n ? g.m : g.n);
+  const bool x3 = sizeof(T) == sizeof(float) && g.x3 && tile != kTile32;
+  const int kind = x3 ? kTimerX3 + (EPI == EPI_GJ ? 1 : 0)
+                      : (EPI == EPI_GJ ? 1 : 0) + (sizeof(T) == sizeof(float) ? 2 : 0) + (g.m3 ? 4 : 0);
   // a kind's samples are one kernel instance: the plain products (both operands
   // untransposed) and the Gauss-Jordan updates; the transposed and adjoint products
   // (small block-row products and traces) are not sampled
@@ -411,12 +419,14 @@ static hipError_t launch(hipStream_t s, const GemmBatch& g, int batch, cx<T> alp
     const hipError_t er = hipEventRecord(ev0, s);
     if (er != hipSuccess) return er;
   }
-  // Workgroup tile from the tuning table (gemm_tile): 64x64 by default -- 128x64 and
-  // 64x128 measured within 3% of it at blocks of 1024 and more, and their FP64
-  // variants do not fit two workgroups per CU without spills -- or 32x32, four
-  // times the workgroups for small blocks.
-  const int tile = gemm_tile(sizeof(T) == sizeof(float), g.m > g.n ? g.m : g.n);
   hipError_t e;
+  if constexpr (sizeof(T) == sizeof(float)) {
+    if (x3) {
+      e = gemm_x3(s, OPA, OPB, EPI, g, batch, alpha, beta);
+      if (timed && e == hipSuccess) return hipEventRecord(ev1, s);
+      return e;
+    }
+  }
   if (tile == kTile32)
     e = g.m3 ? launch_cfg<T, OPA, OPB, EPI, 1, 1, 1>(s, g, batch, alpha, beta)
              : launch_cfg<T, OPA, OPB, EPI, 1, 1, 0>(s, g, batch, alpha, beta);

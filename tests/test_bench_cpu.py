@@ -59,3 +59,19 @@ def test_dominant_kind_is_largest_summed_duration():
     t = b.kernel_table(prof, "c4")
     assert "gbytes_per_s" in t["fp64_gj_panel"] and "tflops" in t["fp64_gemm_3m"]
     assert "fp32_gj_panel" not in t
+
+
+def test_profile_kinds_match_the_c_abi():
+    # negf_profile_end fills one entry per kind; the Python buffers must be exactly
+    # as long as the arrays the header documents (a shorter buffer is overwritten)
+    import re
+    from nonequilibriumgreensfunction_amd import capi
+    with open(os.path.join(ROOT, "include", "negf_capi.h")) as f:
+        text = f.read()
+    m = re.search(r"each argument is an array of (\d+)", text)
+    assert m and int(m.group(1)) == len(capi.PROFILE_KINDS)
+    src = open(os.path.join(ROOT, "nonequilibriumgreensfunction_amd", "csrc", "negf_device.h")).read()
+    k = re.search(r"constexpr int kTimerKinds = (\d+);", src)
+    assert k and int(k.group(1)) == len(capi.PROFILE_KINDS)
+    b = _bench()
+    assert set(b.GEMM_KINDS) <= set(capi.PROFILE_KINDS)

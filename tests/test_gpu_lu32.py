@@ -100,6 +100,17 @@ def test_lu32_matches_oracle(case):
     print(f"energies {ne}, sweeps {sweeps}, escalated {esc}", flush=True)
 
 
+def test_lu32_fp32_matrix_cores_match_oracle():
+    # NEGF_FLAG_LU32_FP32_MFMA: the complex64 products on the FP32 matrix cores (3M)
+    # instead of the bf16 split form, to the same bounds
+    w = build_wire(8, 3, 2, disorder=0.5, seed=7, flux=0.4, zeeman=0.03)
+    energies = [-1.7, -0.3, 0.05, 0.25, 1.1]
+    ref, noise = P.reference(w, energies, DE, mu=MU, muT=MUT)
+    acc, it, (ne, sweeps, esc) = _run(w, energies, DE, flags=capi.FLAG_LU32_FP32_MFMA)
+    assert np.array_equal(it, ref["iters"])
+    check(_collect(acc), ref, noise, "FP32 matrix cores")
+
+
 CASES = [(name, tag, dims, kw) for name, lst in sorted(MB.CASES.items())
          for tag, dims, kw, _, _ in lst]
 

@@ -75,13 +75,45 @@ def test_invert_reports_singular(rng, n):
     assert info > 0
 
 
-@pytest.mark.parametrize("complex64", [True, False])
-@pytest.mark.parametrize("m3", [True, False])
+@pytest.mark.parametrize("complex64,m3", [(True, 1), (True, 0), (True, 2), (False, 1), (False, 0)])
 def test_bench_gemm_instrumentation_runs(complex64, m3):
     # negf_bench_gemm times the product kernel alone (counter runs); it must run
     # and report a positive launch time in every form
     ms = capi.bench_gemm(256, 3, complex64=complex64, m3=m3, reps=2)
     assert ms > 0.0
+
+
+def test_bf16_split_form_is_complex64_only():
+    with pytest.raises(capi.NegfError):
+        capi.gemm_batch(np.zeros((1, 4, 4), complex), np.zeros((1, 4, 4), complex), complex64=False, m3=2)
+
+
+# The complex64 product on the bf16 matrix cores (every FP32 operand split three
+# ways, four real products): against numpy complex128 with the componentwise bound
+# of the 4-product form, |Re err| <= 4 n eps32 (|Re A||Re B| + |Im A||Im B|) and
+# |Im err| <= 4 n eps32 (|Re A||Im B| + |Im A||Re B|): its split products and FP32
+# accumulation are at the FP32 rounding level.  Sizes: partial and whole 64x64
+# tiles and 32-deep K slabs, up to the benchmark's block.
+@pytest.mark.parametrize("n,batch", [(1, 2), (5, 2), (31, 3), (64, 2), (100, 2), (257, 2), (1024, 2), (2304, 3)])
+def test_gemm_bf16_split_matches_numpy(rng, n, batch):
+    print(f"bf16 split gemm n={n} batch={batch}", flush=True)
+    A = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))).astype(np.complex64)
+    B = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))).astype(np.complex64)
+    A[0, :, 0] *= 1e-3  # a column of small entries: componentwise, not normwise
+    got = capi.gemm_batch(A, B, complex64=True, m3=2)
+    eps = np.finfo(np.float32).eps
+    for b in range(batch):
+        a, bb = A[b].astype(np.complex128), B[b].astype(np.complex128)
+        ref = a @ bb
+        ar, ai, br, bi = np.abs(a.real), np.abs(a.imag), np.abs(bb.real), np.abs(bb.imag)
+        mre = ar @ br + ai @ bi
+        mim = ar @ bi + ai @ br
+        ere = np.abs(got[b].real - ref.real)
+        eim = np.abs(got[b].imag - ref.imag)
+        print(f"  b={b}: max err/bound re {(ere / (n * eps * mre)).max():.3f} im {(eim / (n * eps * mim)).max():.3f}",
+              flush=True)
+        assert np.all(ere <= 4 * n * eps * mre), (b, float((ere / mre).max() / eps))
+        assert np.all(eim <= 4 * n * eps * mim), (b, float((eim / mim).max() / eps))
 
 
 # The sweep's product kernel in each of its forms at the sweep's block sizes and
@@ -135,20 +167,21 @@ def _block_shuffled(rng, n, kb, dt):
 
 
 @pytest.mark.parametrize("n", [1, 7, 64, 100, 128, 129, 300, 1024, 2304])
-@pytest.mark.parametrize("c64", [True, False], ids=["c64", "c128"])
-def test_invert_local_matches_lapack(rng, n, c64):
+@pytest.mark.parametrize("c64,m3", [(True, 1), (True, 2), (False, 1)], ids=["c64", "c64_bf16x3", "c128"])
+def test_invert_local_matches_lapack(rng, n, c64, m3):
     """negf_invert_batch_local (block Gauss-Jordan pivoting inside 128-column
     diagonal blocks in complex64, 64-column in complex128; the complex64 block-LU
     form's inverse) against LAPACK: one partial block up to the benchmark's 2304
     (18 and 36 blocks).  complex128: 1e-12 relative like the partially pivoted
     inverse; complex64: within 16x LAPACK's own complex64 inverse (cgetrf/cgetri
-    through numpy) on the same matrix, or 16 eps."""
+    through numpy) on the same matrix, or 16 eps; the trailing updates in the 3M
+    form (m3 = 1) or on the bf16 matrix cores (m3 = 2)."""
     kb = 128 if c64 else 64
     dt = np.complex64 if c64 else np.complex128
     batch = 2
-    print(f"local inverse c64={c64} n={n} batch={batch}", flush=True)
+    print(f"local inverse c64={c64} m3={m3} n={n} batch={batch}", flush=True)
     G = np.stack([_block_shuffled(rng, n, kb, dt) for _ in range(batch)])
-    X, info = capi.invert_batch(G, complex64=c64, m3=True, local=True)
+    X, info = capi.invert_batch(G, complex64=c64, m3=m3, local=True)
     assert np.all(info == 0), info
     for b in range(batch):
         ref = np.linalg.inv(G[b])
