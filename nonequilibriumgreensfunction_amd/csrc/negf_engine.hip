@@ -42,6 +42,10 @@ hipError_t launch_assemble(hipStream_t, int, int, cx<T>*, long long, const cx<T>
                            const double*, double, const cx<T>*, long long, const cx<T>*, long long,
                            const cx<T>*, long long, const cx<T>*, int, int, const int*,
                            const double*);
+hipError_t launch_assemble_blk(hipStream_t s, int norb, double2* M, long long sM, const double2* H, int add_diag,
+                               double hsign, const double* energy, double eta, const double2* S1, long long sS1,
+                               const double2* S2, long long sS2, const double2* g, const float2* g32, long long sg,
+                               const double2* X, int n, int batch, const double* V);
 template <typename T>
 hipError_t launch_bd_mul(hipStream_t, int, int, cx<T>*, long long, const cx<T>*, long long,
                          const cx<T>*, int, int, int, const int*, int accum = 0);
@@ -1305,9 +1309,16 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
     auto hx = [&](int x) { return c->compact ? (const zd*)c->H0 : (const zd*)(c->H + (size_t)x * nn); };
     auto hv = [&](int x) { return c->compact ? (const double*)(c->V + (size_t)x * ns) : nullptr; };
     // Assemble M = zI - H_x - [SigL if x==0] - [SigR if x==L-1] - sandwich(g) into o_M.
-    auto assemble = [&](int x, const zd* g, int mode) -> int {
+    // g (complex128) or g32 (complex64, energy stride sg32) is the previous slice's block
+    auto assemble = [&](int x, const zd* g, int mode, const cx<float>* g32 = nullptr, long long sg32 = 0) -> int {
       const zd* S1 = (x == 0) ? Q(y.o_SigL) : nullptr;
       const zd* S2 = (x == L - 1) ? Q(y.o_SigR) : nullptr;
+      if (bd && mode == 1 && (g || g32)) {
+        // site-block form: one thread per site block, g read once in its own precision
+        TRY(launch_assemble_blk(s, no, Q(y.o_M), Es, hx(x), 1, -1.0, ln.d_energy, c->cfg.eta, S1, Es, S2, Es, g,
+                                g32, g32 ? sg32 : Es, c->X, n, nb, hv(x)));
+        return 0;
+      }
       if (g && !bd) {
         // dense HopX: P = X' g X (mode 1) or X g X' (mode 2) with two GEMMs, then subtract
         zd* T1 = Q(y.o_P);
@@ -1515,8 +1526,7 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       // a tail, the first-row chain (W_Lb' G_1x)' = g_x' X' (...)_{x-1} (start g_1' W_Lb),
       // complex64; after slice L it holds (W_Lb' G_1L)'
       for (int x = 0; x < L; ++x) {
-        if (x > 0) TRY((launch_convert<double, float>(s, Q(y.o_P), Es, gs(x - 1), Es32, nn, nb, nullptr)));
-        if ((r = assemble(x, x > 0 ? Q(y.o_P) : nullptr, 1))) return r;
+        if ((r = assemble(x, nullptr, 1, x > 0 ? gs(x - 1) : nullptr, Es32))) return r;
         TRY(to_c64(s, gs(x), Es32, Q(y.o_M), Es, nn, nb));
         TRY(gj_invert_local<float>(s, gs(x), Es32, n, n, nb, w32, nullptr));
         if (x == 0) {

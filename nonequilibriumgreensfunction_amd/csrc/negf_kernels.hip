@@ -82,6 +82,78 @@ __global__ void k_assemble(cx<T>* M, long long sM, const cx<T>* H, int add_diag,
   M[(long long)b * sM + e] = v;
 }
 
+// The same slice matrix as k_assemble MODE 1 (z I - H_x - Sigma - X' g X), one thread
+// per NORB x NORB site block (a, c): the g block is read once (complex64 or complex128:
+// the widening to FP64 is exact, so both give the bits of k_assemble on the
+// complex128 copy of g) and X' g X takes 2 NORB^3 complex multiply-adds per block
+// instead of 4 per element per orbital pair; the column site's X block is staged in
+// LDS (one site column per workgroup row).  Each element's operations and their order
+// are those of k_assemble.
+template <int NORB, typename TG>
+__global__ __launch_bounds__(128) void k_assemble_blk(double2* M, long long sM, const double2* H, int add_diag, double hsign,
+                               const double* energy, double eta, const double2* S1, long long sS1,
+                               const double2* S2, long long sS2, const TG* g, long long sg, const double2* X,
+                               int n, int nsite, const double* V) {
+  const int b = blockIdx.z;
+  const int sc = blockIdx.y;
+  const int sa = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double2 xc[NORB][NORB];  // X[c0 + q][c0 + s] as xc[q][s]
+  const int c0 = sc * NORB;
+  if (threadIdx.x < NORB * NORB) {
+    const int q = threadIdx.x % NORB, t = threadIdx.x / NORB;
+    xc[q][t] = X[(long long)(c0 + t) * n + c0 + q];
+  }
+  __syncthreads();
+  if (sa >= nsite) return;
+  const int a0 = sa * NORB;
+  const TG* G = g + (long long)b * sg;
+  // t[p][s] = (g X)[a0 + p][c0 + s]
+  double2 t[NORB][NORB];
+#pragma unroll
+  for (int p = 0; p < NORB; ++p)
+#pragma unroll
+    for (int s2 = 0; s2 < NORB; ++s2) t[p][s2] = mk<double>(0, 0);
+#pragma unroll
+  for (int q = 0; q < NORB; ++q) {
+    double2 gq[NORB];
+#pragma unroll
+    for (int p = 0; p < NORB; ++p) {
+      const TG v = G[(long long)(c0 + q) * n + a0 + p];
+      gq[p] = mk<double>((double)v.x, (double)v.y);
+    }
+#pragma unroll
+    for (int p = 0; p < NORB; ++p)
+#pragma unroll
+      for (int s2 = 0; s2 < NORB; ++s2) t[p][s2] = cadd(t[p][s2], cmul(gq[p], xc[q][s2]));
+  }
+#pragma unroll
+  for (int r = 0; r < NORB; ++r) {
+    const int a = a0 + r;
+    double2 xa[NORB];  // X[a0 + p][a0 + r]
+#pragma unroll
+    for (int p = 0; p < NORB; ++p) xa[p] = X[(long long)(a0 + r) * n + a0 + p];
+#pragma unroll
+    for (int s2 = 0; s2 < NORB; ++s2) {
+      const int c = c0 + s2;
+      const long long e = (long long)c * n + a;
+      double2 v = H[e];
+      if (V && a == c) v.x += V[sa];
+      v.x *= hsign;
+      v.y *= hsign;
+      if (add_diag && a == c) {
+        v.x += energy[b];
+        v.y += eta;
+      }
+      if (S1) v = csub(v, S1[(long long)b * sS1 + e]);
+      if (S2) v = csub(v, S2[(long long)b * sS2 + e]);
+      double2 d = mk<double>(0, 0);
+#pragma unroll
+      for (int p = 0; p < NORB; ++p) d = cadd(d, cmulc(t[p][s2], xa[p]));  // conj(X[p][r]) t
+      M[(long long)b * sM + e] = csub(v, d);
+    }
+  }
+}
+
 // Y_b = X G_b (CONJ = 0) or X' G_b (CONJ = 1), X site-block-diagonal; accum: Y_b +=.
 template <typename T, int NORB, int CONJ>
 __global__ void k_bd_mul(cx<T>* Y, long long sY, const cx<T>* G, long long sG, const cx<T>* X,
@@ -978,6 +1050,30 @@ hipError_t launch_assemble(hipStream_t s, int norb, int mode, cx<T>* M, long lon
   NEGF_ASM_N(1) NEGF_ASM_N(2) NEGF_ASM_N(4)
 #undef NEGF_ASM_N
 #undef NEGF_ASM
+  return hipErrorInvalidValue;
+}
+
+// k_assemble_blk for a batch of `batch` slices (g complex128 or, g32 given, complex64).
+hipError_t launch_assemble_blk(hipStream_t s, int norb, double2* M, long long sM, const double2* H, int add_diag,
+                               double hsign, const double* energy, double eta, const double2* S1, long long sS1,
+                               const double2* S2, long long sS2, const double2* g, const float2* g32, long long sg,
+                               const double2* X, int n, int batch, const double* V) {
+  if (batch <= 0) return hipSuccess;
+  if (norb < 1 || norb > 4 || n % norb) return hipErrorInvalidValue;
+  const int nsite = n / norb;
+  dim3 grid((nsite + 127) / 128, nsite, batch);
+#define NEGF_ASB(NO)                                                                                  \
+  if (norb == NO) {                                                                                   \
+    if (g32)                                                                                          \
+      hipLaunchKernelGGL((k_assemble_blk<NO, float2>), grid, dim3(128), 0, s, M, sM, H, add_diag, hsign, \
+                         energy, eta, S1, sS1, S2, sS2, g32, sg, X, n, nsite, V);                     \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_assemble_blk<NO, double2>), grid, dim3(128), 0, s, M, sM, H, add_diag,     \
+                         hsign, energy, eta, S1, sS1, S2, sS2, g, sg, X, n, nsite, V);                \
+    return hipGetLastError();                                                                         \
+  }
+  NEGF_ASB(1) NEGF_ASB(2) NEGF_ASB(3) NEGF_ASB(4)
+#undef NEGF_ASB
   return hipErrorInvalidValue;
 }
 
