@@ -132,6 +132,13 @@ typedef struct {
  * 1.3 times the FP32 kernel's throughput at block 2304).  The block inverses' rank-128
  * updates run on the FP32 matrix cores either way. */
 #define NEGF_FLAG_LU32_FP32_MFMA 32
+/* Paired decimation only (equal end slices): by default, when the end slice and HopX
+ * both keep a mirror of the cross-section (negf_find_mirror), every Sancho-Rubio
+ * matrix is block diagonal in the mirror's two sectors, and the decimation runs on
+ * the n/2 x n/2 sector blocks (a quarter of the arithmetic); its stop test
+ * (max|change| <= sr_tol) and Sigma are taken in the site basis as
+ * Contact_Self_Energy.F90:178-209 does.  SR_NO_MIRROR keeps the n x n decimation. */
+#define NEGF_FLAG_SR_NO_MIRROR 64
 
 /* Energy-integrated and per-energy outputs, in the reference's layouts.  Every
  * pointer may be NULL (that output is skipped).  All accumulate (+=), as the
@@ -195,6 +202,22 @@ int negf_contact_cache_stats(negf_ctx* ctx, int* entries, long long* hits);
  * inverse.  NEGF_PREC_LU32: energies solved, FP64 refinement sweeps (summed over
  * energies) and energies escalated to the plain FP64 form. */
 int negf_refine_stats(negf_ctx* ctx, long long* blocks, long long* steps, long long* escalated);
+/* No counterpart in the reference (whose Sancho_Rubio, Contact_Self_Energy.F90:141-209,
+ * works on the whole n x n blocks): the mirror P of the lead cross-section that
+ * the end slice Hend and HopX (n x n, column-major complex, n = Npy*Npz*Norb, site
+ * y + Npy*z, orbital fastest) both keep to 1e-14 of their largest entry, tried as
+ * z -> Npz-1-z first, then y -> Npy-1-y, on an even extent only:
+ *     P e(site, o) = i^phase[o] e(mirror site, perm[o]),
+ * perm an involution of the orbitals and i^phase[o] i^phase[perm[o]] = 1 (P is a
+ * Hermitian unitary involution).  axis = 0 (none), 1 (y) or 2 (z); perm and phase
+ * (Norb entries each, may be NULL) are filled when axis > 0.  Norb <= 4.  Host
+ * only: needs no device. */
+int negf_find_mirror(int npy, int npz, int norb, const double* Hend, const double* HopX, int* axis, int* perm,
+                     int* phase);
+/* The mirror the context's paired decimation uses for the current Hamiltonian
+ * (axis 0: none -- no mirror kept, end slices unequal, SR_SEPARATE_LEADS or
+ * SR_NO_MIRROR set).  Same outputs as negf_find_mirror. */
+int negf_contact_mirror(negf_ctx* ctx, int* axis, int* perm, int* phase);
 /* NEGF_FLAG_GRAPH, no counterpart in the reference: graphs captured and graph
  * replays (batches launched from a graph) since negf_init. */
 int negf_graph_stats(negf_ctx* ctx, long long* built, long long* replays);

@@ -40,6 +40,7 @@ EXPORTS = (
     "negf_server_wait", "negf_server_destroy", "negf_bench_gemm", "negf_gemm_batch",
     "negf_invert_batch", "negf_refine_stats", "negf_graph_stats", "negf_bench_gemm_tile",
     "negf_set_gemm_tiles", "negf_set_lane_table", "negf_invert_batch_noalloc", "negf_invert_batch_local",
+    "negf_find_mirror", "negf_contact_mirror",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -60,6 +61,7 @@ FLAG_SINGLE_LANE = 4
 FLAG_CACHE_CONTACTS = 8
 FLAG_GRAPH = 16
 FLAG_LU32_FP32_MFMA = 32
+FLAG_SR_NO_MIRROR = 64
 
 
 class NegfAccum(ctypes.Structure):
@@ -136,6 +138,8 @@ def lib():
         L.negf_set_lane_table.argtypes = [ctypes.c_int, _ip, _ip]
         L.negf_invert_batch_noalloc.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 4
         L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 12
+        L.negf_find_mirror.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _ip]
+        L.negf_contact_mirror.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -264,6 +268,22 @@ def invert_batch(G, complex64=False, m3=True, local=False):
     return np.swapaxes(out, 1, 2), info
 
 
+def find_mirror(npy, npz, norb, Hend, HopX):
+    """negf_find_mirror (host only): the mirror of the lead cross-section that both the
+    end slice and HopX keep, as (axis, perm, phase); axis 0 when there is none."""
+    H = np.asfortranarray(Hend, dtype=complex)
+    X = np.asfortranarray(HopX, dtype=complex)
+    n = npy * npz * norb
+    if H.shape != (n, n) or X.shape != (n, n):
+        raise ValueError(f"Hend and HopX must be {n} x {n}")
+    ax = ctypes.c_int()
+    perm = np.zeros(max(norb, 4), np.int32)
+    ph = np.zeros(max(norb, 4), np.int32)
+    _check(lib().negf_find_mirror(npy, npz, norb, _p(H), _p(X), ctypes.byref(ax), perm.ctypes.data_as(_ip),
+                                  ph.ctypes.data_as(_ip)))
+    return ax.value, perm[:norb].copy(), ph[:norb].copy()
+
+
 def device_count() -> int:
     return lib().negf_device_count()
 
@@ -381,6 +401,17 @@ class Context:
         b, st, e = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
         _check(lib().negf_refine_stats(self._h, ctypes.byref(b), ctypes.byref(st), ctypes.byref(e)))
         return b.value, st.value, e.value
+
+    def contact_mirror(self):
+        """The cross-section mirror the paired decimation uses: (axis, perm, phase),
+        axis 0 (none), 1 (y) or 2 (z); P e(site, o) = i^phase[o] e(mirror site, perm[o])."""
+        no = self.dims[3]
+        ax = ctypes.c_int()
+        perm = np.zeros(max(no, 4), np.int32)
+        ph = np.zeros(max(no, 4), np.int32)
+        _check(lib().negf_contact_mirror(self._h, ctypes.byref(ax), perm.ctypes.data_as(_ip),
+                                         ph.ctypes.data_as(_ip)))
+        return ax.value, perm[:no].copy(), ph[:no].copy()
 
     def contact_cache_stats(self):
         """(stored energies, energies served from the contact cache)."""

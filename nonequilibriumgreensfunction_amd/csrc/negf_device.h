@@ -112,6 +112,48 @@ struct SpinMats {
   double im[3][4][4];
 };
 
+// A mirror of a lead's cross-section, P e(s,o) = i^ph[o] e(R s, perm[o]), with R the
+// site reflection y -> Npy-1-y (axis 1) or z -> Npz-1-z (axis 2) of the slice's
+// sites (site = y + Npy z, orbital fastest), perm an involution of the orbitals and
+// i^ph[o] i^ph[perm[o]] = 1, so P is a Hermitian unitary involution.  With Npy
+// (Npz) even no site lies on the mirror, and the +1 and -1 eigenspaces of P (the
+// sectors) have the orthonormal bases v(a)+- = (e(r_a) +- i^ph e(m_a)) / sqrt 2:
+// a = r norb + o runs over the representative half of the sites (y < Npy/2, r = y +
+// Npy/2 z; or z < Npz/2, r = y + Npy z) and every orbital, r_a is that site and
+// orbital, m_a = (R site, perm[o]).  A matrix M that commutes with P is block
+// diagonal there: M+- (a, b) = M(r_a, r_b) +- i^ph_b M(r_a, m_b).
+constexpr int kSymMaxOrb = 4;
+struct SymMap {
+  int axis;  // 0: no mirror, 1: y, 2: z
+  int npy, npz, norb;
+  int perm[kSymMaxOrb];
+  int ph[kSymMaxOrb];  // power of i, 0..3
+};
+__host__ __device__ inline void sym_index(const SymMap& m, int a, int& ir, int& im, int& ph) {
+  const int o = a % m.norb, r = a / m.norb;
+  int y, z, ym, zm;
+  if (m.axis == 1) {
+    const int hy = m.npy / 2;
+    y = r % hy; z = r / hy; ym = m.npy - 1 - y; zm = z;
+  } else {
+    y = r % m.npy; z = r / m.npy; ym = y; zm = m.npz - 1 - z;
+  }
+  ir = (y + m.npy * z) * m.norb + o;
+  im = (ym + m.npy * zm) * m.norb + m.perm[o];
+  ph = m.ph[o];
+}
+// i^k z (exact: a swap of parts and sign changes)
+template <typename V> __host__ __device__ inline V rot_i(V z, int k) {
+  V r;
+  switch (k & 3) {
+    case 0: r = z; break;
+    case 1: r.x = -z.y; r.y = z.x; break;
+    case 2: r.x = -z.x; r.y = -z.y; break;
+    default: r.x = z.y; r.y = -z.x; break;
+  }
+  return r;
+}
+
 // Per-matrix workspace of the batched Gauss-Jordan inverse (gj_inverse.hip).
 struct GjWork {
   void* prow;       // kGjNBO x n complex, row-major (row m at prow + m*n)

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <iterator>
 #include <atomic>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -85,6 +86,10 @@ hipError_t launch_maxabs(hipStream_t, const cx<T>*, long long, long long, double
 template <typename T>
 hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long long, int, int,
                             double*, long long);
+hipError_t launch_sym_expand(hipStream_t, const double2*, long long, long long, int, double2*, long long, int, int,
+                             const SymMap&);
+hipError_t launch_sym_maxabs(hipStream_t, const double2*, long long, long long, long long, double*, int,
+                             const int*);
 template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
                           const int*);
@@ -179,6 +184,9 @@ struct Layout {
   // per-chain Sancho-Rubio region (complex elements)
   long long Cs, c_alpha, c_work, c_tmp, c_t, c_tt, c_Toldt, c_CapT, c_W1, c_W2, c_W3, c_W4, c_prow, c_gjt;
   long long Is;  // ints per matrix slot: pivstep n, pivrow n, info 1 (padded)
+  long long Is_sec;  // the same for an n/2 x n/2 mirror-sector slot
+  // chain ints per energy: two whole slots or four sector slots
+  long long chain_ints() const { return std::max(2 * Is, 4 * Is_sec); }
   // Site outputs per energy: kSiteQ planes, each (lead, slice, site), at
   // ((q * 2 + lead) * L + x) * nsites + s.  Planes: 0 Re tr of the site block,
   // 1..3 its spin traces, 4..7 its diagonal per orbital, 8..10 the bond-current
@@ -240,6 +248,7 @@ struct Layout {
     c_gjt = o; o += (long long)n * 64;  // the block-local inverse's n x 64 workspace
     Cs = (o + 31) & ~31LL;
     Is = (gj_ints(n) + 63) & ~63LL;
+    Is_sec = (gj_ints(n / 2) + 63) & ~63LL;
     site_q = 2LL * L * nsites;
     site_per_e = kSiteQ * site_q;
   }
@@ -271,13 +280,13 @@ struct Lane {
   zd* E = nullptr;       // energy arena (cap * Es)
   zd* C = nullptr;       // chain arena (2cap * Cs)
   int* Ie = nullptr;     // energy ints (cap * Is)
-  int* Ic = nullptr;     // chain ints (2cap * Is)
+  int* Ic = nullptr;     // chain ints (cap * Layout::chain_ints())
   double* d_energy = nullptr;  // cap
   double* d_scal = nullptr;    // cap * 8
   double* d_site = nullptr;    // cap * site_per_e
   double* d_mc = nullptr;      // 2cap
   double* d_ref = nullptr;     // 8cap: LU32 refinement norms (per system: per sweep, and the solution's)
-  int* d_active = nullptr;     // 4cap: Sancho-Rubio active lists
+  int* d_active = nullptr;     // 8cap: Sancho-Rubio active lists
   // contact-cache entries written by this lane's current batch: (energy, slot,
   // iters L/R, status L/R), published once the batch's stream has drained
   struct CachePending { double energy; int slot, it0, it1, st0, st1; };
@@ -350,6 +359,13 @@ struct negf_ctx {
   zd* tmpP = nullptr;    // dense-HopX fallback workspace (n x n)
   cx<float>* X32 = nullptr;  // HopX in complex64 (mixed-precision form)
   cx<float>* omega = nullptr;  // LU32: the sketch matrix (n x kLu32Sketch)
+  // A mirror that the end slice and HopX keep (axis 0: none found, or the leads'
+  // decimations not paired): the end slice and the couplings in its two sectors,
+  // (n/2, n/2, 2) each, sector + first
+  SymMap sym = {};
+  zd* sHend = nullptr;
+  zd* sBetaL = nullptr;  // -HopX'
+  zd* sBetaR = nullptr;  // -HopX
   SpinMats sp;
   Layout lay;
   Lane lanes[kMaxLanes];
@@ -500,6 +516,7 @@ extern "C" int negf_finalize(negf_ctx* c) {
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
   hipFree(c->X32);
   hipFree(c->omega);
+  hipFree(c->sHend); hipFree(c->sBetaL); hipFree(c->sBetaR);
   hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
   hipFree(c->cc.buf);
   for (Lane& l : c->lanes)
@@ -592,6 +609,170 @@ static void contact_key_update(negf_ctx* c, const double* HopX, const zd* end0, 
   c->cc.key.swap(key);
 }
 
+// ---------------------------------------------------------------------------
+// Mirror of the contacts.  When the end slice and HopX both commute with a mirror P
+// of the cross-section (SymMap), so do alpha = E + i eta - H_end, beta = -HopX' and
+// every matrix of the decimation (sums, products and inverses of them), which is
+// then block diagonal in the two sectors of P: each n x n product and inverse of
+// Contact_Self_Energy.F90:141-209 becomes two n/2 x n/2 ones, a quarter of the
+// arithmetic.  The sector blocks are M+- (a, b) = M(r_a, r_b) +- i^ph_b M(r_a, m_b)
+// (exact for the unit phases: one rounded sum per entry), and the decimation's
+// stop test and Sigma are taken in the site basis again (k_sym_maxabs, k_sym_expand).
+
+// Full-index map of P: P e(i) = i^ph[i] e(p[i]).
+static void mirror_index(const SymMap& m, int n, std::vector<int>& p, std::vector<int>& ph) {
+  p.assign(n, 0);
+  ph.assign(n, 0);
+  for (int a = 0; a < n / 2; ++a) {
+    int ir, im, k;
+    sym_index(m, a, ir, im, k);
+    p[ir] = im; ph[ir] = k;
+    p[im] = ir; ph[im] = (4 - k) & 3;  // P e(m_a) = i^-ph_a e(r_a), as P^2 = I
+  }
+}
+
+// P A P' = A over the rows [r0, r1): A(p_i, p_j) = i^(ph_i - ph_j) A(i, j) to within tol.
+static bool mirror_holds(const std::vector<int>& p, const std::vector<int>& ph, int n, const zd* A, double tol,
+                         int r0, int r1) {
+  for (int j = 0; j < n; ++j) {
+    const zd* col = A + (long long)j * n;
+    const zd* pcol = A + (long long)p[j] * n;
+    for (int i = r0; i < r1; ++i) {
+      const zd v = rot_i(col[i], ph[i] + 4 - ph[j]);
+      const zd w = pcol[p[i]];
+      if (!(std::fabs(v.x - w.x) <= tol && std::fabs(v.y - w.y) <= tol)) return false;
+    }
+  }
+  return true;
+}
+
+// The first mirror (z before y; within an axis, orbital maps in a fixed order:
+// identity first) that Hend and X both keep to 1e-14 of their largest entry.  The
+// orbital maps tried are the monomial involutions: perm an involution, phases powers
+// of i with i^ph[o] i^ph[perm o] = 1.  Returns axis 0 when none holds, when the
+// mirrored extent is odd (a site would lie on the mirror) or norb > kSymMaxOrb.
+static SymMap find_mirror(int npy, int npz, int norb, const zd* Hend, const zd* X) {
+  SymMap none = {};
+  if (npy < 1 || npz < 1 || norb < 1 || norb > kSymMaxOrb) return none;
+  const int n = npy * npz * norb;
+  double mx = 0.0;
+  for (long long k = 0; k < (long long)n * n; ++k)
+    mx = std::max(mx, std::max(std::max(std::fabs(Hend[k].x), std::fabs(Hend[k].y)),
+                               std::max(std::fabs(X[k].x), std::fabs(X[k].y))));
+  if (!(mx < HUGE_VAL)) return none;
+  const double tol = 1e-14 * mx;
+  // orbital maps
+  std::vector<SymMap> cand;
+  SymMap cur = {};
+  for (int o = 0; o < kSymMaxOrb; ++o) cur.perm[o] = -1;
+  std::function<void()> rec = [&]() {
+    int o = 0;
+    while (o < norb && cur.perm[o] >= 0) ++o;
+    if (o == norb) { cand.push_back(cur); return; }
+    for (int k = 0; k <= 2; k += 2) {  // fixed orbital: sign +-1
+      cur.perm[o] = o; cur.ph[o] = k;
+      rec();
+    }
+    for (int q = o + 1; q < norb; ++q) {
+      if (cur.perm[q] >= 0) continue;
+      for (int k = 0; k < 4; ++k) {  // swapped pair: phases i^k, i^-k
+        cur.perm[o] = q; cur.perm[q] = o;
+        cur.ph[o] = k; cur.ph[q] = (4 - k) & 3;
+        rec();
+      }
+      cur.perm[q] = -1;
+    }
+    cur.perm[o] = -1;
+  };
+  rec();
+  std::vector<int> p, ph;
+  for (int axis = 2; axis >= 1; --axis) {
+    const int ext = axis == 1 ? npy : npz;
+    if (ext < 2 || ext % 2) continue;
+    for (SymMap m : cand) {
+      m.axis = axis; m.npy = npy; m.npz = npz; m.norb = norb;
+      mirror_index(m, n, p, ph);
+      const int r1 = std::min(n, 2 * norb);  // a quick test on two sites' rows first
+      if (!mirror_holds(p, ph, n, Hend, tol, 0, r1) || !mirror_holds(p, ph, n, X, tol, 0, r1)) continue;
+      if (mirror_holds(p, ph, n, Hend, tol, 0, n) && mirror_holds(p, ph, n, X, tol, 0, n)) return m;
+    }
+  }
+  return none;
+}
+
+// Sector blocks of M (n x n): out + s d^2 holds M_s (s = 0: +, 1: -), d = n/2.
+static void sector_blocks(const SymMap& m, int n, const zd* M, zd* out) {
+  const int d = n / 2;
+  std::vector<int> ir(d), im(d), ph(d);
+  for (int a = 0; a < d; ++a) sym_index(m, a, ir[a], im[a], ph[a]);
+  for (int b = 0; b < d; ++b)
+    for (int a = 0; a < d; ++a) {
+      const zd u = M[ir[a] + (long long)ir[b] * n];
+      const zd w = rot_i(M[ir[a] + (long long)im[b] * n], ph[b]);
+      out[a + (long long)b * d] = mk<double>(u.x + w.x, u.y + w.y);
+      out[(long long)d * d + a + (long long)b * d] = mk<double>(u.x - w.x, u.y - w.y);
+    }
+}
+
+// Find the contacts' mirror for this Hamiltonian and upload the sector blocks of the
+// end slice and of -HopX', -HopX; the paired decimation (equal end slices) takes it.
+static int mirror_update(negf_ctx* c, const zd* hopx, const zd* end0) {
+  c->sym = SymMap{};
+  if (!c->ends_equal || (c->cfg.flags & (NEGF_FLAG_SR_SEPARATE_LEADS | NEGF_FLAG_SR_NO_MIRROR))) return 0;
+  const SymMap m = find_mirror(c->cfg.npy, c->cfg.npz, c->cfg.norb, end0, hopx);
+  if (!m.axis) return 0;
+  const int n = c->n, d = n / 2;
+  const size_t dd = (size_t)d * d;
+  std::vector<zd> blk(2 * dd), adj(2 * dd);
+  if (!c->sHend) {
+    HIPCHK(hipMalloc(&c->sHend, sizeof(zd) * 2 * dd));
+    HIPCHK(hipMalloc(&c->sBetaL, sizeof(zd) * 2 * dd));
+    HIPCHK(hipMalloc(&c->sBetaR, sizeof(zd) * 2 * dd));
+  }
+  sector_blocks(m, n, end0, blk.data());
+  HIPCHK(hipMemcpy(c->sHend, blk.data(), sizeof(zd) * 2 * dd, hipMemcpyHostToDevice));
+  sector_blocks(m, n, hopx, blk.data());
+  for (int sg = 0; sg < 2; ++sg)
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i) {
+        const zd v = blk[sg * dd + i + (size_t)j * d];
+        adj[sg * dd + j + (size_t)i * d] = mk<double>(-v.x, v.y);  // -(X_s)'
+      }
+  HIPCHK(hipMemcpy(c->sBetaL, adj.data(), sizeof(zd) * 2 * dd, hipMemcpyHostToDevice));
+  for (zd& v : blk) v = mk<double>(-v.x, -v.y);
+  HIPCHK(hipMemcpy(c->sBetaR, blk.data(), sizeof(zd) * 2 * dd, hipMemcpyHostToDevice));
+  c->sym = m;
+  return 0;
+}
+
+extern "C" int negf_find_mirror(int npy, int npz, int norb, const double* Hend, const double* HopX, int* axis,
+                                int* perm, int* phase) {
+  if (!Hend || !HopX || !axis) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  const SymMap m = find_mirror(npy, npz, norb, (const zd*)Hend, (const zd*)HopX);
+  *axis = m.axis;
+  for (int o = 0; o < norb && o < kSymMaxOrb; ++o) {
+    if (perm) perm[o] = m.axis ? m.perm[o] : o;
+    if (phase) phase[o] = m.axis ? m.ph[o] : 0;
+  }
+  return 0;
+}
+
+extern "C" int negf_contact_mirror(negf_ctx* c, int* axis, int* perm, int* phase) {
+  if (!c || !axis) {
+    negf_set_error("null argument");
+    return NEGF_ERR_ARG;
+  }
+  *axis = c->sym.axis;
+  for (int o = 0; o < c->cfg.norb && o < kSymMaxOrb; ++o) {
+    if (perm) perm[o] = c->sym.axis ? c->sym.perm[o] : o;
+    if (phase) phase[o] = c->sym.axis ? c->sym.ph[o] : 0;
+  }
+  return 0;
+}
+
 extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const double* HopX,
                                     const double* gM) {
   if (!c || !Htotd || !HopX) {
@@ -610,6 +791,7 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
   c->compact = false;
   c->ends_equal = std::memcmp(Htotd, Htotd + 2 * (size_t)(L - 1) * nn, sizeof(zd) * nn) == 0;
   contact_key_update(c, HopX, (const zd*)Htotd, (const zd*)Htotd + (size_t)(L - 1) * nn);
+  if ((rc = mirror_update(c, (const zd*)HopX, (const zd*)Htotd))) return rc;
   if (!c->H) HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
   HIPCHK(hipMemcpy(c->H, Htotd, sizeof(zd) * nn * L, hipMemcpyHostToDevice));
   c->have_h = true;
@@ -664,6 +846,7 @@ static int set_potentials(negf_ctx* c, const double* onsite) {
   }
   c->ends_equal = std::memcmp(ends.data(), ends.data() + nn, sizeof(zd) * nn) == 0;
   contact_key_update(c, (const double*)c->hopx_host.data(), ends.data(), ends.data() + nn);
+  if (int rc = mirror_update(c, c->hopx_host.data(), ends.data())) return rc;
   HIPCHK(hipMemcpy(c->Hends, ends.data(), sizeof(zd) * 2 * nn, hipMemcpyHostToDevice));
   c->have_h = true;
   return 0;
@@ -726,8 +909,8 @@ static int ensure_arena(negf_ctx* c, int want) {
             (c->lanes[li].cap >= per_lane || c->lanes[li].cap == c->cap_budget);
   if (ready) return 0;
   const Layout& y = c->lay;
-  size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(3 * y.Is) +
-                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2 + 8) + 2 * sizeof(int);
+  size_t per_e = sizeof(zd) * (size_t)(y.Es + 2 * y.Cs) + sizeof(int) * (size_t)(y.Is + y.chain_ints()) +
+                 sizeof(double) * (size_t)(y.site_per_e + 8 + 2 + 8) + 8 * sizeof(int);
   free_arena(c);
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
@@ -745,13 +928,13 @@ static int ensure_arena(negf_ctx* c, int want) {
     HIPCHK(hipMalloc(&l.E, sizeof(zd) * (size_t)y.Es * cap));
     HIPCHK(hipMalloc(&l.C, sizeof(zd) * (size_t)y.Cs * 2 * cap));
     HIPCHK(hipMalloc(&l.Ie, sizeof(int) * (size_t)y.Is * cap));
-    HIPCHK(hipMalloc(&l.Ic, sizeof(int) * (size_t)y.Is * 2 * cap));
+    HIPCHK(hipMalloc(&l.Ic, sizeof(int) * (size_t)y.chain_ints() * cap));
     HIPCHK(hipMalloc(&l.d_energy, sizeof(double) * cap));
     HIPCHK(hipMalloc(&l.d_scal, sizeof(double) * 8 * cap));
     HIPCHK(hipMalloc(&l.d_site, sizeof(double) * (size_t)y.site_per_e * cap));
     HIPCHK(hipMalloc(&l.d_mc, sizeof(double) * 2 * cap));
     HIPCHK(hipMalloc(&l.d_ref, sizeof(double) * 8 * cap));
-    HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 4 * cap));
+    HIPCHK(hipMalloc(&l.d_active, sizeof(int) * 8 * cap));
     l.cap = cap;
   }
   c->cap_budget = cap_budget;
@@ -957,6 +1140,35 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
   return 0;
 }
 
+// Slots of the paired decimation: the whole n x n matrices (one slot of stride Cs
+// per energy and lead), or, with a mirror of the contacts, the two n/2 x n/2 sector
+// blocks of each energy and lead in half slots (stride Cs/2, sector-major: slot
+// s*nb + b), the matrices in the same order within a slot.
+struct SrSlots {
+  int m;              // matrix order
+  long long Cs, Is;   // slot strides (complex elements, ints)
+  long long alpha, work, tmp, t, tt, Toldt, CapT, W1, W2, W3, W4, prow, gjt;
+};
+static SrSlots sr_slots(const Layout& y, int n, bool sectors) {
+  SrSlots v;
+  if (!sectors) {
+    v.m = n; v.Cs = y.Cs; v.Is = y.Is;
+    v.alpha = y.c_alpha; v.work = y.c_work; v.tmp = y.c_tmp; v.t = y.c_t; v.tt = y.c_tt;
+    v.Toldt = y.c_Toldt; v.CapT = y.c_CapT; v.W1 = y.c_W1; v.W2 = y.c_W2; v.W3 = y.c_W3; v.W4 = y.c_W4;
+    v.prow = y.c_prow; v.gjt = y.c_gjt;
+    return v;
+  }
+  const int d = n / 2;
+  const long long dd = (long long)d * d;
+  v.m = d; v.Cs = y.Cs / 2; v.Is = y.Is_sec;
+  long long* mats[] = {&v.alpha, &v.work, &v.tmp, &v.t, &v.tt, &v.Toldt, &v.CapT, &v.W1, &v.W2, &v.W3, &v.W4};
+  long long o = 0;
+  for (long long* q : mats) { *q = o; o += dd; }
+  v.prow = o; o += (long long)kGjNBO * d;
+  v.gjt = o;  // + 64 d: the half slot holds it (Cs/2 >= 5.5 n^2 + 96 n > 2.75 n^2 + 96 n)
+  return v;
+}
+
 // Sancho-Rubio for both leads of nb energies whose end slices are equal
 // (alpha_L = alpha_R).  With beta_L = -X', beta_R = -X the reference's two
 // decimations (Contact_Self_Energy.F90:141-180) start from t_R = tt_L, tt_R = t_L,
@@ -966,37 +1178,66 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
 // change_R = Toldt_R tt' with Toldt_R = prod t.  Each lead stops accumulating at
 // its own convergence (its own iteration count, as separate calls would); the
 // shared steps run while either lead is active.  Per iteration: 1 inverse and 10
-// GEMMs instead of 2 and 16.  Chain b (< nb) holds the shared state and the left
-// lead, chain nb + b the right lead's Toldt, CapT and work matrices.
+// GEMMs instead of 2 and 16.  The left lead's slots hold the shared state and the
+// left lead, the right lead's (from R) its Toldt, CapT and work matrices.  With a
+// mirror of the contacts (negf_ctx::sym) every matrix is taken as its two sector
+// blocks; the stop test reads max|change| of the site-basis matrix and Sigma is
+// formed in the site basis from its sector blocks.
 static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long long sSig,
                          std::vector<int>& iters, std::vector<int>& status) {
-  const int n = c->n;
-  const Layout& y = c->lay;
-  const long long Cs = y.Cs, nn = y.nn;
-  const long long R = (long long)nb * Cs;  // right-lead chain offset
+  const int ns = c->sym.axis ? 2 : 1;  // slots per energy and lead
+  const SrSlots v = sr_slots(c->lay, c->n, ns == 2);
+  const int m = v.m;
+  const long long Cs = v.Cs, mm = (long long)m * m;
+  const int nh = ns * nb;                  // slots per lead
+  const long long R = (long long)nh * Cs;  // right-lead slot offset
+  const long long off1 = (long long)nb * Cs;  // second sector's slots
   hipStream_t s = ln.st;
   zd* C0 = ln.C;
   auto P = [&](long long off) { return C0 + off; };
-  GjWork w = gjw(P(y.c_prow), Cs, ln.Ic, y.Is, n);
+  GjWork w = gjw(P(v.prow), Cs, ln.Ic, v.Is, m);
   w.m3 = sr_m3(c);
-  w.tmp = P(y.c_gjt);
+  w.tmp = P(v.gjt);
   w.sTmp = Cs;
   iters.assign(2 * nb, 0);
   status.assign(2 * nb, 0);
-  TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * 2 * nb));
+  TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)v.Is * 2 * nh));
+  // alpha = (E + i eta) I - H_end and the seed (+ i eta I with sr_extra_eta) of the
+  // left lead's slots (Contact_Self_Energy.F90:68-101, :141); the right lead's alpha
+  // is the same.  In the sectors the end slice's sector blocks take H_end's place.
+  if (ns == 2)
+    TRY(launch_sr_alpha<double>(s, P(v.alpha), P(v.work), Cs, c->sHend, c->sHend + mm, ln.d_energy, nb,
+                                c->cfg.eta, c->cfg.sr_extra_eta, m));
+  else
+    TRY(launch_sr_alpha<double>(s, P(v.alpha), P(v.work), Cs, slice_end(c, 0), slice_end(c, 1), ln.d_energy,
+                                nb, c->cfg.eta, c->cfg.sr_extra_eta, m));
+  // C = a op(A) op(B) over one lead's slots (from base) with the coupling -HopX
+  // (right) or -HopX' as A (cp_left) or B; in the sectors each sector's nb slots
+  // take its block.  Cx: the result goes there (stride sSig) instead of slot matrix offC.
+  auto cpl = [&](bool right, bool cp_left, int opb, long long base, long long offX, long long offC, zd* Cx,
+                 zd a) -> int {
+    for (int sg = 0; sg < ns; ++sg) {
+      const zd* cp = ns == 2 ? (right ? c->sBetaR : c->sBetaL) + sg * mm : (right ? c->betaR : c->betaL);
+      const long long b0 = base + sg * off1;
+      zd* Cp = Cx ? Cx : P(b0 + offC);
+      const long long sC = Cx ? sSig : Cs;
+      const GemmBatch g = cp_left ? gbs(c, cp, 0, m, P(b0 + offX), Cs, m, Cp, sC, m, m)
+                                  : gbs(c, P(b0 + offX), Cs, m, cp, 0, m, Cp, sC, m, m);
+      TRY(gemm_batched<double>(s, OP_N, opb, g, nb, a, kZero));
+    }
+    return 0;
+  };
   // tmp = inv(seed); t = -tmp*betad_L = tmp X, tt = -tmp*beta_L = tmp X'  (:141-149)
-  if (int rc = sr_invert(c, s, P(y.c_work), P(y.c_tmp), Cs, n, nb, w, nullptr)) return rc;
-  TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, c->betaR, 0, n, P(y.c_t), Cs, n, n), nb,
-                           kMone, kZero));
-  TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, c->betaL, 0, n, P(y.c_tt), Cs, n, n),
-                           nb, kMone, kZero));
+  if (int rc = sr_invert(c, s, P(v.work), P(v.tmp), Cs, m, nh, w, nullptr)) return rc;
+  if (int rc = cpl(true, false, OP_N, 0, v.tmp, v.t, nullptr, kMone)) return rc;
+  if (int rc = cpl(false, false, OP_N, 0, v.tmp, v.tt, nullptr, kMone)) return rc;
   // CapT_L = t_L = t, CapT_R = t_R = tt; Toldt = I for both
-  TRY(launch_copy3<double>(s, P(y.c_CapT), P(y.c_t), P(R + y.c_CapT), P(y.c_tt), nullptr, nullptr, Cs,
-                           nn, nb, nullptr));
-  TRY(launch_eye<double>(s, P(y.c_Toldt), Cs, n, 2 * nb, nullptr));
+  TRY(launch_copy3<double>(s, P(v.CapT), P(v.t), P(R + v.CapT), P(v.tt), nullptr, nullptr, Cs, mm, nh, nullptr));
+  TRY(launch_eye<double>(s, P(v.Toldt), Cs, m, 2 * nh, nullptr));
   std::vector<int> actL(nb), actR(nb);
   for (int i = 0; i < nb; ++i) actL[i] = actR[i] = i;
   std::vector<double> mc(2 * nb);
+  std::vector<int> lists(8 * (size_t)nb, 0);
   int it = 0;
   const double tol = c->cfg.sr_tol;
   while ((!actL.empty() || !actR.empty()) && it < c->cfg.sr_max_iter) {  // (:154)
@@ -1004,56 +1245,66 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
     std::vector<int> U;
     std::set_union(actL.begin(), actL.end(), actR.begin(), actR.end(), std::back_inserter(U));
     const int nL = (int)actL.size(), nR = (int)actR.size(), nU = (int)U.size();
+    const int sL = ns * nL, sR = ns * nR, sU = ns * nU;
+    // slot lists (every sector of each active energy), then the energy lists
     int* AL = ln.d_active;
-    int* AR = ln.d_active + nb;
-    int* AU = ln.d_active + 2 * nb;
-    std::vector<int> lists(3 * (size_t)nb, 0);
-    std::copy(actL.begin(), actL.end(), lists.begin());
-    std::copy(actR.begin(), actR.end(), lists.begin() + nb);
-    std::copy(U.begin(), U.end(), lists.begin() + 2 * nb);
+    int* AR = ln.d_active + 2 * nb;
+    int* AU = ln.d_active + 4 * nb;
+    int* EL = ln.d_active + 6 * nb;
+    int* ER = ln.d_active + 7 * nb;
+    auto put = [&](const std::vector<int>& e, size_t at) {
+      for (int sg = 0; sg < ns; ++sg)
+        for (size_t k = 0; k < e.size(); ++k) lists[at + sg * e.size() + k] = sg * nb + e[k];
+    };
+    put(actL, 0);
+    put(actR, 2 * (size_t)nb);
+    put(U, 4 * (size_t)nb);
+    std::copy(actL.begin(), actL.end(), lists.begin() + 6 * nb);
+    std::copy(actR.begin(), actR.end(), lists.begin() + 7 * nb);
     TRY(hipMemcpyAsync(ln.d_active, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice, s));
     // W1_L = Toldt_L*tt, W1_R = Toldt_R*t   (:157)
     if (nL)
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_Toldt), Cs, n, P(y.c_tt), Cs, n, P(y.c_W1), Cs, n,
-                                                 n, AL), nL, kOne, kZero));
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.Toldt), Cs, m, P(v.tt), Cs, m, P(v.W1), Cs, m, m, AL), sL,
+                               kOne, kZero));
     if (nR)
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + y.c_Toldt), Cs, n, P(y.c_t), Cs, n,
-                                                 P(R + y.c_W1), Cs, n, n, AR), nR, kOne, kZero));
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + v.Toldt), Cs, m, P(v.t), Cs, m, P(R + v.W1), Cs, m, m,
+                                                 AR), sR, kOne, kZero));
     // work = I - t*tt - tt*t; tmp = inv(work)   (:160-163)
-    TRY(launch_eye<double>(s, P(y.c_work), Cs, n, nU, AU));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_t), Cs, n, P(y.c_tt), Cs, n, P(y.c_work), Cs, n, n,
-                                               AU), nU, kMone, kOne));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tt), Cs, n, P(y.c_t), Cs, n, P(y.c_work), Cs, n, n,
-                                               AU), nU, kMone, kOne));
-    if (int rc = sr_invert(c, s, P(y.c_work), P(y.c_tmp), Cs, n, nU, w, AU)) return rc;
+    TRY(launch_eye<double>(s, P(v.work), Cs, m, sU, AU));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.t), Cs, m, P(v.tt), Cs, m, P(v.work), Cs, m, m, AU), sU,
+                             kMone, kOne));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.tt), Cs, m, P(v.t), Cs, m, P(v.work), Cs, m, m, AU), sU,
+                             kMone, kOne));
+    if (int rc = sr_invert(c, s, P(v.work), P(v.tmp), Cs, m, sU, w, AU)) return rc;
     // t' = tmp*t*t -> W3, tt' = tmp*tt*tt -> W4   (:170-176)
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, P(y.c_t), Cs, n, P(y.c_W2), Cs, n, n,
-                                               AU), nU, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W2), Cs, n, P(y.c_t), Cs, n, P(y.c_W3), Cs, n, n,
-                                               AU), nU, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_tmp), Cs, n, P(y.c_tt), Cs, n, P(y.c_W2), Cs, n, n,
-                                               AU), nU, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W2), Cs, n, P(y.c_tt), Cs, n, P(y.c_W4), Cs, n, n,
-                                               AU), nU, kOne, kZero));
-    // change_L = Toldt_L*t', change_R = Toldt_R*tt'; CapT += change   (:178-180)
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.tmp), Cs, m, P(v.t), Cs, m, P(v.W2), Cs, m, m, AU), sU,
+                             kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.W2), Cs, m, P(v.t), Cs, m, P(v.W3), Cs, m, m, AU), sU,
+                             kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.tmp), Cs, m, P(v.tt), Cs, m, P(v.W2), Cs, m, m, AU), sU,
+                             kOne, kZero));
+    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.W2), Cs, m, P(v.tt), Cs, m, P(v.W4), Cs, m, m, AU), sU,
+                             kOne, kZero));
+    // change_L = Toldt_L*t', change_R = Toldt_R*tt'; CapT += change   (:178-180);
+    // max|change| of the site-basis matrix per energy and lead
     if (nL) {
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(y.c_W1), Cs, n, P(y.c_W3), Cs, n, P(y.c_W2), Cs, n,
-                                                 n, AL), nL, kOne, kZero));
-      TRY(launch_add<double>(s, P(y.c_CapT), Cs, P(y.c_CapT), Cs, P(y.c_W2), Cs, nn, nL, AL));
-      TRY(launch_maxabs<double>(s, P(y.c_W2), Cs, nn, ln.d_mc, nL, AL));
-      TRY(launch_copy3<double>(s, P(y.c_Toldt), P(y.c_W1), nullptr, nullptr, nullptr, nullptr, Cs, nn,
-                               nL, AL));
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.W1), Cs, m, P(v.W3), Cs, m, P(v.W2), Cs, m, m, AL), sL,
+                               kOne, kZero));
+      TRY(launch_add<double>(s, P(v.CapT), Cs, P(v.CapT), Cs, P(v.W2), Cs, mm, sL, AL));
+      if (ns == 2) TRY(launch_sym_maxabs(s, P(v.W2), Cs, off1, mm, ln.d_mc, nL, EL));
+      else TRY(launch_maxabs<double>(s, P(v.W2), Cs, mm, ln.d_mc, nL, AL));
+      TRY(launch_copy3<double>(s, P(v.Toldt), P(v.W1), nullptr, nullptr, nullptr, nullptr, Cs, mm, sL, AL));
     }
     if (nR) {
-      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + y.c_W1), Cs, n, P(y.c_W4), Cs, n, P(R + y.c_W2),
-                                                 Cs, n, n, AR), nR, kOne, kZero));
-      TRY(launch_add<double>(s, P(R + y.c_CapT), Cs, P(R + y.c_CapT), Cs, P(R + y.c_W2), Cs, nn, nR, AR));
-      TRY(launch_maxabs<double>(s, P(R + y.c_W2), Cs, nn, ln.d_mc + nb, nR, AR));
-      TRY(launch_copy3<double>(s, P(R + y.c_Toldt), P(R + y.c_W1), nullptr, nullptr, nullptr, nullptr,
-                               Cs, nn, nR, AR));
+      TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + v.W1), Cs, m, P(v.W4), Cs, m, P(R + v.W2), Cs, m, m,
+                                                 AR), sR, kOne, kZero));
+      TRY(launch_add<double>(s, P(R + v.CapT), Cs, P(R + v.CapT), Cs, P(R + v.W2), Cs, mm, sR, AR));
+      if (ns == 2) TRY(launch_sym_maxabs(s, P(R + v.W2), Cs, off1, mm, ln.d_mc + nb, nR, ER));
+      else TRY(launch_maxabs<double>(s, P(R + v.W2), Cs, mm, ln.d_mc + nb, nR, AR));
+      TRY(launch_copy3<double>(s, P(R + v.Toldt), P(R + v.W1), nullptr, nullptr, nullptr, nullptr, Cs, mm, sR,
+                               AR));
     }
-    TRY(launch_copy3<double>(s, P(y.c_t), P(y.c_W3), P(y.c_tt), P(y.c_W4), nullptr, nullptr, Cs, nn, nU,
-                             AU));
+    TRY(launch_copy3<double>(s, P(v.t), P(v.W3), P(v.tt), P(v.W4), nullptr, nullptr, Cs, mm, sU, AU));
     TRY(hipMemcpyAsync(mc.data(), ln.d_mc, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
     TRY(hipStreamSynchronize(s));
     for (int lead = 0; lead < 2; ++lead) {
@@ -1062,10 +1313,10 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
       for (int b : act) {
         const int ch = lead * nb + b;
         iters[ch] = it;
-        const double m = mc[ch];
-        if (m != m) {
+        const double mv = mc[ch];
+        if (mv != mv) {
           status[ch] |= NEGF_STATUS_SR_NAN;  // (:182-191): stop this lead
-        } else if (m > tol) {
+        } else if (mv > tol) {
           keep.push_back(b);
         }
       }
@@ -1076,29 +1327,36 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   for (int b : actR) status[nb + b] |= NEGF_STATUS_SR_NOT_CONVERGED;
   // g = inv(alpha + beta*CapT), Sigma = beta*g*beta' for each lead   (:195-209)
   for (int grp = 0; grp < 2; ++grp) {
-    const zd* beta = grp ? c->betaR : c->betaL;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, beta, 0, n, P(grp * R + y.c_CapT), Cs, n,
-                                               P(grp * R + y.c_W1), Cs, n, n), nb, kOne, kZero));
+    if (int rc = cpl(grp == 1, true, OP_N, grp * R, v.CapT, v.W1, nullptr, kOne)) return rc;
+    TRY(launch_add<double>(s, P(grp * R + v.work), Cs, P(grp * R + v.W1), Cs, P(v.alpha), Cs, mm, nh, nullptr));
   }
-  TRY(launch_add<double>(s, P(y.c_work), Cs, P(y.c_W1), Cs, P(y.c_alpha), Cs, nn, 2 * nb, nullptr));
-  if (int rc = sr_invert(c, s, P(y.c_work), P(y.c_W2), Cs, n, 2 * nb, w, nullptr)) return rc;
+  if (int rc = sr_invert(c, s, P(v.work), P(v.W2), Cs, m, 2 * nh, w, nullptr)) return rc;
   for (int grp = 0; grp < 2; ++grp) {
-    const zd* beta = grp ? c->betaR : c->betaL;
-    TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, beta, 0, n, P(grp * R + y.c_W2), Cs, n,
-                                               P(grp * R + y.c_W3), Cs, n, n), nb, kOne, kZero));
-    TRY(gemm_batched<double>(s, OP_N, OP_C, gbs(c, P(grp * R + y.c_W3), Cs, n, beta, 0, n, grp ? sig1 : sig0,
-                                               sSig, n, n), nb, kOne, kZero));
+    zd* sig = grp ? sig1 : sig0;
+    if (int rc = cpl(grp == 1, true, OP_N, grp * R, v.W2, v.W3, nullptr, kOne)) return rc;
+    if (ns == 1) {
+      if (int rc = cpl(grp == 1, false, OP_C, grp * R, v.W3, 0, sig, kOne)) return rc;
+    } else {
+      if (int rc = cpl(grp == 1, false, OP_C, grp * R, v.W3, v.W4, nullptr, kOne)) return rc;
+      TRY(launch_sym_expand(s, P(grp * R + v.W4), Cs, off1, m, sig, sSig, c->n, nb, c->sym));
+    }
   }
-  std::vector<int> info(2 * nb);
-  for (int ch = 0; ch < 2 * nb; ++ch)
-    TRY(hipMemcpyAsync(&info[ch], ln.Ic + (size_t)ch * y.Is + 2 * n, sizeof(int),
+  // LAPACK INFO of the slots' inverses (in the sectors: the first zero pivot of the
+  // + block, else n/2 + that of the - block)
+  std::vector<int> info(2 * (size_t)nh);
+  TRY(hipMemcpy2DAsync(info.data(), sizeof(int), ln.Ic + 2 * m, sizeof(int) * v.Is, sizeof(int), 2 * nh,
                        hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
+  auto lead_info = [&](int lead, int b) {
+    for (int sg = 0; sg < ns; ++sg)
+      if (const int f = info[(size_t)lead * nh + sg * nb + b]) return sg * m + f;
+    return 0;
+  };
   for (int b = 0; b < nb; ++b) {
     // the shared inverses belong to both leads' decimations
-    const int shared = info[b];
+    const int shared = lead_info(0, b);
     if (shared && !(status[b] & 0xFFFFF)) status[b] |= (shared & 0xFFFFF);
-    const int right = info[nb + b] ? info[nb + b] : shared;
+    const int right = lead_info(1, b) ? lead_info(1, b) : shared;
     if (right && !(status[nb + b] & 0xFFFFF)) status[nb + b] |= (right & 0xFFFFF);
   }
   return 0;
@@ -1218,12 +1476,11 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       iters[b] = m[0]; iters[nb + b] = m[1]; sst[b] = m[2]; sst[nb + b] = m[3];
     }
   } else {
-    const zd* Hend0 = slice_end(c, 0);
-    const zd* Hend1 = slice_end(c, 1);
-    TRY(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, Hend0, Hend1,
-                                ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
     // left: beta = -HopX', betad = -HopX;  right: beta = -HopX, betad = -HopX'
     const bool paired = c->ends_equal && !(c->cfg.flags & NEGF_FLAG_SR_SEPARATE_LEADS);
+    if (!paired)  // (the paired decimation forms its own alpha)
+      TRY(launch_sr_alpha<double>(s, ln.C + y.c_alpha, ln.C + y.c_work, y.Cs, slice_end(c, 0), slice_end(c, 1),
+                                  ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, n));
     int rc = paired ? run_sr_paired(c, ln, nb, Q(y.o_SigL), Q(y.o_SigR), Es, iters, sst)
                     : run_sr(c, ln, nb, nb, c->betaL, c->betaR, c->betaR, c->betaL, Q(y.o_SigL),
                              Q(y.o_SigR), Es, iters, sst);

@@ -442,6 +442,70 @@ __global__ void k_add(cx<T>* Y, long long sY, const cx<T>* X, long long sX, cons
     Y[(long long)b * sY + i] = cadd(X[(long long)b * sX + i], Z[(long long)b * sZ + i]);
 }
 
+// Mirror sectors (SymMap): for matrix b the sector blocks M+ at M + b*sM and M- at
+// M + b*sM + off1 (d x d each) give the n x n matrix V+ M+ V+' + V- M- V-'.  With
+// S = (M+ + M-)/2 and D = (M+ - M-)/2 at (a, c): Y(r_a, r_c) = S, Y(r_a, m_c) =
+// i^-ph_c D, Y(m_a, r_c) = i^ph_a D, Y(m_a, m_c) = i^(ph_a - ph_c) S.  The halving
+// and the unit phases are exact, so every entry is one rounded sum or difference.
+__global__ __launch_bounds__(256) void k_sym_expand(const double2* M, long long sM, long long off1, int d,
+                                                    double2* Y, long long sY, int n, SymMap map) {
+  const int b = blockIdx.y;
+  const double2* M0 = M + (long long)b * sM;
+  const double2* M1 = M0 + off1;
+  double2* Yb = Y + (long long)b * sY;
+  const long long dd = (long long)d * d;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < dd; i += (long long)gridDim.x * 256) {
+    const int a = (int)(i % d), cc = (int)(i / d);
+    const double2 p = M0[i], q = M1[i];
+    double2 S, D;
+    S.x = 0.5 * (p.x + q.x); S.y = 0.5 * (p.y + q.y);
+    D.x = 0.5 * (p.x - q.x); D.y = 0.5 * (p.y - q.y);
+    int ra, ma, pa, rc, mc, pc;
+    sym_index(map, a, ra, ma, pa);
+    sym_index(map, cc, rc, mc, pc);
+    Yb[ra + (long long)rc * n] = S;
+    Yb[ra + (long long)mc * n] = rot_i(D, 4 - pc);
+    Yb[ma + (long long)rc * n] = rot_i(D, pa);
+    Yb[ma + (long long)mc * n] = rot_i(S, pa + 4 - pc);
+  }
+}
+
+// out[b] = max |entry| of the matrix k_sym_expand would form from the sector blocks
+// of matrix b (through `active`): the phases have unit modulus, so it is the larger
+// of max|S| and max|D|; NaN as k_maxabs.  out[b] cleared by k_maxabs_clear first.
+__global__ __launch_bounds__(256) void k_sym_maxabs(const double2* M, long long sM, long long off1,
+                                                    long long dd, double* out, const int* active) {
+  const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
+  const double2* M0 = M + (long long)b * sM;
+  const double2* M1 = M0 + off1;
+  double m = 0.0;
+  bool nan = false;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < dd; i += (long long)gridDim.x * 256) {
+    const double2 p = M0[i], q = M1[i];
+    const double vs = hypot(0.5 * (p.x + q.x), 0.5 * (p.y + q.y));
+    const double vd = hypot(0.5 * (p.x - q.x), 0.5 * (p.y - q.y));
+    if (vs != vs || vd != vd) nan = true;
+    m = fmax(m, fmax(vs, vd));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmax(m, __shfl_xor(m, off));
+    nan = nan || __shfl_xor((int)nan, off);
+  }
+  __shared__ double red[4];
+  __shared__ int rn[4];
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = m;
+    rn[threadIdx.x >> 6] = nan;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    const bool an = rn[0] | rn[1] | rn[2] | rn[3];
+    const unsigned long long bits = an ? 0x7ff8000000000000ull : (unsigned long long)__double_as_longlong(t);
+    atomicMax((unsigned long long*)(out + b), bits);
+  }
+}
+
 // up to three batched copies Y_k = X_k in one pass
 template <typename T>
 __global__ void k_copy3(cx<T>* Y0, const cx<T>* X0, cx<T>* Y1, const cx<T>* X1, cx<T>* Y2,
@@ -1218,6 +1282,24 @@ hipError_t launch_maxabs(hipStream_t s, const cx<T>* X, long long sX, long long 
   const long long chunks = std::min<long long>(512, std::max<long long>(1, (nn + 4095) / 4096));
   hipLaunchKernelGGL((k_maxabs<T>), dim3((unsigned)chunks, batch), dim3(256), 0, s, X, sX, nn, out,
                      active);
+  return hipGetLastError();
+}
+
+hipError_t launch_sym_expand(hipStream_t s, const double2* M, long long sM, long long off1, int d, double2* Y,
+                             long long sY, int n, int batch, const SymMap& map) {
+  if (batch <= 0) return hipSuccess;
+  if (map.axis < 1 || map.axis > 2 || map.norb < 1 || map.norb > kSymMaxOrb || 2 * d != n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sym_expand, grid_nn((long long)d * d, batch), dim3(256), 0, s, M, sM, off1, d, Y, sY, n,
+                     map);
+  return hipGetLastError();
+}
+
+hipError_t launch_sym_maxabs(hipStream_t s, const double2* M, long long sM, long long off1, long long dd,
+                             double* out, int batch, const int* active) {
+  if (batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_maxabs_clear, dim3((batch + 63) / 64), dim3(64), 0, s, out, active, batch);
+  const long long chunks = std::min<long long>(512, std::max<long long>(1, (dd + 4095) / 4096));
+  hipLaunchKernelGGL(k_sym_maxabs, dim3((unsigned)chunks, batch), dim3(256), 0, s, M, sM, off1, dd, out, active);
   return hipGetLastError();
 }
 

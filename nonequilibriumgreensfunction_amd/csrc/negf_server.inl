@@ -34,9 +34,9 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // Device bytes one energy of this layout needs in a lane pool.
 size_t lane_bytes_per_energy(const Layout& y) {
   return align256(sizeof(zd) * (size_t)y.Es) + align256(sizeof(zd) * 2 * (size_t)y.Cs) +
-         align256(sizeof(int) * (size_t)y.Is) + align256(sizeof(int) * 2 * (size_t)y.Is) +
+         align256(sizeof(int) * (size_t)y.Is) + align256(sizeof(int) * (size_t)y.chain_ints()) +
          align256(sizeof(double) * (size_t)y.site_per_e) + 3 * 256 + 2 * align256(sizeof(double) * 8) +
-         align256(sizeof(int) * 4);
+         align256(sizeof(int) * 8);
 }
 
 // Point the lane's buffers at `pool` for nb energies of layout y (nb * per-energy
@@ -51,13 +51,13 @@ void bind_lane(Lane& ln, const Layout& y, int nb, char* pool) {
   ln.E = (zd*)take(sizeof(zd) * (size_t)y.Es * nb);
   ln.C = (zd*)take(sizeof(zd) * (size_t)y.Cs * 2 * nb);
   ln.Ie = (int*)take(sizeof(int) * (size_t)y.Is * nb);
-  ln.Ic = (int*)take(sizeof(int) * (size_t)y.Is * 2 * nb);
+  ln.Ic = (int*)take(sizeof(int) * (size_t)y.chain_ints() * nb);
   ln.d_site = (double*)take(sizeof(double) * (size_t)y.site_per_e * nb);
   ln.d_energy = (double*)take(sizeof(double) * nb);
   ln.d_scal = (double*)take(sizeof(double) * 8 * nb);
   ln.d_mc = (double*)take(sizeof(double) * 2 * nb);
   ln.d_ref = (double*)take(sizeof(double) * 8 * nb);
-  ln.d_active = (int*)take(sizeof(int) * 4 * nb);
+  ln.d_active = (int*)take(sizeof(int) * 8 * nb);
   ln.cap = nb;
 }
 
@@ -65,8 +65,9 @@ void bind_lane(Lane& ln, const Layout& y, int nb, char* pool) {
 size_t resident_bytes(int n, int L, int nsites, int precision, bool compact) {
   const size_t nn = (size_t)n * n;
   // X, betaL, betaR, and tmpP (2 n x n, allocated only for a dense HopX; counted
-  // always, since admission runs before HopX is inspected)
-  size_t b = 5 * sizeof(zd) * nn;
+  // always, since admission runs before HopX is inspected); the mirror sectors of the
+  // end slice and the couplings (6 n/2 x n/2, likewise counted always)
+  size_t b = 5 * sizeof(zd) * nn + 6 * sizeof(zd) * (nn / 4);
   if (compact)
     b += 3 * sizeof(zd) * nn + sizeof(double) * (size_t)L * nsites;  // H0, two end slices, V
   else

@@ -1,0 +1,144 @@
+"""CPU checks of the lead-mirror search (negf_find_mirror, host only) and of the
+sector algebra the paired decimation runs on.
+
+The mirror is pinned by what defines it, not by its own formula: the unitary P
+built densely from (axis, perm, phase) must be a Hermitian involution that commutes
+with the end slice and HopX, and a perturbation that breaks the symmetry must make
+the search drop it.  The sector blocks M+- (a, b) = M(r_a, r_b) +- i^ph_b M(r_a, m_b)
+and the site-basis expansion are checked against explicit eigenbases of P."""
+import numpy as np
+import pytest
+
+from nonequilibriumgreensfunction_amd import capi
+from nonequilibriumgreensfunction_amd.hamiltonian import build_wire
+
+
+def dense_mirror(npy, npz, norb, axis, perm, phase):
+    """P with P e(site, o) = i^phase[o] e(R site, perm[o]) (site = y + npy z)."""
+    n = npy * npz * norb
+    P = np.zeros((n, n), complex)
+    for z in range(npz):
+        for y in range(npy):
+            ym, zm = (npy - 1 - y, z) if axis == 1 else (y, npz - 1 - z)
+            for o in range(norb):
+                P[(ym + npy * zm) * norb + perm[o], (y + npy * z) * norb + o] = 1j ** phase[o]
+    return P
+
+
+def rep_index(npy, npz, norb, axis, perm, phase):
+    """(r_a, m_a, ph_a) over the representative half, in the sector order."""
+    out = []
+    if axis == 1:
+        sites = [(y, z) for z in range(npz) for y in range(npy // 2)]
+    else:
+        sites = [(y, z) for z in range(npz // 2) for y in range(npy)]
+    for (y, z) in sites:
+        ym, zm = (npy - 1 - y, z) if axis == 1 else (y, npz - 1 - z)
+        for o in range(norb):
+            out.append(((y + npy * z) * norb + o, (ym + npy * zm) * norb + perm[o], phase[o]))
+    return out
+
+
+WIRES = [
+    ((3, 4, 4), dict()),
+    ((3, 6, 4), dict(disorder=0.7, seed=3)),  # disorder stays off the end slices
+    ((3, 4, 3), dict()),                       # odd Npz: the y mirror
+    ((3, 2, 1), dict(model="metal", params=dict(t=1.0, eps=0.1))),
+]
+
+
+@pytest.mark.parametrize("case", range(len(WIRES)))
+def test_found_mirror_commutes_with_the_lead(case):
+    dims, kw = WIRES[case]
+    w = build_wire(*dims, **kw)
+    H, X = w.htotd[:, :, 0], w.hopx
+    axis, perm, phase = capi.find_mirror(w.npy, w.npz, w.norb, H, X)
+    assert axis in (1, 2)
+    P = dense_mirror(w.npy, w.npz, w.norb, axis, perm, phase)
+    I = np.eye(w.n)
+    assert np.abs(P @ P - I).max() == 0.0
+    assert np.abs(P.conj().T - P).max() == 0.0
+    for A in (H, X, w.htotd[:, :, -1]):
+        assert np.abs(P @ A - A @ P).max() <= 1e-14 * np.abs(A).max()
+    # z is tried before y: the y mirror only where the z extent is odd
+    assert axis == (2 if w.npz % 2 == 0 else 1)
+
+
+def test_symmetry_breaking_drops_the_mirror():
+    w = build_wire(3, 4, 4)
+    H, X = w.htotd[:, :, 0].copy(), w.hopx.copy()
+    assert capi.find_mirror(4, 4, 4, H, X)[0] == 2
+    # a potential that grows along z breaks the z mirror; the y mirror remains
+    site = np.arange(w.n) // 4
+    Hz = H + np.diag(0.05 * (site // 4))
+    axis, perm, phase = capi.find_mirror(4, 4, 4, Hz, X)
+    assert axis == 1
+    P = dense_mirror(4, 4, 4, axis, perm, phase)
+    assert np.abs(P @ Hz - Hz @ P).max() <= 1e-14
+    # ... and one along y as well: no mirror
+    Hyz = Hz + np.diag(0.03 * (site % 4))
+    assert capi.find_mirror(4, 4, 4, Hyz, X)[0] == 0
+    # one entry of HopX off by 1e-9: no mirror; off by 1e-17 of its scale: kept
+    for eps, keep in ((1e-9, False), (1e-17, True)):
+        X2 = X.copy()
+        X2[5, 5] += eps * np.abs(X).max()
+        assert (capi.find_mirror(4, 4, 4, H, X2)[0] != 0) == keep, eps
+    # flux threading the wire (Peierls phases on the y bonds) breaks both mirrors
+    wf = build_wire(3, 4, 4, flux=0.3)
+    assert capi.find_mirror(4, 4, 4, wf.htotd[:, :, 0], wf.hopx)[0] == 0
+
+
+def test_sector_blocks_and_expansion_against_eigenbases():
+    """V+- (n x n/2) orthonormal eigenbases of P; for M commuting with P, V' M V is
+    block diagonal with the blocks M+- the engine forms, and V+ S+ V+' + V- S- V-'
+    equals the expansion the engine's kernel writes (entries S, i^-ph_b D, i^ph_a D,
+    i^(ph_a - ph_b) S with S, D the half sum and difference)."""
+    rng = np.random.default_rng(5)
+    for dims in ((3, 4, 4), (3, 4, 3)):
+        w = build_wire(*dims)
+        npy, npz, no = w.npy, w.npz, w.norb
+        axis, perm, phase = capi.find_mirror(npy, npz, no, w.htotd[:, :, 0], w.hopx)
+        P = dense_mirror(npy, npz, no, axis, perm, phase)
+        idx = rep_index(npy, npz, no, axis, perm, phase)
+        n, d = w.n, w.n // 2
+        V = {s: np.zeros((n, d), complex) for s in (1, -1)}
+        for a, (r, m, ph) in enumerate(idx):
+            for s in (1, -1):
+                V[s][r, a] = 1 / np.sqrt(2)
+                V[s][m, a] = s * 1j ** ph / np.sqrt(2)
+        for s in (1, -1):
+            assert np.abs(P @ V[s] - s * V[s]).max() < 1e-15
+            assert np.abs(V[s].conj().T @ V[s] - np.eye(d)).max() < 1e-15
+        # a random matrix commuting with P
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        M = A + P @ A @ P
+        r = np.array([t[0] for t in idx])
+        mi = np.array([t[1] for t in idx])
+        ph = np.array([t[2] for t in idx])
+        for s in (1, -1):
+            blk = M[np.ix_(r, r)] + s * M[np.ix_(r, mi)] * (1j ** ph)[None, :]
+            assert np.abs(V[s].conj().T @ M @ V[s] - blk).max() < 1e-12
+        assert np.abs(V[1].conj().T @ M @ V[-1]).max() < 1e-12
+        # expansion from two arbitrary sector blocks
+        Sp = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+        Sm = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+        full = V[1] @ Sp @ V[1].conj().T + V[-1] @ Sm @ V[-1].conj().T
+        S, D = 0.5 * (Sp + Sm), 0.5 * (Sp - Sm)
+        Y = np.zeros((n, n), complex)
+        pa = (1j ** ph)
+        Y[np.ix_(r, r)] = S
+        Y[np.ix_(r, mi)] = D * np.conj(pa)[None, :]
+        Y[np.ix_(mi, r)] = D * pa[:, None]
+        Y[np.ix_(mi, mi)] = S * pa[:, None] * np.conj(pa)[None, :]
+        assert np.abs(Y - full).max() < 1e-12
+        # the largest entry of the site-basis matrix is max(max|S|, max|D|)
+        assert np.isclose(np.abs(full).max(), max(np.abs(S).max(), np.abs(D).max()), rtol=1e-14)
+
+
+def test_find_mirror_rejects_bad_arguments():
+    w = build_wire(3, 4, 4)
+    with pytest.raises(ValueError):
+        capi.find_mirror(4, 4, 4, w.htotd[:8, :8, 0], w.hopx)
+    # more orbitals than the search handles: no mirror, no error
+    n = 2 * 2 * 5
+    assert capi.find_mirror(2, 2, 5, np.eye(n), np.eye(n))[0] == 0
