@@ -135,7 +135,8 @@ typedef struct {
 /* Paired decimation only (equal end slices): by default, when the end slice and HopX
  * both keep a mirror of the cross-section (negf_find_mirror), every Sancho-Rubio
  * matrix is block diagonal in the mirror's two sectors, and the decimation runs on
- * the n/2 x n/2 sector blocks (a quarter of the arithmetic); its stop test
+ * the n/2 x n/2 sector blocks (a quarter of the arithmetic; an eighth when a second,
+ * anticommuting mirror makes the two sectors equivalent); its stop test
  * (max|change| <= sr_tol) and Sigma are taken in the site basis as
  * Contact_Self_Energy.F90:178-209 does.  SR_NO_MIRROR keeps the n x n decimation. */
 #define NEGF_FLAG_SR_NO_MIRROR 64
@@ -210,14 +211,16 @@ int negf_refine_stats(negf_ctx* ctx, long long* blocks, long long* steps, long l
  *     P e(site, o) = i^phase[o] e(mirror site, perm[o]),
  * perm an involution of the orbitals and i^phase[o] i^phase[perm[o]] = 1 (P is a
  * Hermitian unitary involution).  axis = 0 (none), 1 (y) or 2 (z); perm and phase
- * (Norb entries each, may be NULL) are filled when axis > 0.  Norb <= 4.  Host
- * only: needs no device. */
+ * (Norb entries each, may be NULL) are filled when axis > 0.  paired (may be NULL)
+ * = 1 when a mirror on the other axis also holds and anticommutes with P: it maps
+ * one sector of P onto the other, and one n/2 x n/2 decimation serves both.
+ * Norb <= 4.  Host only: needs no device. */
 int negf_find_mirror(int npy, int npz, int norb, const double* Hend, const double* HopX, int* axis, int* perm,
-                     int* phase);
+                     int* phase, int* paired);
 /* The mirror the context's paired decimation uses for the current Hamiltonian
  * (axis 0: none -- no mirror kept, end slices unequal, SR_SEPARATE_LEADS or
  * SR_NO_MIRROR set).  Same outputs as negf_find_mirror. */
-int negf_contact_mirror(negf_ctx* ctx, int* axis, int* perm, int* phase);
+int negf_contact_mirror(negf_ctx* ctx, int* axis, int* perm, int* phase, int* paired);
 /* NEGF_FLAG_GRAPH, no counterpart in the reference: graphs captured and graph
  * replays (batches launched from a graph) since negf_init. */
 int negf_graph_stats(negf_ctx* ctx, long long* built, long long* replays);

@@ -138,8 +138,8 @@ def lib():
         L.negf_set_lane_table.argtypes = [ctypes.c_int, _ip, _ip]
         L.negf_invert_batch_noalloc.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 4
         L.negf_profile_end.argtypes = [_dp, _dp, ctypes.POINTER(ctypes.c_longlong)]  # arrays of 12
-        L.negf_find_mirror.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _ip]
-        L.negf_contact_mirror.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        L.negf_find_mirror.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _ip, _ip]
+        L.negf_contact_mirror.argtypes = [ctypes.c_void_p, _ip, _ip, _ip, _ip]
         if hasattr(L, "negf_retarded"):
             L.negf_retarded.argtypes = [ctypes.c_void_p] + [_dp] * 6
             L.negf_gnp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] * 12
@@ -270,18 +270,19 @@ def invert_batch(G, complex64=False, m3=True, local=False):
 
 def find_mirror(npy, npz, norb, Hend, HopX):
     """negf_find_mirror (host only): the mirror of the lead cross-section that both the
-    end slice and HopX keep, as (axis, perm, phase); axis 0 when there is none."""
+    end slice and HopX keep, as (axis, perm, phase, paired); axis 0 when there is none;
+    paired when a second, anticommuting mirror makes its two sectors equivalent."""
     H = np.asfortranarray(Hend, dtype=complex)
     X = np.asfortranarray(HopX, dtype=complex)
     n = npy * npz * norb
     if H.shape != (n, n) or X.shape != (n, n):
         raise ValueError(f"Hend and HopX must be {n} x {n}")
-    ax = ctypes.c_int()
+    ax, pr = ctypes.c_int(), ctypes.c_int()
     perm = np.zeros(max(norb, 4), np.int32)
     ph = np.zeros(max(norb, 4), np.int32)
     _check(lib().negf_find_mirror(npy, npz, norb, _p(H), _p(X), ctypes.byref(ax), perm.ctypes.data_as(_ip),
-                                  ph.ctypes.data_as(_ip)))
-    return ax.value, perm[:norb].copy(), ph[:norb].copy()
+                                  ph.ctypes.data_as(_ip), ctypes.byref(pr)))
+    return ax.value, perm[:norb].copy(), ph[:norb].copy(), bool(pr.value)
 
 
 def device_count() -> int:
@@ -403,15 +404,16 @@ class Context:
         return b.value, st.value, e.value
 
     def contact_mirror(self):
-        """The cross-section mirror the paired decimation uses: (axis, perm, phase),
-        axis 0 (none), 1 (y) or 2 (z); P e(site, o) = i^phase[o] e(mirror site, perm[o])."""
+        """The cross-section mirror the paired decimation uses: (axis, perm, phase, paired),
+        axis 0 (none), 1 (y) or 2 (z); P e(site, o) = i^phase[o] e(mirror site, perm[o]);
+        paired: one sector's decimation serves both (a second, anticommuting mirror)."""
         no = self.dims[3]
-        ax = ctypes.c_int()
+        ax, pr = ctypes.c_int(), ctypes.c_int()
         perm = np.zeros(max(no, 4), np.int32)
         ph = np.zeros(max(no, 4), np.int32)
         _check(lib().negf_contact_mirror(self._h, ctypes.byref(ax), perm.ctypes.data_as(_ip),
-                                         ph.ctypes.data_as(_ip)))
-        return ax.value, perm[:no].copy(), ph[:no].copy()
+                                         ph.ctypes.data_as(_ip), ctypes.byref(pr)))
+        return ax.value, perm[:no].copy(), ph[:no].copy(), bool(pr.value)
 
     def contact_cache_stats(self):
         """(stored energies, energies served from the contact cache)."""

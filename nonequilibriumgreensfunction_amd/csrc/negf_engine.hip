@@ -86,9 +86,9 @@ hipError_t launch_maxabs(hipStream_t, const cx<T>*, long long, long long, double
 template <typename T>
 hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long long, int, int,
                             double*, long long);
-hipError_t launch_sym_expand(hipStream_t, const double2*, long long, long long, int, double2*, long long, int, int,
-                             const SymMap&);
-hipError_t launch_sym_maxabs(hipStream_t, const double2*, long long, long long, long long, double*, int,
+hipError_t launch_sym_expand(hipStream_t, const double2*, long long, long long, const int2*, int, double2*,
+                             long long, int, int, const SymMap&);
+hipError_t launch_sym_maxabs(hipStream_t, const double2*, long long, long long, const int2*, int, double*, int,
                              const int*);
 template <typename Td, typename Ts>
 hipError_t launch_convert(hipStream_t, cx<Td>*, long long, const cx<Ts>*, long long, long long, int,
@@ -363,6 +363,11 @@ struct negf_ctx {
   // decimations not paired): the end slice and the couplings in its two sectors,
   // (n/2, n/2, 2) each, sector + first
   SymMap sym = {};
+  // With a second mirror that anticommutes with the first, the - sector's blocks are
+  // those of the + sector, permuted with phases (sPair, n/2 int2; see find_mirror):
+  // one sector is decimated
+  bool sym_paired = false;
+  int2* sPair = nullptr;
   zd* sHend = nullptr;
   zd* sBetaL = nullptr;  // -HopX'
   zd* sBetaR = nullptr;  // -HopX
@@ -516,7 +521,7 @@ extern "C" int negf_finalize(negf_ctx* c) {
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
   hipFree(c->X32);
   hipFree(c->omega);
-  hipFree(c->sHend); hipFree(c->sBetaL); hipFree(c->sBetaR);
+  hipFree(c->sHend); hipFree(c->sBetaL); hipFree(c->sBetaR); hipFree(c->sPair);
   hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
   hipFree(c->cc.buf);
   for (Lane& l : c->lanes)
@@ -618,6 +623,11 @@ static void contact_key_update(negf_ctx* c, const double* HopX, const zd* end0, 
 // arithmetic.  The sector blocks are M+- (a, b) = M(r_a, r_b) +- i^ph_b M(r_a, m_b)
 // (exact for the unit phases: one rounded sum per entry), and the decimation's
 // stop test and Sigma are taken in the site basis again (k_sym_maxabs, k_sym_expand).
+// When a second mirror Q on the other axis also holds and anticommutes with P (the
+// spin-orbit bands of the 4-band model: the z mirror acts on the orbitals as
+// diag(1, -1, -1, 1), the y mirror swaps them in pairs with phases i, -i), Q carries
+// one sector onto the other and every - block is the + block permuted with phases:
+// one n/2 x n/2 decimation, an eighth of the arithmetic.
 
 // Full-index map of P: P e(i) = i^ph[i] e(p[i]).
 static void mirror_index(const SymMap& m, int n, std::vector<int>& p, std::vector<int>& ph) {
@@ -646,22 +656,9 @@ static bool mirror_holds(const std::vector<int>& p, const std::vector<int>& ph, 
   return true;
 }
 
-// The first mirror (z before y; within an axis, orbital maps in a fixed order:
-// identity first) that Hend and X both keep to 1e-14 of their largest entry.  The
-// orbital maps tried are the monomial involutions: perm an involution, phases powers
-// of i with i^ph[o] i^ph[perm o] = 1.  Returns axis 0 when none holds, when the
-// mirrored extent is odd (a site would lie on the mirror) or norb > kSymMaxOrb.
-static SymMap find_mirror(int npy, int npz, int norb, const zd* Hend, const zd* X) {
-  SymMap none = {};
-  if (npy < 1 || npz < 1 || norb < 1 || norb > kSymMaxOrb) return none;
-  const int n = npy * npz * norb;
-  double mx = 0.0;
-  for (long long k = 0; k < (long long)n * n; ++k)
-    mx = std::max(mx, std::max(std::max(std::fabs(Hend[k].x), std::fabs(Hend[k].y)),
-                               std::max(std::fabs(X[k].x), std::fabs(X[k].y))));
-  if (!(mx < HUGE_VAL)) return none;
-  const double tol = 1e-14 * mx;
-  // orbital maps
+// The orbital maps tried for a mirror: the monomial involutions (perm an involution,
+// phases powers of i with i^ph[o] i^ph[perm o] = 1), identity first.
+static std::vector<SymMap> orbital_maps(int norb) {
   std::vector<SymMap> cand;
   SymMap cur = {};
   for (int o = 0; o < kSymMaxOrb; ++o) cur.perm[o] = -1;
@@ -685,17 +682,92 @@ static SymMap find_mirror(int npy, int npz, int norb, const zd* Hend, const zd* 
     cur.perm[o] = -1;
   };
   rec();
-  std::vector<int> p, ph;
-  for (int axis = 2; axis >= 1; --axis) {
-    const int ext = axis == 1 ? npy : npz;
-    if (ext < 2 || ext % 2) continue;
-    for (SymMap m : cand) {
-      m.axis = axis; m.npy = npy; m.npz = npz; m.norb = norb;
-      mirror_index(m, n, p, ph);
-      const int r1 = std::min(n, 2 * norb);  // a quick test on two sites' rows first
-      if (!mirror_holds(p, ph, n, Hend, tol, 0, r1) || !mirror_holds(p, ph, n, X, tol, 0, r1)) continue;
-      if (mirror_holds(p, ph, n, Hend, tol, 0, n) && mirror_holds(p, ph, n, X, tol, 0, n)) return m;
+  return cand;
+}
+
+// Q a second mirror that anticommutes with P (QP = -PQ): Q maps the + sector of P
+// onto the - sector, Q v+_b = i^k_b v-_sigma(b) with sigma a permutation (both are
+// monomial), so for every M that commutes with Q, M-(sigma a, sigma c) =
+// i^(k_a - k_c) M+(a, c).  Fills pair[sigma(b)] = (b, k_b); false when Q does not
+// anticommute with P or the map is not of that form.
+static bool sector_pair(const SymMap& P, const SymMap& Q, int n, std::vector<int2>& pair) {
+  std::vector<int> p, pp, q, qp;
+  mirror_index(P, n, p, pp);
+  mirror_index(Q, n, q, qp);
+  for (int i = 0; i < n; ++i)  // PQ e_i = -QP e_i
+    if (p[q[i]] != q[p[i]] || ((qp[i] + pp[q[i]]) - (pp[i] + qp[p[i]]) + 8) % 4 != 2) return false;
+  const int d = n / 2;
+  std::vector<int> ir(d), im(d), ph(d), rep(n, -1), mir(n, -1);
+  for (int a = 0; a < d; ++a) {
+    sym_index(P, a, ir[a], im[a], ph[a]);
+    rep[ir[a]] = a;
+    mir[im[a]] = a;
+  }
+  pair.assign(d, make_int2(-1, 0));
+  for (int b = 0; b < d; ++b) {
+    // Q v+_b = (i^c1 e_j1 + i^c2 e_j2) / sqrt 2 against v-_a = (e_r_a - i^ph_a e_m_a) / sqrt 2
+    const int j1 = q[ir[b]], c1 = qp[ir[b]];
+    const int j2 = q[im[b]], c2 = (ph[b] + qp[im[b]]) & 3;
+    int a, k;
+    if (rep[j1] >= 0) {
+      a = rep[j1];
+      k = c1;
+      if (j2 != im[a] || c2 != ((c1 + 2 + ph[a]) & 3)) return false;
+    } else {
+      a = mir[j1];
+      k = c2;
+      if (a < 0 || j2 != ir[a] || c1 != ((c2 + 2 + ph[a]) & 3)) return false;
     }
+    if (pair[a].x >= 0) return false;
+    pair[a] = make_int2(b, k);
+  }
+  return true;
+}
+
+// The first mirror (z before y; within an axis, orbital maps in the order of
+// orbital_maps) that Hend and X both keep to 1e-14 of their largest entry, and, when
+// a mirror on the other axis also holds and anticommutes with it, the sector pair
+// map it gives (else pair is left empty).  Returns axis 0 when no mirror holds,
+// where the mirrored extent is odd (a site would lie on the mirror) or norb >
+// kSymMaxOrb.
+static SymMap find_mirror(int npy, int npz, int norb, const zd* Hend, const zd* X,
+                          std::vector<int2>* pair = nullptr) {
+  SymMap none = {};
+  if (pair) pair->clear();
+  if (npy < 1 || npz < 1 || norb < 1 || norb > kSymMaxOrb) return none;
+  const int n = npy * npz * norb;
+  double mx = 0.0;
+  for (long long k = 0; k < (long long)n * n; ++k)
+    mx = std::max(mx, std::max(std::max(std::fabs(Hend[k].x), std::fabs(Hend[k].y)),
+                               std::max(std::fabs(X[k].x), std::fabs(X[k].y))));
+  if (!(mx < HUGE_VAL)) return none;
+  const double tol = 1e-14 * mx;
+  const std::vector<SymMap> cand = orbital_maps(norb);
+  std::vector<int> p, ph;
+  auto holds = [&](SymMap m, int axis, SymMap& out) {
+    const int ext = axis == 1 ? npy : npz;
+    if (ext < 2 || ext % 2) return false;
+    m.axis = axis; m.npy = npy; m.npz = npz; m.norb = norb;
+    mirror_index(m, n, p, ph);
+    const int r1 = std::min(n, 2 * norb);  // a quick test on two sites' rows first
+    if (!mirror_holds(p, ph, n, Hend, tol, 0, r1) || !mirror_holds(p, ph, n, X, tol, 0, r1)) return false;
+    if (!mirror_holds(p, ph, n, Hend, tol, 0, n) || !mirror_holds(p, ph, n, X, tol, 0, n)) return false;
+    out = m;
+    return true;
+  };
+  for (int axis = 2; axis >= 1; --axis) {
+    SymMap P;
+    bool found = false;
+    for (const SymMap& m : cand)
+      if ((found = holds(m, axis, P))) break;
+    if (!found) continue;
+    if (pair) {
+      SymMap Q;
+      for (const SymMap& m : cand)
+        if (holds(m, 3 - axis, Q) && sector_pair(P, Q, n, *pair)) break;
+        else pair->clear();
+    }
+    return P;
   }
   return none;
 }
@@ -719,7 +791,9 @@ static void sector_blocks(const SymMap& m, int n, const zd* M, zd* out) {
 static int mirror_update(negf_ctx* c, const zd* hopx, const zd* end0) {
   c->sym = SymMap{};
   if (!c->ends_equal || (c->cfg.flags & (NEGF_FLAG_SR_SEPARATE_LEADS | NEGF_FLAG_SR_NO_MIRROR))) return 0;
-  const SymMap m = find_mirror(c->cfg.npy, c->cfg.npz, c->cfg.norb, end0, hopx);
+  c->sym_paired = false;
+  std::vector<int2> pair;
+  const SymMap m = find_mirror(c->cfg.npy, c->cfg.npz, c->cfg.norb, end0, hopx, &pair);
   if (!m.axis) return 0;
   const int n = c->n, d = n / 2;
   const size_t dd = (size_t)d * d;
@@ -728,10 +802,27 @@ static int mirror_update(negf_ctx* c, const zd* hopx, const zd* end0) {
     HIPCHK(hipMalloc(&c->sHend, sizeof(zd) * 2 * dd));
     HIPCHK(hipMalloc(&c->sBetaL, sizeof(zd) * 2 * dd));
     HIPCHK(hipMalloc(&c->sBetaR, sizeof(zd) * 2 * dd));
+    HIPCHK(hipMalloc(&c->sPair, sizeof(int2) * d));
   }
+  // the pair map, checked once more on the blocks themselves: M- = the remapped M+
+  auto pair_holds = [&](const std::vector<zd>& B) {
+    double mx = 0.0;
+    for (const zd& v : B) mx = std::max(mx, std::max(std::fabs(v.x), std::fabs(v.y)));
+    for (int cc = 0; cc < d; ++cc)
+      for (int a = 0; a < d; ++a) {
+        const int2 pa = pair[a], pc = pair[cc];
+        const zd u = rot_i(B[pa.x + (size_t)pc.x * d], pa.y + 4 - pc.y);
+        const zd w = B[dd + a + (size_t)cc * d];
+        if (!(std::fabs(u.x - w.x) <= 1e-14 * mx && std::fabs(u.y - w.y) <= 1e-14 * mx)) return false;
+      }
+    return true;
+  };
   sector_blocks(m, n, end0, blk.data());
+  bool paired = !pair.empty() && pair_holds(blk);
   HIPCHK(hipMemcpy(c->sHend, blk.data(), sizeof(zd) * 2 * dd, hipMemcpyHostToDevice));
   sector_blocks(m, n, hopx, blk.data());
+  paired = paired && pair_holds(blk);
+  if (paired) HIPCHK(hipMemcpy(c->sPair, pair.data(), sizeof(int2) * d, hipMemcpyHostToDevice));
   for (int sg = 0; sg < 2; ++sg)
     for (int j = 0; j < d; ++j)
       for (int i = 0; i < d; ++i) {
@@ -742,17 +833,20 @@ static int mirror_update(negf_ctx* c, const zd* hopx, const zd* end0) {
   for (zd& v : blk) v = mk<double>(-v.x, -v.y);
   HIPCHK(hipMemcpy(c->sBetaR, blk.data(), sizeof(zd) * 2 * dd, hipMemcpyHostToDevice));
   c->sym = m;
+  c->sym_paired = paired;
   return 0;
 }
 
 extern "C" int negf_find_mirror(int npy, int npz, int norb, const double* Hend, const double* HopX, int* axis,
-                                int* perm, int* phase) {
+                                int* perm, int* phase, int* paired) {
   if (!Hend || !HopX || !axis) {
     negf_set_error("null argument");
     return NEGF_ERR_ARG;
   }
-  const SymMap m = find_mirror(npy, npz, norb, (const zd*)Hend, (const zd*)HopX);
+  std::vector<int2> pair;
+  const SymMap m = find_mirror(npy, npz, norb, (const zd*)Hend, (const zd*)HopX, &pair);
   *axis = m.axis;
+  if (paired) *paired = m.axis && !pair.empty();
   for (int o = 0; o < norb && o < kSymMaxOrb; ++o) {
     if (perm) perm[o] = m.axis ? m.perm[o] : o;
     if (phase) phase[o] = m.axis ? m.ph[o] : 0;
@@ -760,12 +854,13 @@ extern "C" int negf_find_mirror(int npy, int npz, int norb, const double* Hend, 
   return 0;
 }
 
-extern "C" int negf_contact_mirror(negf_ctx* c, int* axis, int* perm, int* phase) {
+extern "C" int negf_contact_mirror(negf_ctx* c, int* axis, int* perm, int* phase, int* paired) {
   if (!c || !axis) {
     negf_set_error("null argument");
     return NEGF_ERR_ARG;
   }
   *axis = c->sym.axis;
+  if (paired) *paired = c->sym.axis && c->sym_paired;
   for (int o = 0; o < c->cfg.norb && o < kSymMaxOrb; ++o) {
     if (perm) perm[o] = c->sym.axis ? c->sym.perm[o] : o;
     if (phase) phase[o] = c->sym.axis ? c->sym.ph[o] : 0;
@@ -1141,9 +1236,10 @@ static int run_sr(negf_ctx* c, Lane& ln, int nl, int nr, const zd* betaA, const 
 }
 
 // Slots of the paired decimation: the whole n x n matrices (one slot of stride Cs
-// per energy and lead), or, with a mirror of the contacts, the two n/2 x n/2 sector
+// per energy and lead), or, with a mirror of the contacts, the n/2 x n/2 sector
 // blocks of each energy and lead in half slots (stride Cs/2, sector-major: slot
-// s*nb + b), the matrices in the same order within a slot.
+// s*nb + b; the + sector alone when the sectors are paired), the matrices in the
+// same order within a slot.
 struct SrSlots {
   int m;              // matrix order
   long long Cs, Is;   // slot strides (complex elements, ints)
@@ -1181,17 +1277,20 @@ static SrSlots sr_slots(const Layout& y, int n, bool sectors) {
 // GEMMs instead of 2 and 16.  The left lead's slots hold the shared state and the
 // left lead, the right lead's (from R) its Toldt, CapT and work matrices.  With a
 // mirror of the contacts (negf_ctx::sym) every matrix is taken as its two sector
-// blocks; the stop test reads max|change| of the site-basis matrix and Sigma is
-// formed in the site basis from its sector blocks.
+// blocks (or its + block alone, the - block being its remap, negf_ctx::sPair); the
+// stop test reads max|change| of the site-basis matrix and Sigma is formed in the
+// site basis from its sector blocks.
 static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long long sSig,
                          std::vector<int>& iters, std::vector<int>& status) {
-  const int ns = c->sym.axis ? 2 : 1;  // slots per energy and lead
-  const SrSlots v = sr_slots(c->lay, c->n, ns == 2);
+  const bool sec = c->sym.axis != 0;                       // on the mirror's sectors
+  const int2* pair = sec && c->sym_paired ? c->sPair : nullptr;  // - sector from the + one
+  const int ns = sec && !pair ? 2 : 1;                       // decimated slots per energy and lead
+  const SrSlots v = sr_slots(c->lay, c->n, sec);
   const int m = v.m;
   const long long Cs = v.Cs, mm = (long long)m * m;
   const int nh = ns * nb;                  // slots per lead
   const long long R = (long long)nh * Cs;  // right-lead slot offset
-  const long long off1 = (long long)nb * Cs;  // second sector's slots
+  const long long off1 = (long long)nb * Cs;  // the - sector's slots (ns = 2)
   hipStream_t s = ln.st;
   zd* C0 = ln.C;
   auto P = [&](long long off) { return C0 + off; };
@@ -1205,9 +1304,9 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   // alpha = (E + i eta) I - H_end and the seed (+ i eta I with sr_extra_eta) of the
   // left lead's slots (Contact_Self_Energy.F90:68-101, :141); the right lead's alpha
   // is the same.  In the sectors the end slice's sector blocks take H_end's place.
-  if (ns == 2)
-    TRY(launch_sr_alpha<double>(s, P(v.alpha), P(v.work), Cs, c->sHend, c->sHend + mm, ln.d_energy, nb,
-                                c->cfg.eta, c->cfg.sr_extra_eta, m));
+  if (sec)
+    TRY(launch_sr_alpha<double>(s, P(v.alpha), P(v.work), Cs, c->sHend, c->sHend + (ns == 2 ? mm : 0),
+                                ln.d_energy, nb, c->cfg.eta, c->cfg.sr_extra_eta, m));
   else
     TRY(launch_sr_alpha<double>(s, P(v.alpha), P(v.work), Cs, slice_end(c, 0), slice_end(c, 1), ln.d_energy,
                                 nb, c->cfg.eta, c->cfg.sr_extra_eta, m));
@@ -1217,7 +1316,7 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   auto cpl = [&](bool right, bool cp_left, int opb, long long base, long long offX, long long offC, zd* Cx,
                  zd a) -> int {
     for (int sg = 0; sg < ns; ++sg) {
-      const zd* cp = ns == 2 ? (right ? c->sBetaR : c->sBetaL) + sg * mm : (right ? c->betaR : c->betaL);
+      const zd* cp = sec ? (right ? c->sBetaR : c->sBetaL) + sg * mm : (right ? c->betaR : c->betaL);
       const long long b0 = base + sg * off1;
       zd* Cp = Cx ? Cx : P(b0 + offC);
       const long long sC = Cx ? sSig : Cs;
@@ -1291,7 +1390,7 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
       TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(v.W1), Cs, m, P(v.W3), Cs, m, P(v.W2), Cs, m, m, AL), sL,
                                kOne, kZero));
       TRY(launch_add<double>(s, P(v.CapT), Cs, P(v.CapT), Cs, P(v.W2), Cs, mm, sL, AL));
-      if (ns == 2) TRY(launch_sym_maxabs(s, P(v.W2), Cs, off1, mm, ln.d_mc, nL, EL));
+      if (sec) TRY(launch_sym_maxabs(s, P(v.W2), Cs, off1, pair, m, ln.d_mc, nL, EL));
       else TRY(launch_maxabs<double>(s, P(v.W2), Cs, mm, ln.d_mc, nL, AL));
       TRY(launch_copy3<double>(s, P(v.Toldt), P(v.W1), nullptr, nullptr, nullptr, nullptr, Cs, mm, sL, AL));
     }
@@ -1299,7 +1398,7 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
       TRY(gemm_batched<double>(s, OP_N, OP_N, gbs(c, P(R + v.W1), Cs, m, P(v.W4), Cs, m, P(R + v.W2), Cs, m, m,
                                                  AR), sR, kOne, kZero));
       TRY(launch_add<double>(s, P(R + v.CapT), Cs, P(R + v.CapT), Cs, P(R + v.W2), Cs, mm, sR, AR));
-      if (ns == 2) TRY(launch_sym_maxabs(s, P(R + v.W2), Cs, off1, mm, ln.d_mc + nb, nR, ER));
+      if (sec) TRY(launch_sym_maxabs(s, P(R + v.W2), Cs, off1, pair, m, ln.d_mc + nb, nR, ER));
       else TRY(launch_maxabs<double>(s, P(R + v.W2), Cs, mm, ln.d_mc + nb, nR, AR));
       TRY(launch_copy3<double>(s, P(R + v.Toldt), P(R + v.W1), nullptr, nullptr, nullptr, nullptr, Cs, mm, sR,
                                AR));
@@ -1334,11 +1433,11 @@ static int run_sr_paired(negf_ctx* c, Lane& ln, int nb, zd* sig0, zd* sig1, long
   for (int grp = 0; grp < 2; ++grp) {
     zd* sig = grp ? sig1 : sig0;
     if (int rc = cpl(grp == 1, true, OP_N, grp * R, v.W2, v.W3, nullptr, kOne)) return rc;
-    if (ns == 1) {
+    if (!sec) {
       if (int rc = cpl(grp == 1, false, OP_C, grp * R, v.W3, 0, sig, kOne)) return rc;
     } else {
       if (int rc = cpl(grp == 1, false, OP_C, grp * R, v.W3, v.W4, nullptr, kOne)) return rc;
-      TRY(launch_sym_expand(s, P(grp * R + v.W4), Cs, off1, m, sig, sSig, c->n, nb, c->sym));
+      TRY(launch_sym_expand(s, P(grp * R + v.W4), Cs, off1, pair, m, sig, sSig, c->n, nb, c->sym));
     }
   }
   // LAPACK INFO of the slots' inverses (in the sectors: the first zero pivot of the

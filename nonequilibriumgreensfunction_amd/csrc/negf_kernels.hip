@@ -447,8 +447,18 @@ __global__ void k_add(cx<T>* Y, long long sY, const cx<T>* X, long long sX, cons
 // S = (M+ + M-)/2 and D = (M+ - M-)/2 at (a, c): Y(r_a, r_c) = S, Y(r_a, m_c) =
 // i^-ph_c D, Y(m_a, r_c) = i^ph_a D, Y(m_a, m_c) = i^(ph_a - ph_c) S.  The halving
 // and the unit phases are exact, so every entry is one rounded sum or difference.
-__global__ __launch_bounds__(256) void k_sym_expand(const double2* M, long long sM, long long off1, int d,
-                                                    double2* Y, long long sY, int n, SymMap map) {
+// With `pair` (sectors related by a second mirror) M- is not stored: M-(a, c) =
+// i^(k_a - k_c) M+(s_a, s_c) with (s_a, k_a) = pair[a].
+__device__ inline double2 sym_minus(const double2* M0, const double2* M1, const int2* pair, long long i, int a,
+                                    int cc, int d) {
+  if (!pair) return M1[i];
+  const int2 pa = pair[a], pc = pair[cc];
+  return rot_i(M0[pa.x + (long long)pc.x * d], pa.y + 4 - pc.y);
+}
+
+__global__ __launch_bounds__(256) void k_sym_expand(const double2* M, long long sM, long long off1,
+                                                    const int2* pair, int d, double2* Y, long long sY, int n,
+                                                    SymMap map) {
   const int b = blockIdx.y;
   const double2* M0 = M + (long long)b * sM;
   const double2* M1 = M0 + off1;
@@ -456,7 +466,7 @@ __global__ __launch_bounds__(256) void k_sym_expand(const double2* M, long long 
   const long long dd = (long long)d * d;
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < dd; i += (long long)gridDim.x * 256) {
     const int a = (int)(i % d), cc = (int)(i / d);
-    const double2 p = M0[i], q = M1[i];
+    const double2 p = M0[i], q = sym_minus(M0, M1, pair, i, a, cc, d);
     double2 S, D;
     S.x = 0.5 * (p.x + q.x); S.y = 0.5 * (p.y + q.y);
     D.x = 0.5 * (p.x - q.x); D.y = 0.5 * (p.y - q.y);
@@ -474,14 +484,15 @@ __global__ __launch_bounds__(256) void k_sym_expand(const double2* M, long long 
 // of matrix b (through `active`): the phases have unit modulus, so it is the larger
 // of max|S| and max|D|; NaN as k_maxabs.  out[b] cleared by k_maxabs_clear first.
 __global__ __launch_bounds__(256) void k_sym_maxabs(const double2* M, long long sM, long long off1,
-                                                    long long dd, double* out, const int* active) {
+                                                    const int2* pair, int d, double* out, const int* active) {
   const int b = active ? active[blockIdx.y] : (int)blockIdx.y;
   const double2* M0 = M + (long long)b * sM;
   const double2* M1 = M0 + off1;
+  const long long dd = (long long)d * d;
   double m = 0.0;
   bool nan = false;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < dd; i += (long long)gridDim.x * 256) {
-    const double2 p = M0[i], q = M1[i];
+    const double2 p = M0[i], q = sym_minus(M0, M1, pair, i, (int)(i % d), (int)(i / d), d);
     const double vs = hypot(0.5 * (p.x + q.x), 0.5 * (p.y + q.y));
     const double vd = hypot(0.5 * (p.x - q.x), 0.5 * (p.y - q.y));
     if (vs != vs || vd != vd) nan = true;
@@ -1285,21 +1296,23 @@ hipError_t launch_maxabs(hipStream_t s, const cx<T>* X, long long sX, long long 
   return hipGetLastError();
 }
 
-hipError_t launch_sym_expand(hipStream_t s, const double2* M, long long sM, long long off1, int d, double2* Y,
-                             long long sY, int n, int batch, const SymMap& map) {
+hipError_t launch_sym_expand(hipStream_t s, const double2* M, long long sM, long long off1, const int2* pair,
+                             int d, double2* Y, long long sY, int n, int batch, const SymMap& map) {
   if (batch <= 0) return hipSuccess;
   if (map.axis < 1 || map.axis > 2 || map.norb < 1 || map.norb > kSymMaxOrb || 2 * d != n) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_sym_expand, grid_nn((long long)d * d, batch), dim3(256), 0, s, M, sM, off1, d, Y, sY, n,
-                     map);
+  hipLaunchKernelGGL(k_sym_expand, grid_nn((long long)d * d, batch), dim3(256), 0, s, M, sM, off1, pair, d, Y, sY,
+                     n, map);
   return hipGetLastError();
 }
 
-hipError_t launch_sym_maxabs(hipStream_t s, const double2* M, long long sM, long long off1, long long dd,
-                             double* out, int batch, const int* active) {
+hipError_t launch_sym_maxabs(hipStream_t s, const double2* M, long long sM, long long off1, const int2* pair,
+                             int d, double* out, int batch, const int* active) {
   if (batch <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_maxabs_clear, dim3((batch + 63) / 64), dim3(64), 0, s, out, active, batch);
+  const long long dd = (long long)d * d;
   const long long chunks = std::min<long long>(512, std::max<long long>(1, (dd + 4095) / 4096));
-  hipLaunchKernelGGL(k_sym_maxabs, dim3((unsigned)chunks, batch), dim3(256), 0, s, M, sM, off1, dd, out, active);
+  hipLaunchKernelGGL(k_sym_maxabs, dim3((unsigned)chunks, batch), dim3(256), 0, s, M, sM, off1, pair, d, out,
+                     active);
   return hipGetLastError();
 }
 

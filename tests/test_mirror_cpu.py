@@ -52,8 +52,10 @@ def test_found_mirror_commutes_with_the_lead(case):
     dims, kw = WIRES[case]
     w = build_wire(*dims, **kw)
     H, X = w.htotd[:, :, 0], w.hopx
-    axis, perm, phase = capi.find_mirror(w.npy, w.npz, w.norb, H, X)
+    axis, perm, phase, paired = capi.find_mirror(w.npy, w.npz, w.norb, H, X)
     assert axis in (1, 2)
+    # the 4-band model keeps both mirrors where both extents are even, and they anticommute
+    assert paired == (w.norb == 4 and w.npy % 2 == 0 and w.npz % 2 == 0)
     P = dense_mirror(w.npy, w.npz, w.norb, axis, perm, phase)
     I = np.eye(w.n)
     assert np.abs(P @ P - I).max() == 0.0
@@ -71,8 +73,8 @@ def test_symmetry_breaking_drops_the_mirror():
     # a potential that grows along z breaks the z mirror; the y mirror remains
     site = np.arange(w.n) // 4
     Hz = H + np.diag(0.05 * (site // 4))
-    axis, perm, phase = capi.find_mirror(4, 4, 4, Hz, X)
-    assert axis == 1
+    axis, perm, phase, paired = capi.find_mirror(4, 4, 4, Hz, X)
+    assert axis == 1 and not paired
     P = dense_mirror(4, 4, 4, axis, perm, phase)
     assert np.abs(P @ Hz - Hz @ P).max() <= 1e-14
     # ... and one along y as well: no mirror
@@ -97,7 +99,7 @@ def test_sector_blocks_and_expansion_against_eigenbases():
     for dims in ((3, 4, 4), (3, 4, 3)):
         w = build_wire(*dims)
         npy, npz, no = w.npy, w.npz, w.norb
-        axis, perm, phase = capi.find_mirror(npy, npz, no, w.htotd[:, :, 0], w.hopx)
+        axis, perm, phase, _ = capi.find_mirror(npy, npz, no, w.htotd[:, :, 0], w.hopx)
         P = dense_mirror(npy, npz, no, axis, perm, phase)
         idx = rep_index(npy, npz, no, axis, perm, phase)
         n, d = w.n, w.n // 2
@@ -133,6 +135,46 @@ def test_sector_blocks_and_expansion_against_eigenbases():
         assert np.abs(Y - full).max() < 1e-12
         # the largest entry of the site-basis matrix is max(max|S|, max|D|)
         assert np.isclose(np.abs(full).max(), max(np.abs(S).max(), np.abs(D).max()), rtol=1e-14)
+
+
+def test_anticommuting_mirrors_make_the_sectors_equivalent():
+    """The 4-band lead keeps the z mirror P and the y mirror Q with QP = -PQ: Q maps
+    the +1 eigenspace of P onto the -1 one, W = V-' Q V+ is a permutation with unit
+    phases, and V-' M V- = W (V+' M V+) W' for the end slice and HopX -- so one
+    sector's decimation gives the other's (the claim behind the paired sectors)."""
+    w = build_wire(3, 4, 4)
+    H, X = w.htotd[:, :, 0], w.hopx
+    axis, perm, phase, paired = capi.find_mirror(4, 4, 4, H, X)
+    assert axis == 2 and paired
+    P = dense_mirror(4, 4, 4, axis, perm, phase)
+    # the y mirror of the same lead: what the search finds once a potential along z
+    # breaks the z mirror
+    site = np.arange(w.n) // 4
+    ay, py, phy, _ = capi.find_mirror(4, 4, 4, H + np.diag(0.05 * (site // 4)), X)
+    assert ay == 1
+    Q = dense_mirror(4, 4, 4, ay, py, phy)
+    for A in (H, X):
+        assert np.abs(Q @ A - A @ Q).max() <= 1e-14 * np.abs(A).max()
+    assert np.abs(Q @ P + P @ Q).max() == 0.0
+    idx = rep_index(4, 4, 4, axis, perm, phase)
+    n, d = w.n, w.n // 2
+    V = {s: np.zeros((n, d), complex) for s in (1, -1)}
+    for a, (r, m, ph) in enumerate(idx):
+        for s in (1, -1):
+            V[s][r, a] = 1 / np.sqrt(2)
+            V[s][m, a] = s * 1j ** ph / np.sqrt(2)
+    W = V[-1].conj().T @ Q @ V[1]
+    nz = np.abs(W) > 1e-12
+    assert np.all(nz.sum(axis=0) == 1) and np.all(nz.sum(axis=1) == 1)
+    assert np.allclose(np.abs(W[nz]), 1.0, atol=1e-15)
+    assert np.allclose(np.round(np.angle(W[nz]) / (np.pi / 2)), np.angle(W[nz]) / (np.pi / 2), atol=1e-12)
+    for A in (H, X):
+        Ap, Am = V[1].conj().T @ A @ V[1], V[-1].conj().T @ A @ V[-1]
+        assert np.abs(Am - W @ Ap @ W.conj().T).max() <= 1e-13 * np.abs(A).max()
+    # planted: a mirror that commutes with P gives no pairing (the metal ladder's two
+    # plain reflections commute)
+    m = build_wire(3, 2, 2, model="metal", params=dict(t=1.0, eps=0.1))
+    assert capi.find_mirror(2, 2, 1, m.htotd[:, :, 0], m.hopx)[3] is False
 
 
 def test_find_mirror_rejects_bad_arguments():
