@@ -1090,10 +1090,12 @@ static GjWork gjw(zd* prow_base, long long sProw, int* ints, long long sInt, int
 // sig0 + c*sSig (c < nl) or sig1 + (c-nl)*sSig.  iters/status per chain.
 // Sancho-Rubio products: the 4-product form in the plain FP64 form, whose
 // componentwise accuracy keeps the O(eta) anti-Hermitian part of Sigma in a gap to
-// 1e-10 relative; the 3M form in the mixed form, where the normwise error it adds
-// to Sigma (~1e-16 |Sigma|, ~1e-10 of an eta-scale DOSc) is far inside that form's
-// bounds.
-static int sr_m3(const negf_ctx* c) { return c->cfg.precision != NEGF_PREC_FP64 ? 1 : 0; }
+// 1e-10 relative; the 3M form in the reduced forms, where the normwise error it adds
+// to Sigma (~1e-16 |Sigma|, ~1e-10 of an eta-scale DOSc) is inside their bounds --
+// except on the mirror sectors (an eighth or a quarter of the work), which keep the
+// 4-product form in every form: near a subband edge the decimation amplifies the 3M
+// form's error to ~1e-9 of T12.
+static int sr_m3(const negf_ctx* c) { return c->cfg.precision != NEGF_PREC_FP64 && !c->sym.axis ? 1 : 0; }
 static GemmBatch gbs(const negf_ctx* c, const zd* A, long long sA, int lda, const zd* B, long long sB,
                      int ldb, zd* C, long long sC, int ldc, int n, const int* active = nullptr) {
   GemmBatch g = gb(A, sA, lda, B, sB, ldb, C, sC, ldc, n, active);
