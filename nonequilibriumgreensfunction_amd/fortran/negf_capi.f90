@@ -12,6 +12,9 @@ module negf_capi
   integer(c_int), parameter :: NEGF_STATUS_SR_NOT_CONVERGED = 1048576, NEGF_STATUS_SR_NAN = 2097152
   integer(c_int), parameter :: NEGF_PREC_FP64 = 0, NEGF_PREC_MIXED = 1, NEGF_PREC_REFINED = 2, &
                                 NEGF_PREC_LU32 = 3
+  integer(c_int), parameter :: NEGF_FLAG_RGF_THREE_PASS = 1, NEGF_FLAG_SR_SEPARATE_LEADS = 2, &
+      NEGF_FLAG_SINGLE_LANE = 4, NEGF_FLAG_CACHE_CONTACTS = 8, NEGF_FLAG_GRAPH = 16, &
+      NEGF_FLAG_LU32_FP32_MFMA = 32, NEGF_FLAG_SR_NO_MIRROR = 64
 
   type, bind(C) :: negf_config
     integer(c_int) :: npx, npy, npz, norb
@@ -153,6 +156,24 @@ module negf_capi
       complex(c_double_complex), intent(in) :: G(*)
       complex(c_double_complex), intent(out) :: invG(*)
       integer(c_int), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+
+    ! The lead mirror the paired decimation uses (axis 0: none, 1: y, 2: z); paired = 1
+    ! when one sector's decimation serves both.  No counterpart in the reference.
+    function negf_contact_mirror(ctx, axis, perm, phase, paired) result(rc) bind(C, name="negf_contact_mirror")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: axis, perm(*), phase(*), paired
+      integer(c_int) :: rc
+    end function
+
+    function negf_find_mirror(npy, npz, norb, Hend, HopX, axis, perm, phase, paired) result(rc) &
+        bind(C, name="negf_find_mirror")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: npy, npz, norb
+      complex(c_double_complex), intent(in) :: Hend(*), HopX(*)
+      integer(c_int), intent(out) :: axis, perm(*), phase(*), paired
       integer(c_int) :: rc
     end function
 
