@@ -207,12 +207,18 @@ def inverse_only(args):
     rng = np.random.default_rng(7)
     G = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))) / np.sqrt(n)
     G += 4.0 * np.eye(n)
+    G64 = G.astype(np.complex64)
     for _ in range(2):
         t0 = time.perf_counter()
-        _, info = capi.invert_batch(G.astype(np.complex64), complex64=True, m3=True if args.fp32_mfma else 2,
-                                    local=True)
+        capi.profile_begin(1)
+        _, info = capi.invert_batch(G64, complex64=True, m3=True if args.fp32_mfma else 2, local=True,
+                                    group=args.gj_group)
+        prof = capi.profile_end()
+    upd = {k: {"tflops": v[0] / v[1] / 1e9 if v[1] > 0 else None, "ms": v[1], "launches": v[2]}
+           for k, v in prof.items() if v[2] and k in ("fp32_gj_update_3m", "fp32x3_gj_update")}
     print(json.dumps({"kernel": "k_gemm<float,0,1,1,2,2,0,1> inside negf_invert_batch_local", "n": n, "batch": batch,
-                      "outer_blocks": (n + 127) // 128, "seconds_incl_host_copies": time.perf_counter() - t0,
+                      "outer_blocks": (n + 127) // 128, "group": args.gj_group,
+                      "seconds_incl_host_copies": time.perf_counter() - t0, "trailing_updates": upd,
                       "info": info.tolist()}))
 
 
@@ -474,6 +480,8 @@ def main():
     ap.add_argument("--inverse-only", action="store_true",
                     help="run only the complex64 Gauss-Jordan inverse at the block size and batch of one "
                          "lane, for rocprofv3 --pmc counter runs of its trailing updates")
+    ap.add_argument("--gj-group", type=int, default=1,
+                    help="--inverse-only: diagonal blocks per delayed trailing update")
     ap.add_argument("--fp32-mfma", action="store_true",
                     help="lu32 form: complex64 products on the FP32 matrix cores (NEGF_FLAG_LU32_FP32_MFMA) "
                          "instead of the bf16 split form; with --inverse-only, the same for the inverse")

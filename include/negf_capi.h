@@ -336,8 +336,12 @@ int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, vo
  *                      diagonal block met during the elimination is well conditioned,
  *                      and is meant for callers that check their results.  info[b] =
  *                      the first zero pivot inside a diagonal block (1-based) or 0;
- *                      the inverse is then undefined. */
-int negf_invert_batch_local(int complex64, int m3, int n, int batch, const void* G, void* invG, int* info);
+ *                      the inverse is then undefined.  group (1..8; complex64 only,
+ *                      else taken as 1): diagonal blocks eliminated against their own
+ *                      columns before one rank-(128 group) update of the others --
+ *                      the same arithmetic, longer products. */
+int negf_invert_batch_local(int complex64, int m3, int group, int n, int batch, const void* G, void* invG,
+                            int* info);
 
 /* Allocation-free form of negf_invert_batch.  No counterpart in the reference; for
  * callers that cannot allocate device memory -- code under HIP graph capture, tight

@@ -121,7 +121,7 @@ def lib():
         L.negf_profile_begin.argtypes = [ctypes.c_int]
         L.negf_gemm_batch.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3
         L.negf_invert_batch.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [_ip]
-        L.negf_invert_batch_local.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [_ip]
+        L.negf_invert_batch_local.argtypes = [ctypes.c_int] * 5 + [ctypes.c_void_p] * 2 + [_ip]
         L.negf_server_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(NegfServerConfig)]
         L.negf_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(NegfRequest),
                                          ctypes.POINTER(ctypes.c_longlong)]
@@ -252,19 +252,21 @@ def gemm_batch(A, B, complex64=False, m3=True):
     return np.swapaxes(c, 1, 2)
 
 
-def invert_batch(G, complex64=False, m3=True, local=False):
+def invert_batch(G, complex64=False, m3=True, local=False, group=1):
     """(inverses, info) of a stack G of shape (batch, n, n) with the sweep's
     Gauss-Jordan inverse in the given form (negf_invert_batch), or with local=True
     the block Gauss-Jordan with pivoting inside diagonal blocks
-    (negf_invert_batch_local)."""
+    (negf_invert_batch_local; group diagonal blocks per delayed trailing update)."""
     dt = np.complex64 if complex64 else np.complex128
     batch, n, _ = G.shape
     g = np.ascontiguousarray(np.swapaxes(G, 1, 2), dtype=dt)
     out = np.zeros_like(g)
     info = np.zeros(batch, np.int32)
-    fn = lib().negf_invert_batch_local if local else lib().negf_invert_batch
-    _check(fn(int(bool(complex64)), _form(m3), n, batch, g.ctypes.data_as(ctypes.c_void_p),
-              out.ctypes.data_as(ctypes.c_void_p), info.ctypes.data_as(_ip)))
+    args = (g.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), info.ctypes.data_as(_ip))
+    if local:
+        _check(lib().negf_invert_batch_local(int(bool(complex64)), _form(m3), int(group), n, batch, *args))
+    else:
+        _check(lib().negf_invert_batch(int(bool(complex64)), _form(m3), n, batch, *args))
     return np.swapaxes(out, 1, 2), info
 
 

@@ -494,6 +494,8 @@ hipError_t gj_invert(hipStream_t s, cx<T>* S, long long sS, int ld, cx<T>* out, 
 // for callers that check their results (the complex64 block LU whose solves are
 // refined in FP64, which falls back to the partially pivoted FP64 inverse when
 // the refinement does not contract).  4 launches per 128 columns instead of 50.
+// With GjWork::group > 1 (complex64) the updates are delayed over groups of blocks
+// (see gj_invert_local).
 
 // Pivot keys of the diagonal-block inverse: the magnitude |re|+|im| with its low
 // bits replaced by the row's rank (127 - i), so the larger magnitude wins and ties
@@ -781,6 +783,28 @@ __global__ void k_gjl_cols(cx<T>* S, long long sS, int ld, int n, int k0, int kb
   if (inK) Rt[(long long)b * sR + (long long)m * n + i] = Dinv[(long long)b * sD + (long long)(i - k0) * kb + m];
 }
 
+// Bt[j + m n] = S[g0 + m, j] for m < w, j < n: a band of rows of S, transposed (32 x 32
+// tiles through LDS, so both the read and the write are coalesced)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gjl_rows_t(const cx<T>* S, long long sS, int ld, int n, int g0, int w,
+                                                    cx<T>* Bt, long long sB, const int* active) {
+  __shared__ cx<T> tile[32][33];
+  const int b = active ? active[blockIdx.z] : (int)blockIdx.z;
+  const int j0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const cx<T>* Sb = S + (long long)b * sS;
+  cx<T>* Bb = Bt + (long long)b * sB;
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + r, m = m0 + tx;
+    if (j < n && m < w) tile[r][tx] = Sb[(long long)(g0 + m) + (long long)j * ld];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int m = m0 + r, j = j0 + tx;
+    if (j < n && m < w) Bb[(long long)j + (long long)m * n] = tile[tx][r];
+  }
+}
+
 // pivstep = identity: the epilogue's row mask for "rows of K" (info is left to the
 // caller, which clears it before a sequence of inversions and reads the first failure)
 __global__ void k_gjl_init(int* pivstep, long long sInt, int n, const int* active) {
@@ -794,41 +818,72 @@ hipError_t gj_invert_local(hipStream_t s, cx<T>* S, long long sS, int ld, int n,
                            const int* active) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   if (!w.tmp) return hipErrorInvalidValue;
-  // workspaces: A' (n x kb) at w.tmp; R^T (n x kb) then Dinv (kb x kb) in the pivot-row
-  // panel (kGjNBO x n complex128 holds 2 kGjNBO x n of complex64, and both fit)
+  // workspaces: A' (n x kb) at w.tmp, then (grouped) Bt (n x group kb); R^T (n x kb) then
+  // Dinv (kb x kb) in the pivot-row panel (kGjNBO x n complex128 holds 2 kGjNBO x n of
+  // complex64, and both fit)
   constexpr int KB = LocalBlock<T>::kb;
   cx<T>* Ap = (cx<T>*)w.tmp;
+  cx<T>* Bt = Ap + (long long)n * KB;
   cx<T>* Rt = (cx<T>*)w.prow;
   cx<T>* Di = Rt + (long long)n * (n < KB ? n : KB);
+  // Delayed updates (complex64, group > 1): the blocks of a group of columns [g0, g0 + wg)
+  // update only the group's own columns; the accumulated transform M, which differs from
+  // the identity only in those columns and is what they then hold, is applied to the
+  // other columns at once: S[:, j] = keep(S[:, j]) + S[:, G] S0[G, j], with S0[G, j] the
+  // group's rows as they were before the group (Bt), one rank-wg product.
+  const int G = (sizeof(T) == sizeof(float) && w.group > 1) ? w.group : 1;
   hipLaunchKernelGGL(k_gjl_init, dim3((n + 255) / 256, batch), dim3(256), 0, s, w.pivstep, w.sInt, n, active);
-  for (int k0 = 0; k0 < n; k0 += KB) {
-    const int kb = (n - k0 < KB) ? n - k0 : KB;
-    hipLaunchKernelGGL((k_gjl_diag<T, KB>), dim3(batch), dim3(KB * 8), 0, s, S, sS, ld, k0, kb, Di, w.sProw,
-                       w.info, w.sInt, active);
-    GemmBatch g = {};  // R^T = S[K, :]^T Dinv^T
-    g.A = S + k0; g.sA = sS; g.lda = ld;
-    g.B = Di; g.sB = w.sProw; g.ldb = kb;
-    g.C = Rt; g.sC = w.sProw; g.ldc = n;
-    g.m = n; g.n = kb; g.k = kb;
-    g.active = active;
-    g.m3 = w.m3 ? 1 : 0;
-    g.x3 = w.m3 == 2;
-    hipError_t e = gemm_batched<T>(s, OP_T, OP_T, g, batch, mk<T>(1, 0), mk<T>(0, 0));
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_gjl_cols<T>), dim3((n + 255) / 256, kb, batch), dim3(256), 0, s, S, sS, ld, n, k0, kb,
-                       Ap, w.sTmp, Rt, w.sProw, Di, w.sProw, active);
-    GemmBatch u = {};  // S = keep(S) + A' R
-    u.A = Ap; u.sA = w.sTmp; u.lda = n;
-    u.B = Rt; u.sB = w.sProw; u.ldb = n;
-    u.C = S; u.sC = sS; u.ldc = ld;
-    u.m = n; u.n = n; u.k = kb;
-    u.active = active; u.pivstep = w.pivstep; u.sPiv = w.sInt;
-    u.pv_lo = k0; u.pv_hi = k0 + kb;
-    u.skip_lo = u.skip_hi = 0;
-    u.m3 = w.m3 ? 1 : 0;
-    u.x3 = w.m3 == 2;
-    e = gemm_gj_update<T>(s, u, batch);
-    if (e != hipSuccess) return e;
+  for (int g0 = 0; g0 < n; g0 += G * KB) {
+    const int wg = (n - g0 < G * KB) ? n - g0 : G * KB;
+    const bool grouped = G > 1 && wg > KB;
+    if (grouped && wg < n)
+      hipLaunchKernelGGL((k_gjl_rows_t<T>), dim3((n + 31) / 32, (wg + 31) / 32, batch), dim3(256), 0, s, S, sS, ld,
+                         n, g0, wg, Bt, w.sTmp, active);
+    for (int k0 = g0; k0 < g0 + wg; k0 += KB) {
+      const int kb = (g0 + wg - k0 < KB) ? g0 + wg - k0 : KB;
+      hipLaunchKernelGGL((k_gjl_diag<T, KB>), dim3(batch), dim3(KB * 8), 0, s, S, sS, ld, k0, kb, Di, w.sProw,
+                         w.info, w.sInt, active);
+      // R^T = S[K, c]^T Dinv^T over the columns c the update below reaches
+      const int c0 = grouped ? g0 : 0, nc = grouped ? wg : n;
+      GemmBatch g = {};
+      g.A = S + k0 + (long long)c0 * ld; g.sA = sS; g.lda = ld;
+      g.B = Di; g.sB = w.sProw; g.ldb = kb;
+      g.C = Rt + c0; g.sC = w.sProw; g.ldc = n;
+      g.m = nc; g.n = kb; g.k = kb;
+      g.active = active;
+      g.m3 = w.m3 ? 1 : 0;
+      g.x3 = w.m3 == 2;
+      hipError_t e = gemm_batched<T>(s, OP_T, OP_T, g, batch, mk<T>(1, 0), mk<T>(0, 0));
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_gjl_cols<T>), dim3((n + 255) / 256, kb, batch), dim3(256), 0, s, S, sS, ld, n, k0, kb,
+                         Ap, w.sTmp, Rt, w.sProw, Di, w.sProw, active);
+      GemmBatch u = {};  // S[:, c] = keep(S[:, c]) + A' R[:, c]
+      u.A = Ap; u.sA = w.sTmp; u.lda = n;
+      u.B = Rt + c0; u.sB = w.sProw; u.ldb = n;
+      u.C = S + (long long)c0 * ld; u.sC = sS; u.ldc = ld;
+      u.m = n; u.n = nc; u.k = kb;
+      u.active = active; u.pivstep = w.pivstep; u.sPiv = w.sInt;
+      u.pv_lo = k0; u.pv_hi = k0 + kb;
+      u.skip_lo = u.skip_hi = 0;
+      u.m3 = w.m3 ? 1 : 0;
+      u.x3 = w.m3 == 2;
+      e = gemm_gj_update<T>(s, u, batch);
+      if (e != hipSuccess) return e;
+    }
+    if (grouped && wg < n) {
+      GemmBatch u = {};  // S[:, j] = keep(S[:, j]) + S[:, G] Bt^T[G, j] outside the group
+      u.A = S + (long long)g0 * ld; u.sA = sS; u.lda = ld;
+      u.B = Bt; u.sB = w.sTmp; u.ldb = n;
+      u.C = S; u.sC = sS; u.ldc = ld;
+      u.m = n; u.n = n; u.k = wg;
+      u.active = active; u.pivstep = w.pivstep; u.sPiv = w.sInt;
+      u.pv_lo = g0; u.pv_hi = g0 + wg;
+      u.skip_lo = g0; u.skip_hi = g0 + wg;
+      u.m3 = w.m3 ? 1 : 0;
+      u.x3 = w.m3 == 2;
+      hipError_t e = gemm_gj_update<T>(s, u, batch);
+      if (e != hipSuccess) return e;
+    }
   }
   return hipGetLastError();
 }

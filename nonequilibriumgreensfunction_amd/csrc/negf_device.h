@@ -164,8 +164,9 @@ struct GjWork {
   long long sInt;   // batch stride of pivstep, pivrow and info (ints)
   int m3 = 0;       // complex product form of the trailing updates: 0 4-product, 1 3M,
                     // 2 (complex64) the bf16 three-way split products (GemmBatch::x3)
-  void* tmp = nullptr;  // gj_invert_local: n x 128 (complex64) or n x 64 (complex128) per matrix
-  long long sTmp = 0;
+  void* tmp = nullptr;  // gj_invert_local: n x 128 (complex64) or n x 64 (complex128) per matrix,
+  long long sTmp = 0;   //   n x 128 (1 + group) complex64 with group > 1
+  int group = 1;        // gj_invert_local, complex64: diagonal blocks per delayed trailing update
 };
 
 template <typename T>

@@ -1863,11 +1863,15 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
       w32.prow = Q(y.o_prow);
       w32.sProw = Es32;
-      // the trailing updates (rank 128) stay on the FP32 matrix cores: at this short K the
-      // split form is no faster alone and slower beside the other lanes' kernels
-      w32.m3 = 1;
-      w32.tmp = F(y.f_P);
+      // four diagonal blocks per delayed trailing update (rank 512, after three rank-128
+      // updates of the group's own 512 columns), on the bf16 matrix cores with split
+      // operands: at block 2304 under four lanes 2.02-2.05 energy-points/s against 1.85
+      // for rank-128 updates on the FP32 cores (groups of 2 and 8: 1.97, 1.96; groups of
+      // 4 on the FP32 cores: 1.95)
+      w32.m3 = 2;
+      w32.tmp = F(y.f_P);  // n x n complex64: A' and the group's rows (n x 128 (1 + group))
       w32.sTmp = Es32;
+      w32.group = std::max(1, std::min(4, n / LocalBlock<float>::kb - 1));
       TRY(launch_zero(s, ln.Ic, sizeof(int) * (size_t)y.Is * nb));  // the first zero pivot stays
       int r;
       // W_L whole in complex64 (its leading columns start the first-row chain, its
@@ -2885,12 +2889,14 @@ extern "C" int negf_gemm_batch(int complex64, int m3, int n, int batch, const vo
 // copies of the host matrices: the inverse the complex64 block-LU form's RGF and its
 // decimation take.
 template <typename T>
-static int invert_local_host(int m3, int n, int batch, const void* G, void* invG, int* info) {
+static int invert_local_host(int m3, int group, int n, int batch, const void* G, void* invG, int* info) {
   typedef cx<T> V;
   constexpr int KB = LocalBlock<T>::kb;
   const long long nn = (long long)n * n;
   const size_t bytes = sizeof(V) * (size_t)nn * batch;
   const long long kb = n < KB ? n : KB;
+  if (sizeof(T) != sizeof(float)) group = 1;
+  const long long tcols = kb * (group > 1 ? 1 + group : 1);  // A', then the group's rows
   const long long sWork = (long long)n * kb + kb * kb;  // R^T (n x kb), then Dinv (kb x kb)
   const long long sInt = gj_ints(n);
   V *dA = nullptr, *work = nullptr, *tmp = nullptr;
@@ -2898,7 +2904,7 @@ static int invert_local_host(int m3, int n, int batch, const void* G, void* invG
   hipError_t e = hipSuccess;
   auto step = [&](hipError_t r) { if (e == hipSuccess) e = r; return e == hipSuccess; };
   if (step(hipMalloc(&dA, bytes)) && step(hipMalloc(&work, sizeof(V) * (size_t)sWork * batch)) &&
-      step(hipMalloc(&tmp, sizeof(V) * (size_t)n * kb * batch)) &&
+      step(hipMalloc(&tmp, sizeof(V) * (size_t)n * tcols * batch)) &&
       step(hipMalloc(&ints, sizeof(int) * (size_t)sInt * batch)) &&
       step(hipMemcpy(dA, G, bytes, hipMemcpyHostToDevice)) &&
       step(hipMemset(ints, 0, sizeof(int) * (size_t)sInt * batch))) {
@@ -2911,7 +2917,8 @@ static int invert_local_host(int m3, int n, int batch, const void* G, void* invG
     w.sInt = sInt;
     w.m3 = m3 == 2 ? 2 : m3 ? 1 : 0;
     w.tmp = tmp;
-    w.sTmp = (long long)n * kb;
+    w.sTmp = (long long)n * tcols;
+    w.group = group;
     std::vector<int> h((size_t)sInt * batch);
     if (step(gj_invert_local<T>(nullptr, dA, nn, n, n, batch, w, nullptr)) &&
         step(hipMemcpy(invG, dA, bytes, hipMemcpyDeviceToHost)) &&
@@ -2931,13 +2938,17 @@ static int invert_local_host(int m3, int n, int batch, const void* G, void* invG
   return 0;
 }
 
-extern "C" int negf_invert_batch_local(int complex64, int m3, int n, int batch, const void* G, void* invG,
-                                       int* info) {
+extern "C" int negf_invert_batch_local(int complex64, int m3, int group, int n, int batch, const void* G,
+                                       void* invG, int* info) {
   int rc = batch_args_ok(n, batch, G, invG, G);
   if (rc) return rc;
   if ((rc = form_ok(complex64, m3))) return rc;
-  return complex64 ? invert_local_host<float>(m3, n, batch, G, invG, info)
-                   : invert_local_host<double>(m3, n, batch, G, invG, info);
+  if (group < 1 || group > 8) {
+    negf_set_error("negf_invert_batch_local: group %d outside 1..8", group);
+    return NEGF_ERR_ARG;
+  }
+  return complex64 ? invert_local_host<float>(m3, group, n, batch, G, invG, info)
+                   : invert_local_host<double>(m3, group, n, batch, G, invG, info);
 }
 
 extern "C" int negf_invert_batch(int complex64, int m3, int n, int batch, const void* G, void* invG,

@@ -167,21 +167,26 @@ def _block_shuffled(rng, n, kb, dt):
 
 
 @pytest.mark.parametrize("n", [1, 7, 64, 100, 128, 129, 300, 1024, 2304])
-@pytest.mark.parametrize("c64,m3", [(True, 1), (True, 2), (False, 1)], ids=["c64", "c64_bf16x3", "c128"])
-def test_invert_local_matches_lapack(rng, n, c64, m3):
+@pytest.mark.parametrize("c64,m3,group", [(True, 1, 1), (True, 2, 1), (False, 1, 1), (True, 1, 2), (True, 1, 4),
+                                          (True, 2, 4)],
+                         ids=["c64", "c64_bf16x3", "c128", "c64_group2", "c64_group4", "c64_bf16x3_group4"])
+def test_invert_local_matches_lapack(rng, n, c64, m3, group):
     """negf_invert_batch_local (block Gauss-Jordan pivoting inside 128-column
     diagonal blocks in complex64, 64-column in complex128; the complex64 block-LU
     form's inverse) against LAPACK: one partial block up to the benchmark's 2304
     (18 and 36 blocks).  complex128: 1e-12 relative like the partially pivoted
     inverse; complex64: within 16x LAPACK's own complex64 inverse (cgetrf/cgetri
     through numpy) on the same matrix, or 16 eps; the trailing updates in the 3M
-    form (m3 = 1) or on the bf16 matrix cores (m3 = 2)."""
+    form (m3 = 1) or on the bf16 matrix cores (m3 = 2); with group > 1 the blocks of
+    each group of 128 group columns update only the group's columns and one
+    rank-(128 group) product updates the rest (groups that are partial or the whole
+    matrix included)."""
     kb = 128 if c64 else 64
     dt = np.complex64 if c64 else np.complex128
     batch = 2
-    print(f"local inverse c64={c64} m3={m3} n={n} batch={batch}", flush=True)
+    print(f"local inverse c64={c64} m3={m3} group={group} n={n} batch={batch}", flush=True)
     G = np.stack([_block_shuffled(rng, n, kb, dt) for _ in range(batch)])
-    X, info = capi.invert_batch(G, complex64=c64, m3=m3, local=True)
+    X, info = capi.invert_batch(G, complex64=c64, m3=m3, local=True, group=group)
     assert np.all(info == 0), info
     for b in range(batch):
         ref = np.linalg.inv(G[b])
@@ -197,8 +202,8 @@ def test_invert_local_matches_lapack(rng, n, c64, m3):
             assert np.abs(G[b] @ X[b] - np.eye(n)).max() < 1e-11
 
 
-@pytest.mark.parametrize("c64", [True, False], ids=["c64", "c128"])
-def test_invert_local_reports_zero_pivot_in_a_diagonal_block(rng, c64):
+@pytest.mark.parametrize("c64,group", [(True, 1), (False, 1), (True, 2)], ids=["c64", "c128", "c64_group2"])
+def test_invert_local_reports_zero_pivot_in_a_diagonal_block(rng, c64, group):
     # [[0, I], [I, 0]] is its own inverse, but its first diagonal block is zero: the
     # block Gauss-Jordan, which pivots only inside diagonal blocks, meets a zero pivot
     # in column 1 (info 1).  The second matrix of the batch is regular: its inverse and
@@ -209,7 +214,7 @@ def test_invert_local_reports_zero_pivot_in_a_diagonal_block(rng, c64):
     swap[:kb, kb:] = np.eye(kb)
     swap[kb:, :kb] = np.eye(kb)
     good = _block_shuffled(rng, n, kb, np.complex64 if c64 else np.complex128)
-    X, info = capi.invert_batch(np.stack([swap, good]), complex64=c64, m3=True, local=True)
+    X, info = capi.invert_batch(np.stack([swap, good]), complex64=c64, m3=True, local=True, group=group)
     assert info.tolist() == [1, 0], info
     ref = np.linalg.inv(good)
     rel = np.abs(X[1] - ref).max() / np.abs(ref).max()
