@@ -125,8 +125,11 @@ def cpu_baseline(mp_ctx, cfg, energies, k_mean, ranks, mode):
 # counters were collected, `traffic_bytes_per_launch` (rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE passes, each scaled by the same counter's reading for a copy
 # kernel of known size).
+# (config, kind) -> counter summary of that kind's launches at the config's shapes; the
+# 3M rank-128 update is measured with one block per update (the sweep groups four)
 TRAFFIC_PROFILES = {("c4", "fp64_gemm_3m"): "traffic_fp64_gemm3m_n2304_b3.md",
-                    ("c4", "fp32_gj_update_3m"): "traffic_c64_gj_update3m_n2304_b3.md"}
+                    ("c4", "fp32_gj_update_3m"): "traffic_c64_gj_update3m_n2304_b3.md",
+                    ("c4", "fp32x3_gj_update"): "traffic_c64x3_gj_update_g4_n2304_b3.md"}
 GEMM_KINDS = ("fp64_gemm", "fp64_gj_update", "fp32_gemm", "fp32_gj_update", "fp64_gemm_3m",
               "fp64_gj_update_3m", "fp32_gemm_3m", "fp32_gj_update_3m", "fp32x3_gemm", "fp32x3_gj_update")
 KERNEL_NAMES = {

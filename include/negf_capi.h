@@ -126,11 +126,11 @@ typedef struct {
  * Batches of more than about 20000 launches (e.g. blocks of 2304 over 200 slices)
  * run without a graph. */
 #define NEGF_FLAG_GRAPH 16
-/* NEGF_PREC_LU32 only: take the complex64 column-chain products on the FP32 matrix
- * cores (3M form) instead of the default bf16 matrix cores with every FP32 operand
- * split three ways (four real products of six split products each, FP32 accuracy;
- * 1.3 times the FP32 kernel's throughput at block 2304).  The block inverses' rank-128
- * updates run on the FP32 matrix cores either way. */
+/* NEGF_PREC_LU32 only: take the complex64 column-chain products and the block
+ * inverses' trailing updates on the FP32 matrix cores (3M form) instead of the default
+ * bf16 matrix cores with every FP32 operand split three ways (four real products of
+ * six split products each, FP32 accuracy; 1.3 times the FP32 kernel's throughput for
+ * the chains at block 2304). */
 #define NEGF_FLAG_LU32_FP32_MFMA 32
 /* Paired decimation only (equal end slices): by default, when the end slice and HopX
  * both keep a mirror of the cross-section (negf_find_mirror), every Sancho-Rubio

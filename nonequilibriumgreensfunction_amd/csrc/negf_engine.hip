@@ -1863,12 +1863,12 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
       GjWork w32 = gjw(nullptr, 0, ln.Ic, y.Is, n);
       w32.prow = Q(y.o_prow);
       w32.sProw = Es32;
-      // four diagonal blocks per delayed trailing update (rank 512, after three rank-128
+      // four diagonal blocks per delayed trailing update (rank 512, after rank-128
       // updates of the group's own 512 columns), on the bf16 matrix cores with split
-      // operands: at block 2304 under four lanes 2.02-2.05 energy-points/s against 1.85
-      // for rank-128 updates on the FP32 cores (groups of 2 and 8: 1.97, 1.96; groups of
-      // 4 on the FP32 cores: 1.95)
-      w32.m3 = 2;
+      // operands unless NEGF_FLAG_LU32_FP32_MFMA: at block 2304 under four lanes 2.02-2.05
+      // energy-points/s against 1.85 for rank-128 updates on the FP32 cores (groups of 2
+      // and 8: 1.97, 1.96; groups of 4 on the FP32 cores: 1.95)
+      w32.m3 = x3 ? 2 : 1;
       w32.tmp = F(y.f_P);  // n x n complex64: A' and the group's rows (n x 128 (1 + group))
       w32.sTmp = Es32;
       w32.group = std::max(1, std::min(4, n / LocalBlock<float>::kb - 1));

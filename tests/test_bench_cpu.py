@@ -38,14 +38,23 @@ def test_traffic_summary_parses_and_exceeds_algorithmic_bytes():
         cfg = b.CONFIGS[config]
         n = cfg["npy"] * cfg["npz"] * 4
         batch = cfg["per_step"] // 4  # energies per lane
-        if "gj_update" in kind:
-            # rank-128 update: both thin operands read once, C read and written once
-            alg = batch * 8 * (2 * n * 128 + 2 * n * n)
+
+        def update_bytes(cols, k):  # the m x k and k x cols operands read once, C read and written
+            return batch * 8 * (n * k + k * cols + 2 * n * cols)
+        if kind == "fp32x3_gj_update":
+            # groups of four 128-column blocks (18 blocks: four groups of 512, one of 256):
+            # per inverse 16 updates (512 x 128) and 2 (256 x 128) of a group's own columns,
+            # 4 (1792 x 512) and 1 (2048 x 256) of the rest; the average launch
+            shapes = [(16, 512, 128), (2, 256, 128), (4, n - 512, 512), (1, n - 256, 256)]
+            alg = sum(c * update_bytes(cols, k) for c, cols, k in shapes) / sum(c for c, _, _ in shapes)
+        elif "gj_update" in kind:
+            alg = update_bytes(n, 128)  # rank-128 update over every column
         else:  # A and B read once, C written once
             alg = 3 * batch * n * n * (8 if kind.startswith("fp32") else 16)
         assert t >= alg, (t, alg)
     assert b.traffic_from_profiles("c4", "fp64_gemm_3m") is not None
     assert b.traffic_from_profiles("c4", "fp32_gj_update_3m") is not None
+    assert b.traffic_from_profiles("c4", "fp32x3_gj_update") is not None
     assert b.traffic_from_profiles("c2", "fp64_gemm_3m") is None
 
 
