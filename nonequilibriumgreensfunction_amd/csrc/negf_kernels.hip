@@ -209,7 +209,27 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
     // the sum order does not depend on the rest of the batch
     const int k_lo = kfirst, k_n = kcols > kfirst ? kcols - kfirst : 0;
     const int kbeg = k_lo + (k_n * kq) / KQ, kend = k_lo + (k_n * (kq + 1)) / KQ;
-    for (int k = kbeg; k < kend; ++k) {
+    // U columns' loads issued together, then added in column order (the same sums as
+    // one column at a time): the loop is latency-bound, not bandwidth-bound
+    constexpr int U = sizeof(T) == sizeof(float) ? 8 : 4;
+    int k = kbeg;
+    for (; k + U <= kend; k += U) {
+      cx<T> bv[U], gv[U][NORB];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        bv[u] = Bb[(long long)(k + u) * n + a];
+#pragma unroll
+        for (int q = 0; q < NORB; ++q) gv[u][q] = Gb[(long long)(k + u) * n + a0 + q];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < NORB; ++q) {
+          accr[q] += (double)bv[u].x * gv[u][q].x + (double)bv[u].y * gv[u][q].y;
+          acci[q] += (double)bv[u].y * gv[u][q].x - (double)bv[u].x * gv[u][q].y;
+        }
+    }
+    for (; k < kend; ++k) {
       const cx<T> bv = Bb[(long long)k * n + a];
 #pragma unroll
       for (int q = 0; q < NORB; ++q) {
@@ -232,39 +252,32 @@ __global__ __launch_bounds__(256) void k_siteblocks(const cx<T>* Bm, long long s
     const int sl = threadIdx.x;
     const int s = (row0 / NORB) + sl;
     if (s * NORB < n) {
-      double dr[NORB][NORB], di[NORB][NORB];
+      // each entry D(r, q) once, in chunk order, into the trace, the per-orbital
+      // diagonal and the three spin traces (summed over r, then q, as written out)
+      double* o = out + (long long)b * sOut + s;
+      auto put = [&](long long off, double v) { o[off] = accum ? o[off] + v : v; };
+      double trace = 0.0, spin[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int r = 0; r < NORB; ++r)
 #pragma unroll
         for (int q = 0; q < NORB; ++q) {
           const int t = sl * NORB + r;
           double xr = 0.0, xi = 0.0;
+#pragma unroll 1
           for (int c = 0; c < KQ; ++c) {
             xr += sr[c][t][q];
             xi += si[c][t][q];
           }
-          dr[r][q] = xr;
-          di[r][q] = xi;
-        }
-      double* o = out + (long long)b * sOut + s;
-      auto put = [&](long long off, double v) { o[off] = accum ? o[off] + v : v; };
-      double trace = 0.0;
+          if (r == q) {
+            trace += xr;
+            put((4 + r) * qstride, xr);  // per orbital (the orbital-resolved density)
+          }
 #pragma unroll
-      for (int r = 0; r < NORB; ++r) {
-        trace += dr[r][r];
-        put((4 + r) * qstride, dr[r][r]);  // per orbital (the orbital-resolved density)
-      }
+          for (int m = 0; m < 3; ++m) spin[m] += xr * sp.re[m][q][r] - xi * sp.im[m][q][r];
+        }
       put(0, trace);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        double acc = 0.0;
-#pragma unroll
-        for (int r = 0; r < NORB; ++r)
-#pragma unroll
-          for (int q = 0; q < NORB; ++q)
-            acc += dr[r][q] * sp.re[m][q][r] - di[r][q] * sp.im[m][q][r];
-        put((m + 1) * qstride, acc);
-      }
+      for (int m = 0; m < 3; ++m) put((m + 1) * qstride, spin[m]);
     }
   }
 }
@@ -310,27 +323,48 @@ __global__ __launch_bounds__(256) void k_bondblocks(const cx<T>* V, long long sV
   if (a < n) {
     const int k_lo = kfirst, k_n = kcols > kfirst ? kcols - kfirst : 0;
     const int kbeg = k_lo + (k_n * kq) / KQ, kend = k_lo + (k_n * (kq + 1)) / KQ;
-    for (int k = kbeg; k < kend; ++k) {
-      const long long col = (long long)k * n;
-      const cx<T> va = Vb[col + a];
+    // one column's terms, in the order of the sums (as k_siteblocks, U columns' loads
+    // are issued together and their terms then added in column order)
+    auto terms = [&](const cx<T>& va, const cx<T>* wx, const cx<T>* wy, const cx<T>* wz) {
 #pragma unroll
       for (int q = 0; q < NORB; ++q) {
         if (hx) {
-          const cx<T> w = Vnb[col + a0 + q];
-          accr[q] += (double)w.x * va.x + (double)w.y * va.y;
-          acci[q] += (double)w.y * va.x - (double)w.x * va.y;
+          accr[q] += (double)wx[q].x * va.x + (double)wx[q].y * va.y;
+          acci[q] += (double)wx[q].y * va.x - (double)wx[q].x * va.y;
         }
         if (hy) {
-          const cx<T> w = Vb[col + a0 + NORB + q];
-          accr[NORB + q] += (double)w.x * va.x + (double)w.y * va.y;
-          acci[NORB + q] += (double)w.y * va.x - (double)w.x * va.y;
+          accr[NORB + q] += (double)wy[q].x * va.x + (double)wy[q].y * va.y;
+          acci[NORB + q] += (double)wy[q].y * va.x - (double)wy[q].x * va.y;
         }
         if (hz) {
-          const cx<T> w = Vb[col + a0 + NN + q];
-          accr[2 * NORB + q] += (double)w.x * va.x + (double)w.y * va.y;
-          acci[2 * NORB + q] += (double)w.y * va.x - (double)w.x * va.y;
+          accr[2 * NORB + q] += (double)wz[q].x * va.x + (double)wz[q].y * va.y;
+          acci[2 * NORB + q] += (double)wz[q].y * va.x - (double)wz[q].x * va.y;
         }
       }
+    };
+    auto load = [&](int k, cx<T>& va, cx<T>* wx, cx<T>* wy, cx<T>* wz) {
+      const long long col = (long long)k * n;
+      va = Vb[col + a];
+#pragma unroll
+      for (int q = 0; q < NORB; ++q) {
+        if (hx) wx[q] = Vnb[col + a0 + q];
+        if (hy) wy[q] = Vb[col + a0 + NORB + q];
+        if (hz) wz[q] = Vb[col + a0 + NN + q];
+      }
+    };
+    constexpr int U = sizeof(T) == sizeof(float) ? 2 : 1;  // (more would halve the occupancy)
+    int k = kbeg;
+    for (; k + U <= kend; k += U) {
+      cx<T> va[U], wx[U][NORB], wy[U][NORB], wz[U][NORB];
+#pragma unroll
+      for (int u = 0; u < U; ++u) load(k + u, va[u], wx[u], wy[u], wz[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) terms(va[u], wx[u], wy[u], wz[u]);
+    }
+    for (; k < kend; ++k) {
+      cx<T> va, wx[NORB], wy[NORB], wz[NORB];
+      load(k, va, wx, wy, wz);
+      terms(va, wx, wy, wz);
     }
   }
   __shared__ double sr[KQ][RW][NQ], si[KQ][RW][NQ];
