@@ -88,6 +88,9 @@ hipError_t launch_gamma_dos(hipStream_t, const cx<T>*, long long, cx<T>*, long l
                             double*, long long);
 hipError_t launch_sym_expand(hipStream_t, const double2*, long long, long long, const int2*, int, double2*,
                              long long, int, int, const SymMap&);
+hipError_t launch_stencil_check(hipStream_t, const double2*, long long, int, int, int, int, int, int*);
+hipError_t launch_stencil_mul(hipStream_t, int, const double2*, long long, const double2*, long long, long long,
+                              double2*, long long, long long, int, int, int, int, int, int);
 hipError_t launch_sym_maxabs(hipStream_t, const double2*, long long, long long, const int2*, int, double*, int,
                              const int*);
 template <typename Td, typename Ts>
@@ -368,6 +371,10 @@ struct negf_ctx {
   // one sector is decimated
   bool sym_paired = false;
   int2* sPair = nullptr;
+  // Every slice's nonzeros lie in the site-block stencil (the site and its y, z
+  // neighbours): the LU32 refinement's products H_x V_x take the stencil kernel
+  bool stencil = false;
+  int* d_flag = nullptr;
   zd* sHend = nullptr;
   zd* sBetaL = nullptr;  // -HopX'
   zd* sBetaR = nullptr;  // -HopX
@@ -521,7 +528,7 @@ extern "C" int negf_finalize(negf_ctx* c) {
   hipFree(c->H); hipFree(c->X); hipFree(c->betaL); hipFree(c->betaR); hipFree(c->tmpP);
   hipFree(c->X32);
   hipFree(c->omega);
-  hipFree(c->sHend); hipFree(c->sBetaL); hipFree(c->sBetaR); hipFree(c->sPair);
+  hipFree(c->sHend); hipFree(c->sBetaL); hipFree(c->sBetaR); hipFree(c->sPair); hipFree(c->d_flag);
   hipFree(c->H0); hipFree(c->V); hipFree(c->Hends);
   hipFree(c->cc.buf);
   for (Lane& l : c->lanes)
@@ -868,6 +875,22 @@ extern "C" int negf_contact_mirror(negf_ctx* c, int* axis, int* perm, int* phase
   return 0;
 }
 
+// Whether every slice (L of them at stride sH on the device) keeps to the site-block
+// stencil; the site potentials of a compact Hamiltonian sit on the diagonal, inside it.
+static int stencil_update(negf_ctx* c, const zd* H, long long sH, int nslices) {
+  c->stencil = false;
+  const int no = c->cfg.norb;
+  if (!(no == 1 || no == 2 || no == 4)) return 0;
+  if (!c->d_flag) HIPCHK(hipMalloc(&c->d_flag, sizeof(int)));
+  int f = 0;
+  HIPCHK(hipMemcpy(c->d_flag, &f, sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(launch_stencil_check(c->st, H, sH, nslices, c->n, c->cfg.npy, c->cfg.npz, no, c->d_flag));
+  HIPCHK(hipMemcpyAsync(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->stencil = f == 0;
+  return 0;
+}
+
 extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const double* HopX,
                                     const double* gM) {
   if (!c || !Htotd || !HopX) {
@@ -889,6 +912,7 @@ extern "C" int negf_set_hamiltonian(negf_ctx* c, const double* Htotd, const doub
   if ((rc = mirror_update(c, (const zd*)HopX, (const zd*)Htotd))) return rc;
   if (!c->H) HIPCHK(hipMalloc(&c->H, sizeof(zd) * nn * L));
   HIPCHK(hipMemcpy(c->H, Htotd, sizeof(zd) * nn * L, hipMemcpyHostToDevice));
+  if ((rc = stencil_update(c, c->H, nn, L))) return rc;
   c->have_h = true;
   return 0;
 }
@@ -915,6 +939,7 @@ extern "C" int negf_set_hamiltonian_compact(negf_ctx* c, const double* H0, const
     HIPCHK(hipMalloc(&c->Hends, sizeof(zd) * nn * 2));
   }
   HIPCHK(hipMemcpy(c->H0, H0, sizeof(zd) * nn, hipMemcpyHostToDevice));
+  if ((rc = stencil_update(c, c->H0, 0, 1))) return rc;
   c->h0_host.assign((const zd*)H0, (const zd*)H0 + nn);
   c->hopx_host.assign((const zd*)HopX, (const zd*)HopX + nn);
   c->compact = true;
@@ -1971,10 +1996,14 @@ static int run_batch(negf_ctx* c, Lane& ln, int nb, const double* energies, std:
         const zd* Hb = c->compact ? (const zd*)c->H0 : (const zd*)c->H;
         const long long sH = c->compact ? 0 : nn;
         for (int sw = 0; sw < kLu32Sweeps; ++sw) {
-          for (int b = 0; b < nb; ++b) {
-            GemmBatch g = shape(gbr(Hb, sH, n, Vr(0) + (long long)b * Es, rs, n, P0 + (long long)b * Es, rs, n,
-                                    n), n, R);
-            TRY(gemm_batched<double>(s, OP_N, OP_N, g, L, kOne, kZero));
+          if (c->stencil) {  // the slices keep to the site-block stencil: 20 n R terms, not n^2 R
+            TRY(launch_stencil_mul(s, no, Hb, sH, Vr(0), Es, rs, P0, Es, rs, n, R, L, nb, c->cfg.npy, c->cfg.npz));
+          } else {
+            for (int b = 0; b < nb; ++b) {
+              GemmBatch g = shape(gbr(Hb, sH, n, Vr(0) + (long long)b * Es, rs, n, P0 + (long long)b * Es, rs, n,
+                                      n), n, R);
+              TRY(gemm_batched<double>(s, OP_N, OP_N, g, L, kOne, kZero));
+            }
           }
           TRY(gemm_batched<double>(s, OP_N, OP_N, shape(gbr(Q(y.o_SigL), Es, n, Vr(0), Es, n, P0, Es, n, n), n, R),
                                    nb, kOne, kOne));

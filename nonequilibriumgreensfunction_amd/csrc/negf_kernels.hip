@@ -551,6 +551,85 @@ __global__ __launch_bounds__(256) void k_sym_maxabs(const double2* M, long long 
   }
 }
 
+// Site-block stencil of a slice (site = y + Npy z): the site itself and its y and z
+// neighbours, periodic wrap included.  Slice Hamiltonians built from on-site blocks
+// and nearest-neighbour y/z hoppings (Build_Hamiltonian.F90:66-95, with or without
+// periodic boundaries, Peierls phases and Zeeman terms) have no other nonzero blocks.
+__device__ inline bool stencil_pair(int si, int sj, int npy, int npz) {
+  if (si == sj) return true;
+  const int yi = si % npy, zi = si / npy, yj = sj % npy, zj = sj / npy;
+  if (zi == zj) {
+    const int d = yi > yj ? yi - yj : yj - yi;
+    return d == 1 || d == npy - 1;
+  }
+  if (yi == yj) {
+    const int d = zi > zj ? zi - zj : zj - zi;
+    return d == 1 || d == npz - 1;
+  }
+  return false;
+}
+
+// flag[0] = 1 when some slice x < nslices (stride sH) has a nonzero entry outside the
+// stencil (a plain store of the same value by every finder; flag cleared before).
+__global__ __launch_bounds__(256) void k_stencil_check(const double2* H, long long sH, int n, int npy, int npz,
+                                                       int norb, int* flag) {
+  const double2* Hx = H + (long long)blockIdx.y * sH;
+  const int nn = n * n;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < nn; e += gridDim.x * 256) {
+    const double2 h = Hx[e];
+    const int i = e % n, j = e / n;
+    if ((h.x != 0.0 || h.y != 0.0) && !stencil_pair(i / norb, j / norb, npy, npz)) flag[0] = 1;
+  }
+}
+
+// P_x = H_x V_x (n x R, FP64) for stencil-sparse slices: grid z = b L + x over the slices
+// of every matrix b; each thread holds its row's stencil entries of H_x in registers and
+// takes kStencilCols columns.  H_x at H + x sH (sH = 0: one slice for all), V_x at V + b sVb
+// + x sVx, P_x likewise, both n x R with leading dimension n.
+constexpr int kStencilCols = 16;
+template <int NORB>
+__global__ __launch_bounds__(256) void k_stencil_mul(const double2* H, long long sH, const double2* V,
+                                                     long long sVb, long long sVx, double2* P, long long sPb,
+                                                     long long sPx, int n, int R, int L, int npy, int npz) {
+  const int b = blockIdx.z / L, x = blockIdx.z % L;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double2* Hx = H + (long long)x * sH;
+  const double2* Vx = V + (long long)b * sVb + (long long)x * sVx;
+  double2* Px = P + (long long)b * sPb + (long long)x * sPx;
+  // the distinct stencil sites of row i's site (wrap-around neighbours may coincide)
+  const int si = i / NORB, yi = si % npy, zi = si / npy;
+  int nb[5];
+  int cnt = 0;
+  auto add = [&](int t) {
+    for (int k = 0; k < cnt; ++k)
+      if (nb[k] == t) return;
+    nb[cnt++] = t;
+  };
+  add(si);
+  add(zi * npy + (yi + 1) % npy);
+  add(zi * npy + (yi + npy - 1) % npy);
+  add(((zi + 1) % npz) * npy + yi);
+  add(((zi + npz - 1) % npz) * npy + yi);
+  double2 h[5][NORB];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int o = 0; o < NORB; ++o)
+      h[k][o] = k < cnt ? Hx[i + (long long)(nb[k] * NORB + o) * n] : make_double2(0.0, 0.0);
+  const int c0 = blockIdx.y * kStencilCols, c1 = min(R, c0 + kStencilCols);
+  for (int c = c0; c < c1; ++c) {
+    const double2* v = Vx + (long long)c * n;
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (k < cnt)
+#pragma unroll
+        for (int o = 0; o < NORB; ++o) acc = cadd(acc, cmul(h[k][o], v[nb[k] * NORB + o]));
+    Px[i + (long long)c * n] = acc;
+  }
+}
+
 // up to three batched copies Y_k = X_k in one pass
 template <typename T>
 __global__ void k_copy3(cx<T>* Y0, const cx<T>* X0, cx<T>* Y1, const cx<T>* X1, cx<T>* Y2,
@@ -1328,6 +1407,32 @@ hipError_t launch_maxabs(hipStream_t s, const cx<T>* X, long long sX, long long 
   hipLaunchKernelGGL((k_maxabs<T>), dim3((unsigned)chunks, batch), dim3(256), 0, s, X, sX, nn, out,
                      active);
   return hipGetLastError();
+}
+
+hipError_t launch_stencil_check(hipStream_t s, const double2* H, long long sH, int nslices, int n, int npy,
+                                int npz, int norb, int* flag) {
+  if (nslices <= 0 || n <= 0) return hipSuccess;
+  if (npy < 1 || npz < 1 || norb < 1 || npy * npz * norb != n) return hipErrorInvalidValue;
+  const int blocks = (int)std::min<long long>(2048, ((long long)n * n + 255) / 256);
+  hipLaunchKernelGGL(k_stencil_check, dim3(blocks, nslices), dim3(256), 0, s, H, sH, n, npy, npz, norb, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_stencil_mul(hipStream_t s, int norb, const double2* H, long long sH, const double2* V,
+                              long long sVb, long long sVx, double2* P, long long sPb, long long sPx, int n, int R,
+                              int L, int batch, int npy, int npz) {
+  if (batch <= 0 || L <= 0 || R <= 0) return hipSuccess;
+  if (npy * npz * norb != n) return hipErrorInvalidValue;
+  const dim3 grid((n + 255) / 256, (R + kStencilCols - 1) / kStencilCols, batch * L);
+#define NEGF_ST(NO)                                                                                        \
+  if (norb == NO) {                                                                                        \
+    hipLaunchKernelGGL((k_stencil_mul<NO>), grid, dim3(256), 0, s, H, sH, V, sVb, sVx, P, sPb, sPx, n, R, L, \
+                       npy, npz);                                                                          \
+    return hipGetLastError();                                                                              \
+  }
+  NEGF_ST(1) NEGF_ST(2) NEGF_ST(4)
+#undef NEGF_ST
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_sym_expand(hipStream_t s, const double2* M, long long sM, long long off1, const int2* pair,

@@ -100,6 +100,25 @@ def test_lu32_matches_oracle(case):
     print(f"energies {ne}, sweeps {sweeps}, escalated {esc}", flush=True)
 
 
+def test_lu32_slice_couplings_off_the_stencil_match_oracle():
+    # The refinement's products H_x V_x take a stencil kernel when every slice keeps to
+    # the site blocks of a site and its y, z neighbours, and the dense product otherwise:
+    # a hopping between opposite corners of the cross-section in the interior slices
+    # (the leads keep the stencil) takes the second path, to the same bounds
+    w = build_wire(7, 3, 3, disorder=0.4, seed=5)
+    no = w.norb
+    a, b = 0, (w.npy * w.npz - 1) * no  # site (0, 0) and site (2, 2)
+    t = 0.15 * np.eye(no) + 0.05j * np.roll(np.eye(no), 1, axis=1)
+    for x in range(1, w.npx - 1):
+        w.htotd[a:a + no, b:b + no, x] += t
+        w.htotd[b:b + no, a:a + no, x] += t.conj().T
+    energies = [-0.9, 0.2, 1.2]
+    ref, noise = P.reference(w, energies, DE, mu=MU, muT=MUT)
+    acc, it, (ne, sweeps, esc) = _run(w, energies, DE)
+    assert np.array_equal(it, ref["iters"])
+    check(_collect(acc), ref, noise, "off-stencil slice couplings")
+
+
 def test_lu32_fp32_matrix_cores_match_oracle():
     # NEGF_FLAG_LU32_FP32_MFMA: the complex64 products on the FP32 matrix cores (3M)
     # instead of the bf16 split form, to the same bounds
